@@ -1,0 +1,553 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of OpenCLIPER's NESTA-TV path (FP32 "reference-faithful" form and FP64 oracle form, one
+// template).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
+// only as the checker; the product path (opencliper_amd) never links or calls it.
+//
+// What each function follows (paths relative to the reference checkout):
+//   fft1d / fft2d ........ clFFT conventions used by src/processes/FFT.cpp:127-146,336-449,510-525
+//                          (2-D C2C, forward e^{-2*pi*i}, unscaled; backward e^{+2*pi*i}, scaled by 1/(H*W);
+//                          no fftshift).  The clFFT kernels themselves are third-party and not in the checkout.
+//   ttv_fwd / ttv_adj .... src/kernels/temporalTV.cl:49-76 (operator_tTV) and :78-107 (operator_tTVadj = -D^T)
+//   vecnorm .............. src/kernels/vectorNormalization.cl:42-56
+//   cep .................. src/kernels/complexElementProd.cl:42-76 + src/processes/ComplexElementProd.cpp:117
+//   apply_mask ........... src/kernels/applyMask.cl:5-40
+//   ximage_sum ........... src/kernels/xImageSum.cl:42-61
+//   abs / abspow2 ........ src/kernels/complexAbs.cl:43-70, src/kernels/complexAbsPow2.cl:43-71
+//   nrm2/dotc/csum/asum .. CLBlast Scnrm2 / Cdotc (conjugates x) / Scsum / Scasum as called from
+//                          src/processes/nesta/NestaUp.cpp (summation order here: sequential, in double)
+//   op_A / op_At ......... src/processes/nesta/NestaUp.cpp:162-190 / :117-151
+//   normest .............. src/processes/nesta/NestaUp.cpp:291-439 (power method, drand48 restart)
+//   nesta ................ src/processes/nesta/NestaUp.cpp:442-910 (continuation NESTA, stop test 1)
+//   simple_recon ......... src/processes/examples/SimpleMRIRecon.cpp:59-101
+//
+// Pinning (each checked in tests/test_oracle.py): fft vs brute-force DFT and numpy.fft; ttv vs closed form on
+// integers and the adjoint identity; normest's LCG vs libc drand48 from the POSIX default state and its result vs
+// the closed-form sigma_max of the periodic difference; vecnorm vs the Huber identity; op_A/op_At adjointness and
+// E^H E = I when fully sampled; nesta vs an independent numpy FP64 implementation (tests/golden/make_golden.py).
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <climits>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+template <class T> struct Cx { T re, im; };
+
+// ------------------------------------------------------------------------------------------------------------------
+// Mixed-radix Stockham FFT (natural order in and out).  Twiddles computed in double, rounded to T.
+// ------------------------------------------------------------------------------------------------------------------
+template <class T> struct Plan {
+    int n = 0;
+    std::vector<int> radix;
+    std::vector<Cx<T>> w;   // w[m] = exp(-2*pi*i*m/n)
+};
+
+static std::vector<int> factorize(int n) {
+    std::vector<int> r;
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    for (int p : {2, 3, 5, 7}) while (n % p == 0) { r.push_back(p); n /= p; }
+    for (int p = 11; n > 1; p += 2) while (n % p == 0) { r.push_back(p); n /= p; }
+    return r;
+}
+
+template <class T> Plan<T> make_plan(int n) {
+    Plan<T> p;
+    p.n = n;
+    p.radix = factorize(n);
+    p.w.resize(n);
+    for (int m = 0; m < n; ++m) {
+        double a = -2.0 * M_PI * (double)m / (double)n;
+        p.w[m].re = (T)std::cos(a);
+        p.w[m].im = (T)std::sin(a);
+    }
+    return p;
+}
+
+// Forward transform of one sequence (in -> out, both length n, unit stride).  tmp has length n.
+template <class T> void fft_forward(const Plan<T>& P, const Cx<T>* in, Cx<T>* out, Cx<T>* tmp) {
+    const int n = P.n;
+    if (n == 1) { out[0] = in[0]; return; }
+    const Cx<T>* src = in;
+    Cx<T>* bufs[2] = {out, tmp};
+    // Choose the first destination so that the last stage lands in `out`.
+    int ns = (int)P.radix.size();
+    int cur = (ns % 2 == 1) ? 0 : 1;
+    int Ls = 1;
+    Cx<T> v[64], u[64];
+    for (int s = 0; s < ns; ++s) {
+        const int R = P.radix[s];
+        const int m = n / R;
+        Cx<T>* dst = bufs[cur];
+        for (int j = 0; j < m; ++j) {
+            const int k = j % Ls;
+            for (int r = 0; r < R; ++r) {
+                Cx<T> a = src[j + r * m];
+                long tix = ((long)r * k * (n / (Ls * R))) % n;
+                Cx<T> t = P.w[tix];
+                v[r].re = a.re * t.re - a.im * t.im;
+                v[r].im = a.re * t.im + a.im * t.re;
+            }
+            for (int q = 0; q < R; ++q) {
+                T sr = 0, si = 0;
+                for (int r = 0; r < R; ++r) {
+                    Cx<T> t = P.w[((long)(q * r) % R) * m];
+                    sr += v[r].re * t.re - v[r].im * t.im;
+                    si += v[r].re * t.im + v[r].im * t.re;
+                }
+                u[q].re = sr; u[q].im = si;
+            }
+            const int base = (j / Ls) * Ls * R + k;
+            for (int q = 0; q < R; ++q) dst[base + q * Ls] = u[q];
+        }
+        src = dst;
+        cur ^= 1;
+        Ls *= R;
+    }
+}
+
+// 2-D transform of `batch` images [H][W] (W fastest).  dir < 0: forward, unscaled.  dir > 0: backward, 1/(H*W).
+template <class T> void fft2d(const Cx<T>* in, Cx<T>* out, long batch, int H, int W, int dir) {
+    Plan<T> pw = make_plan<T>(W), ph = make_plan<T>(H);
+    const long N = (long)H * W;
+    const bool inv = dir > 0;
+    const T scale = inv ? (T)(1.0 / (double)N) : (T)1;
+#pragma omp parallel
+    {
+        std::vector<Cx<T>> a(std::max(H, W)), b(std::max(H, W)), t(std::max(H, W));
+#pragma omp for schedule(static)
+        for (long img = 0; img < batch; ++img) {
+            const Cx<T>* I = in + img * N;
+            Cx<T>* O = out + img * N;
+            for (int h = 0; h < H; ++h) {
+                for (int x = 0; x < W; ++x) { a[x] = I[(long)h * W + x]; if (inv) a[x].im = -a[x].im; }
+                fft_forward(pw, a.data(), b.data(), t.data());
+                for (int x = 0; x < W; ++x) O[(long)h * W + x] = b[x];
+            }
+            for (int x = 0; x < W; ++x) {
+                for (int h = 0; h < H; ++h) a[h] = O[(long)h * W + x];
+                fft_forward(ph, a.data(), b.data(), t.data());
+                for (int h = 0; h < H; ++h) {
+                    Cx<T> z = b[h];
+                    if (inv) { z.im = -z.im; z.re *= scale; z.im *= scale; }
+                    O[(long)h * W + x] = z;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Element-wise operators (layouts: XData [F][N]; KData [F][C][N]; sensitivity maps [C][N]; masks [F][N] uint8)
+// ------------------------------------------------------------------------------------------------------------------
+
+// temporalTV.cl:49-76: out_0 = in_0 - in_{F-1}; out_f = in_f - in_{f-1}
+template <class T> void ttv_fwd(const Cx<T>* in, Cx<T>* out, int F, long N) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p) {
+        for (int f = 0; f < F; ++f) {
+            int g = (f == 0) ? F - 1 : f - 1;
+            out[f * N + p].re = in[f * N + p].re - in[g * N + p].re;
+            out[f * N + p].im = in[f * N + p].im - in[g * N + p].im;
+        }
+    }
+}
+
+// temporalTV.cl:78-107: out_f = in_{f+1} - in_f (f < F-1); out_{F-1} = in_0 - in_{F-1}
+template <class T> void ttv_adj(const Cx<T>* in, Cx<T>* out, int F, long N) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p) {
+        for (int f = 0; f < F; ++f) {
+            int g = (f == F - 1) ? 0 : f + 1;
+            out[f * N + p].re = in[g * N + p].re - in[f * N + p].re;
+            out[f * N + p].im = in[g * N + p].im - in[f * N + p].im;
+        }
+    }
+}
+
+// vectorNormalization.cl:42-56: out = in / max(mu, |in|)
+template <class T> void vecnorm(const Cx<T>* in, Cx<T>* out, long n, T mu) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        T len = std::sqrt(in[i].re * in[i].re + in[i].im * in[i].im);
+        T fac = std::max(mu, len);
+        out[i].re = in[i].re / fac;
+        out[i].im = in[i].im / fac;
+    }
+}
+
+// complexElementProd.cl:42-76.  in_coils == false: input is an XData [F][N] broadcast over coils.
+template <class T> void cep(const Cx<T>* in, const Cx<T>* S, Cx<T>* out, int F, int C, long N, bool in_coils,
+                            bool conj) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p) {
+        for (int f = 0; f < F; ++f)
+            for (int c = 0; c < C; ++c) {
+                Cx<T> a = in_coils ? in[((long)f * C + c) * N + p] : in[(long)f * N + p];
+                Cx<T> s = S[(long)c * N + p];
+                if (conj) s.im = -s.im;
+                Cx<T> r;
+                r.re = a.re * s.re - a.im * s.im;
+                r.im = a.re * s.im + a.im * s.re;
+                out[((long)f * C + c) * N + p] = r;
+            }
+    }
+}
+
+// applyMask.cl:5-40: zero where mask == 0
+template <class T> void apply_mask(Cx<T>* d, const uint8_t* M, int F, int C, long N) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p)
+        for (int f = 0; f < F; ++f)
+            if (M[(long)f * N + p] == 0)
+                for (int c = 0; c < C; ++c) { d[((long)f * C + c) * N + p].re = 0; d[((long)f * C + c) * N + p].im = 0; }
+}
+
+// xImageSum.cl:42-61: x_f = sum_c in_{f,c}, sequential over c
+template <class T> void ximage_sum(const Cx<T>* in, Cx<T>* out, int F, int C, long N) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p)
+        for (int f = 0; f < F; ++f) {
+            T sr = 0, si = 0;
+            for (int c = 0; c < C; ++c) { sr += in[((long)f * C + c) * N + p].re; si += in[((long)f * C + c) * N + p].im; }
+            out[(long)f * N + p].re = sr;
+            out[(long)f * N + p].im = si;
+        }
+}
+
+// BLAS-1 reductions: sequential summation in double, result rounded to T.
+template <class T> T nrm2(const Cx<T>* x, long n) {
+    double s = 0;
+    for (long i = 0; i < n; ++i) s += (double)x[i].re * x[i].re + (double)x[i].im * x[i].im;
+    return (T)std::sqrt(s);
+}
+template <class T> T dotc_re(const Cx<T>* x, const Cx<T>* y, long n) {   // Re sum conj(x) y
+    double s = 0;
+    for (long i = 0; i < n; ++i) s += (double)x[i].re * y[i].re + (double)x[i].im * y[i].im;
+    return (T)s;
+}
+template <class T> T asum(const Cx<T>* x, long n) {
+    double s = 0;
+    for (long i = 0; i < n; ++i) s += std::fabs((double)x[i].re) + std::fabs((double)x[i].im);
+    return (T)s;
+}
+template <class T> T sum_abspow2(const Cx<T>* x, long n) {  // ComplexAbsPow2 (rounded to T per element) + Scsum
+    double s = 0;
+    for (long i = 0; i < n; ++i) { T v = x[i].re * x[i].re + x[i].im * x[i].im; s += (double)v; }
+    return (T)s;
+}
+template <class T> T max_abs(const Cx<T>* x, long n) {   // ComplexAbs + host max loop (NestaUp.cpp:521-567)
+    T m = (T)LONG_MIN;
+    for (long i = 0; i < n; ++i) { T v = std::sqrt(x[i].re * x[i].re + x[i].im * x[i].im); if (v > m) m = v; }
+    return m;
+}
+
+// y = a*x + y, x = a*x, copy
+template <class T> void axpy(long n, T a, const Cx<T>* x, Cx<T>* y) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) { y[i].re += a * x[i].re; y[i].im += a * x[i].im; }
+}
+template <class T> void scal(long n, T a, Cx<T>* x) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) { x[i].re *= a; x[i].im *= a; }
+}
+template <class T> void copy(long n, const Cx<T>* x, Cx<T>* y) { std::memcpy(y, x, sizeof(Cx<T>) * n); }
+
+// ------------------------------------------------------------------------------------------------------------------
+// Encoding operators (NestaUp.cpp:117-190)
+// ------------------------------------------------------------------------------------------------------------------
+template <class T> struct Prob {
+    int F, C, H, W;
+    long N;
+    const Cx<T>* y;       // [F][C][N]
+    const Cx<T>* S;       // [C][N]
+    const uint8_t* M;     // [F][N]
+};
+
+// operatorA: out = mask(FFT_fwd(S * x)); aux is KData-sized scratch
+template <class T> void op_A(const Prob<T>& P, const Cx<T>* x, Cx<T>* out, Cx<T>* aux) {
+    cep(x, P.S, aux, P.F, P.C, P.N, false, false);
+    fft2d(aux, out, (long)P.F * P.C, P.H, P.W, -1);
+    apply_mask(out, P.M, P.F, P.C, P.N);
+}
+
+// operatorAt: x = sum_c conj(S_c) * FFT_bwd(r)  (single coil: plain copy)
+template <class T> void op_At(const Prob<T>& P, const Cx<T>* r, Cx<T>* x, Cx<T>* aux) {
+    fft2d(r, aux, (long)P.F * P.C, P.H, P.W, +1);
+    cep(aux, P.S, aux, P.F, P.C, P.N, true, true);
+    if (P.C != 1) ximage_sum(aux, x, P.F, P.C, P.N);
+    else copy((long)P.F * P.N, aux, x);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// drand48 from the POSIX default state (X0 = 0x1234ABCD330E, a = 0x5DEECE66D, c = 0xB, modulus 2^48).
+// ------------------------------------------------------------------------------------------------------------------
+struct Drand48 {
+    uint64_t x = 0x1234ABCD330EULL;
+    double next() {
+        x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+        return (double)x / 281474976710656.0;
+    }
+};
+
+// myNormest (NestaUp.cpp:291-439): power method for the spectral norm of U = tTV, FP-typed like the reference.
+template <class T> T normest(int F, long N, int* iters_out) {
+    const long n = (long)F * N;
+    const T tol = (T)1e-3;
+    const unsigned maxiter = 100;
+    unsigned cnt = 0;
+    std::vector<Cx<T>> x(n, Cx<T>{(T)1, (T)0}), Sx(n), aux(n);
+    T e = nrm2(x.data(), n);
+    scal(n, (T)1 / e, x.data());
+    T e0 = 0;
+    Drand48 rng;
+    while ((std::abs(e - e0) > (tol * e)) && (cnt < maxiter)) {
+        e0 = e;
+        ttv_fwd(x.data(), Sx.data(), F, N);
+        T nnz = asum(Sx.data(), n);
+        if (nnz == 0) {
+            for (long i = 0; i < n; ++i) { Sx[i].re = (T)rng.next(); Sx[i].im = 0; }
+        }
+        e = nrm2(Sx.data(), n);
+        ttv_adj(Sx.data(), aux.data(), F, N);
+        copy(n, aux.data(), x.data());
+        T nx = nrm2(x.data(), n);
+        scal(n, (T)1 / nx, x.data());
+        cnt = cnt + 1;
+    }
+    if (iters_out) *iters_out = (int)cnt;
+    return e;
+}
+
+}  // namespace
+
+// ======================================================================================================================
+// C ABI (ctypes).  Complex arrays are interleaved (re, im).
+// ======================================================================================================================
+extern "C" {
+
+struct orc_params {          // NestaUp::LaunchParameters field order (include/.../nesta/NestaUp.hpp:67-83)
+    double lambda_i, mu_f, La;
+    int maxIntIter;
+    double tolVar;
+    int verbose, maxIter, stoptest, miniter;
+};
+
+#define ORC_MAX_STAGES 16
+#define ORC_TRACE 4096
+struct orc_report {
+    double normU, maxXref, maxUXref, lambda, muf, mu0, gamma, gammat, final_tolVar, final_mu;
+    int normest_iters, stages;
+    int iters[ORC_MAX_STAGES];       // inner iterations executed per stage
+    int ntrace;                      // number of f values recorded
+    double trace_f[ORC_TRACE];       // f per inner iteration (all stages, in order)
+    double trace_l2[ORC_TRACE];      // ||r||^2 per inner iteration
+    double trace_l1[ORC_TRACE];      // fx - mu/2 ||u||^2 per inner iteration
+};
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
+void orc_fft2d_f32(const float* in, float* out, long batch, int H, int W, int dir) {
+    fft2d((const Cx<float>*)in, (Cx<float>*)out, batch, H, W, dir);
+}
+void orc_fft2d_f64(const double* in, double* out, long batch, int H, int W, int dir) {
+    fft2d((const Cx<double>*)in, (Cx<double>*)out, batch, H, W, dir);
+}
+void orc_ttv_f32(const float* in, float* out, int F, long N, int dir) {
+    if (dir) ttv_fwd((const Cx<float>*)in, (Cx<float>*)out, F, N); else ttv_adj((const Cx<float>*)in, (Cx<float>*)out, F, N);
+}
+void orc_ttv_f64(const double* in, double* out, int F, long N, int dir) {
+    if (dir) ttv_fwd((const Cx<double>*)in, (Cx<double>*)out, F, N); else ttv_adj((const Cx<double>*)in, (Cx<double>*)out, F, N);
+}
+void orc_vecnorm_f32(const float* in, float* out, long n, float mu) { vecnorm((const Cx<float>*)in, (Cx<float>*)out, n, mu); }
+void orc_cep_f32(const float* in, const float* S, float* out, int F, int C, long N, int in_coils, int conj) {
+    cep((const Cx<float>*)in, (const Cx<float>*)S, (Cx<float>*)out, F, C, N, in_coils != 0, conj != 0);
+}
+void orc_apply_mask_f32(float* d, const uint8_t* M, int F, int C, long N) { apply_mask((Cx<float>*)d, M, F, C, N); }
+void orc_ximage_sum_f32(const float* in, float* out, int F, int C, long N) {
+    ximage_sum((const Cx<float>*)in, (Cx<float>*)out, F, C, N);
+}
+double orc_nrm2_f32(const float* x, long n) { return nrm2((const Cx<float>*)x, n); }
+double orc_dotc_re_f32(const float* x, const float* y, long n) { return dotc_re((const Cx<float>*)x, (const Cx<float>*)y, n); }
+double orc_asum_f32(const float* x, long n) { return asum((const Cx<float>*)x, n); }
+
+void orc_op_A_f64(const double* x, const double* S, const uint8_t* M, double* out, int F, int C, int H, int W) {
+    Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, M};
+    std::vector<Cx<double>> aux((long)F * C * H * W);
+    op_A(P, (const Cx<double>*)x, (Cx<double>*)out, aux.data());
+}
+void orc_op_At_f64(const double* r, const double* S, double* x, int F, int C, int H, int W) {
+    Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, nullptr};
+    std::vector<Cx<double>> aux((long)F * C * H * W);
+    op_At(P, (const Cx<double>*)r, (Cx<double>*)x, aux.data());
+}
+
+double orc_normest_f32(int F, long N, int* iters) { return normest<float>(F, N, iters); }
+double orc_normest_f64(int F, long N, int* iters) { return normest<double>(F, N, iters); }
+double orc_drand48_nth(long k) {  // value of the k-th (1-based) drand48() call from the default state
+    Drand48 r; double v = 0; for (long i = 0; i < k; ++i) v = r.next(); return v;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------------------
+// NESTA (NestaUp::launch, NestaUp.cpp:442-910)
+// ------------------------------------------------------------------------------------------------------------------
+namespace {
+
+template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& prm, orc_report* rep) {
+    const int F = P.F, C = P.C;
+    const long N = P.N, nX = (long)F * N, nK = (long)F * C * N;
+    std::vector<Cx<T>> yS(nK), aux(nK), Res(nK), AuxRes(nK);
+    std::vector<Cx<T>> Ux(nX), Wk(nX), Xk(nX), Yk(nX), Zk(nX), Uk(nX), AuxFx(nX), Df(nX), ARes(nX), tmp(nX);
+
+    // y_s = sqrt(N) * y ; x_ref = A^H(y_s)                                              (:468-486)
+    copy(nK, P.y, yS.data());
+    const T sqrtN = (T)std::sqrt((T)(P.H * P.W));
+    scal(nK, sqrtN, yS.data());
+    op_At(P, yS.data(), xref, aux.data());
+    ttv_fwd(xref, Ux.data(), F, N);                                                        // (:491)
+
+    const T maxXref = max_abs(xref, nX);                                                   // (:521-545)
+    const T maxUXref = max_abs(Ux.data(), nX);                                             // (:547-568)
+    const int miniter = prm.miniter;
+    const T lambda = (T)prm.lambda_i * maxXref;                                            // (:570-583)
+    const T muf = (T)prm.mu_f * maxXref;
+    T mu0 = (T)(0.9 * (double)maxUXref);                                                 // double product, as in C++
+    int ne_it = 0;
+    const T normU = normest<T>(F, N, &ne_it);
+    const T muL = lambda / (T)prm.La;
+    if (muL > mu0) mu0 = muL;
+    const T gamma = (T)std::pow((double)(muf / mu0), 1.0 / prm.maxIntIter);
+    T mu = mu0;
+    const T gammat = (T)std::pow((double)(T)prm.tolVar / 0.1, 1.0 / prm.maxIntIter);  // float / double literal
+    T tolVar = (T)0.1;
+
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->normU = normU; rep->maxXref = maxXref; rep->maxUXref = maxUXref; rep->lambda = lambda;
+        rep->muf = muf; rep->mu0 = mu0; rep->gamma = gamma; rep->gammat = gammat; rep->normest_iters = ne_it;
+    }
+    std::vector<T> fmean(std::max(miniter, 1), (T)0);
+    T qp = 0;
+    const T invSqrtN = (T)(1.0 / std::sqrt((double)(P.H * P.W)));
+
+    for (int nl = 1; nl <= prm.maxIntIter; ++nl) {
+        T Ak, Lmu, Lmu1;
+        mu = mu * gamma;                                                                   // (:608-609)
+        tolVar = tolVar * gammat;
+        std::fill(Wk.begin(), Wk.end(), Cx<T>{0, 0});                                      // (:615-641)
+        copy(nX, xref, Xk.data());
+        std::fill(Yk.begin(), Yk.end(), Cx<T>{0, 0});
+        std::fill(Zk.begin(), Zk.end(), Cx<T>{0, 0});
+        Ak = 0;                                                                            // (:643-649)
+        Lmu = normU / mu;
+        fmean[miniter - 1] = (T)LONG_MAX;
+        Lmu = lambda * Lmu + (T)prm.La;
+        Lmu1 = (T)1 / Lmu;
+        bool qpDone = false;
+        int done_k = 0;
+        for (int k = 0; k < prm.maxIter; ++k) {
+            ++done_k;
+            ttv_fwd(Xk.data(), Uk.data(), F, N);                                           // (:669)
+            copy(nX, Uk.data(), AuxFx.data());                                             // (:675)
+            vecnorm(Uk.data(), Uk.data(), nX, mu);                                         // (:680-683)
+            const T normUk = nrm2(Uk.data(), nX);                                          // (:686)
+            T fx = dotc_re(Uk.data(), AuxFx.data(), nX);                                   // (:691)
+            ttv_adj(Uk.data(), tmp.data(), F, N);                                          // (:696) Ut = adj kernel + copy
+            copy(nX, tmp.data(), Df.data());
+            scal(nX, (T)-1, Df.data());                                                    // (:698-704)
+            op_A(P, Xk.data(), Res.data(), aux.data());                                    // (:710)
+            scal(nK, invSqrtN, Res.data());                                                // (:712-718)
+            axpy(nK, (T)-1, P.y, Res.data());                                              // (:720-727)
+            copy(nK, Res.data(), AuxRes.data());                                           // (:732)
+            const T l2 = sum_abspow2(Res.data(), nK);                                      // (:738-746)
+            scal(nK, sqrtN, AuxRes.data());                                                // (:749-753)
+            op_At(P, AuxRes.data(), ARes.data(), aux.data());                              // (:756)
+            scal(nX, lambda, Df.data());                                                   // (:759-764)
+            axpy(nX, (T)1, ARes.data(), Df.data());                                        // (:766-772)
+            copy(nX, Xk.data(), Yk.data());                                                // (:776-779)
+            axpy(nX, -Lmu1, Df.data(), Yk.data());                                         // (:781-786)
+
+            const T l1 = (T)(fx - (mu / (T)2) * std::pow((double)normUk, 2));              // (:797-798)
+            fx = (T)0.5 * l2 + lambda * l1;
+            if (rep && rep->ntrace < ORC_TRACE) {
+                rep->trace_f[rep->ntrace] = fx; rep->trace_l2[rep->ntrace] = l2; rep->trace_l1[rep->ntrace] = l1;
+                rep->ntrace++;
+            }
+            if (prm.stoptest == 1) {                                                       // (:799-824)
+                T fmeanMean = 0;
+                bool skip_write = false;
+                if (k >= miniter) {
+                    for (int ne = 0; ne < miniter; ++ne) fmeanMean += fmean[miniter - ne - 1];
+                    fmeanMean = fmeanMean / (T)miniter;
+                    qp = (fmeanMean - fx) / fmeanMean;
+                    if (qp <= tolVar) {
+                        if (qpDone) k = prm.maxIter;
+                        qpDone = true;
+                        skip_write = true;
+                    }
+                } else {
+                    qp = fmean[miniter - 1];
+                }
+                if (!skip_write) fmean[(miniter - 1) - ((k + 1) % miniter)] = fx;
+            }
+            const T apk = (T)0.5 * ((T)k + (T)1);                                          // (:829-831)
+            Ak = Ak + apk;
+            const T tauk = (T)2 / ((T)k + (T)3);
+            axpy(nX, apk, Df.data(), Wk.data());                                           // (:837)
+            copy(nX, xref, Zk.data());                                                     // (:843)
+            axpy(nX, -Lmu1, Wk.data(), Zk.data());                                         // (:850)
+            copy(nX, Zk.data(), Xk.data());                                                // (:858)
+            scal(nX, tauk, Xk.data());                                                     // (:865)
+            axpy(nX, (T)1 - tauk, Yk.data(), Xk.data());                                   // (:873)
+        }
+        copy(nX, Xk.data(), xref);                                                         // (:889)
+        if (rep && nl <= ORC_MAX_STAGES) { rep->iters[nl - 1] = done_k; rep->stages = nl; }
+    }
+    if (rep) { rep->final_tolVar = tolVar; rep->final_mu = mu; }
+    (void)qp;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Full NESTA-TV reconstruction.  y: [F][C][H][W], S: [C][H][W], M: [F][H][W] u8, x_out: [F][H][W].
+int orc_nesta_f32(const float* y, const float* S, const uint8_t* M, float* x_out, int F, int C, int H, int W,
+                  const orc_params* prm, orc_report* rep) {
+    if (!prm || prm->miniter < 1 || F < 1 || C < 1) return -1;
+    Prob<float> P{F, C, H, W, (long)H * W, (const Cx<float>*)y, (const Cx<float>*)S, M};
+    return nesta(P, (Cx<float>*)x_out, *prm, rep);
+}
+int orc_nesta_f64(const double* y, const double* S, const uint8_t* M, double* x_out, int F, int C, int H, int W,
+                  const orc_params* prm, orc_report* rep) {
+    if (!prm || prm->miniter < 1 || F < 1 || C < 1) return -1;
+    Prob<double> P{F, C, H, W, (long)H * W, (const Cx<double>*)y, (const Cx<double>*)S, M};
+    return nesta(P, (Cx<double>*)x_out, *prm, rep);
+}
+
+// SimpleMRIRecon (SimpleMRIRecon.cpp:59-101): x_f = sum_c conj(S_c) * IFFT2_{1/N}(y_{f,c})
+int orc_simple_recon_f32(const float* y, const float* S, float* x, int F, int C, int H, int W) {
+    long nK = (long)F * C * H * W;
+    std::vector<Cx<float>> aux(nK);
+    Prob<float> P{F, C, H, W, (long)H * W, nullptr, (const Cx<float>*)S, nullptr};
+    op_At(P, (const Cx<float>*)y, (Cx<float>*)x, aux.data());
+    return 0;
+}
+int orc_simple_recon_f64(const double* y, const double* S, double* x, int F, int C, int H, int W) {
+    long nK = (long)F * C * H * W;
+    std::vector<Cx<double>> aux(nK);
+    Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, nullptr};
+    op_At(P, (const Cx<double>*)y, (Cx<double>*)x, aux.data());
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""Pins the oracle (oracle/nesta_oracle.cpp) against closed forms, libc, numpy and the golden fixtures."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from opencliper_amd import synth
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def brute_dft2(a, inverse=False):
+    H, W = a.shape[-2:]
+    s = 1 if inverse else -1
+    Fh = np.exp(s * 2j * np.pi * np.outer(np.arange(H), np.arange(H)) / H)
+    Fw = np.exp(s * 2j * np.pi * np.outer(np.arange(W), np.arange(W)) / W)
+    out = Fh @ a @ Fw.T
+    return out / (H * W) if inverse else out
+
+
+@pytest.mark.parametrize("H,W", [(2, 3), (4, 4), (5, 7), (8, 6), (9, 10), (12, 14), (16, 15)])
+def test_fft_vs_bruteforce(oracle, H, W):
+    rng = np.random.default_rng(H * 100 + W)
+    a = rng.normal(size=(2, H, W)) + 1j * rng.normal(size=(2, H, W))
+    for inv in (False, True):
+        ref = np.stack([brute_dft2(a[i], inv) for i in range(2)])
+        got = oracle.fft2d(a, inverse=inv)
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("H,W", [(128, 128), (160, 192), (224, 256), (288, 320)])
+def test_fft_vs_numpy(oracle, H, W):
+    rng = np.random.default_rng(H + W)
+    a = rng.normal(size=(2, H, W)) + 1j * rng.normal(size=(2, H, W))
+    assert np.abs(oracle.fft2d(a) - np.fft.fft2(a)).max() <= 1e-12 * np.abs(np.fft.fft2(a)).max()
+    assert np.abs(oracle.fft2d(a, inverse=True) - np.fft.ifft2(a)).max() <= 1e-12 * np.abs(np.fft.ifft2(a)).max()
+    f32 = oracle.fft2d(a, dtype=np.complex64)
+    assert np.abs(f32 - np.fft.fft2(a)).max() <= 1e-5 * np.abs(np.fft.fft2(a)).max()
+
+
+def test_ttv_closed_form_integers(oracle):
+    rng = np.random.default_rng(1)
+    F, N = 7, 33
+    x = (rng.integers(-50, 50, (F, N)) + 1j * rng.integers(-50, 50, (F, N))).astype(np.complex64)
+    fwd = oracle.ttv(x, True, np.complex64)
+    adj = oracle.ttv(x, False, np.complex64)
+    exp_fwd = np.empty_like(x)
+    exp_adj = np.empty_like(x)
+    for f in range(F):
+        exp_fwd[f] = x[f] - x[f - 1]            # f = 0 wraps to F-1
+        exp_adj[f] = x[(f + 1) % F] - x[f]
+    assert np.array_equal(fwd, exp_fwd) and np.array_equal(adj, exp_adj)
+
+
+def test_ttv_adjoint_identity(oracle):
+    rng = np.random.default_rng(2)
+    F, N = 6, 40
+    x = rng.normal(size=(F, N)) + 1j * rng.normal(size=(F, N))
+    v = rng.normal(size=(F, N)) + 1j * rng.normal(size=(F, N))
+    Dx = oracle.ttv(x, True)
+    negDtv = oracle.ttv(v, False)
+    assert abs(np.vdot(v, Dx) - np.vdot(-negDtv, x)) < 1e-10 * np.abs(np.vdot(v, Dx))
+
+
+def test_drand48_matches_libc(oracle):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.seed48((ctypes.c_ushort * 3)(0x330E, 0xABCD, 0x1234))   # the POSIX default state
+    ref = [libc.drand48() for _ in range(1000)]
+    got = [oracle.lib().orc_drand48_nth(k) for k in (1, 2, 3, 500, 1000)]
+    assert got == [ref[0], ref[1], ref[2], ref[499], ref[999]]
+
+
+@pytest.mark.parametrize("F", [3, 4, 7, 8, 20, 25])
+def test_normest_power_method(oracle, F):
+    N = 256
+    e64, it64 = oracle.normest(F, N, double=True)
+    e32, it32 = oracle.normest(F, N, double=False)
+    exact = 2.0 if F % 2 == 0 else 2 * np.cos(np.pi / (2 * F))   # sigma_max of the periodic difference
+    assert it64 == it32
+    assert abs(e32 - e64) < 1e-6 * e64
+    assert e64 <= exact + 1e-12 and e64 > 0.97 * exact     # power method converges from below
+
+
+def test_vecnorm_huber_identity(oracle):
+    rng = np.random.default_rng(3)
+    z = (rng.normal(size=4096) + 1j * rng.normal(size=4096)).astype(np.complex64)
+    mu = np.float32(0.7)
+    u = oracle.vecnorm(z, mu)
+    t = np.abs(z.astype(np.complex128))
+    lhs = np.real(np.vdot(u.astype(np.complex128), z)) - mu / 2 * np.linalg.norm(u.astype(np.complex128)) ** 2
+    huber = np.where(t < mu, t * t / (2 * mu), t - mu / 2).sum()
+    assert abs(lhs - huber) < 1e-5 * huber
+
+
+def test_encoding_adjoint_and_identity(oracle):
+    sl = synth.make_cine(H=16, W=12, C=3, F=4, accel=2.0, n_centre=4, slice_id=9)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(4, 16, 12)) + 1j * rng.normal(size=(4, 16, 12))
+    r = rng.normal(size=(4, 3, 16, 12)) + 1j * rng.normal(size=(4, 3, 16, 12))
+    Ax = oracle.op_A(x, sl.S, sl.M)                  # mask * FFT2 (unscaled)
+    r = r * sl.M[:, None]
+    AHr = oracle.op_At(r, sl.S) * (16 * 12)          # op_At uses the 1/N backward FFT
+    assert abs(np.vdot(r, Ax) - np.vdot(AHr, x)) < 1e-10 * abs(np.vdot(r, Ax))
+    ones = np.ones_like(sl.M)
+    x_back = oracle.op_At(oracle.op_A(x, sl.S, ones), sl.S)   # fully sampled, RSS-normalised maps: E^H E = I
+    assert np.abs(x_back - x).max() < 1e-6
+
+
+def test_simple_recon_closed_form(oracle):
+    sl = synth.make_fully_sampled(H=32, W=24, C=4, F=1)
+    k = np.fft.fft2(sl.S[None].astype(np.complex128) * sl.x_true[:, None], axes=(-2, -1))
+    x = oracle.simple_recon(k, sl.S)
+    assert np.abs(x - sl.x_true).max() < 1e-6
+    x32 = oracle.simple_recon(sl.y, sl.S, double=False)
+    assert oracle.rel_l2(x32, sl.x_true) < 2e-2          # noise at 1e-3 of the k-space peak
+
+
+@pytest.mark.parametrize("name", sorted(json.load(open(os.path.join(GOLD, "golden.json"))).keys()))
+def test_nesta_golden(oracle, name):
+    info = json.load(open(os.path.join(GOLD, "golden.json")))[name]
+    d = np.load(os.path.join(GOLD, f"{name}.npz"))
+    prm = oracle.params(**info["nesta"])
+    x64, rep = oracle.nesta(d["y"], d["S"], d["M"], prm, double=True)
+    assert rep["iters"] == info["iters"]
+    assert abs(rep["normU"] - info["normU"]) < 1e-12
+    assert rep["normest_iters"] == info["normest_iters"]
+    assert np.abs(rep["trace_f"] - np.array(info["trace_f"])).max() < 1e-9 * max(1, np.abs(info["trace_f"]).max())
+    assert oracle.rel_l2(x64, d["x"]) < 1e-12
+    x32, rep32 = oracle.nesta(d["y"], d["S"], d["M"], prm, double=False)
+    assert rep32["iters"] == info["iters"]
+    assert oracle.rel_l2(x32, d["x"]) < 1e-4
+
+
+def test_planted_mistake_is_caught(oracle):
+    """A plausible slip (dropping the sqrt(N) prescale of y) must move the result far beyond the parity bars."""
+    d = np.load(os.path.join(GOLD, "g2_f4c3_12x10_fixed.npz"))
+    info = json.load(open(os.path.join(GOLD, "golden.json")))["g2_f4c3_12x10_fixed"]
+    prm = oracle.params(**info["nesta"])
+    x_bad, _ = oracle.nesta(d["y"] * np.sqrt(12 * 10), d["S"], d["M"], prm, double=True)
+    assert oracle.rel_l2(x_bad, d["x"]) > 1e-2
