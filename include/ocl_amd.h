@@ -1,0 +1,137 @@
+/*
+ * C ABI of libocl_amd.so — the MI355X (gfx950) implementation of OpenCLIPER's NESTA-TV hot path.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success or a negative OCL_E* code; the message of the
+ * last failure on the calling thread is returned by ocl_last_error().  "Device pointers" are HBM addresses obtained
+ * from ocl_malloc (or any hipMalloc'ed memory of the same device).  Complex data is interleaved float (re, im).
+ *
+ * Layouts (W fastest, no padding, no in-band header):
+ *   XData  [F][H][W] complex64        KData  [F][C][H][W] complex64
+ *   SensitivityMapsData [C][H][W]     SamplingMasksData [F][H][W] uint8 (non-zero = sampled)
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference checkout).
+ */
+#ifndef OCL_AMD_H
+#define OCL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCL_OK 0
+#define OCL_E_INVALID -1     /* bad argument (std::invalid_argument in the reference) */
+#define OCL_E_HIP -2         /* HIP runtime failure (CLError in the reference) */
+#define OCL_E_UNSUPPORTED -3 /* e.g. FFT length not 2^a 3^b 5^c 7^d (src/processes/FFT.cpp:374-379) */
+#define OCL_E_NOMEM -4       /* device memory exhausted / admission refused */
+
+typedef struct ocl_ctx ocl_ctx;
+typedef struct ocl_batch ocl_batch;
+
+/* NestaUp::LaunchParameters (include/OpenCLIPER/processes/nesta/NestaUp.hpp:67-83), same field order.
+ * argsMC is not carried: the motion-compensated operator is outside this library. */
+typedef struct {
+    double lambda_i, mu_f, La;
+    int maxIntIter;
+    double tolVar;
+    int verbose, maxIter, stoptest, miniter;
+} ocl_nesta_params;
+
+#define OCL_MAX_STAGES 16
+#define OCL_TRACE 4096
+/* What NestaUp::launch computes along the way (NestaUp.cpp:521-583, per-iteration f at :797-798). */
+typedef struct {
+    double normU, maxXref, maxUXref, lambda, muf, mu0, gamma, gammat, final_tolVar, final_mu;
+    int normest_iters, stages;
+    int iters[OCL_MAX_STAGES];
+    int ntrace;
+    double trace_f[OCL_TRACE];
+    double trace_l2[OCL_TRACE];
+    double trace_l1[OCL_TRACE];
+} ocl_nesta_report;
+
+/* One slice of a batch: device pointers plus sizes. */
+typedef struct {
+    const void* y;       /* KData [F][C][H][W] */
+    const void* S;       /* [C][H][W] */
+    const uint8_t* M;    /* [F][H][W] */
+    void* x_out;         /* XData [F][H][W] */
+    int F, C, H, W;
+} ocl_slice;
+
+/* ---- context: replaces CLapp::create/init (src/CLapp.cpp:139-569): one device, one in-order stream ---- */
+const char* ocl_last_error(void);
+int ocl_ctx_create(int device, ocl_ctx** out);
+int ocl_ctx_destroy(ocl_ctx* ctx);
+int ocl_ctx_sync(ocl_ctx* ctx);
+void* ocl_ctx_stream(ocl_ctx* ctx);                       /* CLapp::getCommandQueue -> hipStream_t */
+int ocl_device_info(ocl_ctx* ctx, char* name, size_t name_len, size_t* total_mem, int* num_cus);
+
+/* ---- device buffers: replace Data::host2Device / device2Host (src/Data.cpp:1493-1495) ---- */
+int ocl_malloc(ocl_ctx* ctx, size_t bytes, void** dptr);
+int ocl_free(ocl_ctx* ctx, void* dptr);
+int ocl_memcpy_h2d(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int ocl_memcpy_d2h(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int ocl_memcpy_d2d(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /* stream-ordered */
+int ocl_memset_zero(ocl_ctx* ctx, void* dptr, size_t bytes);                  /* stream-ordered fill kernel */
+
+/* ---- individual Processes (stream-ordered; outputs may alias inputs where the reference allows it) ---- */
+/* TemporalTV::launch (src/processes/nesta/TemporalTV.cpp:49-86, kernels src/kernels/temporalTV.cl:49-107);
+ * dir = 1 FORWARD, 0 ADJOINT (the reference's adjoint kernel, i.e. -D^T).  in and out must not alias. */
+int ocl_ttv(ocl_ctx* ctx, const void* in, void* out, int F, long N, int dir);
+/* VectorNormalization::launch (src/processes/nesta/VectorNormalization.cpp:50-87): out = in / max(mu, |in|) */
+int ocl_vecnorm(ocl_ctx* ctx, const void* in, void* out, long n, float mu);
+/* ComplexElementProd::launch (src/processes/ComplexElementProd.cpp:60-131); in_coils = 0 for XData input */
+int ocl_cep(ocl_ctx* ctx, const void* in, const void* S, void* out, int F, int C, long N, int in_coils, int conj);
+/* ApplyMask::launch (src/processes/ApplyMask.cpp:21-51), in place */
+int ocl_apply_mask(ocl_ctx* ctx, void* data, const uint8_t* M, int F, int C, long N);
+/* XImageSum::launch (src/processes/XImageSum.cpp:66-96) */
+int ocl_ximage_sum(ocl_ctx* ctx, const void* in, void* out, int F, int C, long N);
+/* ComplexAbs / ComplexAbsPow2::launch (src/processes/ComplexAbs.cpp:48-78, ComplexAbsPow2.cpp:48-78) */
+int ocl_complex_abs(ocl_ctx* ctx, const void* in, void* out, long n, int pow2);
+/* CopyDataGPU::launch (src/processes/groupwiseRegistration/initialize/CopyDataGPU.cpp:53-74) */
+int ocl_copy(ocl_ctx* ctx, const void* in, void* out, long n);
+/* FFT::launch with dim = -1 (src/processes/FFT.cpp:127-146,510-525): batch of 2-D C2C transforms of [H][W];
+ * dir = -1 FORWARD (unscaled), +1 BACKWARD (scaled by 1/(H*W)).  in == out is allowed. */
+int ocl_fft2d(ocl_ctx* ctx, const void* in, void* out, long batch, int H, int W, int dir);
+/* SimpleMRIRecon::launch (src/processes/examples/SimpleMRIRecon.cpp:59-101): x = sum_c conj(S_c) IFFT2(y_c).
+ * Like the reference it transforms y in place. */
+int ocl_simple_mri_recon(ocl_ctx* ctx, void* y, const void* S, void* x, int F, int C, int H, int W);
+
+/* ---- BLAS-1 as NestaUp calls it through CLBlast (NestaUp.cpp call sites listed in the docs) ---- */
+int ocl_scal(ocl_ctx* ctx, void* x, long n, float ar, float ai);                          /* CLBlastCscal */
+int ocl_axpy(ocl_ctx* ctx, long n, float ar, float ai, const void* x, void* y);           /* CLBlastCaxpy */
+/* result[0..1] on the host (blocking, like the reference's clEnqueueReadBuffer):
+ * op 0 Scnrm2, 1 Cdotc (conj(x).y, re/im), 2 Scsum (sum re+im), 3 Scasum, 4 max |x| */
+int ocl_reduce(ocl_ctx* ctx, const void* x, const void* y, long n, int op, double* result);
+
+/* ---- NESTA-TV: NestaUp::launch (src/processes/nesta/NestaUp.cpp:442-910) ---- */
+#define OCL_FORM_PLAIN 0   /* the reference's op sequence on the GPU, host stop test after each iteration */
+#define OCL_FORM_FUSED 1   /* two fused passes per iteration, device-side stop test, no host round trip */
+/* myNormest (NestaUp.cpp:291-439) for an F-frame image of N pixels, from a fresh POSIX-default drand48 stream. */
+int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
+int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
+                  int W, const ocl_nesta_params* prm, int form, ocl_nesta_report* report);
+
+/* ---- batched NESTA-TV: many uneven slices advanced together, one launch per pass per iteration ---- */
+int ocl_batch_create(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
+                     ocl_batch** out);
+int ocl_batch_setup(ocl_batch* b);                    /* E^H y, maxima, normU, state (per slice) */
+int ocl_batch_iterate(ocl_batch* b, int n_iters);     /* enqueue n_iters iterations (no host sync) */
+int ocl_batch_capture(ocl_batch* b, int n_iters);     /* record n_iters iterations into a hipGraph */
+int ocl_batch_replay(ocl_batch* b);                   /* launch the recorded graph */
+int ocl_batch_active(ocl_batch* b, int* n_active);    /* synchronises; number of slices still iterating */
+int ocl_batch_finalize(ocl_batch* b);                 /* copies each slice's result to its x_out */
+int ocl_batch_report(ocl_batch* b, int slice, ocl_nesta_report* report);
+int ocl_batch_run(ocl_batch* b, int chunk);           /* setup + iterate in chunks until done + finalize */
+/* enqueue n_iters iterations with HIP events around each launch; average ms of pass B and pass CA and the total
+ * span (synchronises).  Used to price the fused iteration against the HBM roofline. */
+int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total);
+int ocl_batch_destroy(ocl_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCL_AMD_H */

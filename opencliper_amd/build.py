@@ -1,0 +1,66 @@
+"""Builds libocl_amd.so (HIP kernels for gfx950 + the C ABI) and libOpenCLIPER_amd.so (the OpenCLIPER-shaped C++ API
+over that C ABI) in-tree, with hipcc/g++ directly (no CMake).  Run: python -m opencliper_amd.build"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+CPP = os.path.join(PKG, "cpp")
+INC = os.path.join(ROOT, "include")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libocl_amd.so")
+LIBCPP = os.path.join(PKG, "libOpenCLIPER_amd.so")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = os.environ.get("OCL_AMD_ARCH", "gfx950")
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
+             # packed FP32 (v_pk_*_f32) is kept out of every kernel on this platform
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-parameter", "-Wno-unused-variable",
+             f"-I{INC}", f"-I{CSRC}"]
+HIP_SOURCES = ["ocl_ops.hip", "ocl_nesta.hip", "ocl_capi.cpp"]
+CPP_SOURCES = []   # filled below if the C++ API sources exist
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[-1])}")
+    return r
+
+
+def _obj(src):
+    return os.path.join(BUILD, os.path.basename(src) + ".o")
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    jobs = []
+    for s in srcs:
+        cmd = [HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", "-o", _obj(s), s]
+        jobs.append(cmd)
+    with cf.ThreadPoolExecutor(max_workers=min(len(jobs), 4)) as ex:
+        list(ex.map(_run, jobs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [_obj(s) for s in srcs])
+    os.replace(tmp, LIB)
+    cpp_srcs = sorted(os.path.join(CPP, f) for f in os.listdir(CPP) if f.endswith(".cpp")) if os.path.isdir(CPP) else []
+    if cpp_srcs:
+        tmp = LIBCPP + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INC}", "-o", tmp] + cpp_srcs +
+             [f"-L{PKG}", "-locl_amd", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, LIBCPP)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build())

@@ -1,0 +1,325 @@
+"""ctypes binding of libocl_amd.so (the C ABI in include/ocl_amd.h).
+
+There is no fallback: if the library is missing or the GPU is absent, calls raise.  Device memory is owned by
+DeviceArray (freed when collected); host arrays are numpy.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libocl_amd.so")
+HEADER = os.path.join(os.path.dirname(_PKG), "include", "ocl_amd.h")
+
+OCL_FORM_PLAIN = 0
+OCL_FORM_FUSED = 1
+MAX_STAGES = 16
+TRACE = 4096
+RED = {"nrm2": 0, "dotc": 1, "sum": 2, "asum": 3, "maxabs": 4}
+
+
+class OclError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ocl error {code}: {msg}")
+        self.code = code
+
+
+class NestaParams(ctypes.Structure):
+    """NestaUp::LaunchParameters, same field order (include/OpenCLIPER/processes/nesta/NestaUp.hpp:67-83)."""
+    _fields_ = [("lambda_i", ctypes.c_double), ("mu_f", ctypes.c_double), ("La", ctypes.c_double),
+                ("maxIntIter", ctypes.c_int), ("tolVar", ctypes.c_double), ("verbose", ctypes.c_int),
+                ("maxIter", ctypes.c_int), ("stoptest", ctypes.c_int), ("miniter", ctypes.c_int)]
+
+
+def nesta_params(lambda_i=1e-2, mu_f=3e-4, La=1.0, maxIntIter=4, tolVar=3e-4, verbose=1, maxIter=100, stoptest=1,
+                 miniter=7) -> NestaParams:
+    """Defaults are the values tests/MRIRecon.cpp:105-114 of the reference passes to NestaUp."""
+    return NestaParams(lambda_i, mu_f, La, maxIntIter, tolVar, verbose, maxIter, stoptest, miniter)
+
+
+class NestaReport(ctypes.Structure):
+    _fields_ = [("normU", ctypes.c_double), ("maxXref", ctypes.c_double), ("maxUXref", ctypes.c_double),
+                ("lambda_", ctypes.c_double), ("muf", ctypes.c_double), ("mu0", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("gammat", ctypes.c_double), ("final_tolVar", ctypes.c_double),
+                ("final_mu", ctypes.c_double), ("normest_iters", ctypes.c_int), ("stages", ctypes.c_int),
+                ("iters", ctypes.c_int * MAX_STAGES), ("ntrace", ctypes.c_int),
+                ("trace_f", ctypes.c_double * TRACE), ("trace_l2", ctypes.c_double * TRACE),
+                ("trace_l1", ctypes.c_double * TRACE)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("trace") and k != "iters"}
+        d["lambda"] = d.pop("lambda_")
+        d["iters"] = list(self.iters)[: self.stages]
+        n = self.ntrace
+        d["trace_f"] = np.array(self.trace_f[:n])
+        d["trace_l2"] = np.array(self.trace_l2[:n])
+        d["trace_l1"] = np.array(self.trace_l1[:n])
+        return d
+
+
+class Slice(ctypes.Structure):
+    _fields_ = [("y", ctypes.c_void_p), ("S", ctypes.c_void_p), ("M", ctypes.c_void_p), ("x_out", ctypes.c_void_p),
+                ("F", ctypes.c_int), ("C", ctypes.c_int), ("H", ctypes.c_int), ("W", ctypes.c_int)]
+
+
+_lib = None
+
+
+def header_symbols():
+    """Every function the C header declares."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void\*?|const char\*)\s+\**(ocl_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OclError(-100, f"{LIB_PATH} is not built; run python -m opencliper_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, LG, D, F32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_float
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "ocl_last_error": ([], ctypes.c_char_p),
+        "ocl_ctx_create": ([I, PP], I), "ocl_ctx_destroy": ([P], I), "ocl_ctx_sync": ([P], I),
+        "ocl_ctx_stream": ([P], P),
+        "ocl_device_info": ([P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                             ctypes.POINTER(I)], I),
+        "ocl_malloc": ([P, ctypes.c_size_t, PP], I), "ocl_free": ([P, P], I),
+        "ocl_memcpy_h2d": ([P, P, P, ctypes.c_size_t], I), "ocl_memcpy_d2h": ([P, P, P, ctypes.c_size_t], I),
+        "ocl_memcpy_d2d": ([P, P, P, ctypes.c_size_t], I), "ocl_memset_zero": ([P, P, ctypes.c_size_t], I),
+        "ocl_ttv": ([P, P, P, I, LG, I], I), "ocl_vecnorm": ([P, P, P, LG, F32], I),
+        "ocl_cep": ([P, P, P, P, I, I, LG, I, I], I), "ocl_apply_mask": ([P, P, P, I, I, LG], I),
+        "ocl_ximage_sum": ([P, P, P, I, I, LG], I), "ocl_complex_abs": ([P, P, P, LG, I], I),
+        "ocl_copy": ([P, P, P, LG], I), "ocl_fft2d": ([P, P, P, LG, I, I, I], I),
+        "ocl_simple_mri_recon": ([P, P, P, P, I, I, I, I], I),
+        "ocl_scal": ([P, P, LG, F32, F32], I), "ocl_axpy": ([P, LG, F32, F32, P, P], I),
+        "ocl_reduce": ([P, P, P, LG, I, ctypes.POINTER(D)], I),
+        "ocl_normest": ([P, I, LG, ctypes.POINTER(D), ctypes.POINTER(I)], I),
+        "ocl_nesta_run": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(NestaReport)],
+                          I),
+        "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
+        "ocl_batch_setup": ([P], I), "ocl_batch_iterate": ([P, I], I), "ocl_batch_capture": ([P, I], I),
+        "ocl_batch_replay": ([P], I), "ocl_batch_active": ([P, ctypes.POINTER(I)], I),
+        "ocl_batch_finalize": ([P], I), "ocl_batch_report": ([P, I, ctypes.POINTER(NestaReport)], I),
+        "ocl_batch_run": ([P, I], I), "ocl_batch_destroy": ([P], I),
+        "ocl_batch_time": ([P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)], I),
+    }
+    for name, (a, r) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = a
+        fn.restype = r
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise OclError(rc, lib().ocl_last_error().decode(errors="replace"))
+
+
+class DeviceArray:
+    """A device allocation with a numpy-like shape/dtype tag."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().ocl_malloc(ctx.handle, max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().ocl_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def set(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, "size mismatch"
+        _check(lib().ocl_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            lib().ocl_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device + one stream (CLapp)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().ocl_ctx_create(device, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            lib().ocl_ctx_destroy(self.handle)
+            self.handle = None
+
+    def sync(self):
+        _check(lib().ocl_ctx_sync(self.handle))
+
+    def info(self):
+        name = ctypes.create_string_buffer(256)
+        mem = ctypes.c_size_t()
+        cus = ctypes.c_int()
+        _check(lib().ocl_device_info(self.handle, name, 256, ctypes.byref(mem), ctypes.byref(cus)))
+        return dict(name=name.value.decode(), total_mem=mem.value, num_cus=cus.value)
+
+    # --- memory ---
+    def empty(self, shape, dtype=np.complex64) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        return DeviceArray(self, a.shape, a.dtype).set(a)
+
+    def zeros(self, shape, dtype=np.complex64) -> DeviceArray:
+        d = DeviceArray(self, shape, dtype)
+        _check(lib().ocl_memset_zero(self.handle, d.ptr, d.nbytes))
+        return d
+
+    # --- Processes ---
+    def ttv(self, x: DeviceArray, out: DeviceArray, forward=True):
+        F = x.shape[0]
+        _check(lib().ocl_ttv(self.handle, x.ptr, out.ptr, F, x.size // F, 1 if forward else 0))
+
+    def vecnorm(self, x, out, mu):
+        _check(lib().ocl_vecnorm(self.handle, x.ptr, out.ptr, x.size, float(mu)))
+
+    def cep(self, x, S, out, F, C, in_coils, conj):
+        _check(lib().ocl_cep(self.handle, x.ptr, S.ptr, out.ptr, F, C, S.size // C, int(in_coils), int(conj)))
+
+    def apply_mask(self, k, M):
+        F, C = k.shape[:2]
+        _check(lib().ocl_apply_mask(self.handle, k.ptr, M.ptr, F, C, k.size // (F * C)))
+
+    def ximage_sum(self, k, out):
+        F, C = k.shape[:2]
+        _check(lib().ocl_ximage_sum(self.handle, k.ptr, out.ptr, F, C, k.size // (F * C)))
+
+    def complex_abs(self, x, out, pow2=False):
+        _check(lib().ocl_complex_abs(self.handle, x.ptr, out.ptr, x.size, int(pow2)))
+
+    def copy(self, x, out):
+        _check(lib().ocl_copy(self.handle, x.ptr, out.ptr, x.size))
+
+    def fft2d(self, x, out, inverse=False):
+        H, W = x.shape[-2:]
+        _check(lib().ocl_fft2d(self.handle, x.ptr, out.ptr, x.size // (H * W), H, W, 1 if inverse else -1))
+
+    def simple_mri_recon(self, y, S, x):
+        F, C, H, W = y.shape
+        _check(lib().ocl_simple_mri_recon(self.handle, y.ptr, S.ptr, x.ptr, F, C, H, W))
+
+    def scal(self, x, a):
+        a = complex(a)
+        _check(lib().ocl_scal(self.handle, x.ptr, x.size, a.real, a.imag))
+
+    def axpy(self, a, x, y):
+        a = complex(a)
+        _check(lib().ocl_axpy(self.handle, x.size, a.real, a.imag, x.ptr, y.ptr))
+
+    def reduce(self, x, op, y=None):
+        r = (ctypes.c_double * 2)()
+        _check(lib().ocl_reduce(self.handle, x.ptr, y.ptr if y is not None else None, x.size, RED[op], r))
+        return r[0], r[1]
+
+    def normest(self, F, N):
+        v = ctypes.c_double()
+        it = ctypes.c_int()
+        _check(lib().ocl_normest(self.handle, F, N, ctypes.byref(v), ctypes.byref(it)))
+        return v.value, it.value
+
+    # --- NESTA ---
+    def nesta(self, y: DeviceArray, S: DeviceArray, M: DeviceArray, x: DeviceArray, prm: NestaParams | None = None,
+              form=OCL_FORM_FUSED):
+        prm = prm or nesta_params()
+        F, C, H, W = y.shape
+        rep = NestaReport()
+        _check(lib().ocl_nesta_run(self.handle, y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W, ctypes.byref(prm), form,
+                                   ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def batch(self, slices, prm: NestaParams | None = None) -> "Batch":
+        return Batch(self, slices, prm or nesta_params())
+
+
+class Batch:
+    """Many uneven slices advanced together (ocl_batch_*).  `slices` = list of (y, S, M, x_out) DeviceArrays."""
+
+    def __init__(self, ctx: Context, slices, prm: NestaParams):
+        self.ctx = ctx
+        self.keep = list(slices)
+        arr = (Slice * len(slices))()
+        for i, (y, S, M, x) in enumerate(slices):
+            F, C, H, W = y.shape
+            arr[i] = Slice(y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W)
+        self.prm = prm
+        h = ctypes.c_void_p()
+        _check(lib().ocl_batch_create(ctx.handle, len(slices), arr, ctypes.byref(prm), ctypes.byref(h)))
+        self.handle = h.value
+
+    def setup(self):
+        _check(lib().ocl_batch_setup(self.handle))
+
+    def iterate(self, n):
+        _check(lib().ocl_batch_iterate(self.handle, int(n)))
+
+    def capture(self, n):
+        _check(lib().ocl_batch_capture(self.handle, int(n)))
+
+    def replay(self):
+        _check(lib().ocl_batch_replay(self.handle))
+
+    def active(self):
+        n = ctypes.c_int()
+        _check(lib().ocl_batch_active(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def finalize(self):
+        _check(lib().ocl_batch_finalize(self.handle))
+
+    def run(self, chunk=16):
+        _check(lib().ocl_batch_run(self.handle, int(chunk)))
+
+    def time_iters(self, n):
+        """Runs n iterations with HIP events around each launch: (ms of pass B, ms of pass CA, total ms)."""
+        b, c, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().ocl_batch_time(self.handle, int(n), ctypes.byref(b), ctypes.byref(c), ctypes.byref(t)))
+        return b.value, c.value, t.value
+
+    def report(self, i):
+        rep = NestaReport()
+        _check(lib().ocl_batch_report(self.handle, i, ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().ocl_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,94 @@
+// Types shared by the host side (ocl_capi.cpp) and the gfx950 kernels (ocl_ops.hip, ocl_nesta.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ocl_device.hpp"
+
+enum { RED_NRM2 = 0, RED_DOTC = 1, RED_SUM = 2, RED_ASUM = 3, RED_MAXABS = 4 };
+
+#define OCL_MAX_MINITER 32
+#define OCL_MAX_NSTAGES 32
+#define OCL_MAXE 32          // most owned elements per thread in pass CA: H * tile_cols <= OCL_MAXE * OCL_NT
+
+// Device-resident NESTA state of one slice (NestaUp.cpp:570-889 scalars).  Scalars are kept in double.
+struct NState {
+    double lambda, La, normU, gamma, gammat;
+    int maxIntIter, maxIter, stoptest, miniter;
+    int active;        // 0 once every continuation stage has finished
+    int stage;         // 1-based continuation stage of the iteration about to run
+    int k;             // inner iteration index of the iteration about to run
+    int stage_start;   // 1: the next iteration is the first of its stage (x_ref := x_k, W_k := 0)
+    int cur;           // which x buffer holds x_k
+    int qpDone;
+    int iters_stage, total_iters, ntrace, trace_cap;
+    double mu, tolVar, Lmu, Lmu1;
+    double last_f, last_qp;
+    // coefficients of the iteration prepared by pass B and consumed by pass CA
+    int it_pending;    // 1 between the scalar step of B(k) and the end of CA(k)
+    int it_cur, it_sstart;
+    double it_mu, it_L1, it_apk, it_tau;
+    double fmean[OCL_MAX_MINITER];
+    int iters[OCL_MAX_NSTAGES];
+};
+
+// One slice of a (possibly uneven) batch.  All pointers are device pointers.
+struct SliceDesc {
+    int F, C, H, W;
+    long N;
+    int nmax;           // sampled-row capacity per frame in kbuf
+    int tile_cols;      // column tile width of pass CA (power of two)
+    int rows_per_wg;    // rows per workgroup of pass B (divides OCL_NT)
+    int ncol_tiles;     // ceil(W / tile_cols)
+    int nrow_groups;    // ceil(nmax / rows_per_wg)
+    int wg_B, wg_CA;    // workgroups of this slice in passes B (k-space rows) and CA
+    int wg_U;           // extra workgroups of pass B that reduce the temporal-TV terms of x_k
+    int u_chunk;        // pixels per U workgroup (per frame)
+    float inv_sqrtN;
+    FftPlan ph, pw;     // column (length H) and row (length W) plans
+    const float2* y;    // [F][C][H][W]
+    const float2* S;    // [C][H][W]
+    const uint8_t* M;   // [F][H][W]
+    float2* x0;         // ping-pong x_k buffers, [F][H][W] each
+    float2* x1;
+    float2* xref;       // [F][H][W]
+    float2* Wk;         // [F][H][W]
+    float2* kbuf;       // [F][C][nmax][W]  sampled-row intermediates
+    const int16_t* rows;// [F][nmax] sampled rows of each frame
+    const int* nrows;   // [F]
+    NState* st;
+    double* partB;      // [wg_B]        sum |r|^2 per row workgroup
+    double* partU;      // [wg_U][2]     sum |u|^2, sum Re(conj(u) Ux) per U workgroup
+    unsigned* counter;  // [2] arrival counters of passes B and CA (reset by their last workgroup)
+    double* trace;      // [trace_cap][3] : f, ||r||^2, l1 per inner iteration (may be null)
+    double* maxpart;    // [wg_CA][2] partial maxima of |x_ref| and |U x_ref| (setup)
+};
+
+// kernels (ocl_ops.hip)
+__global__ void k_ttv(const float2* in, float2* out, int F, long N, int dir);
+__global__ void k_vecnorm(const float2* in, float2* out, long n, float mu);
+__global__ void k_cep(const float2* in, const float2* S, float2* out, int F, int C, long N, int in_coils, int conj);
+__global__ void k_apply_mask(float2* d, const uint8_t* M, int F, int C, long N);
+__global__ void k_ximage_sum(const float2* in, float2* out, int F, int C, long N);
+__global__ void k_abs(const float2* in, float2* out, long n, int pow2);
+__global__ void k_copy(const float2* x, float2* y, long n);
+__global__ void k_scal(float2* x, long n, float ar, float ai);
+__global__ void k_axpy(long n, float ar, float ai, const float2* x, float2* y);
+__global__ void k_fill(float2* x, long n, float2 v);
+__global__ void k_fill_bytes(uint8_t* x, long n, uint8_t v);
+__global__ void k_reduce_partial(const float2* x, const float2* y, long n, int op, double* partials);
+__global__ void k_reduce_final(const double* partials, int nparts, int op, double* out);
+__global__ void k_fft_rows(const float2* in, float2* out, long nrows, FftPlan P, int rows_per_wg, int conj_io,
+                           float scale);
+__global__ void k_fft_cols(const float2* in, float2* out, int H, int W, FftPlan P, int tw_cols, int conj_io,
+                           float scale);
+__global__ void k_drand48_fill(float2* out, long n, unsigned long long x0, long start);
+
+// kernels (ocl_nesta.hip)
+__global__ void k_rowflags(const float2* y, const uint8_t* M, int F, int C, int H, int W, uint8_t* flags);
+__global__ void k_nesta_B(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+__global__ void k_nesta_CA_4(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+__global__ void k_nesta_CA_8(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+__global__ void k_nesta_CA_16(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+__global__ void k_nesta_CA_32(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+__global__ void k_setup_max(const SliceDesc* descs, const int* wg_off, int nslices, double* out);
+__global__ void k_finalize(const SliceDesc* descs, int nslices, float2* const* outs);

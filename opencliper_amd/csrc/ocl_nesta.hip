@@ -1,0 +1,421 @@
+// Fused NESTA-TV iteration for gfx950 (src/processes/nesta/NestaUp.cpp:651-889 restated as two passes).
+//
+// The 2-D FFT of the encoding operator E = M F S is split column-first so that only the k-space rows that carry
+// data ever leave the workgroup:
+//   pass CA(k)   per (slice, frame, column tile):  for each coil  Z = IFFT_H(V) (input pruned to the sampled rows),
+//                A^H r = sum_c conj(S_c) Z / sqrt(N); fused NESTA update of x, W (temporal TV from x_{f-1}, x_f,
+//                x_{f+1}); then, for iteration k+1, U = FFT_H(S_c x_{k+1}) keeping only the sampled rows.
+//   pass B(k+1)  per (slice, frame, coil, row group):  K = FFT_W(U row); r = M K / sqrt(N) - y; ||r||^2;
+//                V = IFFT_W(r) written over U.
+// The last workgroup of CA folds the per-workgroup partial sums in index order (deterministic), evaluates
+// f = 1/2 ||r||^2 + lambda (Re<u,Ux> - mu/2 ||u||^2), runs the reference's stop test and advances the device state
+// (NestaUp.cpp:797-889), so no host round trip is needed between iterations.
+#include "ocl_kernels.hpp"
+
+__device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, int b) {
+    int lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Row flags for the sampled-row lists: flag[f][h] = 1 if the mask row or any coil's k-space row is non-zero.
+__global__ void k_rowflags(const float2* __restrict__ y, const uint8_t* __restrict__ M, int F, int C, int H, int W,
+                           uint8_t* __restrict__ flags) {
+    const int fh = blockIdx.x;
+    const int f = fh / H, h = fh - f * H;
+    int any = 0;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) any |= (M[((long)f * H + h) * W + i] != 0);
+    for (int c = 0; c < C && !__syncthreads_or(any); ++c)
+        for (int i = threadIdx.x; i < W; i += blockDim.x) {
+            const float2 v = y[(((long)f * C + c) * H + h) * W + i];
+            any |= (v.x != 0.f || v.y != 0.f);
+        }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[fh] = (uint8_t)(any != 0);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// scalar step (NestaUp.cpp:797-889), run by the last workgroup of pass B(k) once ||r_k||^2 and the temporal-TV
+// terms of x_k are complete.  It fixes the coefficients pass CA(k) applies and advances the state to k+1.
+// Like the reference, an iteration that ends its stage through the stop test takes apk and tau with k = maxIter
+// (NestaUp.cpp:809-831).
+// ------------------------------------------------------------------------------------------------------------------
+__device__ void nesta_scalar_step(const SliceDesc& d, double* red) {
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < d.wg_B; i += blockDim.x) s[0] += d.partB[i];
+    for (int i = threadIdx.x; i < d.wg_U; i += blockDim.x) { s[1] += d.partU[2 * i]; s[2] += d.partU[2 * i + 1]; }
+    block_sum<3>(s, red);
+    if (threadIdx.x != 0) return;
+    NState* st = d.st;
+    const double l2 = s[0], u2 = s[1], fxs = s[2];
+    const double l1 = fxs - (st->mu / 2.0) * u2;
+    const double fv = 0.5 * l2 + st->lambda * l1;
+    st->last_f = fv;
+    if (d.trace && st->ntrace < st->trace_cap) {
+        d.trace[3 * st->ntrace] = fv;
+        d.trace[3 * st->ntrace + 1] = l2;
+        d.trace[3 * st->ntrace + 2] = l1;
+        st->ntrace++;
+    }
+    int k = st->k;
+    const int mi = st->miniter;
+    bool end_stage = false;
+    if (st->stoptest == 1) {
+        bool skip_write = false;
+        if (k >= mi) {
+            double mean = 0.0;
+            for (int ne = 0; ne < mi; ++ne) mean += st->fmean[mi - ne - 1];
+            mean = mean / mi;
+            const double qp = (mean - fv) / mean;
+            st->last_qp = qp;
+            if (qp <= st->tolVar) {
+                if (st->qpDone) end_stage = true;
+                st->qpDone = 1;
+                skip_write = true;
+            }
+        }
+        if (!skip_write) st->fmean[(mi - 1) - ((k + 1) % mi)] = fv;
+    }
+    // coefficients of iteration k for pass CA
+    const double keff = end_stage ? (double)st->maxIter : (double)k;
+    st->it_mu = st->mu;
+    st->it_L1 = st->Lmu1;
+    st->it_apk = 0.5 * (keff + 1.0);
+    st->it_tau = 2.0 / (keff + 3.0);
+    st->it_cur = st->cur;
+    st->it_sstart = st->stage_start;
+    st->it_pending = 1;
+    // advance to iteration k+1
+    st->iters_stage++;
+    st->total_iters++;
+    st->cur ^= 1;
+    st->stage_start = 0;
+    k = k + 1;
+    if (end_stage || k >= st->maxIter) {
+        if (st->stage - 1 < OCL_MAX_NSTAGES) st->iters[st->stage - 1] = st->iters_stage;
+        st->stage++;
+        st->iters_stage = 0;
+        if (st->stage > st->maxIntIter) {
+            st->active = 0;
+        } else {
+            st->mu = st->mu * st->gamma;
+            st->tolVar = st->tolVar * st->gammat;
+            st->Lmu = st->lambda * (st->normU / st->mu) + st->La;
+            st->Lmu1 = 1.0 / st->Lmu;
+            st->qpDone = 0;
+            st->fmean[mi - 1] = 9223372036854775807.0;
+            st->stage_start = 1;
+            k = 0;
+        }
+    }
+    st->k = k;
+    __threadfence();
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// pass B.  Workgroups [0, wg_B) of a slice handle k-space rows; [wg_B, wg_B + wg_U) reduce the temporal-TV terms
+// of x_k (mode 0 only).  mode 0: NESTA residual + scalar step.  mode 1: setup (V = IFFT_W(y row)).
+// ------------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(OCL_NT) void k_nesta_B(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
+                                                    int nslices, int mode) {
+    extern __shared__ float2 lds[];
+    __shared__ double red[64];
+    __shared__ int is_last;
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    NState* st = d.st;
+    if (mode == 0 && !st->active) return;
+    const int local = blockIdx.x - wg_off[si];
+    const int C = d.C, W = d.W, H = d.H, nmax = d.nmax, RB = d.rows_per_wg;
+    if (local >= d.wg_B) {
+        // ---- temporal-TV terms of x_k: sum |u|^2 and sum Re(conj(u) Ux), u = Ux / max(mu, |Ux|) ----
+        if (mode != 0) return;
+        const int uid = local - d.wg_B;
+        const int per_f = (int)((d.N + d.u_chunk - 1) / d.u_chunk);
+        const int f = uid / per_f;
+        const long p0 = (long)(uid - f * per_f) * d.u_chunk;
+        const long p1 = min(d.N, p0 + d.u_chunk);
+        const int fm = f == 0 ? d.F - 1 : f - 1;
+        const float2* xk = st->cur ? d.x1 : d.x0;
+        const float mu = (float)st->mu;
+        double v[2] = {0.0, 0.0};
+        for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+            const float2 ux = c_sub(xk[(long)f * d.N + p], xk[(long)fm * d.N + p]);
+            const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
+            const float2 u = make_float2(ux.x / nf, ux.y / nf);
+            v[0] += (double)u.x * u.x + (double)u.y * u.y;
+            v[1] += (double)u.x * ux.x + (double)u.y * ux.y;
+        }
+        block_sum<2>(v, red);
+        if (threadIdx.x == 0) { d.partU[2 * uid] = v[0]; d.partU[2 * uid + 1] = v[1]; }
+    } else {
+        const int g = local % d.nrow_groups;
+        const int fc = local / d.nrow_groups;
+        const int f = fc / C, c = fc - f * C;
+        const int nr = d.nrows[f];
+        const int j0 = g * RB;
+        double acc[1] = {0.0};
+        if (j0 < nr) {
+            const int ld = W + 1;
+            float2* a = lds;
+            float2* b = a + RB * ld;
+            float2* tw = b + RB * ld;
+            const float invW = 1.0f / (float)W;
+            const int16_t* rows = d.rows + (long)f * nmax;
+            float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+            const float2* yfc = d.y + ((long)f * C + c) * H * W;
+            load_twiddles(tw, d.pw);
+            float2* res;
+            if (mode == 0) {
+                for (int e = threadIdx.x; e < RB * W; e += blockDim.x) {
+                    const int r = fdiv(e, invW), i = e - r * W;
+                    a[r * ld + i] = (j0 + r < nr) ? kb[(long)(j0 + r) * W + i] : make_float2(0.f, 0.f);
+                }
+                __syncthreads();
+                res = lds_fft(a, b, tw, d.pw, RB, ld, 1, false);
+                const uint8_t* Mf = d.M + (long)f * H * W;
+                const float s = d.inv_sqrtN;
+                for (int e = threadIdx.x; e < RB * W; e += blockDim.x) {
+                    const int r = fdiv(e, invW), i = e - r * W;
+                    float2 rv = make_float2(0.f, 0.f);
+                    if (j0 + r < nr) {
+                        const int ky = rows[j0 + r];
+                        const float2 yv = yfc[(long)ky * W + i];
+                        const float2 K = res[r * ld + i];
+                        rv = Mf[(long)ky * W + i] ? make_float2(K.x * s - yv.x, K.y * s - yv.y)
+                                                  : make_float2(-yv.x, -yv.y);
+                        acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
+                    }
+                    res[r * ld + i] = c_conj(rv);
+                }
+            } else {
+                for (int e = threadIdx.x; e < RB * W; e += blockDim.x) {
+                    const int r = fdiv(e, invW), i = e - r * W;
+                    float2 v = make_float2(0.f, 0.f);
+                    if (j0 + r < nr) v = yfc[(long)rows[j0 + r] * W + i];
+                    a[r * ld + i] = c_conj(v);
+                }
+                res = a;
+            }
+            __syncthreads();
+            float2* other = (res == a) ? b : a;
+            float2* out = lds_fft(res, other, tw, d.pw, RB, ld, 1, false);
+            for (int e = threadIdx.x; e < RB * W; e += blockDim.x) {
+                const int r = fdiv(e, invW), i = e - r * W;
+                if (j0 + r < nr) kb[(long)(j0 + r) * W + i] = c_conj(out[r * ld + i]);
+            }
+        }
+        if (mode != 0) return;
+        block_sum<1>(acc, red);
+        if (threadIdx.x == 0) d.partB[local] = acc[0];
+    }
+    // ---- the last workgroup of this slice runs the scalar step ----
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned old = atomicAdd(&d.counter[0], 1u);
+        is_last = (old == (unsigned)(d.wg_B + d.wg_U - 1));
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        nesta_scalar_step(d, red);
+        if (threadIdx.x == 0) atomicExch(&d.counter[0], 0u);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// pass CA.  mode 0: NESTA iteration k with the coefficients prepared by B(k), then the forward columns of x_{k+1}.
+// mode 1: setup (x_ref = A^H y, forward columns of iteration 0).
+// ------------------------------------------------------------------------------------------------------------------
+template <int MAXE>
+__device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
+                                              int nslices, int mode) {
+    extern __shared__ float2 lds[];
+    __shared__ int is_last;
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    NState* st = d.st;
+    if (mode == 0 && !st->it_pending) return;
+    const int local = blockIdx.x - wg_off[si];
+    const int tile = local % d.ncol_tiles;
+    const int f = local / d.ncol_tiles;
+    const int F = d.F, C = d.C, H = d.H, W = d.W, TW = d.tile_cols, nmax = d.nmax;
+    const long N = d.N;
+    const int w0 = tile * TW;
+    const int ld = TW + 1;
+    float2* a = lds;
+    float2* b = a + H * ld;
+    float2* tw = b + H * ld;
+    const int nr = d.nrows[f];
+    const int16_t* rows = d.rows + (long)f * nmax;
+    const int nel = H * TW;
+    const int lsh = __ffs(TW) - 1;   // TW is a power of two
+    load_twiddles(tw, d.ph);
+
+    float2 acc[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) acc[i] = make_float2(0.f, 0.f);
+
+    // ---- A^H r: inverse column DFTs of the sampled rows, conj(S) multiply, coil sum ----
+    for (int c = 0; c < C; ++c) {
+        for (int e = threadIdx.x; e < nel; e += blockDim.x) a[(e >> lsh) * ld + (e & (TW - 1))] = make_float2(0.f, 0.f);
+        __syncthreads();
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        for (int e = threadIdx.x; e < nr * TW; e += blockDim.x) {
+            const int j = e >> lsh, wl = e & (TW - 1);
+            if (w0 + wl < W) a[rows[j] * ld + wl] = c_conj(kb[(long)j * W + w0 + wl]);
+        }
+        __syncthreads();
+        float2* res = lds_fft(a, b, tw, d.ph, TW, 1, ld, true);
+        const float2* Sc = d.S + (long)c * N;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * OCL_NT;
+            if (e < nel) {
+                const int h = e >> lsh, wl = e & (TW - 1);
+                if (w0 + wl < W) acc[i] = c_add(acc[i], c_mulconj(c_conj(res[h * ld + wl]), Sc[(long)h * W + w0 + wl]));
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- NESTA update (NestaUp.cpp:669-875 fused) ----
+    float2 xn[MAXE];
+    const float s = d.inv_sqrtN;
+    if (mode == 1) {
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * OCL_NT;
+            xn[i] = c_scale(acc[i], s);
+            if (e < nel) {
+                const int h = e >> lsh, wl = e & (TW - 1);
+                if (w0 + wl < W) d.x0[(long)f * N + (long)h * W + w0 + wl] = xn[i];
+            }
+        }
+    } else {
+        const float2* xk = st->it_cur ? d.x1 : d.x0;
+        float2* xo = st->it_cur ? d.x0 : d.x1;
+        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+        const float mu = (float)st->it_mu, lam = (float)st->lambda, L1 = (float)st->it_L1;
+        const float apk = (float)st->it_apk;
+        const float tau = (float)st->it_tau;
+        const int sstart = st->it_sstart;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * OCL_NT;
+            xn[i] = make_float2(0.f, 0.f);
+            if (e < nel) {
+                const int h = e >> lsh, wl = e & (TW - 1);
+                if (w0 + wl < W) {
+                    const long p = (long)h * W + w0 + wl;
+                    const float2 xf = xk[(long)f * N + p], xm = xk[(long)fm * N + p], xp = xk[(long)fp * N + p];
+                    const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
+                    const float nf = fmaxf(mu, sqrtf(c_abs2(uxf))), np = fmaxf(mu, sqrtf(c_abs2(uxp)));
+                    const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
+                    const float2 ares = c_scale(acc[i], s);
+                    const float2 df = make_float2(lam * (uf.x - up.x) + ares.x, lam * (uf.y - up.y) + ares.y);
+                    float2 xr, wo;
+                    if (sstart) {
+                        xr = xf;
+                        d.xref[(long)f * N + p] = xf;
+                        wo = make_float2(0.f, 0.f);
+                    } else {
+                        xr = d.xref[(long)f * N + p];
+                        wo = d.Wk[(long)f * N + p];
+                    }
+                    const float2 yk = make_float2(xf.x - L1 * df.x, xf.y - L1 * df.y);
+                    const float2 wn = make_float2(wo.x + apk * df.x, wo.y + apk * df.y);
+                    d.Wk[(long)f * N + p] = wn;
+                    const float2 z = make_float2(xr.x - L1 * wn.x, xr.y - L1 * wn.y);
+                    xn[i] = make_float2(tau * z.x + (1.0f - tau) * yk.x, tau * z.y + (1.0f - tau) * yk.y);
+                    xo[(long)f * N + p] = xn[i];
+                }
+            }
+        }
+    }
+
+    // ---- forward column DFTs of S_c x_{k+1}, keeping only the sampled rows ----
+    for (int c = 0; c < C; ++c) {
+        const float2* Sc = d.S + (long)c * N;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * OCL_NT;
+            if (e < nel) {
+                const int h = e >> lsh, wl = e & (TW - 1);
+                a[h * ld + wl] = (w0 + wl < W) ? c_mul(xn[i], Sc[(long)h * W + w0 + wl]) : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+        float2* res = lds_fft(a, b, tw, d.ph, TW, 1, ld, true);
+        float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        for (int e = threadIdx.x; e < nr * TW; e += blockDim.x) {
+            const int j = e >> lsh, wl = e & (TW - 1);
+            if (w0 + wl < W) kb[(long)j * W + w0 + wl] = res[rows[j] * ld + wl];
+        }
+        __syncthreads();
+    }
+
+    if (mode == 1) return;
+    // ---- the last workgroup of this slice marks the iteration consumed ----
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned old = atomicAdd(&d.counter[1], 1u);
+        is_last = (old == (unsigned)(d.wg_CA - 1));
+    }
+    __syncthreads();
+    if (is_last && threadIdx.x == 0) {
+        __threadfence();
+        st->it_pending = 0;
+        atomicExch(&d.counter[1], 0u);
+        __threadfence();
+    }
+}
+
+#define OCL_CA_KERNEL(E)                                                                                        \
+    __global__ __launch_bounds__(OCL_NT) void k_nesta_CA_##E(const SliceDesc* __restrict__ descs,               \
+                                                             const int* __restrict__ wg_off, int nslices,        \
+                                                             int mode) {                                        \
+        nesta_CA_body<E>(descs, wg_off, nslices, mode);                                                          \
+    }
+OCL_CA_KERNEL(4)
+OCL_CA_KERNEL(8)
+OCL_CA_KERNEL(16)
+OCL_CA_KERNEL(32)
+
+// Setup maxima (NestaUp.cpp:521-568): per workgroup max |x_ref| and max |U x_ref| over a chunk of x_ref = x0.
+__global__ void k_setup_max(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off, int nslices,
+                            double* unused) {
+    __shared__ double red[32];
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    const int local = blockIdx.x - wg_off[si];
+    const long total = (long)d.F * d.N;
+    const long chunk = (total + d.wg_CA - 1) / d.wg_CA;
+    const long lo = (long)local * chunk, hi = min(total, lo + chunk);
+    double m0 = -1.0, m1 = -1.0;
+    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const int f = (int)(i / d.N);
+        const long p = i - (long)f * d.N;
+        const int fm = f == 0 ? d.F - 1 : f - 1;
+        const float2 x = d.x0[i];
+        m0 = fmax(m0, (double)sqrtf(c_abs2(x)));
+        m1 = fmax(m1, (double)sqrtf(c_abs2(c_sub(x, d.x0[(long)fm * d.N + p]))));
+    }
+    m0 = block_max(m0, red);
+    m1 = block_max(m1, red);
+    if (threadIdx.x == 0) { d.maxpart[2 * local] = m0; d.maxpart[2 * local + 1] = m1; }
+}
+
+// Copies the final x (the buffer the state points at) to each slice's output.
+__global__ void k_finalize(const SliceDesc* __restrict__ descs, int nslices, float2* const* __restrict__ outs) {
+    const int si = blockIdx.y;
+    if (si >= nslices) return;
+    const SliceDesc& d = descs[si];
+    const float2* x = d.st->cur ? d.x1 : d.x0;
+    float2* o = outs[si];
+    const long total = (long)d.F * d.N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) o[i] = x[i];
+}

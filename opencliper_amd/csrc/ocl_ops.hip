@@ -1,0 +1,235 @@
+// gfx950 kernels behind the individual OpenCLIPER Processes of the NESTA-TV path and the BLAS-1 calls NestaUp makes.
+// Each kernel is grid-stride over the flattened HBM layout (W fastest), so every access is a coalesced 8-byte
+// complex load or store.  Launch wrappers live in ocl_capi.cpp.
+#include "ocl_device.hpp"
+#include "ocl_kernels.hpp"
+
+// TemporalTV (src/kernels/temporalTV.cl:49-107).  dir=1: out_f = in_f - in_{f-1 mod F};
+// dir=0 (the reference's "adjoint", i.e. -D^T): out_f = in_{f+1 mod F} - in_f.
+__global__ void k_ttv(const float2* __restrict__ in, float2* __restrict__ out, int F, long N, int dir) {
+    const long total = (long)F * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        if (dir) {
+            const int g = f == 0 ? F - 1 : f - 1;
+            out[i] = c_sub(in[i], in[(long)g * N + p]);
+        } else {
+            const int g = f == F - 1 ? 0 : f + 1;
+            out[i] = c_sub(in[(long)g * N + p], in[i]);
+        }
+    }
+}
+
+// VectorNormalization (src/kernels/vectorNormalization.cl:42-56): out = in / max(mu, |in|)
+__global__ void k_vecnorm(const float2* __restrict__ in, float2* __restrict__ out, long n, float mu) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float2 v = in[i];
+        const float fac = fmaxf(mu, sqrtf(c_abs2(v)));
+        out[i] = make_float2(v.x / fac, v.y / fac);
+    }
+}
+
+// ComplexElementProd (src/kernels/complexElementProd.cl:42-76): out[f][c] = in[f][(c)] * (conj?) S[c].
+// in_coils = 0: the input is an image [F][N] broadcast over the coils.
+__global__ void k_cep(const float2* __restrict__ in, const float2* __restrict__ S, float2* __restrict__ out, int F,
+                      int C, long N, int in_coils, int conj) {
+    const long total = (long)F * C * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long fc = i / N;
+        const long p = i - fc * N;
+        const int f = (int)(fc / C), c = (int)(fc - (long)f * C);
+        const float2 a = in_coils ? in[i] : in[(long)f * N + p];
+        const float2 s = S[(long)c * N + p];
+        out[i] = conj ? c_mulconj(a, s) : c_mul(a, s);
+    }
+}
+
+// ApplyMask (src/kernels/applyMask.cl:5-40): zero k-space samples whose mask byte is 0
+__global__ void k_apply_mask(float2* __restrict__ d, const uint8_t* __restrict__ M, int F, int C, long N) {
+    const long total = (long)F * C * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long fc = i / N;
+        const long p = i - fc * N;
+        const int f = (int)(fc / C);
+        if (M[(long)f * N + p] == 0) d[i] = make_float2(0.f, 0.f);
+    }
+}
+
+// XImageSum (src/kernels/xImageSum.cl:42-61): x_f = sum_c in_{f,c}, coils summed in order
+__global__ void k_ximage_sum(const float2* __restrict__ in, float2* __restrict__ out, int F, int C, long N) {
+    const long total = (long)F * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int c = 0; c < C; ++c) acc = c_add(acc, in[((long)f * C + c) * N + p]);
+        out[i] = acc;
+    }
+}
+
+// ComplexAbs / ComplexAbsPow2 (src/kernels/complexAbs.cl:43-70, complexAbsPow2.cl:43-71): out = (|z| or |z|^2, 0)
+__global__ void k_abs(const float2* __restrict__ in, float2* __restrict__ out, long n, int pow2) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float a2 = c_abs2(in[i]);
+        out[i] = make_float2(pow2 ? a2 : sqrtf(a2), 0.f);
+    }
+}
+
+// CopyDataGPU (initialize.cl:76-80) / BLAS copy, scal, axpy with a complex alpha
+__global__ void k_copy(const float2* __restrict__ x, float2* __restrict__ y, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+}
+__global__ void k_scal(float2* __restrict__ x, long n, float ar, float ai) {
+    const float2 a = make_float2(ar, ai);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = c_mul(a, x[i]);
+}
+__global__ void k_axpy(long n, float ar, float ai, const float2* __restrict__ x, float2* __restrict__ y) {
+    const float2 a = make_float2(ar, ai);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = c_add(y[i], c_mul(a, x[i]));
+}
+__global__ void k_fill(float2* __restrict__ x, long n, float2 v) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+__global__ void k_fill_bytes(uint8_t* __restrict__ x, long n, uint8_t v) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+
+// Two-stage deterministic reductions over complex vectors.  Stage 1 writes one partial per workgroup
+// (partials[op_width * blockIdx.x + q]); stage 2 (one workgroup) folds them in index order.
+//   RED_NRM2: sum |x|^2              RED_DOTC: sum conj(x)*y (2 values)   RED_SUM: sum re+im
+//   RED_ASUM: sum |re|+|im|          RED_MAXABS: max |x|
+__global__ void k_reduce_partial(const float2* __restrict__ x, const float2* __restrict__ y, long n, int op,
+                                 double* __restrict__ partials) {
+    __shared__ double red[32];
+    double v[2] = {0.0, 0.0};
+    if (op == RED_MAXABS) v[0] = -1.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float2 a = x[i];
+        switch (op) {
+            case RED_NRM2: v[0] += (double)a.x * a.x + (double)a.y * a.y; break;
+            case RED_DOTC: {
+                const float2 b = y[i];
+                v[0] += (double)a.x * b.x + (double)a.y * b.y;
+                v[1] += (double)a.x * b.y - (double)a.y * b.x;
+                break;
+            }
+            case RED_SUM: v[0] += (double)a.x + (double)a.y; break;
+            case RED_ASUM: v[0] += fabs((double)a.x) + fabs((double)a.y); break;
+            default: v[0] = fmax(v[0], (double)sqrtf(c_abs2(a))); break;
+        }
+    }
+    if (op == RED_MAXABS) {
+        v[0] = block_max(v[0], red);
+    } else {
+        block_sum<2>(v, red);
+    }
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = v[0]; partials[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partials, int nparts, int op, double* __restrict__ out) {
+    __shared__ double red[32];
+    double v[2] = {0.0, 0.0};
+    if (op == RED_MAXABS) v[0] = -1.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        if (op == RED_MAXABS) v[0] = fmax(v[0], partials[2 * i]);
+        else { v[0] += partials[2 * i]; v[1] += partials[2 * i + 1]; }
+    }
+    if (op == RED_MAXABS) v[0] = block_max(v[0], red);
+    else block_sum<2>(v, red);
+    if (threadIdx.x == 0) {
+        out[0] = op == RED_NRM2 ? sqrt(v[0]) : v[0];
+        out[1] = v[1];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Standalone batched 2-D FFT (FFT Process, src/processes/FFT.cpp): a row pass and a column pass through LDS.
+// conj_io = 1 turns the forward kernel into the unscaled inverse (conj in, conj out); scale multiplies the output.
+// ------------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(OCL_NT) void k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out,
+                                                     long nrows, FftPlan P, int rows_per_wg, int conj_io, float scale) {
+    extern __shared__ float2 lds[];
+    const int n = P.n, ld = n + 1;
+    float2* a = lds;
+    float2* b = a + rows_per_wg * ld;
+    float2* tw = b + rows_per_wg * ld;
+    const long r0 = (long)blockIdx.x * rows_per_wg;
+    const float invn = 1.0f / (float)n;
+    load_twiddles(tw, P);
+    for (int e = threadIdx.x; e < rows_per_wg * n; e += blockDim.x) {
+        const int r = fdiv(e, invn), i = e - r * n;
+        float2 v = make_float2(0.f, 0.f);
+        if (r0 + r < nrows) v = in[(r0 + r) * n + i];
+        if (conj_io) v.y = -v.y;
+        a[r * ld + i] = v;
+    }
+    __syncthreads();
+    float2* res = lds_fft(a, b, tw, P, rows_per_wg, ld, 1, false);
+    for (int e = threadIdx.x; e < rows_per_wg * n; e += blockDim.x) {
+        const int r = fdiv(e, invn), i = e - r * n;
+        if (r0 + r < nrows) {
+            float2 v = res[r * ld + i];
+            if (conj_io) v.y = -v.y;
+            out[(r0 + r) * n + i] = c_scale(v, scale);
+        }
+    }
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_fft_cols(const float2* __restrict__ in, float2* __restrict__ out, int H,
+                                                     int W, FftPlan P, int tw_cols, int conj_io, float scale) {
+    extern __shared__ float2 lds[];
+    const int ld = tw_cols + 1;
+    float2* a = lds;
+    float2* b = a + H * ld;
+    float2* tw = b + H * ld;
+    const long img = blockIdx.y;
+    const int w0 = blockIdx.x * tw_cols;
+    const float2* I = in + img * (long)H * W;
+    float2* O = out + img * (long)H * W;
+    load_twiddles(tw, P);
+    for (int e = threadIdx.x; e < H * tw_cols; e += blockDim.x) {
+        const int h = e / tw_cols, wl = e - h * tw_cols;
+        float2 v = make_float2(0.f, 0.f);
+        if (w0 + wl < W) v = I[(long)h * W + w0 + wl];
+        if (conj_io) v.y = -v.y;
+        a[h * ld + wl] = v;
+    }
+    __syncthreads();
+    float2* res = lds_fft(a, b, tw, P, tw_cols, 1, ld, true);
+    for (int e = threadIdx.x; e < H * tw_cols; e += blockDim.x) {
+        const int h = e / tw_cols, wl = e - h * tw_cols;
+        if (w0 + wl < W) {
+            float2 v = res[h * ld + wl];
+            if (conj_io) v.y = -v.y;
+            O[(long)h * W + w0 + wl] = c_scale(v, scale);
+        }
+    }
+}
+
+// drand48 stream from a given 48-bit state, written as (float(value), 0) pairs: out[i] = X_{start+i+1} / 2^48.
+// Each thread jumps to its chunk with the LCG's affine power, then steps sequentially.
+__global__ void k_drand48_fill(float2* __restrict__ out, long n, unsigned long long x0, long start) {
+    const unsigned long long A = 0x5DEECE66DULL, Cc = 0xBULL, MASK = (1ULL << 48) - 1;
+    const int CH = 64;
+    const long chunk = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long i0 = chunk * CH;
+    if (i0 >= n) return;
+    // state after (start + i0) steps: x = A^k x0 + C (A^{k-1} + ... + 1)
+    unsigned long long k = (unsigned long long)(start + i0);
+    unsigned long long ma = 1, mc = 0;          // accumulated map x -> ma*x + mc
+    unsigned long long pa = A, pc = Cc;         // map for 2^b steps
+    while (k) {
+        if (k & 1) { ma = (ma * pa) & MASK; mc = (mc * pa + pc) & MASK; }
+        pc = (pc * pa + pc) & MASK;
+        pa = (pa * pa) & MASK;
+        k >>= 1;
+    }
+    unsigned long long x = (ma * x0 + mc) & MASK;
+    for (int t = 0; t < CH && i0 + t < n; ++t) {
+        x = (A * x + Cc) & MASK;
+        out[i0 + t] = make_float2((float)((double)x / 281474976710656.0), 0.f);
+    }
+}

@@ -1,0 +1,43 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol the header declares, and refuses to run
+without a GPU instead of falling back to anything."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+from opencliper_amd import capi
+
+
+def test_header_declares_entry_points():
+    syms = capi.header_symbols()
+    for s in ("ocl_ctx_create", "ocl_ttv", "ocl_fft2d", "ocl_nesta_run", "ocl_batch_run", "ocl_normest"):
+        assert s in syms
+    assert len(syms) >= 36
+
+
+def test_library_exports_every_header_symbol():
+    L = ctypes.CDLL(capi.LIB_PATH)
+    missing = [s for s in capi.header_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_nm_shows_gfx950_code_object():
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True, text=True).stdout
+    assert "ocl_nesta_run" in out
+    blob = open(capi.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_null_arguments_are_rejected_without_gpu():
+    L = capi.lib()
+    assert L.ocl_ctx_sync(None) == -1
+    assert b"null context" in L.ocl_last_error()
+    assert L.ocl_ttv(None, None, None, 1, 1, 1) == -1
+    assert L.ocl_batch_setup(None) == -1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="only meaningful without a GPU")
+def test_context_creation_fails_loudly_without_gpu():
+    with pytest.raises(capi.OclError):
+        capi.Context(0)

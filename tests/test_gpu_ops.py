@@ -1,0 +1,131 @@
+"""GPU parity of the individual Processes (TemporalTV, VectorNormalization, ComplexElementProd, ApplyMask,
+XImageSum, ComplexAbs(Pow2), CopyDataGPU, FFT, SimpleMRIRecon, BLAS-1 reductions, myNormest) against the oracle,
+called through the C ABI."""
+import numpy as np
+import pytest
+
+from opencliper_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    print("device:", c.info(), flush=True)
+    yield c
+    c.close()
+
+
+def rnd(shape, seed, scale=1.0):
+    r = np.random.default_rng(seed)
+    return ((r.normal(size=shape) + 1j * r.normal(size=shape)) * scale).astype(np.complex64)
+
+
+@pytest.mark.parametrize("F,N", [(1, 17), (2, 64), (7, 1000), (20, 65536)])
+def test_ttv_bit_exact(ctx, oracle, F, N):
+    x = rnd((F, N), F * N)
+    d = ctx.to_device(x)
+    o = ctx.empty((F, N))
+    for fwd in (True, False):
+        ctx.ttv(d, o, fwd)
+        assert np.array_equal(o.get(), oracle.ttv(x, fwd, np.complex64))
+
+
+def test_vecnorm(ctx, oracle):
+    x = rnd((3, 5000), 1)
+    d = ctx.to_device(x)
+    o = ctx.empty(x.shape)
+    ctx.vecnorm(d, o, 0.8)
+    np.testing.assert_allclose(o.get(), oracle.vecnorm(x, np.float32(0.8)), rtol=2e-7, atol=1e-7)
+
+
+@pytest.mark.parametrize("in_coils,conj", [(0, 0), (1, 1), (1, 0)])
+def test_cep(ctx, oracle, in_coils, conj):
+    F, C, N = 3, 4, 777
+    S = rnd((C, N), 2)
+    x = rnd((F, C, N) if in_coils else (F, N), 3)
+    o = ctx.empty((F, C, N))
+    ctx.cep(ctx.to_device(x), ctx.to_device(S), o, F, C, in_coils, conj)
+    np.testing.assert_allclose(o.get(), oracle.cep(x, S, F, C, in_coils, conj), rtol=1e-6, atol=1e-6)
+
+
+def test_mask_and_ximage_sum_exact(ctx, oracle):
+    F, C, H, W = 4, 3, 16, 24
+    k = rnd((F, C, H, W), 4)
+    M = (np.random.default_rng(5).random((F, H, W)) > 0.5).astype(np.uint8)
+    d = ctx.to_device(k)
+    ctx.apply_mask(d, ctx.to_device(M))
+    km = d.get()
+    assert np.array_equal(km, oracle.apply_mask(k, M))
+    o = ctx.empty((F, H, W))
+    ctx.ximage_sum(d, o)
+    assert np.array_equal(o.get(), oracle.ximage_sum(km))
+
+
+def test_abs_copy(ctx):
+    x = rnd((1000,), 6)
+    d = ctx.to_device(x)
+    o = ctx.empty(x.shape)
+    ctx.complex_abs(d, o)
+    np.testing.assert_allclose(o.get().real, np.abs(x.astype(np.complex128)), rtol=2e-7)
+    assert np.all(o.get().imag == 0)
+    ctx.complex_abs(d, o, pow2=True)
+    np.testing.assert_allclose(o.get().real, np.abs(x.astype(np.complex128)) ** 2, rtol=3e-7)
+    ctx.copy(d, o)
+    assert np.array_equal(o.get(), x)
+
+
+@pytest.mark.parametrize("H,W", [(8, 8), (12, 10), (128, 128), (160, 192), (224, 256), (256, 256), (288, 320),
+                                 (320, 288), (7, 5), (9, 49), (1, 64), (512, 96)])
+def test_fft2d_vs_numpy(ctx, H, W):
+    x = rnd((3, H, W), H * 1000 + W)
+    d = ctx.to_device(x)
+    o = ctx.empty(x.shape)
+    ctx.fft2d(d, o)
+    ref = np.fft.fft2(x.astype(np.complex128))
+    assert np.abs(o.get() - ref).max() <= 3e-6 * np.abs(ref).max() * max(1.0, np.log2(H * W) / 8)
+    ctx.fft2d(d, d, inverse=True)          # in place
+    refi = np.fft.ifft2(x.astype(np.complex128))
+    assert np.abs(d.get() - refi).max() <= 3e-6 * np.abs(refi).max() * max(1.0, np.log2(H * W) / 8)
+
+
+def test_fft_rejects_unsupported_length(ctx):
+    d = ctx.empty((1, 11, 8))
+    with pytest.raises(capi.OclError) as e:
+        ctx.fft2d(d, d)
+    assert e.value.code == -3
+
+
+def test_reductions(ctx):
+    x = rnd((100003,), 7)
+    y = rnd((100003,), 8)
+    dx, dy = ctx.to_device(x), ctx.to_device(y)
+    X, Y = x.astype(np.complex128), y.astype(np.complex128)
+    assert abs(ctx.reduce(dx, "nrm2")[0] - np.linalg.norm(X)) < 1e-9 * np.linalg.norm(X)
+    re, im = ctx.reduce(dx, "dotc", dy)
+    ref = np.vdot(X, Y)
+    assert abs(complex(re, im) - ref) < 1e-9 * abs(ref) + 1e-6
+    assert abs(ctx.reduce(dx, "sum")[0] - (X.real.sum() + X.imag.sum())) < 1e-6
+    assert abs(ctx.reduce(dx, "asum")[0] - (np.abs(X.real).sum() + np.abs(X.imag).sum())) < 1e-6
+    assert abs(ctx.reduce(dx, "maxabs")[0] - np.abs(X).max()) < 1e-6
+
+
+@pytest.mark.parametrize("F,N", [(3, 64), (4, 64), (20, 65536), (25, 36864), (7, 16384)])
+def test_normest_matches_oracle(ctx, oracle, F, N):
+    g, gi = ctx.normest(F, N)
+    e, ei = oracle.normest(F, N, double=False)
+    assert gi == ei
+    assert abs(g - e) < 2e-6 * e
+
+
+def test_simple_mri_recon(ctx, oracle):
+    sl = synth.make_fully_sampled(H=256, W=256, C=8, F=1)
+    y = ctx.to_device(sl.y)
+    S = ctx.to_device(sl.S)
+    x = ctx.empty((1, 256, 256))
+    ctx.simple_mri_recon(y, S, x)
+    ref64 = oracle.simple_recon(sl.y, sl.S, double=True)
+    got = x.get()
+    assert oracle.rel_l2(got, ref64) < 1e-6
+    assert oracle.rel_l2(got, oracle.simple_recon(sl.y, sl.S, double=False)) < 1e-6
