@@ -18,6 +18,7 @@
 
 #include "ocl_amd.h"
 #include "ocl_kernels.hpp"
+#include "ocl_nesta_fast.hpp"
 
 // ---------------------------------------------------------------------------------------------------------------------
 // errors
@@ -75,11 +76,12 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->red_part, sizeof(double) * 2 * RED_PARTS);
     if (e == hipSuccess) e = hipMalloc(&c->red_out, sizeof(double) * 16);
     if (e == hipSuccess) e = hipHostMalloc(&c->red_host, sizeof(double) * 16, hipHostMallocDefault);
-    const void* lds_kernels[] = {(const void*)k_fft_rows,    (const void*)k_fft_cols,    (const void*)k_nesta_B,
-                                 (const void*)k_nesta_CA_4,  (const void*)k_nesta_CA_8,  (const void*)k_nesta_CA_16,
-                                 (const void*)k_nesta_CA_32};
-    if (e == hipSuccess)
+    const void* lds_kernels[] = {(const void*)k_fft_rows, (const void*)k_fft_cols, (const void*)k_fft_rows_fast,
+                                 (const void*)k_fft_cols_fast};
+    if (e == hipSuccess) {
         for (const void* k : lds_kernels) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
+        nesta_kernel_attributes((int)LDS_MAX);
+    }
     (void)hipGetLastError();
     if (e != hipSuccess) {
         std::string m = std::string("ocl_ctx_create: ") + hipGetErrorString(e);
@@ -215,6 +217,13 @@ static int make_plan(ocl_ctx* c, int n, FftPlan* P) {
     return OCL_OK;
 }
 
+// P when n = 16 P takes the register four-step path (P even, 2..20), else 0
+static int fast_p(int n) {
+    if (n % 16) return 0;
+    const int P = n / 16;
+    return (P >= 2 && P <= 20 && P % 2 == 0) ? P : 0;
+}
+
 static int pow2_floor(int v) {
     int p = 1;
     while (p * 2 <= v) p *= 2;
@@ -341,6 +350,11 @@ extern "C" int ocl_reduce(ocl_ctx* c, const void* x, const void* y, long n, int 
 // ---------------------------------------------------------------------------------------------------------------------
 // 2-D FFT
 // ---------------------------------------------------------------------------------------------------------------------
+static int fft_rows_generic(ocl_ctx* c, const void* in, void* out, long nrows, int W, const FftPlan& pw,
+                            int conj_io);
+static int fft_cols_generic(ocl_ctx* c, void* out, long batch, int H, int W, const FftPlan& ph, int conj_io,
+                            float scale2);
+
 static int fft2d(ocl_ctx* c, const void* in, void* out, long batch, int H, int W, int dir) {
     if (batch < 1 || H < 1 || W < 1 || !in || !out) return fail(OCL_E_INVALID, "FFT: bad arguments");
     if (H > 2048 || W > 2048) return fail(OCL_E_UNSUPPORTED, "FFT: lengths above 2048 are not supported");
@@ -349,17 +363,43 @@ static int fft2d(ocl_ctx* c, const void* in, void* out, long batch, int H, int W
     CHK(make_plan(c, H, &ph));
     const int conj_io = dir > 0 ? 1 : 0;
     const float scale2 = dir > 0 ? (float)(1.0 / ((double)H * (double)W)) : 1.0f;
+    const long nrows = batch * H;
+    const int PW = fast_p(W), PH = fast_p(H);
+    if (PW) {
+        const size_t lds = sizeof(float2) * ((size_t)16 * (PW * 17 + 1) + W);
+        hipLaunchKernelGGL(k_fft_rows_fast, dim3((unsigned)((nrows + 15) / 16)), dim3(OCL_NT), lds, c->stream,
+                           (const float2*)in, (float2*)out, nrows, PW, pw.tw, conj_io, 1.0f);
+        HIPCHK(hipGetLastError());
+    }
+    if (PH) {
+        if (!PW) CHK(fft_rows_generic(c, in, out, nrows, W, pw, conj_io));
+        const size_t lds = sizeof(float2) * ((size_t)PH * FAST_COLS_KS + H);
+        hipLaunchKernelGGL(k_fft_cols_fast, dim3((unsigned)((W + 15) / 16), (unsigned)batch), dim3(OCL_NT), lds,
+                           c->stream, (const float2*)out, (float2*)out, W, PH, ph.tw, conj_io, scale2);
+        HIPCHK(hipGetLastError());
+        return OCL_OK;
+    }
+    if (!PW) CHK(fft_rows_generic(c, in, out, nrows, W, pw, conj_io));
+    return fft_cols_generic(c, out, batch, H, W, ph, conj_io, scale2);
+}
+
+static int fft_rows_generic(ocl_ctx* c, const void* in, void* out, long nrows, int W, const FftPlan& pw,
+                            int conj_io) {
     int RB = std::min(16, pow2_floor(std::max(1, 2048 / W)));
     while (RB > 1 && sizeof(float2) * ((size_t)2 * RB * (W + 1) + W) > LDS_MAX) RB >>= 1;
-    const long nrows = batch * H;
     size_t lds = sizeof(float2) * ((size_t)2 * RB * (W + 1) + W);
     if (lds > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "FFT: row length too large for one LDS tile");
     hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)((nrows + RB - 1) / RB)), dim3(OCL_NT), lds, c->stream,
                        (const float2*)in, (float2*)out, nrows, pw, RB, conj_io, 1.0f);
     HIPCHK(hipGetLastError());
+    return OCL_OK;
+}
+
+static int fft_cols_generic(ocl_ctx* c, void* out, long batch, int H, int W, const FftPlan& ph, int conj_io,
+                            float scale2) {
     int TW = 16;
     while (TW > 1 && sizeof(float2) * ((size_t)2 * H * (TW + 1) + H) > LDS_MAX) TW >>= 1;
-    lds = sizeof(float2) * ((size_t)2 * H * (TW + 1) + H);
+    const size_t lds = sizeof(float2) * ((size_t)2 * H * (TW + 1) + H);
     if (lds > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "FFT: column length too large for one LDS tile");
     hipLaunchKernelGGL(k_fft_cols, dim3((unsigned)((W + TW - 1) / TW), (unsigned)batch), dim3(OCL_NT), lds,
                        c->stream, (const float2*)out, (float2*)out, H, W, ph, TW, conj_io, scale2);
@@ -624,6 +664,7 @@ struct ocl_batch {
     int totB = 0, totCA = 0;
     size_t ldsB = 0, ldsCA = 0;
     int maxe = 4;
+    NestaLaunch L{};
     SliceDesc* dd = nullptr;
     int* d_offB = nullptr;
     int* d_offCA = nullptr;
@@ -715,7 +756,11 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
         d.N = (long)s.H * s.W;
         CHK(make_plan(c, s.H, &d.ph));
         CHK(make_plan(c, s.W, &d.pw));
+        d.fastH = fast_p(s.H);
+        d.fastW = fast_p(s.W);
         CHK(choose_tiles(b, s, &d.tile_cols, &d.rows_per_wg));
+        if (d.fastH) d.tile_cols = 16;
+        if (d.fastW) d.rows_per_wg = 16;
         // sampled-row lists
         const size_t fb = (size_t)s.F * s.H;
         if (fb > flags_cap) {
@@ -735,11 +780,14 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
             for (int h = 0; h < s.H; ++h) nrows[f] += flags[(size_t)f * s.H + h];
             nmax = std::max(nmax, nrows[f]);
         }
-        std::vector<int16_t> rows((size_t)s.F * nmax, 0);
+        std::vector<int16_t> rows((size_t)s.F * nmax, 0), rowpos((size_t)s.F * s.H, -1);
         for (int f = 0; f < s.F; ++f) {
             int j = 0;
             for (int h = 0; h < s.H; ++h)
-                if (flags[(size_t)f * s.H + h]) rows[(size_t)f * nmax + j++] = (int16_t)h;
+                if (flags[(size_t)f * s.H + h]) {
+                    rowpos[(size_t)f * s.H + h] = (int16_t)j;
+                    rows[(size_t)f * nmax + j++] = (int16_t)h;
+                }
         }
         d.nmax = nmax;
         d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
@@ -761,6 +809,8 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
         CHK(batch_alloc(b, sizeof(float2) * (size_t)d.F * d.C * nmax * d.W, &p)); d.kbuf = (float2*)p;
         CHK(batch_alloc(b, sizeof(int16_t) * rows.size(), &p)); d.rows = (const int16_t*)p;
         HIPCHK(hipMemcpyAsync(p, rows.data(), sizeof(int16_t) * rows.size(), hipMemcpyHostToDevice, c->stream));
+        CHK(batch_alloc(b, sizeof(int16_t) * rowpos.size(), &p)); d.rowpos = (const int16_t*)p;
+        HIPCHK(hipMemcpyAsync(p, rowpos.data(), sizeof(int16_t) * rowpos.size(), hipMemcpyHostToDevice, c->stream));
         CHK(batch_alloc(b, sizeof(int) * s.F, &p)); d.nrows = (const int*)p;
         HIPCHK(hipMemcpyAsync(p, nrows.data(), sizeof(int) * s.F, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));   // rows / nrows are host temporaries
@@ -773,10 +823,14 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
         CHK(batch_alloc(b, sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
         b->offB[i + 1] = b->offB[i] + d.wg_B + d.wg_U;
         b->offCA[i + 1] = b->offCA[i] + d.wg_CA;
-        b->ldsB = std::max(b->ldsB, sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W));
-        b->ldsCA = std::max(b->ldsCA, sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H));
-        const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
-        while (b->maxe < need) b->maxe *= 2;
+        b->ldsB = std::max(b->ldsB, d.fastW ? sizeof(float2) * fast_b_lds(d.fastW)
+                                            : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W));
+        b->ldsCA = std::max(b->ldsCA, d.fastH ? sizeof(float2) * fast_ca_lds(d.fastH)
+                                              : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H));
+        if (!d.fastH) {
+            const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
+            while (b->maxe < need) b->maxe *= 2;
+        }
     }
     if (flags_d) {
         hipStreamSynchronize(c->stream);
@@ -797,6 +851,12 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
     HIPCHK(hipMemcpyAsync(b->d_offCA, b->offCA.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(b->d_outs, outs.data(), sizeof(float2*) * ns, hipMemcpyHostToDevice, c->stream));
     if (!b->h_st) HIPCHK(hipHostMalloc(&b->h_st, sizeof(NState) * ns, hipHostMallocDefault));
+    int PH = b->hd[0].fastH, PW = b->hd[0].fastW;
+    for (int i = 1; i < ns; ++i) {
+        if (b->hd[i].fastH != PH) PH = 0;
+        if (b->hd[i].fastW != PW) PW = 0;
+    }
+    b->L = NestaLaunch{b->dd, b->d_offB, b->d_offCA, ns, b->totB, b->totCA, b->ldsB, b->ldsCA, b->maxe, PH, PW};
     HIPCHK(hipStreamSynchronize(c->stream));
     b->allocated = true;
     return batch_compute_setup(b);
@@ -811,18 +871,8 @@ static int batch_compute_setup(ocl_batch* b) {
     for (int i = 0; i < ns; ++i) CHK(ocl_memset_zero(c, b->hd[i].counter, sizeof(unsigned) * 4));
 
     // x_ref = A^H y (NestaUp.cpp:468-486) and the forward columns of iteration 0
-    hipLaunchKernelGGL(k_nesta_B, dim3(b->totB), dim3(OCL_NT), b->ldsB, c->stream, (const SliceDesc*)b->dd,
-                       (const int*)b->d_offB, ns, 1);
-    HIPCHK(hipGetLastError());
-    auto launch_CA = [&](int mode) -> int {
-        void (*k)(const SliceDesc*, const int*, int, int) =
-            b->maxe <= 4 ? k_nesta_CA_4 : b->maxe <= 8 ? k_nesta_CA_8 : b->maxe <= 16 ? k_nesta_CA_16 : k_nesta_CA_32;
-        hipLaunchKernelGGL(k, dim3(b->totCA), dim3(OCL_NT), b->ldsCA, c->stream, (const SliceDesc*)b->dd,
-                           (const int*)b->d_offCA, ns, mode);
-        HIPCHK(hipGetLastError());
-        return OCL_OK;
-    };
-    CHK(launch_CA(1));
+    HIPCHK(launch_nesta_B(b->L, c->stream, 1));
+    HIPCHK(launch_nesta_CA(b->L, c->stream, 1));
     hipLaunchKernelGGL(k_setup_max, dim3(b->totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)b->dd,
                        (const int*)b->d_offCA, ns, (double*)nullptr);
     HIPCHK(hipGetLastError());
@@ -874,13 +924,10 @@ static int batch_compute_setup(ocl_batch* b) {
 static int batch_enqueue(ocl_batch* b, int n_iters) {
     ocl_ctx* c = b->ctx;
     const int ns = (int)b->sl.size();
-    void (*kca)(const SliceDesc*, const int*, int, int) =
-        b->maxe <= 4 ? k_nesta_CA_4 : b->maxe <= 8 ? k_nesta_CA_8 : b->maxe <= 16 ? k_nesta_CA_16 : k_nesta_CA_32;
     for (int it = 0; it < n_iters; ++it) {
-        hipLaunchKernelGGL(k_nesta_B, dim3(b->totB), dim3(OCL_NT), b->ldsB, c->stream, (const SliceDesc*)b->dd,
-                           (const int*)b->d_offB, ns, 0);
-        hipLaunchKernelGGL(kca, dim3(b->totCA), dim3(OCL_NT), b->ldsCA, c->stream, (const SliceDesc*)b->dd,
-                           (const int*)b->d_offCA, ns, 0);
+        HIPCHK(launch_nesta_B(b->L, c->stream, 0));
+        HIPCHK(launch_nesta_S(b->L, c->stream));
+        HIPCHK(launch_nesta_CA(b->L, c->stream, 0));
     }
     HIPCHK(hipGetLastError());
     return OCL_OK;
@@ -892,17 +939,14 @@ extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* m
     ocl_ctx* c = b->ctx;
     HIPCHK(hipSetDevice(c->device));
     const int ns = (int)b->sl.size();
-    void (*kca)(const SliceDesc*, const int*, int, int) =
-        b->maxe <= 4 ? k_nesta_CA_4 : b->maxe <= 8 ? k_nesta_CA_8 : b->maxe <= 16 ? k_nesta_CA_16 : k_nesta_CA_32;
     std::vector<hipEvent_t> ev(3 * (size_t)n_iters);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     for (int it = 0; it < n_iters; ++it) {
         HIPCHK(hipEventRecord(ev[3 * it], c->stream));
-        hipLaunchKernelGGL(k_nesta_B, dim3(b->totB), dim3(OCL_NT), b->ldsB, c->stream, (const SliceDesc*)b->dd,
-                           (const int*)b->d_offB, ns, 0);
+        HIPCHK(launch_nesta_B(b->L, c->stream, 0));
+        HIPCHK(launch_nesta_S(b->L, c->stream));
         HIPCHK(hipEventRecord(ev[3 * it + 1], c->stream));
-        hipLaunchKernelGGL(kca, dim3(b->totCA), dim3(OCL_NT), b->ldsCA, c->stream, (const SliceDesc*)b->dd,
-                           (const int*)b->d_offCA, ns, 0);
+        HIPCHK(launch_nesta_CA(b->L, c->stream, 0));
         HIPCHK(hipEventRecord(ev[3 * it + 2], c->stream));
     }
     HIPCHK(hipGetLastError());

@@ -24,7 +24,7 @@ struct NState {
     double mu, tolVar, Lmu, Lmu1;
     double last_f, last_qp;
     // coefficients of the iteration prepared by pass B and consumed by pass CA
-    int it_pending;    // 1 between the scalar step of B(k) and the end of CA(k)
+    int it_pending;    // 1 when the scalar step has prepared an iteration for pass CA
     int it_cur, it_sstart;
     double it_mu, it_L1, it_apk, it_tau;
     double fmean[OCL_MAX_MINITER];
@@ -44,6 +44,7 @@ struct SliceDesc {
     int wg_U;           // extra workgroups of pass B that reduce the temporal-TV terms of x_k
     int u_chunk;        // pixels per U workgroup (per frame)
     float inv_sqrtN;
+    int fastH, fastW;   // P with H (W) = 16 P when the register four-step path applies, else 0
     FftPlan ph, pw;     // column (length H) and row (length W) plans
     const float2* y;    // [F][C][H][W]
     const float2* S;    // [C][H][W]
@@ -54,11 +55,12 @@ struct SliceDesc {
     float2* Wk;         // [F][H][W]
     float2* kbuf;       // [F][C][nmax][W]  sampled-row intermediates
     const int16_t* rows;// [F][nmax] sampled rows of each frame
+    const int16_t* rowpos; // [F][H] slot of row h in kbuf, or -1
     const int* nrows;   // [F]
     NState* st;
     double* partB;      // [wg_B]        sum |r|^2 per row workgroup
     double* partU;      // [wg_U][2]     sum |u|^2, sum Re(conj(u) Ux) per U workgroup
-    unsigned* counter;  // [2] arrival counters of passes B and CA (reset by their last workgroup)
+    unsigned* counter;  // [4] spare device counters (zeroed at setup)
     double* trace;      // [trace_cap][3] : f, ||r||^2, l1 per inner iteration (may be null)
     double* maxpart;    // [wg_CA][2] partial maxima of |x_ref| and |U x_ref| (setup)
 };
@@ -82,13 +84,26 @@ __global__ void k_fft_rows(const float2* in, float2* out, long nrows, FftPlan P,
 __global__ void k_fft_cols(const float2* in, float2* out, int H, int W, FftPlan P, int tw_cols, int conj_io,
                            float scale);
 __global__ void k_drand48_fill(float2* out, long n, unsigned long long x0, long start);
+#define FAST_COLS_KS 272   // per-k1 stride (float2) of the column exchange in the fast FFT kernels
+__global__ void k_fft_rows_fast(const float2* in, float2* out, long nrows, int P, const float2* tw, int conj_io,
+                                float scale);
+__global__ void k_fft_cols_fast(const float2* in, float2* out, int W, int P, const float2* tw, int conj_io,
+                                float scale);
 
 // kernels (ocl_nesta.hip)
 __global__ void k_rowflags(const float2* y, const uint8_t* M, int F, int C, int H, int W, uint8_t* flags);
-__global__ void k_nesta_B(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
-__global__ void k_nesta_CA_4(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
-__global__ void k_nesta_CA_8(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
-__global__ void k_nesta_CA_16(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
-__global__ void k_nesta_CA_32(const SliceDesc* descs, const int* wg_off, int nslices, int mode);
+// launch shape of the fused passes over one batch; PH / PW > 0 when every slice shares that fast column / row size
+struct NestaLaunch {
+    const SliceDesc* dd;
+    const int* offB;
+    const int* offCA;
+    int ns, totB, totCA;
+    size_t ldsB, ldsCA;
+    int maxe, PH, PW;
+};
+hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
+hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);
+hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
+void nesta_kernel_attributes(int lds_bytes);
 __global__ void k_setup_max(const SliceDesc* descs, const int* wg_off, int nslices, double* out);
 __global__ void k_finalize(const SliceDesc* descs, int nslices, float2* const* outs);

@@ -11,6 +11,7 @@
 // f = 1/2 ||r||^2 + lambda (Re<u,Ux> - mu/2 ||u||^2), runs the reference's stop test and advances the device state
 // (NestaUp.cpp:797-889), so no host round trip is needed between iterations.
 #include "ocl_kernels.hpp"
+#include "ocl_nesta_fast.hpp"
 
 __device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, int b) {
     int lo = 0, hi = ns;
@@ -44,6 +45,10 @@ __global__ void k_rowflags(const float2* __restrict__ y, const uint8_t* __restri
 // (NestaUp.cpp:809-831).
 // ------------------------------------------------------------------------------------------------------------------
 __device__ void nesta_scalar_step(const SliceDesc& d, double* red) {
+    if (!d.st->active) {                 // every stage done: pass CA of this iteration must not run
+        if (threadIdx.x == 0) d.st->it_pending = 0;
+        return;
+    }
     double s[3] = {0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < d.wg_B; i += blockDim.x) s[0] += d.partB[i];
     for (int i = threadIdx.x; i < d.wg_U; i += blockDim.x) { s[1] += d.partU[2 * i]; s[2] += d.partU[2 * i + 1]; }
@@ -112,18 +117,25 @@ __device__ void nesta_scalar_step(const SliceDesc& d, double* red) {
         }
     }
     st->k = k;
-    __threadfence();
+}
+
+// One workgroup per slice, between pass B(k) and pass CA(k).  Kernel boundaries order the partial sums written by
+// B before this read and the coefficients written here before CA reads them, so no grid-wide fences are needed.
+__global__ __launch_bounds__(OCL_NT) void k_nesta_S(const SliceDesc* __restrict__ descs, int nslices) {
+    __shared__ double red[64];
+    if ((int)blockIdx.x >= nslices) return;
+    nesta_scalar_step(descs[blockIdx.x], red);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
 // pass B.  Workgroups [0, wg_B) of a slice handle k-space rows; [wg_B, wg_B + wg_U) reduce the temporal-TV terms
 // of x_k (mode 0 only).  mode 0: NESTA residual + scalar step.  mode 1: setup (V = IFFT_W(y row)).
 // ------------------------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(OCL_NT) void k_nesta_B(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
-                                                    int nslices, int mode) {
+template <int PWU>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_B_t(const SliceDesc* __restrict__ descs,
+                                                      const int* __restrict__ wg_off, int nslices, int mode) {
     extern __shared__ float2 lds[];
     __shared__ double red[64];
-    __shared__ int is_last;
     const int si = find_slice(wg_off, nslices, blockIdx.x);
     const SliceDesc& d = descs[si];
     NState* st = d.st;
@@ -151,6 +163,23 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B(const SliceDesc* __restrict_
         }
         block_sum<2>(v, red);
         if (threadIdx.x == 0) { d.partU[2 * uid] = v[0]; d.partU[2 * uid + 1] = v[1]; }
+    } else if (PWU > 0 || d.fastW) {
+        double acc[1];
+        switch (PWU > 0 ? PWU : d.fastW) {
+            case 2: acc[0] = nesta_B_fast<2>(d, local, mode, lds); break;
+            case 4: acc[0] = nesta_B_fast<4>(d, local, mode, lds); break;
+            case 6: acc[0] = nesta_B_fast<6>(d, local, mode, lds); break;
+            case 8: acc[0] = nesta_B_fast<8>(d, local, mode, lds); break;
+            case 10: acc[0] = nesta_B_fast<10>(d, local, mode, lds); break;
+            case 12: acc[0] = nesta_B_fast<12>(d, local, mode, lds); break;
+            case 14: acc[0] = nesta_B_fast<14>(d, local, mode, lds); break;
+            case 16: acc[0] = nesta_B_fast<16>(d, local, mode, lds); break;
+            case 18: acc[0] = nesta_B_fast<18>(d, local, mode, lds); break;
+            default: acc[0] = nesta_B_fast<20>(d, local, mode, lds); break;
+        }
+        if (mode != 0) return;
+        block_sum<1>(acc, red);
+        if (threadIdx.x == 0) d.partB[local] = acc[0];
     } else {
         const int g = local % d.nrow_groups;
         const int fc = local / d.nrow_groups;
@@ -212,18 +241,6 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B(const SliceDesc* __restrict_
         block_sum<1>(acc, red);
         if (threadIdx.x == 0) d.partB[local] = acc[0];
     }
-    // ---- the last workgroup of this slice runs the scalar step ----
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned old = atomicAdd(&d.counter[0], 1u);
-        is_last = (old == (unsigned)(d.wg_B + d.wg_U - 1));
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        nesta_scalar_step(d, red);
-        if (threadIdx.x == 0) atomicExch(&d.counter[0], 0u);
-    }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -231,15 +248,8 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B(const SliceDesc* __restrict_
 // mode 1: setup (x_ref = A^H y, forward columns of iteration 0).
 // ------------------------------------------------------------------------------------------------------------------
 template <int MAXE>
-__device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
-                                              int nslices, int mode) {
-    extern __shared__ float2 lds[];
-    __shared__ int is_last;
-    const int si = find_slice(wg_off, nslices, blockIdx.x);
-    const SliceDesc& d = descs[si];
+__device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, int mode, float2* lds) {
     NState* st = d.st;
-    if (mode == 0 && !st->it_pending) return;
-    const int local = blockIdx.x - wg_off[si];
     const int tile = local % d.ncol_tiles;
     const int f = local / d.ncol_tiles;
     const int F = d.F, C = d.C, H = d.H, W = d.W, TW = d.tile_cols, nmax = d.nmax;
@@ -358,32 +368,100 @@ __device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ desc
         __syncthreads();
     }
 
-    if (mode == 1) return;
-    // ---- the last workgroup of this slice marks the iteration consumed ----
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned old = atomicAdd(&d.counter[1], 1u);
-        is_last = (old == (unsigned)(d.wg_CA - 1));
-    }
-    __syncthreads();
-    if (is_last && threadIdx.x == 0) {
-        __threadfence();
-        st->it_pending = 0;
-        atomicExch(&d.counter[1], 0u);
-        __threadfence();
+}
+
+template <int PHU, int MAXE>
+__device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
+                                              int nslices, int mode) {
+    extern __shared__ float2 lds[];
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    NState* st = d.st;
+    if (mode == 0 && !st->it_pending) return;
+    const int local = blockIdx.x - wg_off[si];
+    if constexpr (PHU > 0) {
+        nesta_CA_fast<PHU>(d, local, mode, lds);
+    } else if (d.fastH) {
+        switch (d.fastH) {
+            case 2: nesta_CA_fast<2>(d, local, mode, lds); break;
+            case 4: nesta_CA_fast<4>(d, local, mode, lds); break;
+            case 6: nesta_CA_fast<6>(d, local, mode, lds); break;
+            case 8: nesta_CA_fast<8>(d, local, mode, lds); break;
+            case 10: nesta_CA_fast<10>(d, local, mode, lds); break;
+            case 12: nesta_CA_fast<12>(d, local, mode, lds); break;
+            case 14: nesta_CA_fast<14>(d, local, mode, lds); break;
+            case 16: nesta_CA_fast<16>(d, local, mode, lds); break;
+            case 18: nesta_CA_fast<18>(d, local, mode, lds); break;
+            default: nesta_CA_fast<20>(d, local, mode, lds); break;
+        }
+    } else {
+        nesta_CA_generic<MAXE>(d, local, mode, lds);
     }
 }
 
-#define OCL_CA_KERNEL(E)                                                                                        \
-    __global__ __launch_bounds__(OCL_NT) void k_nesta_CA_##E(const SliceDesc* __restrict__ descs,               \
-                                                             const int* __restrict__ wg_off, int nslices,        \
-                                                             int mode) {                                        \
-        nesta_CA_body<E>(descs, wg_off, nslices, mode);                                                          \
+template <int PHU, int MAXE>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_CA_t(const SliceDesc* __restrict__ descs,
+                                                       const int* __restrict__ wg_off, int nslices, int mode) {
+    nesta_CA_body<PHU, MAXE>(descs, wg_off, nslices, mode);
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// host-side launch helpers (kernel templates are instantiated in this translation unit)
+// ---------------------------------------------------------------------------------------------------------------------
+typedef void (*NestaKernel)(const SliceDesc*, const int*, int, int);
+
+static NestaKernel pick_B(int PW) {
+    switch (PW) {
+        case 8: return k_nesta_B_t<8>;
+        case 10: return k_nesta_B_t<10>;
+        case 12: return k_nesta_B_t<12>;
+        case 14: return k_nesta_B_t<14>;
+        case 16: return k_nesta_B_t<16>;
+        case 18: return k_nesta_B_t<18>;
+        case 20: return k_nesta_B_t<20>;
+        default: return k_nesta_B_t<0>;
     }
-OCL_CA_KERNEL(4)
-OCL_CA_KERNEL(8)
-OCL_CA_KERNEL(16)
-OCL_CA_KERNEL(32)
+}
+
+static NestaKernel pick_CA(int PH, int maxe) {
+    switch (PH) {
+        case 8: return k_nesta_CA_t<8, 4>;
+        case 10: return k_nesta_CA_t<10, 4>;
+        case 12: return k_nesta_CA_t<12, 4>;
+        case 14: return k_nesta_CA_t<14, 4>;
+        case 16: return k_nesta_CA_t<16, 4>;
+        case 18: return k_nesta_CA_t<18, 4>;
+        case 20: return k_nesta_CA_t<20, 4>;
+        default: break;
+    }
+    return maxe <= 4 ? k_nesta_CA_t<0, 4> : maxe <= 8 ? k_nesta_CA_t<0, 8> : maxe <= 16 ? k_nesta_CA_t<0, 16>
+                                                                                      : k_nesta_CA_t<0, 32>;
+}
+
+hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode) {
+    hipLaunchKernelGGL(pick_B(L.PW), dim3(L.totB), dim3(OCL_NT), L.ldsB, stream, L.dd, L.offB, L.ns, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream) {
+    hipLaunchKernelGGL(k_nesta_S, dim3(L.ns), dim3(OCL_NT), 0, stream, L.dd, L.ns);
+    return hipGetLastError();
+}
+
+hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode) {
+    hipLaunchKernelGGL(pick_CA(L.PH, L.maxe), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
+    return hipGetLastError();
+}
+
+void nesta_kernel_attributes(int lds_bytes) {
+    const int ps[] = {0, 8, 10, 12, 14, 16, 18, 20};
+    for (int p : ps) {
+        (void)hipFuncSetAttribute((const void*)pick_B(p), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+        for (int e : {4, 8, 16, 32})
+            (void)hipFuncSetAttribute((const void*)pick_CA(p, e), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    }
+    (void)hipGetLastError();
+}
 
 // Setup maxima (NestaUp.cpp:521-568): per workgroup max |x_ref| and max |U x_ref| over a chunk of x_ref = x0.
 __global__ void k_setup_max(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off, int nslices,

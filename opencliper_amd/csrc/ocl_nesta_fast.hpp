@@ -1,0 +1,261 @@
+// Register four-step bodies of the fused NESTA passes for H = 16 P_H and W = 16 P_W (see ocl_rfft.hpp).
+// Included by ocl_nesta.hip only.
+#pragma once
+#include "ocl_kernels.hpp"
+#include "ocl_rfft.hpp"
+
+// LDS footprint (float2 elements) of the fast passes
+#define FAST_CA_KS 272                          // per-k1 stride of the column exchange: 16 roles x 16 columns + pad
+#define FAST_B_RS(P) ((P) * 17 + 1)             // per-row stride of the row exchange
+__host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + 16 * P + 4 * P; }
+__host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + 16 * P; }
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass B, rows of length W = 16 P: 16 rows per workgroup, 16 threads per row
+// ---------------------------------------------------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, int mode, float2* lds) {
+    constexpr int W = 16 * P;
+    constexpr int RS = FAST_B_RS(P);
+    const int a = threadIdx.x & 15, r = threadIdx.x >> 4;
+    const int g = local % d.nrow_groups;
+    const int fc = local / d.nrow_groups;
+    const int C = d.C, H = d.H, nmax = d.nmax;
+    const int f = fc / C, c = fc - f * C;
+    const int nr = d.nrows[f];
+    const int j = g * 16 + r;
+    const bool valid = j < nr;
+    float2* tw = lds + 16 * RS;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) tw[i] = d.pw.tw[i];
+    float2* X = lds + r * RS;
+    float2* kb = d.kbuf + (((long)f * C + c) * nmax + (valid ? j : 0)) * W;
+    const int ky = valid ? d.rows[(long)f * nmax + j] : 0;
+    const float2* yrow = d.y + (((long)f * C + c) * H + ky) * W;
+    double acc = 0.0;
+    const uint8_t* mrow = d.M + ((long)f * H + ky) * W;
+    // y and mask at the residual positions kx = k1 + P k2 of this thread's roles, fetched ahead of the FFT
+    constexpr int NQ = P > 16 ? 2 : 1;
+    float2 yq[NQ][16];
+    uint8_t mq[NQ][16];
+    if (mode == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k1 = a + 16 * q;
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) {
+                const bool ok = valid && k1 < P;
+                yq[q][k2] = ok ? yrow[k1 + P * k2] : make_float2(0.f, 0.f);
+                mq[q][k2] = ok ? mrow[k1 + P * k2] : (uint8_t)0;
+            }
+        }
+    }
+    float2 v[P];
+    // ---- FA phase 1 on U (mode 0) or conj(y) (mode 1) ----
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+        float2 x = make_float2(0.f, 0.f);
+        if (valid) x = (mode == 0) ? kb[a + 16 * b] : c_conj(yrow[a + 16 * b]);
+        v[b] = x;
+    }
+    __syncthreads();   // twiddles in LDS
+    rf::rdft<P>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) X[k1 * 17 + a] = (k1 == 0 || a == 0) ? v[k1] : c_mul(v[k1], tw[a * k1]);
+    __syncthreads();
+    // ---- FA phase 2: role k1 in {a, a+16} ----
+    float2 u[NQ][16];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k1 = a + 16 * q;
+        if (k1 < P) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) u[q][i] = X[k1 * 17 + i];
+            rf::rdft<16>(u[q]);
+        }
+    }
+    if (mode == 1) {
+        // V = IFFT_W(y row) = conj(FFT(conj(y)))  at kx = k1 + P k2
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k1 = a + 16 * q;
+            if (k1 < P && valid)
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) kb[k1 + P * k2] = c_conj(u[q][k2]);
+        }
+        return 0.0;
+    }
+    // ---- residual r = M K / sqrt(N) - y at kx = k1 + P k2, then FB phase 1 on conj(r) ----
+    const float s = d.inv_sqrtN;
+    __syncthreads();   // X is rewritten below
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k1 = a + 16 * q;
+        if (k1 < P) {
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) {
+                float2 rv = make_float2(0.f, 0.f);
+                if (valid) {
+                    const float2 yv = yq[q][k2];
+                    const float2 K = u[q][k2];
+                    rv = mq[q][k2] ? make_float2(K.x * s - yv.x, K.y * s - yv.y) : make_float2(-yv.x, -yv.y);
+                    acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
+                }
+                u[q][k2] = c_conj(rv);
+            }
+            rf::rdft<16>(u[q]);
+#pragma unroll
+            for (int n2 = 0; n2 < 16; ++n2)
+                X[k1 * 17 + n2] = (k1 == 0 || n2 == 0) ? u[q][n2] : c_mul(u[q][n2], tw[k1 * n2]);
+        }
+    }
+    __syncthreads();
+    // ---- FB phase 2: role n2 = a, out[n2 + 16 n1] ----
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) v[k1] = X[k1 * 17 + a];
+    rf::rdft<P>(v);
+    if (valid)
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) kb[a + 16 * n1] = c_conj(v[n1]);
+    return acc;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass CA, columns of length H = 16 P: 16 columns per workgroup, 16 roles per column
+// ---------------------------------------------------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int mode, float2* lds) {
+    constexpr int H = 16 * P;
+    const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int tile = local % d.ncol_tiles;
+    const int f = local / d.ncol_tiles;
+    const int F = d.F, C = d.C, W = d.W, nmax = d.nmax;
+    const long N = d.N;
+    const int w = tile * 16 + s;
+    const bool wv = w < W;
+    const int wc = wv ? w : 0;
+    NState* st = d.st;
+    float2* tw = lds + P * FAST_CA_KS;
+    int16_t* rp = (int16_t*)(tw + H);
+    for (int i = threadIdx.x; i < H; i += blockDim.x) {
+        tw[i] = d.ph.tw[i];
+        rp[i] = d.rowpos[(long)f * H + i];
+    }
+    __syncthreads();
+
+    float2 acc[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[i] = make_float2(0.f, 0.f);
+
+    // ---- A^H r: per coil, inverse column DFT (FB on conj(V), input pruned to the sampled rows) ----
+    constexpr int NQ = P > 16 ? 2 : 1;
+    float2 sv[P];
+    for (int c = 0; c < C; ++c) {
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        const float2* Sc = d.S + (long)c * N + wc;
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) sv[n1] = Sc[(long)(t + 16 * n1) * W];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k1 = t + 16 * q;
+            if (k1 < P) {
+                float2 u[16];
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    const int jj = rp[k1 + P * k2];
+                    u[k2] = (jj >= 0 && wv) ? c_conj(kb[(long)jj * W + w]) : make_float2(0.f, 0.f);
+                }
+                rf::rdft<16>(u);
+#pragma unroll
+                for (int n2 = 0; n2 < 16; ++n2)
+                    lds[k1 * FAST_CA_KS + n2 * 16 + s] = (k1 == 0 || n2 == 0) ? u[n2] : c_mul(u[n2], tw[k1 * n2]);
+            }
+        }
+        __syncthreads();
+        float2 v[P];
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) v[k1] = lds[k1 * FAST_CA_KS + t * 16 + s];
+        __syncthreads();
+        rf::rdft<P>(v);
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) {
+            const float2 z = c_conj(v[n1]);                       // IFFT_H(V) at h = t + 16 n1
+            acc[n1] = c_add(acc[n1], c_mulconj(z, sv[n1]));
+        }
+    }
+
+    // ---- NESTA update (NestaUp.cpp:669-875 fused) at h = t + 16 n1, w ----
+    float2 xn[P];
+    const float sc = d.inv_sqrtN;
+    if (mode == 1) {
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) {
+            xn[n1] = c_scale(acc[n1], sc);
+            if (wv) d.x0[(long)f * N + (long)(t + 16 * n1) * W + w] = xn[n1];
+        }
+    } else {
+        const float2* xk = st->it_cur ? d.x1 : d.x0;
+        float2* xo = st->it_cur ? d.x0 : d.x1;
+        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+        const float mu = (float)st->it_mu, lam = (float)st->lambda, L1 = (float)st->it_L1;
+        const float apk = (float)st->it_apk, tau = (float)st->it_tau;
+        const int sstart = st->it_sstart;
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) {
+            xn[n1] = make_float2(0.f, 0.f);
+            if (wv) {
+                const long p = (long)(t + 16 * n1) * W + w;
+                const float2 xf = xk[(long)f * N + p], xm = xk[(long)fm * N + p], xp = xk[(long)fp * N + p];
+                const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
+                const float nf = fmaxf(mu, sqrtf(c_abs2(uxf))), np = fmaxf(mu, sqrtf(c_abs2(uxp)));
+                const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
+                const float2 ares = c_scale(acc[n1], sc);
+                const float2 df = make_float2(lam * (uf.x - up.x) + ares.x, lam * (uf.y - up.y) + ares.y);
+                float2 xr, wo;
+                if (sstart) {
+                    xr = xf;
+                    d.xref[(long)f * N + p] = xf;
+                    wo = make_float2(0.f, 0.f);
+                } else {
+                    xr = d.xref[(long)f * N + p];
+                    wo = d.Wk[(long)f * N + p];
+                }
+                const float2 yk = make_float2(xf.x - L1 * df.x, xf.y - L1 * df.y);
+                const float2 wn = make_float2(wo.x + apk * df.x, wo.y + apk * df.y);
+                d.Wk[(long)f * N + p] = wn;
+                const float2 z = make_float2(xr.x - L1 * wn.x, xr.y - L1 * wn.y);
+                xn[n1] = make_float2(tau * z.x + (1.0f - tau) * yk.x, tau * z.y + (1.0f - tau) * yk.y);
+                xo[(long)f * N + p] = xn[n1];
+            }
+        }
+    }
+
+    // ---- forward column DFTs of S_c x_{k+1} (FA), output pruned to the sampled rows ----
+    for (int c = 0; c < C; ++c) {
+        const float2* Sc = d.S + (long)c * N + wc;
+        float2 v[P];
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) v[n1] = wv ? c_mul(xn[n1], Sc[(long)(t + 16 * n1) * W]) : make_float2(0.f, 0.f);
+        rf::rdft<P>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1)
+            lds[k1 * FAST_CA_KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+        __syncthreads();
+        float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k1 = t + 16 * q;
+            if (k1 < P) {
+                float2 u[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) u[i] = lds[k1 * FAST_CA_KS + i * 16 + s];
+                rf::rdft<16>(u);
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    const int jj = rp[k1 + P * k2];
+                    if (jj >= 0 && wv) kb[(long)jj * W + w] = u[k2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}

@@ -3,6 +3,7 @@
 // complex load or store.  Launch wrappers live in ocl_capi.cpp.
 #include "ocl_device.hpp"
 #include "ocl_kernels.hpp"
+#include "ocl_rfft.hpp"
 
 // TemporalTV (src/kernels/temporalTV.cl:49-107).  dir=1: out_f = in_f - in_{f-1 mod F};
 // dir=0 (the reference's "adjoint", i.e. -D^T): out_f = in_{f+1 mod F} - in_f.
@@ -206,6 +207,128 @@ __global__ __launch_bounds__(OCL_NT) void k_fft_cols(const float2* __restrict__ 
             if (conj_io) v.y = -v.y;
             O[(long)h * W + w0 + wl] = c_scale(v, scale);
         }
+    }
+}
+
+// Register four-step variants for lengths n = 16 P (P even, 2..20): one LDS transpose per transform.
+// Rows: 16 rows of one workgroup, 16 threads per row.  Columns: 16 columns x 16 roles.
+template <int P>
+__device__ __forceinline__ void fft_rows_fast(const float2* __restrict__ in, float2* __restrict__ out, long nrows,
+                                              const float2* __restrict__ twg, int conj_io, float scale) {
+    constexpr int n = 16 * P, RS = P * 17 + 1;
+    extern __shared__ float2 lds[];
+    float2* tw = lds + 16 * RS;
+    const int a = threadIdx.x & 15, r = threadIdx.x >> 4;
+    const long row = (long)blockIdx.x * 16 + r;
+    const bool valid = row < nrows;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    float2 v[P];
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+        float2 x = valid ? in[row * n + a + 16 * b] : make_float2(0.f, 0.f);
+        if (conj_io) x.y = -x.y;
+        v[b] = x;
+    }
+    __syncthreads();
+    rf::rdft<P>(v);
+    float2* X = lds + r * RS;
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) X[k1 * 17 + a] = (k1 == 0 || a == 0) ? v[k1] : c_mul(v[k1], tw[a * k1]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k1 = a + 16 * q;
+        if (k1 < P) {
+            float2 u[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) u[i] = X[k1 * 17 + i];
+            rf::rdft<16>(u);
+            if (valid)
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    float2 y = u[k2];
+                    if (conj_io) y.y = -y.y;
+                    out[row * n + k1 + P * k2] = c_scale(y, scale);
+                }
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void fft_cols_fast(const float2* __restrict__ in, float2* __restrict__ out, int W,
+                                              const float2* __restrict__ twg, int conj_io, float scale) {
+    constexpr int H = 16 * P, KS = FAST_COLS_KS;
+    extern __shared__ float2 lds[];
+    float2* tw = lds + P * KS;
+    const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const long img = blockIdx.y;
+    const int w = blockIdx.x * 16 + s;
+    const bool wv = w < W;
+    const float2* I = in + img * (long)H * W;
+    float2* O = out + img * (long)H * W;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) tw[i] = twg[i];
+    float2 v[P];
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+        float2 x = wv ? I[(long)(t + 16 * b) * W + w] : make_float2(0.f, 0.f);
+        if (conj_io) x.y = -x.y;
+        v[b] = x;
+    }
+    __syncthreads();
+    rf::rdft<P>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k1 = t + 16 * q;
+        if (k1 < P) {
+            float2 u[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) u[i] = lds[k1 * KS + i * 16 + s];
+            rf::rdft<16>(u);
+            if (wv)
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    float2 y = u[k2];
+                    if (conj_io) y.y = -y.y;
+                    O[(long)(k1 + P * k2) * W + w] = c_scale(y, scale);
+                }
+        }
+    }
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_fft_rows_fast(const float2* __restrict__ in, float2* __restrict__ out,
+                                                          long nrows, int P, const float2* __restrict__ tw,
+                                                          int conj_io, float scale) {
+    switch (P) {
+        case 2: fft_rows_fast<2>(in, out, nrows, tw, conj_io, scale); break;
+        case 4: fft_rows_fast<4>(in, out, nrows, tw, conj_io, scale); break;
+        case 6: fft_rows_fast<6>(in, out, nrows, tw, conj_io, scale); break;
+        case 8: fft_rows_fast<8>(in, out, nrows, tw, conj_io, scale); break;
+        case 10: fft_rows_fast<10>(in, out, nrows, tw, conj_io, scale); break;
+        case 12: fft_rows_fast<12>(in, out, nrows, tw, conj_io, scale); break;
+        case 14: fft_rows_fast<14>(in, out, nrows, tw, conj_io, scale); break;
+        case 16: fft_rows_fast<16>(in, out, nrows, tw, conj_io, scale); break;
+        case 18: fft_rows_fast<18>(in, out, nrows, tw, conj_io, scale); break;
+        default: fft_rows_fast<20>(in, out, nrows, tw, conj_io, scale); break;
+    }
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_fft_cols_fast(const float2* __restrict__ in, float2* __restrict__ out,
+                                                          int W, int P, const float2* __restrict__ tw, int conj_io,
+                                                          float scale) {
+    switch (P) {
+        case 2: fft_cols_fast<2>(in, out, W, tw, conj_io, scale); break;
+        case 4: fft_cols_fast<4>(in, out, W, tw, conj_io, scale); break;
+        case 6: fft_cols_fast<6>(in, out, W, tw, conj_io, scale); break;
+        case 8: fft_cols_fast<8>(in, out, W, tw, conj_io, scale); break;
+        case 10: fft_cols_fast<10>(in, out, W, tw, conj_io, scale); break;
+        case 12: fft_cols_fast<12>(in, out, W, tw, conj_io, scale); break;
+        case 14: fft_cols_fast<14>(in, out, W, tw, conj_io, scale); break;
+        case 16: fft_cols_fast<16>(in, out, W, tw, conj_io, scale); break;
+        case 18: fft_cols_fast<18>(in, out, W, tw, conj_io, scale); break;
+        default: fft_cols_fast<20>(in, out, W, tw, conj_io, scale); break;
     }
 }
 

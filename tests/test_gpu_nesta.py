@@ -75,3 +75,67 @@ def test_fused_is_deterministic(ctx):
         ctx.nesta(y, S, M, x, capi.nesta_params(maxIntIter=3, maxIter=8, stoptest=0), form=capi.OCL_FORM_FUSED)
         outs.append(x.get())
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_nesta_fast_path_wide_rows(ctx, oracle):
+    """H = 160 (column four-step P=10), W = 320 (row four-step P=20) against the oracle."""
+    sl = synth.make_cine(H=160, W=320, C=4, F=6, accel=4.0, slice_id=14)
+    prm = dict(maxIntIter=2, maxIter=6, stoptest=0)
+    x64, r64 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    for form in (capi.OCL_FORM_PLAIN, capi.OCL_FORM_FUSED):
+        y, S, M, x = upload(ctx, sl.y, sl.S, sl.M)
+        ctx.nesta(y, S, M, x, capi.nesta_params(**prm), form=form)
+        err = oracle.rel_l2(x.get(), x64)
+        print("form", form, "vs fp64", err, flush=True)
+        assert err < 1e-4
+
+
+def _solo(ctx, sl, prm):
+    y, S, M, x = upload(ctx, sl.y, sl.S, sl.M)
+    rep = ctx.nesta(y, S, M, x, prm, form=capi.OCL_FORM_FUSED)
+    return x.get(), rep
+
+
+@pytest.mark.parametrize("stoptest", [0, 1])
+def test_batch_uneven_equals_solo(ctx, oracle, stoptest):
+    """Uneven slices advanced together give bit-identical results to their solo runs (no cross-slice coupling)."""
+    shapes = [(128, 128, 4, 8), (96, 160, 3, 5), (64, 64, 2, 12), (12, 10, 3, 4), (160, 128, 5, 6)]
+    prm = capi.nesta_params(maxIntIter=3, maxIter=12, stoptest=stoptest, miniter=5)
+    slices = [synth.make_cine(H=h, W=w, C=c, F=f, accel=3.0, n_centre=4, slice_id=20 + i)
+              for i, (h, w, c, f) in enumerate(shapes)]
+    dev = [upload(ctx, sl.y, sl.S, sl.M) for sl in slices]
+    b = ctx.batch(dev, prm)
+    b.run()
+    for i, sl in enumerate(slices):
+        xs, rs = _solo(ctx, sl, prm)
+        xb = dev[i][3].get()
+        rb = b.report(i)
+        print(i, shapes[i], rb["iters"], rs["iters"], flush=True)
+        assert rb["iters"] == rs["iters"]
+        assert np.array_equal(xb, xs)
+    x64, _ = oracle.nesta(slices[1].y, slices[1].S, slices[1].M,
+                          oracle.params(maxIntIter=3, maxIter=12, stoptest=stoptest, miniter=5), double=True)
+    assert oracle.rel_l2(dev[1][3].get(), x64) < 1e-3
+    b.close()
+
+
+def test_batch_graph_replay_matches_eager(ctx):
+    """A captured hipGraph of the iterations, replayed after fresh setups, reproduces the eager result each time."""
+    sl = synth.make_cine(H=128, W=128, C=4, F=8, accel=4.0, slice_id=30)
+    prm = capi.nesta_params(maxIntIter=2, maxIter=5, stoptest=0)
+    dev = [upload(ctx, sl.y, sl.S, sl.M)]
+    b = ctx.batch(dev, prm)
+    b.setup()
+    b.iterate(10)
+    b.finalize()
+    ctx.sync()
+    ref = dev[0][3].get()
+    b.setup()
+    b.capture(10)
+    for rep in range(3):
+        b.setup()
+        b.replay()
+        b.finalize()
+        ctx.sync()
+        assert np.array_equal(dev[0][3].get(), ref), rep
+    b.close()
