@@ -1,0 +1,37 @@
+// KData: multi-coil k-space, NDArrays ordered frame-major with the coil index fastest (index = coil + C * frame, as
+// hostKernelFunctions.cl:175-183 of the reference lays them out), plus the attached sensitivity maps and sampling
+// masks (include/OpenCLIPER/KData.hpp:66-351).
+#pragma once
+#include <OpenCLIPER/SamplingMasksData.hpp>
+#include <OpenCLIPER/SensitivityMapsData.hpp>
+
+namespace OpenCLIPER {
+
+class KData : public Data {
+  public:
+    /// From a group of NDArrays (F*C of them, coil fastest), the number of coils and the temporal dimensions.
+    KData(const std::shared_ptr<CLapp>& pCLapp, std::vector<NDArray*>*& pNDArrays, numCoilsType nCoils,
+          std::vector<dimIndexType>*& pDynDims);
+    /// Uninitialised k-space of the given per-NDArray dims (KData.hpp:101).
+    KData(const std::shared_ptr<CLapp>& pCLapp, std::vector<std::vector<dimIndexType>*>*& pArraysDims,
+          numCoilsType nCoils, std::vector<dimIndexType>*& pDynDims);
+    /// Same shape as sourceData (KData.cpp:145-213): copyData copies the samples, copySensitivityMaps /
+    /// copySamplingMasks share the source's maps / masks.
+    KData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<KData>& sourceData, bool copyData = false,
+          bool copySensitivityMaps = false, bool copySamplingMasks = false);
+
+    numCoilsType getNCoils() const { return nCoils_; }
+    std::shared_ptr<SensitivityMapsData> getSensitivityMapsData() const { return sens_; }
+    std::shared_ptr<SamplingMasksData> getSamplingMasksData() const { return masks_; }
+    void setSensitivityMapsData(SensitivityMapsData*& p) { sens_.reset(p); p = nullptr; }
+    void setSensitivityMapsData(const std::shared_ptr<SensitivityMapsData>& p) { sens_ = p; }
+    void setSamplingMasksData(SamplingMasksData*& p) { masks_.reset(p); p = nullptr; }
+    void setSamplingMasksData(const std::shared_ptr<SamplingMasksData>& p) { masks_ = p; }
+
+  private:
+    numCoilsType nCoils_ = 1;
+    std::shared_ptr<SensitivityMapsData> sens_;
+    std::shared_ptr<SamplingMasksData> masks_;
+};
+
+}  // namespace OpenCLIPER

@@ -1,0 +1,20 @@
+// XData: image-space data, one NDArray per frame (include/OpenCLIPER/XData.hpp:56-196).
+#pragma once
+#include <OpenCLIPER/Data.hpp>
+
+namespace OpenCLIPER {
+
+class KData;
+
+class XData : public Data {
+  public:
+    /// From a group of NDArrays (one per frame) and the temporal dimensions (ownership is taken).
+    XData(const std::shared_ptr<CLapp>& pCLapp, std::vector<NDArray*>*& pNDArrays, std::vector<dimIndexType>*& pDynDims);
+    /// Same shape as sourceData; copyData copies its contents (XData.cpp:403-406).
+    XData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<XData>& sourceData, bool copyData,
+          bool copyDataToDevice = true);
+    /// Uninitialised image with one frame per KData frame (XData.cpp:425-440).
+    XData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<KData>& sourceData);
+};
+
+}  // namespace OpenCLIPER

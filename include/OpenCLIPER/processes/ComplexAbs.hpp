@@ -1,0 +1,14 @@
+// out = (|in|, 0) (include/OpenCLIPER/processes/ComplexAbs.hpp).
+#pragma once
+#include <OpenCLIPER/Process.hpp>
+
+namespace OpenCLIPER {
+
+class ComplexAbs : public Process {
+  public:
+    void launch() override;
+    const std::string getKernelFile() const override { return "complexAbs.cl"; }
+    OCL_PROCESS_BOILERPLATE(ComplexAbs)
+};
+
+}  // namespace OpenCLIPER

@@ -1,0 +1,14 @@
+// Device-to-device copy (include/OpenCLIPER/processes/groupwiseRegistration/initialize/CopyDataGPU.hpp).
+#pragma once
+#include <OpenCLIPER/Process.hpp>
+
+namespace OpenCLIPER {
+
+class CopyDataGPU : public Process {
+  public:
+    void launch() override;
+    const std::string getKernelFile() const override { return "initialize.cl"; }
+    OCL_PROCESS_BOILERPLATE(CopyDataGPU)
+};
+
+}  // namespace OpenCLIPER

@@ -69,6 +69,11 @@ int ocl_ctx_sync(ocl_ctx* ctx);
 void* ocl_ctx_stream(ocl_ctx* ctx);                       /* CLapp::getCommandQueue -> hipStream_t */
 int ocl_device_info(ocl_ctx* ctx, char* name, size_t name_len, size_t* total_mem, int* num_cus);
 
+/* ---- stream-ordered timing: replaces ProcessCore::startProfiling / stopProfiling (src/ProcessCore.cpp:184-241) ---- */
+typedef struct ocl_timer ocl_timer;
+int ocl_timer_begin(ocl_ctx* ctx, ocl_timer** t);              /* records a start event on the context's stream */
+int ocl_timer_end(ocl_ctx* ctx, ocl_timer* t, double* ms);     /* records the stop event, waits, frees the timer */
+
 /* ---- device buffers: replace Data::host2Device / device2Host (src/Data.cpp:1493-1495) ---- */
 int ocl_malloc(ocl_ctx* ctx, size_t bytes, void** dptr);
 int ocl_free(ocl_ctx* ctx, void* dptr);

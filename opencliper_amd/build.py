@@ -16,6 +16,7 @@ INC = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libocl_amd.so")
 LIBCPP = os.path.join(PKG, "libOpenCLIPER_amd.so")
+EXAMPLE = os.path.join(PKG, "examples", "mri_recon")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -53,12 +54,17 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [_obj(s) for s in srcs])
     os.replace(tmp, LIB)
-    cpp_srcs = sorted(os.path.join(CPP, f) for f in os.listdir(CPP) if f.endswith(".cpp")) if os.path.isdir(CPP) else []
-    if cpp_srcs:
-        tmp = LIBCPP + ".tmp"
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INC}", "-o", tmp] + cpp_srcs +
-             [f"-L{PKG}", "-locl_amd", "-Wl,-rpath,$ORIGIN"])
-        os.replace(tmp, LIBCPP)
+    # OpenCLIPER-compatible C++ API: host-only code over the C ABI (g++, no HIP headers)
+    cpp_srcs = sorted(os.path.join(CPP, f) for f in os.listdir(CPP) if f.endswith(".cpp"))
+    tmp = LIBCPP + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INC}", "-o", tmp] + cpp_srcs +
+         [f"-L{PKG}", "-locl_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, LIBCPP)
+    # example driver shaped like the reference's tests/MRIRecon.cpp
+    tmp = EXAMPLE + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "mri_recon.cpp"),
+          f"-L{PKG}", "-lOpenCLIPER_amd", "-locl_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, EXAMPLE)
     return LIB
 
 

@@ -126,6 +126,41 @@ extern "C" int ocl_device_info(ocl_ctx* c, char* name, size_t name_len, size_t* 
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------------------------------------------------------
+struct ocl_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+extern "C" int ocl_timer_begin(ocl_ctx* c, ocl_timer** t) {
+    if (!c || !t) return fail(OCL_E_INVALID, "ocl_timer_begin: null argument");
+    ocl_timer* q = new ocl_timer();
+    if (hipEventCreate(&q->a) != hipSuccess || hipEventCreate(&q->b) != hipSuccess ||
+        hipEventRecord(q->a, c->stream) != hipSuccess) {
+        if (q->a) hipEventDestroy(q->a);
+        if (q->b) hipEventDestroy(q->b);
+        delete q;
+        return fail(OCL_E_HIP, "ocl_timer_begin: event creation failed");
+    }
+    *t = q;
+    return OCL_OK;
+}
+
+extern "C" int ocl_timer_end(ocl_ctx* c, ocl_timer* t, double* ms) {
+    if (!c || !t) return fail(OCL_E_INVALID, "ocl_timer_end: null argument");
+    float v = 0.f;
+    hipError_t e = hipEventRecord(t->b, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(t->b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&v, t->a, t->b);
+    hipEventDestroy(t->a);
+    hipEventDestroy(t->b);
+    delete t;
+    if (e != hipSuccess) return fail(OCL_E_HIP, std::string("ocl_timer_end: ") + hipGetErrorString(e));
+    if (ms) *ms = v;
+    return OCL_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
 // memory
 // ---------------------------------------------------------------------------------------------------------------------
 extern "C" int ocl_malloc(ocl_ctx* c, size_t bytes, void** p) {
