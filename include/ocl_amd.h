@@ -105,6 +105,12 @@ int ocl_fft2d(ocl_ctx* ctx, const void* in, void* out, long batch, int H, int W,
  * Like the reference it transforms y in place. */
 int ocl_simple_mri_recon(ocl_ctx* ctx, void* y, const void* S, void* x, int F, int C, int H, int W);
 
+/* Synthetic-input helper (no reference counterpart): y = M * (y + sigma * n) with complex Gaussian n whose real and
+ * imaginary parts have variance 1/2, drawn from a counter-based hash of (seed, index).  Used to build large k-space
+ * test inputs on the device. */
+int ocl_add_noise_masked(ocl_ctx* ctx, void* y, const uint8_t* M, int F, int C, long N, float sigma,
+                         unsigned long long seed);
+
 /* ---- BLAS-1 as NestaUp calls it through CLBlast (NestaUp.cpp call sites listed in the docs) ---- */
 int ocl_scal(ocl_ctx* ctx, void* x, long n, float ar, float ai);                          /* CLBlastCscal */
 int ocl_axpy(ocl_ctx* ctx, long n, float ar, float ai, const void* x, void* y);           /* CLBlastCaxpy */
@@ -135,6 +141,31 @@ int ocl_batch_run(ocl_batch* b, int chunk);           /* setup + iterate in chun
  * span (synchronises).  Used to price the fused iteration against the HBM roofline. */
 int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total);
 int ocl_batch_destroy(ocl_batch* b);
+
+/* ---- serving: continuous batching of NESTA-TV requests (no counterpart in the reference, which runs one
+ *      reconstruction per NestaUp::launch).  A worker thread with its own stream admits queued requests in arrival
+ *      order while their device footprint fits the HBM budget, co-batches the iterations of every running request,
+ *      and retires each one as its last continuation stage ends.  Inputs and outputs are caller-owned device memory
+ *      of the same device and must stay valid until the request is done. ---- */
+typedef struct ocl_server ocl_server;
+#define OCL_SERVE_QUEUED 0
+#define OCL_SERVE_RUNNING 1
+#define OCL_SERVE_DONE 2
+typedef struct {
+    long completed, failed, queued, active;
+    size_t used_bytes, peak_used_bytes, budget_bytes;
+    long slice_iterations, launches;
+    double busy_s, elapsed_s;
+} ocl_server_stats;
+int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_iters, ocl_server** out);
+/* report may be NULL; when given it is filled when the request completes */
+int ocl_server_submit(ocl_server* s, const ocl_slice* slice, const ocl_nesta_params* prm, ocl_nesta_report* report,
+                      long* request_id);
+int ocl_server_poll(ocl_server* s, long request_id, int* status, double* latency_s);
+int ocl_server_wait(ocl_server* s, long request_id, double timeout_s, int* status);
+int ocl_server_request_times(ocl_server* s, long request_id, double* t_submit, double* t_admit, double* t_done);
+int ocl_server_get_stats(ocl_server* s, ocl_server_stats* stats);
+int ocl_server_destroy(ocl_server* s);      /* finishes every submitted request, then stops */
 
 #ifdef __cplusplus
 }

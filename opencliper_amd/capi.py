@@ -66,6 +66,18 @@ class Slice(ctypes.Structure):
                 ("F", ctypes.c_int), ("C", ctypes.c_int), ("H", ctypes.c_int), ("W", ctypes.c_int)]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_long), ("failed", ctypes.c_long), ("queued", ctypes.c_long),
+                ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
+                ("budget_bytes", ctypes.c_size_t), ("slice_iterations", ctypes.c_long), ("launches", ctypes.c_long),
+                ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SERVE_QUEUED, SERVE_RUNNING, SERVE_DONE = 0, 1, 2
+
 _lib = None
 
 
@@ -109,6 +121,16 @@ def lib():
         "ocl_batch_finalize": ([P], I), "ocl_batch_report": ([P, I, ctypes.POINTER(NestaReport)], I),
         "ocl_batch_run": ([P, I], I), "ocl_batch_destroy": ([P], I),
         "ocl_batch_time": ([P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)], I),
+        "ocl_timer_begin": ([P, PP], I), "ocl_timer_end": ([P, P, ctypes.POINTER(D)], I),
+        "ocl_add_noise_masked": ([P, P, P, I, I, LG, F32, ctypes.c_ulonglong], I),
+        "ocl_server_create": ([I, ctypes.c_size_t, I, PP], I),
+        "ocl_server_submit": ([P, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), ctypes.POINTER(NestaReport),
+                               ctypes.POINTER(LG)], I),
+        "ocl_server_poll": ([P, LG, ctypes.POINTER(I), ctypes.POINTER(D)], I),
+        "ocl_server_wait": ([P, LG, D, ctypes.POINTER(I)], I),
+        "ocl_server_request_times": ([P, LG, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)], I),
+        "ocl_server_get_stats": ([P, ctypes.POINTER(ServerStats)], I),
+        "ocl_server_destroy": ([P], I),
     }
     for name, (a, r) in sig.items():
         fn = getattr(L, name)
@@ -244,6 +266,11 @@ class Context:
         _check(lib().ocl_reduce(self.handle, x.ptr, y.ptr if y is not None else None, x.size, RED[op], r))
         return r[0], r[1]
 
+    def add_noise_masked(self, y, M, sigma, seed):
+        F, C = y.shape[:2]
+        _check(lib().ocl_add_noise_masked(self.handle, y.ptr, M.ptr, F, C, y.size // (F * C), float(sigma),
+                                          int(seed) & ((1 << 64) - 1)))
+
     def normest(self, F, N):
         v = ctypes.c_double()
         it = ctypes.c_int()
@@ -316,6 +343,62 @@ class Batch:
     def close(self):
         if self.handle:
             lib().ocl_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching NESTA-TV server (ocl_server_*).  It runs its own stream on `device`; inputs/outputs are
+    DeviceArrays of any Context on the same device and must outlive their request."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 64 << 30, chunk: int = 8):
+        h = ctypes.c_void_p()
+        _check(lib().ocl_server_create(device, int(budget_bytes), int(chunk), ctypes.byref(h)))
+        self.handle = h.value
+        self.keep = {}
+
+    def submit(self, y, S, M, x, prm: NestaParams, want_report=False):
+        F, C, H, W = y.shape
+        sl = Slice(y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W)
+        rep = NestaReport() if want_report else None
+        rid = ctypes.c_long()
+        _check(lib().ocl_server_submit(self.handle, ctypes.byref(sl), ctypes.byref(prm),
+                                       ctypes.byref(rep) if rep is not None else None, ctypes.byref(rid)))
+        self.keep[rid.value] = (y, S, M, x, prm, rep)
+        return rid.value
+
+    def poll(self, rid):
+        st, lat = ctypes.c_int(), ctypes.c_double()
+        _check(lib().ocl_server_poll(self.handle, rid, ctypes.byref(st), ctypes.byref(lat)))
+        return st.value, lat.value
+
+    def wait(self, rid, timeout=600.0):
+        st = ctypes.c_int()
+        _check(lib().ocl_server_wait(self.handle, rid, float(timeout), ctypes.byref(st)))
+        return st.value
+
+    def times(self, rid):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().ocl_server_request_times(self.handle, rid, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def report(self, rid):
+        rep = self.keep[rid][5]
+        return rep.as_dict() if rep is not None else None
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().ocl_server_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().ocl_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

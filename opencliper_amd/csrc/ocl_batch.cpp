@@ -1,0 +1,807 @@
+// Batched and served NESTA-TV on one GPU.
+//
+// Every slice (one reconstruction) owns its device-resident NESTA state and work buffers (SliceRes).  A launch table
+// (descriptor array + per-slice workgroup offsets) lists the slices a pass covers; the fused passes B, S and CA then
+// advance every slice of the table by one iteration per three launches, whatever their sizes.  The batch API keeps a
+// fixed table (and can record it in a hipGraph); the server rebuilds its table whenever requests are admitted or
+// finish, which is what lets new reconstructions join while others are mid-solve (continuous batching).
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+#include "ocl_internal.hpp"
+
+namespace {
+
+struct SliceSetup {
+    double maxX, maxUX, normU, lambda, muf, mu0, gamma, gammat;
+    int ne_iters;
+};
+
+// Device resources of one slice.
+struct SliceRes {
+    ocl_ctx* c = nullptr;
+    ocl_slice in{};
+    ocl_nesta_params prm{};
+    SliceDesc d{};
+    SliceSetup setup{};
+    std::vector<int16_t> rows, rowpos;
+    std::vector<int> nrows;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    bool allocated = false;
+
+    ~SliceRes() {
+        if (!allocs.empty()) {
+            hipStreamSynchronize(c->stream);
+            for (void* p : allocs) hipFree(p);
+        }
+    }
+    int alloc(size_t n, void** p) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(n, 16));
+        if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
+        HIPCHK(e);
+        allocs.push_back(*p);
+        bytes += std::max<size_t>(n, 16);
+        return OCL_OK;
+    }
+};
+
+// Tile shape of the generic (non four-step) passes; the four-step passes always use 16 x 16.
+void choose_tiles(long total_cols, const ocl_slice& s, int* tile_cols, int* rows_per_wg) {
+    int tc = (total_cols / 16 >= 2048) ? 16 : 8;
+    while (tc > 1 && ((long)s.H * tc > (long)OCL_MAXE * OCL_NT ||
+                      sizeof(float2) * ((size_t)2 * s.H * (tc + 1) + s.H) > LDS_MAX))
+        tc >>= 1;
+    *tile_cols = tc;
+    *rows_per_wg = std::min(16, pow2_floor(std::max(1, 2048 / s.W)));
+}
+
+size_t lds_B(const SliceDesc& d) {
+    return d.fastW ? sizeof(float2) * fast_b_lds(d.fastW)
+                   : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W);
+}
+size_t lds_CA(const SliceDesc& d) {
+    return d.fastH ? sizeof(float2) * fast_ca_lds(d.fastH)
+                   : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
+}
+
+// Step 1 of admission: plans, sampled-row lists (one small device round trip) and the exact device footprint.
+int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long total_cols, SliceRes* r) {
+    if (s.F < 1 || s.C < 1 || s.H < 1 || s.W < 1 || !s.y || !s.S || !s.M || !s.x_out)
+        return fail(OCL_E_INVALID, "NESTA: slice has bad dimensions or null pointers");
+    if (s.H > 512 || s.W > 1024) return fail(OCL_E_UNSUPPORTED, "NESTA: fused form supports H <= 512 and W <= 1024");
+    r->c = c;
+    r->in = s;
+    r->prm = prm;
+    SliceDesc& d = r->d;
+    d.F = s.F; d.C = s.C; d.H = s.H; d.W = s.W;
+    d.N = (long)s.H * s.W;
+    CHK(make_plan(c, s.H, &d.ph));
+    CHK(make_plan(c, s.W, &d.pw));
+    d.fastH = fast_p(s.H);
+    d.fastW = fast_p(s.W);
+    choose_tiles(total_cols, s, &d.tile_cols, &d.rows_per_wg);
+    if (d.fastH) d.tile_cols = 16;
+    if (d.fastW) d.rows_per_wg = 16;
+    if (lds_B(d) > LDS_MAX || lds_CA(d) > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
+    // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
+    const size_t fb = (size_t)s.F * s.H;
+    uint8_t* flags_d = nullptr;
+    HIPCHK(hipMalloc(&flags_d, fb));
+    hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F, s.C,
+                       s.H, s.W, flags_d);
+    std::vector<uint8_t> flags(fb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), flags_d, fb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(flags_d);
+    HIPCHK(e);
+    r->nrows.assign(s.F, 0);
+    int nmax = 1;
+    for (int f = 0; f < s.F; ++f) {
+        for (int h = 0; h < s.H; ++h) r->nrows[f] += flags[(size_t)f * s.H + h];
+        nmax = std::max(nmax, r->nrows[f]);
+    }
+    r->rows.assign((size_t)s.F * nmax, 0);
+    r->rowpos.assign((size_t)s.F * s.H, -1);
+    for (int f = 0; f < s.F; ++f) {
+        int j = 0;
+        for (int h = 0; h < s.H; ++h)
+            if (flags[(size_t)f * s.H + h]) {
+                r->rowpos[(size_t)f * s.H + h] = (int16_t)j;
+                r->rows[(size_t)f * nmax + j++] = (int16_t)h;
+            }
+    }
+    d.nmax = nmax;
+    d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
+    d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
+    d.wg_B = s.F * s.C * d.nrow_groups;
+    d.wg_CA = s.F * d.ncol_tiles;
+    d.u_chunk = (int)std::min<long>(d.N, 4096);
+    d.wg_U = s.F * (int)((d.N + d.u_chunk - 1) / d.u_chunk);
+    d.inv_sqrtN = (float)(1.0 / std::sqrt((double)d.N));
+    d.y = (const float2*)s.y;
+    d.S = (const float2*)s.S;
+    d.M = s.M;
+    return OCL_OK;
+}
+
+// Device footprint that slice_alloc will take (the server's admission test).
+size_t slice_footprint(const SliceRes& r) {
+    const SliceDesc& d = r.d;
+    auto rnd = [](size_t n) { return std::max<size_t>(n, 16); };
+    const size_t xb = sizeof(float2) * (size_t)d.F * d.N;
+    return 4 * rnd(xb) + rnd(sizeof(float2) * (size_t)d.F * d.C * d.nmax * d.W) + rnd(sizeof(int16_t) * r.rows.size()) +
+           rnd(sizeof(int16_t) * r.rowpos.size()) + rnd(sizeof(int) * d.F) + rnd(sizeof(NState)) +
+           rnd(sizeof(double) * d.wg_B) + rnd(sizeof(double) * 2 * d.wg_U) + rnd(sizeof(unsigned) * 4) +
+           rnd(sizeof(double) * 3 * OCL_TRACE) + rnd(sizeof(double) * 2 * d.wg_CA);
+}
+
+// Step 2 of admission: device buffers and row tables.
+int slice_alloc(SliceRes* r) {
+    if (r->allocated) return OCL_OK;
+    ocl_ctx* c = r->c;
+    SliceDesc& d = r->d;
+    const size_t xb = sizeof(float2) * (size_t)d.F * d.N;
+    void* p;
+    CHK(r->alloc(xb, &p)); d.x0 = (float2*)p;
+    CHK(r->alloc(xb, &p)); d.x1 = (float2*)p;
+    CHK(r->alloc(xb, &p)); d.xref = (float2*)p;
+    CHK(r->alloc(xb, &p)); d.Wk = (float2*)p;
+    CHK(r->alloc(sizeof(float2) * (size_t)d.F * d.C * d.nmax * d.W, &p)); d.kbuf = (float2*)p;
+    CHK(r->alloc(sizeof(int16_t) * r->rows.size(), &p)); d.rows = (const int16_t*)p;
+    HIPCHK(hipMemcpyAsync(p, r->rows.data(), sizeof(int16_t) * r->rows.size(), hipMemcpyHostToDevice, c->stream));
+    CHK(r->alloc(sizeof(int16_t) * r->rowpos.size(), &p)); d.rowpos = (const int16_t*)p;
+    HIPCHK(hipMemcpyAsync(p, r->rowpos.data(), sizeof(int16_t) * r->rowpos.size(), hipMemcpyHostToDevice, c->stream));
+    CHK(r->alloc(sizeof(int) * d.F, &p)); d.nrows = (const int*)p;
+    HIPCHK(hipMemcpyAsync(p, r->nrows.data(), sizeof(int) * d.F, hipMemcpyHostToDevice, c->stream));
+    CHK(r->alloc(sizeof(NState), &p)); d.st = (NState*)p;
+    CHK(r->alloc(sizeof(double) * d.wg_B, &p)); d.partB = (double*)p;
+    CHK(r->alloc(sizeof(double) * 2 * d.wg_U, &p)); d.partU = (double*)p;
+    CHK(r->alloc(sizeof(unsigned) * 4, &p)); d.counter = (unsigned*)p;
+    CHK(r->alloc(sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
+    CHK(r->alloc(sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
+    HIPCHK(hipStreamSynchronize(c->stream));    // the host-side row tables are reused by later admissions
+    r->allocated = true;
+    return OCL_OK;
+}
+
+// Launch table over a set of slices.
+struct Table {
+    ocl_ctx* c = nullptr;
+    SliceDesc* dd = nullptr;
+    int* offB = nullptr;
+    int* offCA = nullptr;
+    int cap = 0;
+    NestaLaunch L{};
+    ~Table() {
+        if (dd) hipFree(dd);
+        if (offB) hipFree(offB);
+        if (offCA) hipFree(offCA);
+    }
+};
+
+int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
+    const int ns = (int)v.size();
+    t->c = c;
+    if (ns > t->cap) {
+        if (t->dd) { hipStreamSynchronize(c->stream); hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); }
+        t->dd = nullptr; t->offB = t->offCA = nullptr;
+        const int cap = std::max(ns, 2 * t->cap);
+        HIPCHK(hipMalloc(&t->dd, sizeof(SliceDesc) * cap));
+        HIPCHK(hipMalloc(&t->offB, sizeof(int) * (cap + 1)));
+        HIPCHK(hipMalloc(&t->offCA, sizeof(int) * (cap + 1)));
+        t->cap = cap;
+    }
+    std::vector<SliceDesc> hd(ns);
+    std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0);
+    NestaLaunch& L = t->L;
+    L = NestaLaunch{};
+    L.maxe = 4;
+    int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
+    for (int i = 0; i < ns; ++i) {
+        const SliceDesc& d = v[i]->d;
+        hd[i] = d;
+        oB[i + 1] = oB[i] + d.wg_B + d.wg_U;
+        oC[i + 1] = oC[i] + d.wg_CA;
+        L.ldsB = std::max(L.ldsB, lds_B(d));
+        L.ldsCA = std::max(L.ldsCA, lds_CA(d));
+        if (!d.fastH) {
+            const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
+            while (L.maxe < need) L.maxe *= 2;
+        }
+        if (d.fastH != PH) PH = 0;
+        if (d.fastW != PW) PW = 0;
+    }
+    L.dd = t->dd;
+    L.offB = t->offB;
+    L.offCA = t->offCA;
+    L.ns = ns;
+    L.totB = oB[ns];
+    L.totCA = oC[ns];
+    L.PH = PH;
+    L.PW = PW;
+    if (ns) {
+        HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(t->offB, oB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(t->offCA, oC.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // hd / oB / oC are host temporaries
+    }
+    return OCL_OK;
+}
+
+// Per-reconstruction setup (NestaUp.cpp:442-649): x_ref = A^H y, the forward columns of iteration 0, the maxima of
+// |x_ref| and |U x_ref|, normU, and the initial device state.  Scalars are kept in double.
+int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
+    if (v.empty()) return OCL_OK;
+    for (SliceRes* r : v) CHK(ocl_memset_zero(c, r->d.counter, sizeof(unsigned) * 4));
+    HIPCHK(launch_nesta_B(t.L, c->stream, 1));
+    HIPCHK(launch_nesta_CA(t.L, c->stream, 1));
+    hipLaunchKernelGGL(k_setup_max, dim3(t.L.totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)t.dd,
+                       (const int*)t.offCA, t.L.ns, (double*)nullptr);
+    HIPCHK(hipGetLastError());
+    std::vector<double> mp;
+    std::vector<NState> st(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+        SliceRes* r = v[i];
+        const SliceDesc& d = r->d;
+        mp.resize(2 * d.wg_CA);
+        HIPCHK(hipMemcpyAsync(mp.data(), d.maxpart, sizeof(double) * 2 * d.wg_CA, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double m0 = (double)LONG_MIN, m1 = (double)LONG_MIN;
+        for (int q = 0; q < d.wg_CA; ++q) { m0 = std::max(m0, mp[2 * q]); m1 = std::max(m1, mp[2 * q + 1]); }
+        SliceSetup& u = r->setup;
+        u.maxX = (float)m0;
+        u.maxUX = (float)m1;
+        CHK(ocl_normest(c, d.F, d.N, &u.normU, &u.ne_iters));
+        const ocl_nesta_params& P = r->prm;
+        u.lambda = P.lambda_i * u.maxX;
+        u.muf = P.mu_f * u.maxX;
+        u.mu0 = 0.9 * u.maxUX;
+        const double muL = u.lambda / P.La;
+        if (muL > u.mu0) u.mu0 = muL;
+        u.gamma = std::pow(u.muf / u.mu0, 1.0 / std::max(1, P.maxIntIter));
+        u.gammat = std::pow(P.tolVar / 0.1, 1.0 / std::max(1, P.maxIntIter));
+        NState& S = st[i];
+        std::memset(&S, 0, sizeof(S));
+        S.lambda = u.lambda; S.La = P.La; S.normU = u.normU; S.gamma = u.gamma; S.gammat = u.gammat;
+        S.maxIntIter = P.maxIntIter; S.maxIter = P.maxIter; S.stoptest = P.stoptest; S.miniter = P.miniter;
+        S.active = (P.maxIntIter >= 1 && P.maxIter >= 1) ? 1 : 0;
+        S.stage = 1;
+        S.stage_start = 1;
+        S.trace_cap = OCL_TRACE;
+        S.mu = u.mu0 * u.gamma;
+        S.tolVar = 0.1 * u.gammat;
+        S.Lmu = u.lambda * (u.normU / S.mu) + P.La;
+        S.Lmu1 = 1.0 / S.Lmu;
+        S.fmean[P.miniter - 1] = 9223372036854775807.0;
+        HIPCHK(hipMemcpyAsync(d.st, &S, sizeof(NState), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCL_OK;
+}
+
+int table_iterate(ocl_ctx* c, const Table& t, int n) {
+    if (!t.L.ns) return OCL_OK;
+    for (int it = 0; it < n; ++it) {
+        HIPCHK(launch_nesta_B(t.L, c->stream, 0));
+        HIPCHK(launch_nesta_S(t.L, c->stream));
+        HIPCHK(launch_nesta_CA(t.L, c->stream, 0));
+    }
+    return OCL_OK;
+}
+
+int pull_states(ocl_ctx* c, const std::vector<SliceRes*>& v, NState* h) {
+    for (size_t i = 0; i < v.size(); ++i)
+        HIPCHK(hipMemcpyAsync(&h[i], v[i]->d.st, sizeof(NState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCL_OK;
+}
+
+// Copies the current x of a slice (the buffer its state points at) to the caller's output.
+int slice_finalize(ocl_ctx* c, SliceRes* r, const NState& s) {
+    const SliceDesc& d = r->d;
+    HIPCHK(hipMemcpyAsync(r->in.x_out, s.cur ? d.x1 : d.x0, sizeof(float2) * (size_t)d.F * d.N,
+                          hipMemcpyDeviceToDevice, c->stream));
+    return OCL_OK;
+}
+
+void fill_report(const SliceRes& r, const NState& S, const double* trace, ocl_nesta_report* rep) {
+    const SliceSetup& u = r.setup;
+    std::memset(rep, 0, sizeof(*rep));
+    rep->normU = u.normU; rep->maxXref = u.maxX; rep->maxUXref = u.maxUX; rep->lambda = u.lambda;
+    rep->muf = u.muf; rep->mu0 = u.mu0; rep->gamma = u.gamma; rep->gammat = u.gammat;
+    rep->normest_iters = u.ne_iters;
+    rep->final_tolVar = S.tolVar;
+    rep->final_mu = S.mu;
+    const int stages_done = S.stage - 1 + (S.active ? 1 : 0);
+    rep->stages = std::min(stages_done, OCL_MAX_STAGES);
+    for (int q = 0; q < rep->stages; ++q) rep->iters[q] = (q == S.stage - 1 && S.active) ? S.iters_stage : S.iters[q];
+    rep->ntrace = std::min(S.ntrace, OCL_TRACE);
+    for (int q = 0; q < rep->ntrace; ++q) {
+        rep->trace_f[q] = trace[3 * q];
+        rep->trace_l2[q] = trace[3 * q + 1];
+        rep->trace_l1[q] = trace[3 * q + 2];
+    }
+}
+
+}  // namespace
+
+// =====================================================================================================================
+// batch API: a fixed set of slices
+// =====================================================================================================================
+struct ocl_batch {
+    ocl_ctx* ctx = nullptr;
+    ocl_nesta_params prm{};
+    std::vector<ocl_slice> sl;
+    std::vector<std::unique_ptr<SliceRes>> res;
+    std::vector<SliceRes*> view;
+    Table t;
+    std::vector<NState> h_st;
+    float2** d_outs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool ready = false;
+};
+
+extern "C" int ocl_batch_create(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
+                                ocl_batch** out) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (nslices < 1 || !slices || !prm || !out) return fail(OCL_E_INVALID, "batch: bad arguments");
+    if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER)
+        return fail(OCL_E_INVALID, "NESTA: miniter must be in [1, 32]");
+    if (prm->maxIntIter < 0 || prm->maxIter < 0) return fail(OCL_E_INVALID, "NESTA: negative iteration counts");
+    for (int i = 0; i < nslices; ++i) {
+        const ocl_slice& s = slices[i];
+        if (s.F < 1 || s.C < 1 || s.H < 1 || s.W < 1 || !s.y || !s.S || !s.M || !s.x_out)
+            return fail(OCL_E_INVALID, "batch: slice " + std::to_string(i) + " has bad dimensions or null pointers");
+        if (s.H > 512 || s.W > 1024)
+            return fail(OCL_E_UNSUPPORTED, "batch: fused NESTA supports H <= 512 and W <= 1024");
+    }
+    ocl_batch* b = new ocl_batch();
+    b->ctx = c;
+    b->sl.assign(slices, slices + nslices);
+    b->prm = *prm;
+    *out = b;
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_destroy(ocl_batch* b) {
+    if (!b) return OCL_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_outs) hipFree(b->d_outs);
+    delete b;
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_setup(ocl_batch* b) {
+    if (!b) return fail(OCL_E_INVALID, "null batch");
+    ocl_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if (b->res.empty()) {
+        long total_cols = 0;
+        for (const auto& s : b->sl) total_cols += (long)s.F * s.W;
+        for (const auto& s : b->sl) {
+            b->res.emplace_back(new SliceRes());
+            CHK(slice_plan(c, s, b->prm, total_cols, b->res.back().get()));
+            CHK(slice_alloc(b->res.back().get()));
+            b->view.push_back(b->res.back().get());
+        }
+        CHK(table_build(c, &b->t, b->view));
+        b->h_st.resize(b->view.size());
+        std::vector<float2*> outs;
+        for (SliceRes* r : b->view) outs.push_back((float2*)r->in.x_out);
+        HIPCHK(hipMalloc((void**)&b->d_outs, sizeof(float2*) * outs.size()));
+        HIPCHK(hipMemcpy(b->d_outs, outs.data(), sizeof(float2*) * outs.size(), hipMemcpyHostToDevice));
+    }
+    CHK(slices_setup(c, b->t, b->view));
+    b->ready = true;
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_iterate(ocl_batch* b, int n_iters) {
+    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
+    if (n_iters < 0) return fail(OCL_E_INVALID, "batch: negative iteration count");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    return table_iterate(b->ctx, b->t, n_iters);
+}
+
+extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total) {
+    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
+    if (n_iters < 1) return fail(OCL_E_INVALID, "batch_time: need at least one iteration");
+    ocl_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<hipEvent_t> ev(3 * (size_t)n_iters);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (int it = 0; it < n_iters; ++it) {
+        HIPCHK(hipEventRecord(ev[3 * it], c->stream));
+        HIPCHK(launch_nesta_B(b->t.L, c->stream, 0));
+        HIPCHK(launch_nesta_S(b->t.L, c->stream));
+        HIPCHK(hipEventRecord(ev[3 * it + 1], c->stream));
+        HIPCHK(launch_nesta_CA(b->t.L, c->stream, 0));
+        HIPCHK(hipEventRecord(ev[3 * it + 2], c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tb = 0, tc = 0;
+    for (int it = 0; it < n_iters; ++it) {
+        float a = 0, q = 0;
+        HIPCHK(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
+        HIPCHK(hipEventElapsedTime(&q, ev[3 * it + 1], ev[3 * it + 2]));
+        tb += a;
+        tc += q;
+    }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)n_iters - 1]));
+    for (auto& e : ev) hipEventDestroy(e);
+    if (ms_B) *ms_B = tb / n_iters;
+    if (ms_CA) *ms_CA = tc / n_iters;
+    if (ms_total) *ms_total = tot;
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_capture(ocl_batch* b, int n_iters) {
+    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
+    if (n_iters < 1) return fail(OCL_E_INVALID, "batch: capture needs at least one iteration");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    HIPCHK(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = table_iterate(b->ctx, b->t, n_iters);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
+    if (rc != OCL_OK) { if (g) hipGraphDestroy(g); return rc; }
+    HIPCHK(e);
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_replay(ocl_batch* b) {
+    if (!b || !b->gexec) return fail(OCL_E_INVALID, "batch: no captured graph");
+    HIPCHK(hipGraphLaunch(b->gexec, b->ctx->stream));
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_active(ocl_batch* b, int* n_active) {
+    if (!b || !b->ready || !n_active) return fail(OCL_E_INVALID, "batch: not set up");
+    CHK(pull_states(b->ctx, b->view, b->h_st.data()));
+    int n = 0;
+    for (const auto& s : b->h_st) n += s.active ? 1 : 0;
+    *n_active = n;
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_finalize(ocl_batch* b) {
+    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
+    const int ns = (int)b->view.size();
+    hipLaunchKernelGGL(k_finalize, dim3(256, ns), dim3(OCL_NT), 0, b->ctx->stream, (const SliceDesc*)b->t.dd, ns,
+                       (float2* const*)b->d_outs);
+    HIPCHK(hipGetLastError());
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_report(ocl_batch* b, int slice, ocl_nesta_report* rep) {
+    if (!b || !b->ready || !rep || slice < 0 || slice >= (int)b->view.size())
+        return fail(OCL_E_INVALID, "batch_report: bad arguments");
+    CHK(pull_states(b->ctx, b->view, b->h_st.data()));
+    const NState& S = b->h_st[slice];
+    std::vector<double> tr(3 * (size_t)std::max(1, std::min(S.ntrace, OCL_TRACE)));
+    HIPCHK(hipMemcpy(tr.data(), b->view[slice]->d.trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
+    fill_report(*b->view[slice], S, tr.data(), rep);
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_run(ocl_batch* b, int chunk) {
+    if (!b) return fail(OCL_E_INVALID, "null batch");
+    CHK(ocl_batch_setup(b));
+    const long max_total = (long)std::max(0, b->prm.maxIntIter) * std::max(0, b->prm.maxIter);
+    if (b->prm.stoptest != 1) {
+        CHK(table_iterate(b->ctx, b->t, (int)max_total));
+    } else {
+        if (chunk < 1) chunk = 16;
+        long done = 0;
+        int n_active = 1;
+        while (done < max_total && n_active > 0) {
+            const int step = (int)std::min<long>(chunk, max_total - done);
+            CHK(table_iterate(b->ctx, b->t, step));
+            done += step;
+            CHK(ocl_batch_active(b, &n_active));
+        }
+    }
+    CHK(ocl_batch_finalize(b));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return OCL_OK;
+}
+
+// =====================================================================================================================
+// server: continuous batching with admission by HBM budget
+// =====================================================================================================================
+namespace {
+struct ServeReq {
+    long id = 0;
+    ocl_slice s{};
+    ocl_nesta_params prm{};
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int status = OCL_SERVE_QUEUED;
+    std::unique_ptr<SliceRes> res;
+    size_t footprint = 0;
+    std::string err;
+    ocl_nesta_report* rep = nullptr;   // optional, filled at completion
+};
+}  // namespace
+
+struct ocl_server {
+    ocl_ctx* c = nullptr;                    // the server's own context (stream) on the device
+    size_t budget = 0;
+    int chunk = 8;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::thread th;
+    bool stop = false;
+    std::deque<ServeReq*> queue;
+    std::map<long, std::unique_ptr<ServeReq>> reqs;
+    long next_id = 1;
+    size_t used = 0, peak_used = 0;
+    long completed = 0, failed = 0, iterations = 0, launches = 0, n_active = 0;
+    double busy_s = 0;
+    std::chrono::steady_clock::time_point t0;
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+// Marks every request that has not completed as failed (device error in the worker).
+static void server_fail_all(ocl_server* sv, std::vector<ServeReq*>& active) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    const std::string msg = std::string("server: device error: ") + ocl_last_error();
+    for (ServeReq* r : active) {
+        r->status = OCL_E_HIP;
+        r->err = msg;
+        r->res.reset();
+        sv->failed++;
+    }
+    for (ServeReq* r : sv->queue) {
+        r->status = OCL_E_HIP;
+        r->err = msg;
+        r->res.reset();
+        sv->failed++;
+    }
+    sv->queue.clear();
+    active.clear();
+    sv->used = 0;
+    sv->done_cv.notify_all();
+}
+
+static void server_loop(ocl_server* sv) {
+    ocl_ctx* c = sv->c;
+    hipSetDevice(c->device);
+    Table table;
+    std::vector<ServeReq*> active;
+    std::vector<SliceRes*> view;
+    std::vector<NState> h;
+    bool dirty = false;
+    for (;;) {
+        // ---- admission: first come first served while the footprint fits the budget ----
+        std::vector<ServeReq*> admitted;
+        for (;;) {
+            ServeReq* r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                if (sv->queue.empty()) break;
+                r = sv->queue.front();
+            }
+            if (!r->res) {
+                r->res.reset(new SliceRes());
+                int rc = slice_plan(c, r->s, r->prm, (long)r->s.F * r->s.W, r->res.get());
+                if (rc == OCL_OK) r->footprint = slice_footprint(*r->res);
+                if (rc == OCL_OK && r->footprint > sv->budget) {
+                    rc = OCL_E_NOMEM;
+                    fail(rc, "server: request exceeds the whole HBM budget");
+                }
+                if (rc != OCL_OK) {
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    r->err = ocl_last_error();
+                    r->status = rc;
+                    r->res.reset();
+                    sv->queue.pop_front();
+                    sv->failed++;
+                    sv->done_cv.notify_all();
+                    continue;
+                }
+            }
+            if (sv->used + r->footprint > sv->budget) break;   // wait for running requests to free memory
+            int rc = slice_alloc(r->res.get());
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->queue.pop_front();
+            if (rc != OCL_OK) {
+                r->err = ocl_last_error();
+                r->status = rc;
+                r->res.reset();
+                sv->failed++;
+                sv->done_cv.notify_all();
+                continue;
+            }
+            sv->used += r->footprint;
+            sv->peak_used = std::max(sv->peak_used, sv->used);
+            r->t_admit = sv->now();
+            r->status = OCL_SERVE_RUNNING;
+            admitted.push_back(r);
+        }
+        if (!admitted.empty()) {
+            Table tnew;
+            std::vector<SliceRes*> nv;
+            for (ServeReq* r : admitted) nv.push_back(r->res.get());
+            int rc = table_build(c, &tnew, nv);
+            if (rc == OCL_OK) rc = slices_setup(c, tnew, nv);
+            if (rc != OCL_OK) {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                for (ServeReq* r : admitted) {
+                    r->err = ocl_last_error();
+                    r->status = rc;
+                    sv->used -= r->footprint;
+                    r->res.reset();
+                    sv->failed++;
+                }
+                sv->done_cv.notify_all();
+            } else {
+                for (ServeReq* r : admitted) active.push_back(r);
+                dirty = true;
+            }
+        }
+        if (active.empty()) {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            if (sv->stop && sv->queue.empty()) break;
+            sv->cv.wait_for(lk, std::chrono::milliseconds(1));
+            continue;
+        }
+        // ---- advance every running request by one chunk of iterations ----
+        if (dirty) {
+            view.clear();
+            for (ServeReq* r : active) view.push_back(r->res.get());
+            if (table_build(c, &table, view) != OCL_OK) { server_fail_all(sv, active); break; }
+            h.resize(view.size());
+            dirty = false;
+        }
+        const double tb = sv->now();
+        if (table_iterate(c, table, sv->chunk) != OCL_OK || pull_states(c, view, h.data()) != OCL_OK) {
+            server_fail_all(sv, active);
+            break;
+        }
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->busy_s += sv->now() - tb;
+            sv->iterations += (long)sv->chunk * (long)view.size();
+            sv->launches += 3L * sv->chunk;
+            sv->n_active = (long)view.size();
+        }
+        // ---- completions ----
+        std::vector<ServeReq*> still;
+        for (size_t i = 0; i < active.size(); ++i) {
+            ServeReq* r = active[i];
+            if (h[i].active) {
+                still.push_back(r);
+                continue;
+            }
+            slice_finalize(c, r->res.get(), h[i]);
+            if (r->rep) {
+                std::vector<double> tr(3 * (size_t)std::max(1, std::min(h[i].ntrace, OCL_TRACE)));
+                hipMemcpy(tr.data(), r->res->d.trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost);
+                fill_report(*r->res, h[i], tr.data(), r->rep);
+            }
+            hipStreamSynchronize(c->stream);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            r->res.reset();
+            sv->used -= r->footprint;
+            r->t_done = sv->now();
+            r->status = OCL_SERVE_DONE;
+            sv->completed++;
+            dirty = true;
+        }
+        if (still.size() != active.size()) {
+            active.swap(still);
+            sv->done_cv.notify_all();
+        }
+    }
+}
+
+extern "C" int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_iters, ocl_server** out) {
+    if (!out || hbm_budget_bytes == 0) return fail(OCL_E_INVALID, "server: bad arguments");
+    ocl_server* sv = new ocl_server();
+    int rc = ocl_ctx_create(device, &sv->c);
+    if (rc != OCL_OK) { delete sv; return rc; }
+    sv->budget = hbm_budget_bytes;
+    sv->chunk = chunk_iters > 0 ? chunk_iters : 8;
+    sv->t0 = std::chrono::steady_clock::now();
+    sv->th = std::thread(server_loop, sv);
+    *out = sv;
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_submit(ocl_server* sv, const ocl_slice* s, const ocl_nesta_params* prm,
+                                 ocl_nesta_report* report, long* request_id) {
+    if (!sv || !s || !prm || !request_id) return fail(OCL_E_INVALID, "server_submit: bad arguments");
+    if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER) return fail(OCL_E_INVALID, "NESTA: miniter out of range");
+    std::unique_ptr<ServeReq> r(new ServeReq());
+    r->s = *s;
+    r->prm = *prm;
+    r->rep = report;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) return fail(OCL_E_INVALID, "server_submit: server is stopping");
+    r->id = sv->next_id++;
+    r->t_submit = sv->now();
+    *request_id = r->id;
+    sv->queue.push_back(r.get());
+    sv->reqs[r->id] = std::move(r);
+    sv->cv.notify_all();
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_poll(ocl_server* sv, long id, int* status, double* latency_s) {
+    if (!sv || !status) return fail(OCL_E_INVALID, "server_poll: bad arguments");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(OCL_E_INVALID, "server_poll: unknown request");
+    *status = it->second->status;
+    if (latency_s) *latency_s = it->second->status == OCL_SERVE_DONE ? it->second->t_done - it->second->t_submit : -1;
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_wait(ocl_server* sv, long id, double timeout_s, int* status) {
+    if (!sv || !status) return fail(OCL_E_INVALID, "server_wait: bad arguments");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(OCL_E_INVALID, "server_wait: unknown request");
+    ServeReq* r = it->second.get();
+    sv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                         [&] { return r->status == OCL_SERVE_DONE || r->status < 0; });
+    *status = r->status;
+    if (r->status < 0) return fail(r->status, r->err);
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_request_times(ocl_server* sv, long id, double* t_submit, double* t_admit, double* t_done) {
+    if (!sv) return fail(OCL_E_INVALID, "server: null");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(OCL_E_INVALID, "server: unknown request");
+    if (t_submit) *t_submit = it->second->t_submit;
+    if (t_admit) *t_admit = it->second->t_admit;
+    if (t_done) *t_done = it->second->t_done;
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_get_stats(ocl_server* sv, ocl_server_stats* st) {
+    if (!sv || !st) return fail(OCL_E_INVALID, "server_stats: bad arguments");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    st->completed = sv->completed;
+    st->failed = sv->failed;
+    st->queued = (long)sv->queue.size();
+    st->active = sv->n_active;
+    st->used_bytes = sv->used;
+    st->peak_used_bytes = sv->peak_used;
+    st->budget_bytes = sv->budget;
+    st->slice_iterations = sv->iterations;
+    st->launches = sv->launches;
+    st->busy_s = sv->busy_s;
+    st->elapsed_s = sv->now();
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_destroy(ocl_server* sv) {
+    if (!sv) return OCL_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        sv->cv.notify_all();
+    }
+    if (sv->th.joinable()) sv->th.join();
+    sv->reqs.clear();
+    ocl_ctx_destroy(sv->c);
+    delete sv;
+    return OCL_OK;
+}

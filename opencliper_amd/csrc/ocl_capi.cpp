@@ -16,51 +16,16 @@
 #include <string>
 #include <vector>
 
-#include "ocl_amd.h"
-#include "ocl_kernels.hpp"
-#include "ocl_nesta_fast.hpp"
+#include "ocl_internal.hpp"
 
-// ---------------------------------------------------------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
 
-#define HIPCHK(x)                                                                                   \
-    do {                                                                                            \
-        hipError_t e_ = (x);                                                                        \
-        if (e_ != hipSuccess) return fail(OCL_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define CHK(x)                  \
-    do {                        \
-        int r_ = (x);           \
-        if (r_ != OCL_OK) return r_; \
-    } while (0)
-
 extern "C" const char* ocl_last_error(void) { return g_err.c_str(); }
-
-// ---------------------------------------------------------------------------------------------------------------------
-// context
-// ---------------------------------------------------------------------------------------------------------------------
-struct ocl_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::map<int, float2*> tw;                              // twiddle tables by DFT length
-    double* red_part = nullptr;                             // reduction partials [1024][2]
-    double* red_out = nullptr;                              // reduction results [8][2]
-    double* red_host = nullptr;                             // pinned mirror of red_out
-    std::map<std::pair<int, long>, std::pair<double, int>> normU_cache;
-    std::mutex mu;
-    int num_cus = 256;
-};
-
-static const int RED_PARTS = 1024;
-static const size_t LDS_MAX = 64 * 1024;   // per-workgroup dynamic LDS budget of every launch
 
 extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (!out) return fail(OCL_E_INVALID, "ocl_ctx_create: null output pointer");
@@ -197,7 +162,7 @@ extern "C" int ocl_memcpy_d2d(ocl_ctx* c, void* dst, const void* src, size_t byt
     return OCL_OK;
 }
 
-static int grid_for(long n) { return (int)std::max(1L, std::min((n + OCL_NT - 1) / OCL_NT, 8192L)); }
+int grid_for(long n) { return (int)std::max(1L, std::min((n + OCL_NT - 1) / OCL_NT, 8192L)); }
 
 extern "C" int ocl_memset_zero(ocl_ctx* c, void* p, size_t bytes) {
     if (!c) return fail(OCL_E_INVALID, "null context");
@@ -217,7 +182,7 @@ extern "C" int ocl_memset_zero(ocl_ctx* c, void* p, size_t bytes) {
 // ---------------------------------------------------------------------------------------------------------------------
 // FFT plans
 // ---------------------------------------------------------------------------------------------------------------------
-static int make_plan(ocl_ctx* c, int n, FftPlan* P) {
+int make_plan(ocl_ctx* c, int n, FftPlan* P) {
     if (n < 1) return fail(OCL_E_INVALID, "FFT length must be positive");
     std::vector<int> r;
     int m = n, p2 = 0;
@@ -253,13 +218,13 @@ static int make_plan(ocl_ctx* c, int n, FftPlan* P) {
 }
 
 // P when n = 16 P takes the register four-step path (P even, 2..20), else 0
-static int fast_p(int n) {
+int fast_p(int n) {
     if (n % 16) return 0;
     const int P = n / 16;
     return (P >= 2 && P <= 20 && P % 2 == 0) ? P : 0;
 }
 
-static int pow2_floor(int v) {
+int pow2_floor(int v) {
     int p = 1;
     while (p * 2 <= v) p *= 2;
     return p;
@@ -339,6 +304,16 @@ extern "C" int ocl_copy(ocl_ctx* c, const void* in, void* out, long n) {
     return OCL_OK;
 }
 
+extern "C" int ocl_add_noise_masked(ocl_ctx* c, void* y, const uint8_t* M, int F, int C, long N, float sigma,
+                                    unsigned long long seed) {
+    CTXCHK(c);
+    if (!y || !M || F < 1 || C < 1 || N < 1) return fail(OCL_E_INVALID, "add_noise_masked: bad arguments");
+    hipLaunchKernelGGL(k_add_noise_masked, dim3(grid_for((long)F * C * N)), dim3(OCL_NT), 0, c->stream, (float2*)y, M,
+                       F, C, N, sigma, seed);
+    HIPCHK(hipGetLastError());
+    return OCL_OK;
+}
+
 extern "C" int ocl_scal(ocl_ctx* c, void* x, long n, float ar, float ai) {
     CTXCHK(c);
     if (n < 0 || (n && !x)) return fail(OCL_E_INVALID, "scal: bad arguments");
@@ -359,7 +334,7 @@ extern "C" int ocl_axpy(ocl_ctx* c, long n, float ar, float ai, const void* x, v
 }
 
 // enqueue a reduction whose two doubles land in dev_out (device)
-static int reduce_async(ocl_ctx* c, const void* x, const void* y, long n, int op, double* dev_out) {
+int reduce_async(ocl_ctx* c, const void* x, const void* y, long n, int op, double* dev_out) {
     if (op < 0 || op > RED_MAXABS) return fail(OCL_E_INVALID, "reduce: unknown operation");
     if (n < 1 || !x || (op == RED_DOTC && !y)) return fail(OCL_E_INVALID, "reduce: bad arguments");
     const int parts = (int)std::max(1L, std::min((long)RED_PARTS, (n + OCL_NT - 1) / OCL_NT));
@@ -678,435 +653,6 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     }
     if (rep) { rep->final_tolVar = tolVar; rep->final_mu = mu; }
     HIPCHK(hipStreamSynchronize(c->stream));
-    return OCL_OK;
-}
-
-// ---------------------------------------------------------------------------------------------------------------------
-// batched / fused NESTA
-// ---------------------------------------------------------------------------------------------------------------------
-struct SliceSetup {
-    double maxX, maxUX, normU, lambda, muf, mu0, gamma, gammat;
-    int ne_iters;
-};
-
-struct ocl_batch {
-    ocl_ctx* ctx = nullptr;
-    std::vector<ocl_slice> sl;
-    ocl_nesta_params prm{};
-    std::vector<SliceDesc> hd;
-    std::vector<SliceSetup> setup;
-    std::vector<int> offB, offCA;
-    int totB = 0, totCA = 0;
-    size_t ldsB = 0, ldsCA = 0;
-    int maxe = 4;
-    NestaLaunch L{};
-    SliceDesc* dd = nullptr;
-    int* d_offB = nullptr;
-    int* d_offCA = nullptr;
-    float2** d_outs = nullptr;
-    NState* h_st = nullptr;            // pinned mirror of the states
-    std::vector<void*> allocs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_iters = 0;
-    bool ready = false;
-    bool allocated = false;
-};
-
-static int batch_alloc(ocl_batch* b, size_t bytes, void** p) {
-    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-    if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "batch: out of device memory");
-    HIPCHK(e);
-    b->allocs.push_back(*p);
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_destroy(ocl_batch* b) {
-    if (!b) return OCL_OK;
-    hipSetDevice(b->ctx->device);
-    hipStreamSynchronize(b->ctx->stream);
-    if (b->gexec) hipGraphExecDestroy(b->gexec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    for (void* p : b->allocs) hipFree(p);
-    if (b->h_st) hipHostFree(b->h_st);
-    delete b;
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_create(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
-                                ocl_batch** out) {
-    CTXCHK(c);
-    if (nslices < 1 || !slices || !prm || !out) return fail(OCL_E_INVALID, "batch: bad arguments");
-    if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER)
-        return fail(OCL_E_INVALID, "NESTA: miniter must be in [1, 32]");
-    if (prm->maxIntIter < 0 || prm->maxIter < 0) return fail(OCL_E_INVALID, "NESTA: negative iteration counts");
-    for (int i = 0; i < nslices; ++i) {
-        const ocl_slice& s = slices[i];
-        if (s.F < 1 || s.C < 1 || s.H < 1 || s.W < 1 || !s.y || !s.S || !s.M || !s.x_out)
-            return fail(OCL_E_INVALID, "batch: slice " + std::to_string(i) + " has bad dimensions or null pointers");
-        if (s.H > 512 || s.W > 1024)
-            return fail(OCL_E_UNSUPPORTED, "batch: fused NESTA supports H <= 512 and W <= 1024");
-    }
-    ocl_batch* b = new ocl_batch();
-    b->ctx = c;
-    b->sl.assign(slices, slices + nslices);
-    b->prm = *prm;
-    *out = b;
-    return OCL_OK;
-}
-
-static int choose_tiles(const ocl_batch* b, const ocl_slice& s, int* tile_cols, int* rows_per_wg) {
-    long total_cols = 0;
-    for (const auto& t : b->sl) total_cols += (long)t.F * t.W;
-    int tc = (total_cols / 16 >= 2048) ? 16 : 8;
-    while (tc > 1 && ((long)s.H * tc > (long)OCL_MAXE * OCL_NT ||
-                      sizeof(float2) * ((size_t)2 * s.H * (tc + 1) + s.H) > LDS_MAX))
-        tc >>= 1;
-    *tile_cols = tc;
-    *rows_per_wg = std::min(16, pow2_floor(std::max(1, 2048 / s.W)));
-    return OCL_OK;
-}
-
-static int batch_compute_setup(ocl_batch* b);
-
-extern "C" int ocl_batch_setup(ocl_batch* b) {
-    if (!b) return fail(OCL_E_INVALID, "null batch");
-    if (b->allocated) return batch_compute_setup(b);
-    ocl_ctx* c = b->ctx;
-    HIPCHK(hipSetDevice(c->device));
-    const int ns = (int)b->sl.size();
-    b->hd.assign(ns, SliceDesc{});
-    b->setup.assign(ns, SliceSetup{});
-    b->offB.assign(ns + 1, 0);
-    b->offCA.assign(ns + 1, 0);
-    b->maxe = 4;
-    b->ldsB = b->ldsCA = 0;
-    uint8_t* flags_d = nullptr;
-    size_t flags_cap = 0;
-    std::vector<uint8_t> flags;
-    for (int i = 0; i < ns; ++i) {
-        const ocl_slice& s = b->sl[i];
-        SliceDesc& d = b->hd[i];
-        d.F = s.F; d.C = s.C; d.H = s.H; d.W = s.W;
-        d.N = (long)s.H * s.W;
-        CHK(make_plan(c, s.H, &d.ph));
-        CHK(make_plan(c, s.W, &d.pw));
-        d.fastH = fast_p(s.H);
-        d.fastW = fast_p(s.W);
-        CHK(choose_tiles(b, s, &d.tile_cols, &d.rows_per_wg));
-        if (d.fastH) d.tile_cols = 16;
-        if (d.fastW) d.rows_per_wg = 16;
-        // sampled-row lists
-        const size_t fb = (size_t)s.F * s.H;
-        if (fb > flags_cap) {
-            if (flags_d) hipFree(flags_d);
-            HIPCHK(hipMalloc(&flags_d, fb));
-            flags_cap = fb;
-        }
-        hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F,
-                           s.C, s.H, s.W, flags_d);
-        HIPCHK(hipGetLastError());
-        flags.resize(fb);
-        HIPCHK(hipMemcpyAsync(flags.data(), flags_d, fb, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        std::vector<int> nrows(s.F, 0);
-        int nmax = 1;
-        for (int f = 0; f < s.F; ++f) {
-            for (int h = 0; h < s.H; ++h) nrows[f] += flags[(size_t)f * s.H + h];
-            nmax = std::max(nmax, nrows[f]);
-        }
-        std::vector<int16_t> rows((size_t)s.F * nmax, 0), rowpos((size_t)s.F * s.H, -1);
-        for (int f = 0; f < s.F; ++f) {
-            int j = 0;
-            for (int h = 0; h < s.H; ++h)
-                if (flags[(size_t)f * s.H + h]) {
-                    rowpos[(size_t)f * s.H + h] = (int16_t)j;
-                    rows[(size_t)f * nmax + j++] = (int16_t)h;
-                }
-        }
-        d.nmax = nmax;
-        d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
-        d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
-        d.wg_B = s.F * s.C * d.nrow_groups;
-        d.wg_CA = s.F * d.ncol_tiles;
-        d.u_chunk = (int)std::min<long>(d.N, 4096);
-        d.wg_U = s.F * (int)((d.N + d.u_chunk - 1) / d.u_chunk);
-        d.inv_sqrtN = (float)(1.0 / std::sqrt((double)d.N));
-        d.y = (const float2*)s.y;
-        d.S = (const float2*)s.S;
-        d.M = s.M;
-        const size_t xb = sizeof(float2) * (size_t)d.F * d.N;
-        void* p;
-        CHK(batch_alloc(b, xb, &p)); d.x0 = (float2*)p;
-        CHK(batch_alloc(b, xb, &p)); d.x1 = (float2*)p;
-        CHK(batch_alloc(b, xb, &p)); d.xref = (float2*)p;
-        CHK(batch_alloc(b, xb, &p)); d.Wk = (float2*)p;
-        CHK(batch_alloc(b, sizeof(float2) * (size_t)d.F * d.C * nmax * d.W, &p)); d.kbuf = (float2*)p;
-        CHK(batch_alloc(b, sizeof(int16_t) * rows.size(), &p)); d.rows = (const int16_t*)p;
-        HIPCHK(hipMemcpyAsync(p, rows.data(), sizeof(int16_t) * rows.size(), hipMemcpyHostToDevice, c->stream));
-        CHK(batch_alloc(b, sizeof(int16_t) * rowpos.size(), &p)); d.rowpos = (const int16_t*)p;
-        HIPCHK(hipMemcpyAsync(p, rowpos.data(), sizeof(int16_t) * rowpos.size(), hipMemcpyHostToDevice, c->stream));
-        CHK(batch_alloc(b, sizeof(int) * s.F, &p)); d.nrows = (const int*)p;
-        HIPCHK(hipMemcpyAsync(p, nrows.data(), sizeof(int) * s.F, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));   // rows / nrows are host temporaries
-        CHK(batch_alloc(b, sizeof(NState), &p)); d.st = (NState*)p;
-        CHK(batch_alloc(b, sizeof(double) * d.wg_B, &p)); d.partB = (double*)p;
-        CHK(batch_alloc(b, sizeof(double) * 2 * d.wg_U, &p)); d.partU = (double*)p;
-        CHK(batch_alloc(b, sizeof(unsigned) * 4, &p)); d.counter = (unsigned*)p;
-        CHK(ocl_memset_zero(c, p, sizeof(unsigned) * 4));
-        CHK(batch_alloc(b, sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
-        CHK(batch_alloc(b, sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
-        b->offB[i + 1] = b->offB[i] + d.wg_B + d.wg_U;
-        b->offCA[i + 1] = b->offCA[i] + d.wg_CA;
-        b->ldsB = std::max(b->ldsB, d.fastW ? sizeof(float2) * fast_b_lds(d.fastW)
-                                            : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W));
-        b->ldsCA = std::max(b->ldsCA, d.fastH ? sizeof(float2) * fast_ca_lds(d.fastH)
-                                              : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H));
-        if (!d.fastH) {
-            const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
-            while (b->maxe < need) b->maxe *= 2;
-        }
-    }
-    if (flags_d) {
-        hipStreamSynchronize(c->stream);
-        hipFree(flags_d);
-    }
-    if (b->ldsB > LDS_MAX || b->ldsCA > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "batch: LDS tile too large");
-    b->totB = b->offB[ns];
-    b->totCA = b->offCA[ns];
-    void* p;
-    CHK(batch_alloc(b, sizeof(SliceDesc) * ns, &p)); b->dd = (SliceDesc*)p;
-    CHK(batch_alloc(b, sizeof(int) * (ns + 1), &p)); b->d_offB = (int*)p;
-    CHK(batch_alloc(b, sizeof(int) * (ns + 1), &p)); b->d_offCA = (int*)p;
-    CHK(batch_alloc(b, sizeof(float2*) * ns, &p)); b->d_outs = (float2**)p;
-    std::vector<float2*> outs(ns);
-    for (int i = 0; i < ns; ++i) outs[i] = (float2*)b->sl[i].x_out;
-    HIPCHK(hipMemcpyAsync(b->dd, b->hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->d_offB, b->offB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->d_offCA, b->offCA.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->d_outs, outs.data(), sizeof(float2*) * ns, hipMemcpyHostToDevice, c->stream));
-    if (!b->h_st) HIPCHK(hipHostMalloc(&b->h_st, sizeof(NState) * ns, hipHostMallocDefault));
-    int PH = b->hd[0].fastH, PW = b->hd[0].fastW;
-    for (int i = 1; i < ns; ++i) {
-        if (b->hd[i].fastH != PH) PH = 0;
-        if (b->hd[i].fastW != PW) PW = 0;
-    }
-    b->L = NestaLaunch{b->dd, b->d_offB, b->d_offCA, ns, b->totB, b->totCA, b->ldsB, b->ldsCA, b->maxe, PH, PW};
-    HIPCHK(hipStreamSynchronize(c->stream));
-    b->allocated = true;
-    return batch_compute_setup(b);
-}
-
-// Per-reconstruction setup on already allocated buffers: E^H y, maxima, normU, initial state (NestaUp.cpp:442-649).
-// The sampled-row lists are those found when the batch was allocated.
-static int batch_compute_setup(ocl_batch* b) {
-    ocl_ctx* c = b->ctx;
-    HIPCHK(hipSetDevice(c->device));
-    const int ns = (int)b->sl.size();
-    for (int i = 0; i < ns; ++i) CHK(ocl_memset_zero(c, b->hd[i].counter, sizeof(unsigned) * 4));
-
-    // x_ref = A^H y (NestaUp.cpp:468-486) and the forward columns of iteration 0
-    HIPCHK(launch_nesta_B(b->L, c->stream, 1));
-    HIPCHK(launch_nesta_CA(b->L, c->stream, 1));
-    hipLaunchKernelGGL(k_setup_max, dim3(b->totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)b->dd,
-                       (const int*)b->d_offCA, ns, (double*)nullptr);
-    HIPCHK(hipGetLastError());
-    std::vector<double> mp;
-    std::vector<NState> st(ns);
-    for (int i = 0; i < ns; ++i) {
-        const SliceDesc& d = b->hd[i];
-        mp.resize(2 * d.wg_CA);
-        HIPCHK(hipMemcpyAsync(mp.data(), d.maxpart, sizeof(double) * 2 * d.wg_CA, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        double m0 = (double)LONG_MIN, m1 = (double)LONG_MIN;
-        for (int q = 0; q < d.wg_CA; ++q) { m0 = std::max(m0, mp[2 * q]); m1 = std::max(m1, mp[2 * q + 1]); }
-        SliceSetup& u = b->setup[i];
-        u.maxX = (float)m0;
-        u.maxUX = (float)m1;
-        CHK(ocl_normest(c, d.F, d.N, &u.normU, &u.ne_iters));
-        // NestaUp.cpp:570-583 (scalars kept in double)
-        const ocl_nesta_params& P = b->prm;
-        u.lambda = P.lambda_i * u.maxX;
-        u.muf = P.mu_f * u.maxX;
-        u.mu0 = 0.9 * u.maxUX;
-        const double muL = u.lambda / P.La;
-        if (muL > u.mu0) u.mu0 = muL;
-        u.gamma = std::pow(u.muf / u.mu0, 1.0 / std::max(1, P.maxIntIter));
-        u.gammat = std::pow(P.tolVar / 0.1, 1.0 / std::max(1, P.maxIntIter));
-        NState& S = st[i];
-        std::memset(&S, 0, sizeof(S));
-        S.lambda = u.lambda; S.La = P.La; S.normU = u.normU; S.gamma = u.gamma; S.gammat = u.gammat;
-        S.maxIntIter = P.maxIntIter; S.maxIter = P.maxIter; S.stoptest = P.stoptest; S.miniter = P.miniter;
-        S.active = (P.maxIntIter >= 1 && P.maxIter >= 1) ? 1 : 0;
-        S.stage = 1;
-        S.k = 0;
-        S.stage_start = 1;
-        S.cur = 0;
-        S.qpDone = 0;
-        S.trace_cap = OCL_TRACE;
-        S.mu = u.mu0 * u.gamma;
-        S.tolVar = 0.1 * u.gammat;
-        S.Lmu = u.lambda * (u.normU / S.mu) + P.La;
-        S.Lmu1 = 1.0 / S.Lmu;
-        S.fmean[P.miniter - 1] = 9223372036854775807.0;
-        HIPCHK(hipMemcpyAsync(d.st, &S, sizeof(NState), hipMemcpyHostToDevice, c->stream));
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    b->ready = true;
-    return OCL_OK;
-}
-
-static int batch_enqueue(ocl_batch* b, int n_iters) {
-    ocl_ctx* c = b->ctx;
-    const int ns = (int)b->sl.size();
-    for (int it = 0; it < n_iters; ++it) {
-        HIPCHK(launch_nesta_B(b->L, c->stream, 0));
-        HIPCHK(launch_nesta_S(b->L, c->stream));
-        HIPCHK(launch_nesta_CA(b->L, c->stream, 0));
-    }
-    HIPCHK(hipGetLastError());
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total) {
-    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
-    if (n_iters < 1) return fail(OCL_E_INVALID, "batch_time: need at least one iteration");
-    ocl_ctx* c = b->ctx;
-    HIPCHK(hipSetDevice(c->device));
-    const int ns = (int)b->sl.size();
-    std::vector<hipEvent_t> ev(3 * (size_t)n_iters);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    for (int it = 0; it < n_iters; ++it) {
-        HIPCHK(hipEventRecord(ev[3 * it], c->stream));
-        HIPCHK(launch_nesta_B(b->L, c->stream, 0));
-        HIPCHK(launch_nesta_S(b->L, c->stream));
-        HIPCHK(hipEventRecord(ev[3 * it + 1], c->stream));
-        HIPCHK(launch_nesta_CA(b->L, c->stream, 0));
-        HIPCHK(hipEventRecord(ev[3 * it + 2], c->stream));
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
-    double tb = 0, tc = 0;
-    for (int it = 0; it < n_iters; ++it) {
-        float a = 0, q = 0;
-        HIPCHK(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
-        HIPCHK(hipEventElapsedTime(&q, ev[3 * it + 1], ev[3 * it + 2]));
-        tb += a;
-        tc += q;
-    }
-    float tot = 0;
-    HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)n_iters - 1]));
-    for (auto& e : ev) hipEventDestroy(e);
-    if (ms_B) *ms_B = tb / n_iters;
-    if (ms_CA) *ms_CA = tc / n_iters;
-    if (ms_total) *ms_total = tot;
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_iterate(ocl_batch* b, int n_iters) {
-    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
-    if (n_iters < 0) return fail(OCL_E_INVALID, "batch: negative iteration count");
-    HIPCHK(hipSetDevice(b->ctx->device));
-    return batch_enqueue(b, n_iters);
-}
-
-extern "C" int ocl_batch_capture(ocl_batch* b, int n_iters) {
-    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
-    if (n_iters < 1) return fail(OCL_E_INVALID, "batch: capture needs at least one iteration");
-    HIPCHK(hipSetDevice(b->ctx->device));
-    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
-    HIPCHK(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = batch_enqueue(b, n_iters);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
-    if (rc != OCL_OK) { if (g) hipGraphDestroy(g); return rc; }
-    HIPCHK(e);
-    b->graph = g;
-    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-    b->graph_iters = n_iters;
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_replay(ocl_batch* b) {
-    if (!b || !b->gexec) return fail(OCL_E_INVALID, "batch: no captured graph");
-    HIPCHK(hipGraphLaunch(b->gexec, b->ctx->stream));
-    return OCL_OK;
-}
-
-static int batch_pull_states(ocl_batch* b) {
-    const int ns = (int)b->sl.size();
-    for (int i = 0; i < ns; ++i)
-        HIPCHK(hipMemcpyAsync(&b->h_st[i], b->hd[i].st, sizeof(NState), hipMemcpyDeviceToHost, b->ctx->stream));
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_active(ocl_batch* b, int* n_active) {
-    if (!b || !b->ready || !n_active) return fail(OCL_E_INVALID, "batch: not set up");
-    CHK(batch_pull_states(b));
-    int n = 0;
-    for (size_t i = 0; i < b->sl.size(); ++i) n += b->h_st[i].active ? 1 : 0;
-    *n_active = n;
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_finalize(ocl_batch* b) {
-    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
-    const int ns = (int)b->sl.size();
-    hipLaunchKernelGGL(k_finalize, dim3(256, ns), dim3(OCL_NT), 0, b->ctx->stream, (const SliceDesc*)b->dd, ns,
-                       (float2* const*)b->d_outs);
-    HIPCHK(hipGetLastError());
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_report(ocl_batch* b, int slice, ocl_nesta_report* rep) {
-    if (!b || !b->ready || !rep || slice < 0 || slice >= (int)b->sl.size())
-        return fail(OCL_E_INVALID, "batch_report: bad arguments");
-    CHK(batch_pull_states(b));
-    const NState& S = b->h_st[slice];
-    const SliceSetup& u = b->setup[slice];
-    std::memset(rep, 0, sizeof(*rep));
-    rep->normU = u.normU; rep->maxXref = u.maxX; rep->maxUXref = u.maxUX; rep->lambda = u.lambda;
-    rep->muf = u.muf; rep->mu0 = u.mu0; rep->gamma = u.gamma; rep->gammat = u.gammat;
-    rep->normest_iters = u.ne_iters;
-    rep->final_tolVar = S.tolVar;
-    rep->final_mu = S.mu;
-    const int stages_done = S.stage - 1 + (S.active ? 1 : 0);
-    rep->stages = std::min(stages_done, OCL_MAX_STAGES);
-    for (int q = 0; q < rep->stages; ++q) rep->iters[q] = (q == S.stage - 1 && S.active) ? S.iters_stage : S.iters[q];
-    rep->ntrace = std::min(S.ntrace, OCL_TRACE);
-    std::vector<double> tr(3 * (size_t)rep->ntrace);
-    if (rep->ntrace) {
-        HIPCHK(hipMemcpy(tr.data(), b->hd[slice].trace, sizeof(double) * tr.size(), hipMemcpyDeviceToHost));
-        for (int q = 0; q < rep->ntrace; ++q) {
-            rep->trace_f[q] = tr[3 * q];
-            rep->trace_l2[q] = tr[3 * q + 1];
-            rep->trace_l1[q] = tr[3 * q + 2];
-        }
-    }
-    return OCL_OK;
-}
-
-extern "C" int ocl_batch_run(ocl_batch* b, int chunk) {
-    if (!b) return fail(OCL_E_INVALID, "null batch");
-    if (!b->ready) CHK(ocl_batch_setup(b));
-    const long max_total = (long)std::max(0, b->prm.maxIntIter) * std::max(0, b->prm.maxIter);
-    if (b->prm.stoptest != 1) {
-        CHK(batch_enqueue(b, (int)max_total));
-    } else {
-        if (chunk < 1) chunk = 16;
-        long done = 0;
-        int n_active = 1;
-        while (done < max_total && n_active > 0) {
-            const int step = (int)std::min<long>(chunk, max_total - done);
-            CHK(batch_enqueue(b, step));
-            done += step;
-            CHK(ocl_batch_active(b, &n_active));
-        }
-    }
-    CHK(ocl_batch_finalize(b));
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return OCL_OK;
 }
 

@@ -1,0 +1,55 @@
+// Declarations shared by the host-side translation units of libocl_amd.so (ocl_capi.cpp, ocl_batch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ocl_amd.h"
+#include "ocl_kernels.hpp"
+#include "ocl_nesta_fast.hpp"
+
+// Records msg as the calling thread's last error and returns code.
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return fail(OCL_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(x)                       \
+    do {                             \
+        int r_ = (x);                \
+        if (r_ != OCL_OK) return r_; \
+    } while (0)
+
+// One HIP device + one in-order stream (the role CLapp plays in src/CLapp.cpp:139-569 of the reference).
+struct ocl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, float2*> tw;                              // twiddle tables by DFT length
+    double* red_part = nullptr;                             // reduction partials [1024][2]
+    double* red_out = nullptr;                              // reduction results [8][2]
+    double* red_host = nullptr;                             // pinned mirror of red_out
+    std::map<std::pair<int, long>, std::pair<double, int>> normU_cache;
+    std::mutex mu;
+    int num_cus = 256;
+};
+
+static const int RED_PARTS = 1024;
+static const size_t LDS_MAX = 64 * 1024;   // per-workgroup dynamic LDS budget of every launch
+
+int grid_for(long n);
+int make_plan(ocl_ctx* c, int n, FftPlan* P);
+int fast_p(int n);
+int pow2_floor(int v);
+int reduce_async(ocl_ctx* c, const void* x, const void* y, long n, int op, double* dev_out);

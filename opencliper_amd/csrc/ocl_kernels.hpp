@@ -84,6 +84,8 @@ __global__ void k_fft_rows(const float2* in, float2* out, long nrows, FftPlan P,
 __global__ void k_fft_cols(const float2* in, float2* out, int H, int W, FftPlan P, int tw_cols, int conj_io,
                            float scale);
 __global__ void k_drand48_fill(float2* out, long n, unsigned long long x0, long start);
+__global__ void k_add_noise_masked(float2* y, const uint8_t* M, int F, int C, long N, float sigma,
+                                   unsigned long long seed);
 #define FAST_COLS_KS 272   // per-k1 stride (float2) of the column exchange in the fast FFT kernels
 __global__ void k_fft_rows_fast(const float2* in, float2* out, long nrows, int P, const float2* tw, int conj_io,
                                 float scale);

@@ -332,6 +332,30 @@ __global__ __launch_bounds__(OCL_NT) void k_fft_cols_fast(const float2* __restri
     }
 }
 
+// y = M * (y + sigma n), n complex Gaussian from splitmix64(seed ^ index) and the Box-Muller transform
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__global__ void k_add_noise_masked(float2* __restrict__ y, const uint8_t* __restrict__ M, int F, int C, long N,
+                                   float sigma, unsigned long long seed) {
+    const long total = (long)F * C * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long fc = i / N;
+        const long p = i - fc * N;
+        const int f = (int)(fc / C);
+        if (!M[(long)f * N + p]) { y[i] = make_float2(0.f, 0.f); continue; }
+        const unsigned long long r = mix64(seed ^ (unsigned long long)i);
+        const double u1 = ((double)(r >> 11) + 1.0) * (1.0 / 9007199254740993.0);   // (0, 1]
+        const double u2 = (double)(mix64(r) >> 11) * (1.0 / 9007199254740992.0);
+        const double rad = sqrt(-log(u1)) * (double)sigma;                             // variance sigma^2 / 2 per part
+        const float2 v = y[i];
+        y[i] = make_float2(v.x + (float)(rad * cos(6.283185307179586 * u2)), v.y + (float)(rad * sin(6.283185307179586 * u2)));
+    }
+}
+
 // drand48 stream from a given 48-bit state, written as (float(value), 0) pairs: out[i] = X_{start+i+1} / 2^48.
 // Each thread jumps to its chunk with the LCG's affine power, then steps sequentially.
 __global__ void k_drand48_fill(float2* __restrict__ out, long n, unsigned long long x0, long start) {
