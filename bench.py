@@ -1,16 +1,22 @@
 """NESTA-TV throughput on one MI355X (recons/s), beside the plain FP32 form and the CPU restatement.
 
-Workload (one "step"): a batch of B independent cine slices, each F=20 frames x C=8 coils x 256x256, variable-density
-ky-line mask at acceleration 6 with 16 centre lines, reconstructed with NESTA-TV for 5 continuation stages x 20
-inner iterations (stop test off, fixed work).  The batch is advanced by the fused batched engine (two launches per
-iteration, replayed from a hipGraph); `value` is whole-job recons/s with inputs resident in HBM.
+Headline workload (one "step"): a batch of B=16 independent cine slices, each F=20 frames x C=8 coils x 256x256,
+variable-density ky-line mask at acceleration 6 with 16 centre lines, reconstructed with NESTA-TV for 5 continuation
+stages x 20 inner iterations (stop test off, fixed work).  The batch is advanced by the fused batched engine (three
+launches per iteration, replayed from a hipGraph); `value` is whole-job recons/s with inputs resident in HBM.
 
-Also measured in the same run: the plain single-instance FP32 form (the reference's op sequence, host stop test),
-the fused single-instance form, per-kernel HIP-event timing of the fused iteration (roofline), the FP32 CPU
-restatement on the host cores (cpu_baseline, bounded sample), and the rel-L2 error of the GPU result against the
-FP64 CPU oracle and the FP32 CPU restatement on the full schedule.
+Other legs in the same run (each reported under its own key of the JSON line):
+  simple_recon  fully-sampled 8-coil 256x256 SimpleMRIRecon (inverse FFT per coil, conj(S) coil combine)
+  plain / fused_single   the plain single-instance FP32 form (reference op sequence, host stop test) and the
+                fused single-instance form on the headline slice
+  uneven_batch  64 slices of uneven size (H=W in 128..320 step 32, C in 4..32, F in 8..30) in one batched
+                launch table per iteration, captured in a hipGraph; k-space synthesised on the device
+  served        Poisson arrivals (50 req/s) mixing the sizes above, 5 stages x 10..30 iterations per request,
+                admitted by HBM budget into the continuous-batching server; p50/p99 latency and recons/s
+  cpu           the FP32 CPU restatement on the host cores (cpu_baseline, bounded sample) and rel-L2 of the GPU
+                results against the FP64 and FP32 CPU oracles on full schedules
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--batch 16] [--no-ref]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--batch 16] [--serve-seconds 20] [--legs ...] [--no-ref]
 """
 from __future__ import annotations
 
@@ -31,6 +37,9 @@ H = W = 256
 C, F, ACCEL = 8, 20, 6.0
 STAGES, ITERS = 5, 20
 HBM_PEAK_GBS = 8000.0
+UNEVEN_SLICES = 64
+SERVE_RATE = 50.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_nesta_batched_16x20x8x256.json")
 
 
 def b_alg(F, C, H, W):
@@ -40,64 +49,95 @@ def b_alg(F, C, H, W):
     return 8 * N * (F * C + C + 5 * F) + F * N
 
 
-def _gen(i):
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# host-side input generation (forked workers, before the GPU is touched)
+# ---------------------------------------------------------------------------------------------------------------------
+def _gen_cine(i):
     from opencliper_amd import synth
     sl = synth.make_cine(H=H, W=W, C=C, F=F, accel=ACCEL, slice_id=i)
     return sl.y, sl.S, sl.M
 
 
-def gen_slices(n, workers):
-    with mp.get_context("fork").Pool(max(1, min(workers, n))) as pool:
-        return pool.map(_gen, range(n))
+def _gen_parts(job):
+    from opencliper_amd import synth
+    i, (h, w, c, f) = job
+    return synth.make_image_parts(h, w, c, f, accel=ACCEL, slice_id=1000 + i)
+
+
+def gen_inputs(args, workers):
+    from opencliper_amd import synth
+    jobs3 = list(enumerate(synth.batch_sizes(UNEVEN_SLICES))) if ("uneven" in args.legs or "served" in args.legs) \
+        else []
+    if "served" in args.legs and "uneven" not in args.legs:
+        jobs3 = jobs3[:12]
+    with mp.get_context("fork").Pool(max(1, min(workers, args.batch + len(jobs3)))) as pool:
+        r2 = pool.map_async(_gen_cine, range(args.batch))
+        r3 = pool.map_async(_gen_parts, jobs3)
+        cine, parts = r2.get(), r3.get()
+    full = synth.make_fully_sampled(H=256, W=256, C=8, F=1, slice_id=5000) if "simple" in args.legs else None
+    return cine, parts, full
 
 
 def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def gpu_leg(args, slices):
-    from opencliper_amd import capi
-    ctx = capi.Context(0)
-    info = ctx.info()
-    prm = capi.nesta_params(maxIntIter=STAGES, maxIter=ITERS, stoptest=0)
-    out = {"device": info["name"]}
-    dev = [(ctx.to_device(y), ctx.to_device(S), ctx.to_device(M), ctx.empty((F, H, W))) for (y, S, M) in slices]
-    y0, S0, M0, x0 = dev[0]
+# ---------------------------------------------------------------------------------------------------------------------
+# GPU legs
+# ---------------------------------------------------------------------------------------------------------------------
+def leg_simple(ctx, full, reps=200):
+    """SimpleMRIRecon, 8 coils 256x256: device time of the recon per call (the recon works in place on y, so the
+    input is restored by a device copy outside the timed region before every call)."""
+    y0, S, x = ctx.to_device(full.y), ctx.to_device(full.S), ctx.empty((1, full.H, full.W))
+    work = ctx.empty(full.y.shape)
+    for _ in range(3):
+        ctx.copy(y0, work)
+        ctx.simple_mri_recon(work, S, x)
+    ms = []
+    for _ in range(reps):
+        ctx.copy(y0, work)
+        t = ctx.timer_begin()
+        ctx.simple_mri_recon(work, S, x)
+        ms.append(ctx.timer_end(t))
+    N = full.H * full.W
+    byts = 8 * N * (2 * full.C + 1)
+    mean = float(np.mean(ms))
+    return {"recons_per_s": 1e3 / mean, "ms_mean": mean, "ms_p50": pct(ms, 50), "ms_p99": pct(ms, 99),
+            "bytes_alg": byts, "achieved_GBs": byts / (mean * 1e-3) / 1e9,
+            "frac_hbm": byts / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "workload": "fully sampled, C=8, 256x256, one frame"}, x.get()
 
-    # plain single-instance FP32 form (reference op sequence, host stop test each iteration)
+
+def leg_single(ctx, capi, dev0, prm, form, steps):
+    y0, S0, M0, x0 = dev0
     lat = []
-    for r in range(1 + args.plain_steps):
+    for r in range(1 + steps):
         t = time.perf_counter()
-        ctx.nesta(y0, S0, M0, x0, prm, form=capi.OCL_FORM_PLAIN)
+        ctx.nesta(y0, S0, M0, x0, prm, form=form)
         if r:
             lat.append(time.perf_counter() - t)
-    out["plain"] = {"recons_per_s": 1.0 / np.mean(lat), "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99)}
-    x_plain = x0.get()
+    return {"recons_per_s": 1.0 / np.mean(lat), "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99)}, x0.get()
 
-    # fused single-instance form
-    lat = []
-    for r in range(1 + args.steps):
-        t = time.perf_counter()
-        ctx.nesta(y0, S0, M0, x0, prm, form=capi.OCL_FORM_FUSED)
-        if r:
-            lat.append(time.perf_counter() - t)
-    out["fused_single"] = {"recons_per_s": 1.0 / np.mean(lat), "p50_ms": 1e3 * pct(lat, 50),
-                           "p99_ms": 1e3 * pct(lat, 99)}
-    x_fused = x0.get()
 
-    # batched: B slices, one graph of STAGES*ITERS iterations per step
+def leg_batched(ctx, dev, prm, steps, warmup):
+    """Whole schedule captured once into a hipGraph; each step = setup (x_ref, normU, mu0, state) + replay +
+    finalize for every slice of the batch."""
     bt = ctx.batch(dev, prm)
     bt.setup()
     bt.capture(STAGES * ITERS)
     ctx.sync()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         bt.setup()
         bt.replay()
         bt.finalize()
         ctx.sync()
     lat = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t = time.perf_counter()
         bt.setup()
         bt.replay()
@@ -106,25 +146,175 @@ def gpu_leg(args, slices):
         lat.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
     B = len(dev)
-    out["batched"] = {"recons_per_s": B * args.steps / total, "ms_per_step": 1e3 * total / args.steps,
-                      "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99)}
-    x_batch0 = dev[0][3].get()
-
-    # roofline of the fused iteration: HIP events around each launch on the library's stream
+    res = {"recons_per_s": B * steps / total, "ms_per_step": 1e3 * total / steps,
+           "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99)}
+    # per-launch HIP-event timing of the fused iteration on the library's stream
     bt.setup()
     ms_b, ms_ca, ms_tot = bt.time_iters(STAGES * ITERS)
-    byts = b_alg(F, C, H, W) * B
+    byts = sum(b_alg(*y.shape) for (y, _, _, _) in dev)
     it_ms = ms_b + ms_ca
-    out["iteration"] = {"ms_B": ms_b, "ms_CA": ms_ca, "ms_iter": it_ms, "span_ms": ms_tot,
-                        "bytes_alg_per_iter": byts}
-    out["x"] = {"plain": x_plain, "fused": x_fused, "batched0": x_batch0}
+    res["iteration"] = {"ms_B_plus_S": ms_b, "ms_CA": ms_ca, "ms_iter": it_ms,
+                        "span_ms_per_iter": ms_tot / (STAGES * ITERS), "bytes_alg_per_iter": byts,
+                        "achieved_GBs": byts / (it_ms * 1e-3) / 1e9,
+                        "frac_hbm": byts / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     bt.close()
-    ctx.close()
+    return res
+
+
+def leg_uneven(ctx, synth, parts, prm, steps, warmup):
+    dev = []
+    for i, (x, S, M) in enumerate(parts):
+        y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
+        dev.append((y, dS, dM, ctx.empty(x.shape)))
+    ctx.sync()
+    res = leg_batched(ctx, dev, prm, steps, warmup)
+    shapes = [y.shape for (y, _, _, _) in dev]
+    res["slices"] = len(dev)
+    res["shapes_FCHW"] = shapes
+    res["coil_images"] = int(sum(f * c for (f, c, _, _) in shapes))
+    return res, dev
+
+
+def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
+    """Poisson trace against the continuous-batching server.  Inputs come from a pool of resident slices; each
+    request gets its own output buffer (recycled once the request is done) and 5 stages x U{10..30} iterations."""
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(round(rate * seconds)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    pick = rng.integers(0, len(pool), n_req)
+    iters = rng.integers(10, 31, n_req)
+    free = [[ctx.empty((y.shape[0], y.shape[2], y.shape[3])) for _ in range(6)] for (y, _, _) in pool]
+    ctx.sync()
+    srv = capi.Server(0, budget_bytes=budget, chunk=chunk)
+    inflight = {}
+    done = []
+    meta = {}
+    t0 = time.perf_counter()
+    try:
+        for r in range(n_req):
+            while True:
+                now = time.perf_counter() - t0
+                for rid in [k for k, v in inflight.items() if srv.poll(k)[0] == capi.SERVE_DONE]:
+                    i, xo = inflight.pop(rid)
+                    free[i].append(xo)
+                    done.append(rid)
+                if now >= arrivals[r]:
+                    break
+                time.sleep(min(0.002, arrivals[r] - now))
+            i = int(pick[r])
+            y, S, M = pool[i]
+            xo = free[i].pop() if free[i] else ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
+            prm = capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0)
+            rid = srv.submit(y, S, M, xo, prm)
+            inflight[rid] = (i, xo)
+            meta[rid] = (i, int(iters[r]), xo)
+        for rid in list(inflight):
+            srv.wait(rid, 600.0)
+            done.append(rid)
+        stats = srv.stats()
+        times = {rid: srv.times(rid) for rid in done}
+        # served result == solo result for the last requests (their buffers were not recycled afterwards)
+        last = sorted(meta)[-4:]
+        eq = []
+        for rid in last:
+            i, it, xo = meta[rid]
+            y, S, M = pool[i]
+            xs = ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
+            ctx.nesta(y, S, M, xs, capi.nesta_params(maxIntIter=STAGES, maxIter=it, stoptest=0),
+                      form=capi.OCL_FORM_FUSED)
+            a, b = xo.get(), xs.get()
+            eq.append(float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())))
+    finally:
+        srv.close()
+    lat = [times[k][2] - times[k][0] for k in times]
+    qwait = [times[k][1] - times[k][0] for k in times]
+    t_first = min(v[0] for v in times.values())
+    t_last = max(v[2] for v in times.values())
+    slice_iters = sum(STAGES * meta[k][1] for k in times)
+    return {"requests": len(times), "offered_rate_per_s": rate, "trace_s": float(arrivals[-1]),
+            "recons_per_s": len(times) / (t_last - t_first), "p50_ms": 1e3 * pct(lat, 50),
+            "p99_ms": 1e3 * pct(lat, 99), "max_ms": 1e3 * max(lat), "queue_wait_p50_ms": 1e3 * pct(qwait, 50),
+            "queue_wait_p99_ms": 1e3 * pct(qwait, 99), "slice_iterations": slice_iters,
+            "budget_bytes": budget, "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"],
+            "busy_s": stats["busy_s"], "failed": stats["failed"], "pool_slices": len(pool),
+            "served_vs_solo_rel_l2_max": max(eq) if eq else None}
+
+
+def leg_served_burst(ctx, capi, pool, n_req, budget, chunk, seed=11):
+    """Capacity of the server: n_req requests (same mix as the Poisson trace) submitted at once."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(pool), n_req)
+    iters = rng.integers(10, 31, n_req)
+    outs = [ctx.empty((pool[i][0].shape[0], pool[i][0].shape[2], pool[i][0].shape[3])) for i in pick]
+    ctx.sync()
+    srv = capi.Server(0, budget_bytes=budget, chunk=chunk)
+    try:
+        t0 = time.perf_counter()
+        rids = [srv.submit(*pool[i], outs[r], capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0))
+                for r, i in enumerate(pick)]
+        for rid in rids:
+            srv.wait(rid, 600.0)
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+        times = [srv.times(rid) for rid in rids]
+    finally:
+        srv.close()
+    lat = [b[2] - b[0] for b in times]
+    return {"requests": n_req, "recons_per_s": n_req / wall, "wall_s": wall, "p50_ms": 1e3 * pct(lat, 50),
+            "p99_ms": 1e3 * pct(lat, 99), "slice_iterations": int(STAGES * iters.sum()),
+            "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"]}
+
+
+def gpu_legs(args, cine, parts, full):
+    from opencliper_amd import capi, synth
+    ctx = capi.Context(0)
+    out = {"device": ctx.info()["name"], "x": {}}
+    prm = capi.nesta_params(maxIntIter=STAGES, maxIter=ITERS, stoptest=0)
+    try:
+        if "simple" in args.legs:
+            out["simple_recon"], out["x"]["simple"] = leg_simple(ctx, full)
+            log(f"simple_recon: {out['simple_recon']['recons_per_s']:.0f} recons/s")
+        dev = [(ctx.to_device(y), ctx.to_device(S), ctx.to_device(M), ctx.empty((F, H, W))) for (y, S, M) in cine]
+        out["plain"], out["x"]["plain"] = leg_single(ctx, capi, dev[0], prm, capi.OCL_FORM_PLAIN, args.plain_steps)
+        log(f"plain: {out['plain']['recons_per_s']:.2f} recons/s")
+        out["fused_single"], out["x"]["fused"] = leg_single(ctx, capi, dev[0], prm, capi.OCL_FORM_FUSED, args.steps)
+        log(f"fused single: {out['fused_single']['recons_per_s']:.2f} recons/s")
+        out["batched"] = leg_batched(ctx, dev, prm, args.steps, args.warmup)
+        out["x"]["batched0"] = dev[0][3].get()
+        log(f"batched: {out['batched']['recons_per_s']:.1f} recons/s, iteration {out['batched']['iteration']}")
+        dev3 = []
+        if "uneven" in args.legs:
+            out["uneven_batch"], dev3 = leg_uneven(ctx, synth, parts, prm, args.steps, args.warmup)
+            small = min(range(len(dev3)), key=lambda i: b_alg(*dev3[i][0].shape))
+            out["x"]["uneven_small"] = (small, dev3[small][0].get(), dev3[small][1].get(), dev3[small][2].get(),
+                                        dev3[small][3].get())
+            u = out["uneven_batch"]
+            log(f"uneven batch: {u['recons_per_s']:.1f} recons/s, iteration {u['iteration']}")
+        if "served" in args.legs:
+            if not dev3:
+                for i, (x, S, M) in enumerate(parts):
+                    y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
+                    dev3.append((y, dS, dM, None))
+            pool = [d[:3] for d in dev[:4]] + [d[:3] for d in dev3[:12]]
+            out["served"] = leg_served(ctx, capi, pool, args.serve_seconds, SERVE_RATE, args.serve_budget << 30,
+                                       args.serve_chunk)
+            s = out["served"]
+            log(f"served: {s['requests']} requests, {s['recons_per_s']:.1f} recons/s, p50 {s['p50_ms']:.0f} ms, "
+                f"p99 {s['p99_ms']:.0f} ms")
+            if args.serve_burst > 0:
+                out["served_burst"] = leg_served_burst(ctx, capi, pool, args.serve_burst, args.serve_budget << 30,
+                                                       args.serve_chunk)
+                s = out["served_burst"]
+                log(f"served burst: {s['requests']} requests, {s['recons_per_s']:.1f} recons/s")
+    finally:
+        ctx.close()
     return out
 
 
-def cpu_leg(args, slice0, gpu_x):
-    """Oracle: the FP32 restatement timed on the host cores, and the full-schedule references for slice 0."""
+# ---------------------------------------------------------------------------------------------------------------------
+# CPU leg: oracle (the FP32 restatement as the baseline, FP64 and FP32 as the checkers)
+# ---------------------------------------------------------------------------------------------------------------------
+def cpu_leg(args, slice0, full, g):
     from oracle import oracle as O
     y, S, M = slice0
     threads = O.num_threads()
@@ -139,17 +329,43 @@ def cpu_leg(args, slice0, gpu_x):
     t_setup = max(0.0, ta - 2 * t_iter)
     t_recon = t_setup + STAGES * ITERS * t_iter
     res["cpu_baseline"] = {"value": 1.0 / t_recon, "unit": "recons/s", "cores": threads, "kind": "port",
-                           "sample": f"FP32 restatement (oracle/nesta_oracle.cpp), one config slice, runs of 2 and 8 "
-                                     f"inner iterations ({ta + tb:.1f} s of CPU work); setup + 100 iterations "
-                                     f"extrapolated linearly ({t_setup:.2f} s setup + {t_iter:.3f} s/iteration)"}
+                           "sample": f"FP32 restatement (oracle/nesta_oracle.cpp, OpenMP), one 20x8x256x256 slice, "
+                                     f"runs of 2 and 8 inner iterations ({ta + tb:.1f} s of CPU work); setup + 100 "
+                                     f"iterations extrapolated linearly ({t_setup:.2f} s setup + {t_iter:.3f} "
+                                     f"s/iteration)"}
+    gx = g["x"]
+    if full is not None and "simple" in gx:
+        O.simple_recon(full.y, full.S, double=False)
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            O.simple_recon(full.y, full.S, double=False)
+            ts.append(time.perf_counter() - t)
+        x64 = O.simple_recon(full.y, full.S, double=True)
+        res["simple_recon"] = {"cpu_recons_per_s": 1.0 / float(np.median(ts)), "cores": threads,
+                               "rel_l2_vs_fp64": O.rel_l2(gx["simple"], x64),
+                               "rel_l2_vs_truth": O.rel_l2(gx["simple"], full.x_true)}
     if not args.no_ref:
         prm = O.params(maxIntIter=STAGES, maxIter=ITERS, stoptest=0)
         x64, _ = O.nesta(y, S, M, prm, double=True)
         x32, _ = O.nesta(y, S, M, prm, double=False)
-        res["rel_l2"] = {form: {"vs_fp64": O.rel_l2(x, x64), "vs_fp32_restatement": O.rel_l2(x, x32)}
-                         for form, x in gpu_x.items()}
-        res["rel_l2"]["fp32_restatement_vs_fp64"] = O.rel_l2(x32, x64)
+        rl = {form: {"vs_fp64": O.rel_l2(gx[form], x64), "vs_fp32_restatement": O.rel_l2(gx[form], x32)}
+              for form in ("plain", "fused", "batched0")}
+        rl["fp32_restatement_vs_fp64"] = O.rel_l2(x32, x64)
+        if "uneven_small" in gx:
+            i, yy, SS, MM, xx = gx["uneven_small"]
+            ref, _ = O.nesta(yy, SS, MM, prm, double=True)
+            rl["uneven_batch_smallest"] = {"slice": i, "shape_FCHW": list(yy.shape), "vs_fp64": O.rel_l2(xx, ref)}
+        res["rel_l2"] = rl
     return res
+
+
+def load_traffic():
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -159,23 +375,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--legs", default="simple,uneven,served",
+                    help="comma list of extra legs beside the headline batch: simple, uneven, served")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson trace (300 = 5 min)")
+    ap.add_argument("--serve-budget", type=int, default=64, help="server HBM budget in GiB")
+    ap.add_argument("--serve-chunk", type=int, default=5, help="server iterations between admission points")
+    ap.add_argument("--serve-burst", type=int, default=96, help="requests submitted at once for server capacity")
     ap.add_argument("--no-ref", action="store_true", help="skip the full-schedule CPU references (rel-L2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg entirely")
     args = ap.parse_args()
+    args.legs = set(x for x in args.legs.split(",") if x)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(slices are independent, so N GPUs would each run their own batch).", file=sys.stderr)
         sys.exit(2)
     workers = min(16, os.cpu_count() or 1)
     t = time.perf_counter()
-    slices = gen_slices(args.batch, workers)
-    print(f"# generated {args.batch} slices in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    g = gpu_leg(args, slices)
-    print(f"# gpu leg done: {json.dumps({k: v for k, v in g.items() if k != 'x'}, default=float)}", file=sys.stderr,
-          flush=True)
-    c = {} if args.no_cpu else cpu_leg(args, slices[0], g["x"])
-    it = g["iteration"]
-    achieved = it["bytes_alg_per_iter"] / (it["ms_iter"] * 1e-3) / 1e9
+    cine, parts, full = gen_inputs(args, workers)
+    log(f"generated inputs in {time.perf_counter() - t:.1f} s")
+    g = gpu_legs(args, cine, parts, full)
+    c = {} if args.no_cpu else cpu_leg(args, cine[0], full, g)
+    it = g["batched"]["iteration"]
+    achieved = it["achieved_GBs"]
+    tr = load_traffic()
+    traffic = tr["hbm_bytes_per_iter"] if tr else None
     line = {
         "metric": METRIC,
         "value": g["batched"]["recons_per_s"],
@@ -200,13 +423,29 @@ def main():
         "fused_single_recons_per_s": g["fused_single"]["recons_per_s"],
         "speedup_vs_plain": g["batched"]["recons_per_s"] / g["plain"]["recons_per_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_nesta_B + k_nesta_CA (one fused NESTA iteration of the whole batch)",
-                     "ms_B": it["ms_B"], "ms_CA": it["ms_CA"], "bytes_alg_per_iter": it["bytes_alg_per_iter"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_nesta_B + k_nesta_S + k_nesta_CA (one fused NESTA iteration of the whole batch)",
+                     "ms_B_plus_S": it["ms_B_plus_S"], "ms_CA": it["ms_CA"],
+                     "bytes_alg_per_iter": it["bytes_alg_per_iter"],
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None},
         "cpu_baseline": c.get("cpu_baseline"),
         "rel_l2": c.get("rel_l2"),
         "device": g["device"],
     }
+    if "simple_recon" in g:
+        line["simple_recon"] = dict(g["simple_recon"], **c.get("simple_recon", {}))
+    if "uneven_batch" in g:
+        u = dict(g["uneven_batch"])
+        u.pop("shapes_FCHW", None)
+        line["uneven_batch"] = u
+    if "served" in g:
+        s = dict(g["served"])
+        s["speedup_vs_plain"] = s["recons_per_s"] / g["plain"]["recons_per_s"]
+        line["served"] = s
+    if "served_burst" in g:
+        s = dict(g["served_burst"])
+        s["note"] = "same request mix submitted at once: server capacity"
+        line["served_burst"] = s
     print(json.dumps(line, default=float))
 
 

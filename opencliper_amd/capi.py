@@ -207,6 +207,18 @@ class Context:
         _check(lib().ocl_device_info(self.handle, name, 256, ctypes.byref(mem), ctypes.byref(cus)))
         return dict(name=name.value.decode(), total_mem=mem.value, num_cus=cus.value)
 
+    def timer_begin(self):
+        """Records a start event on the context's stream; pass the handle to timer_end."""
+        h = ctypes.c_void_p()
+        _check(lib().ocl_timer_begin(self.handle, ctypes.byref(h)))
+        return h
+
+    def timer_end(self, h):
+        """Records the stop event, waits for it and returns the elapsed device time in ms."""
+        ms = ctypes.c_double()
+        _check(lib().ocl_timer_end(self.handle, h, ctypes.byref(ms)))
+        return ms.value
+
     # --- memory ---
     def empty(self, shape, dtype=np.complex64) -> DeviceArray:
         return DeviceArray(self, shape, dtype)

@@ -170,3 +170,29 @@ def batch_sizes(n_slices=64, seed=BASE_SEED, sizes=range(128, 321, 32), coils=(4
         n = int(rng.choice(sizes))
         out.append((n, n, int(rng.integers(coils[0], coils[1] + 1)), int(rng.integers(frames[0], frames[1] + 1))))
     return out
+
+
+def make_image_parts(H, W, C, F, accel=6.0, n_centre=16, seed=BASE_SEED, slice_id=0):
+    """Host-side parts of a cine slice (phantom x, maps S, mask M) for k-space synthesis on the device:
+    y = M * (F_u(S x) + n), n at 1e-3 of the peak |F_u(S x)|."""
+    rng = slice_rng(seed, slice_id)
+    x = phantom(H, W, F, rng, cine=True).astype(np.complex64)
+    S = sens_maps(H, W, C).astype(np.complex64)
+    M = line_mask(H, W, F, accel, n_centre, rng)
+    return x, S, M
+
+
+def device_kspace(ctx, x, S, M, noise=1e-3, seed=BASE_SEED):
+    """Builds y = M * (FFT2(S x)/sqrt(N) + n) on the device with the library's own operators; returns the device
+    arrays (y, S, M)."""
+    F, H, W = x.shape
+    C = S.shape[0]
+    dx, dS, dM = ctx.to_device(x), ctx.to_device(S), ctx.to_device(M)
+    y = ctx.empty((F, C, H, W))
+    ctx.cep(dx, dS, y, F, C, in_coils=False, conj=False)
+    ctx.fft2d(y, y)
+    ctx.scal(y, 1.0 / np.sqrt(H * W))
+    peak = ctx.reduce(y, "maxabs")[0]
+    ctx.add_noise_masked(y, dM, noise * peak, seed)
+    del dx
+    return y, dS, dM
