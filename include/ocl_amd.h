@@ -120,7 +120,7 @@ int ocl_reduce(ocl_ctx* ctx, const void* x, const void* y, long n, int op, doubl
 
 /* ---- NESTA-TV: NestaUp::launch (src/processes/nesta/NestaUp.cpp:442-910) ---- */
 #define OCL_FORM_PLAIN 0   /* the reference's op sequence on the GPU, host stop test after each iteration */
-#define OCL_FORM_FUSED 1   /* two fused passes per iteration, device-side stop test, no host round trip */
+#define OCL_FORM_FUSED 1   /* fused row pass, scalar step, fused column pass; device-side stop test, no host sync */
 /* myNormest (NestaUp.cpp:291-439) for an F-frame image of N pixels, from a fresh POSIX-default drand48 stream. */
 int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
 int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
