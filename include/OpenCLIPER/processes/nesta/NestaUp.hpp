@@ -31,8 +31,9 @@ class NestaUp : public Process {
             : lambda_i(lambda_i), mu_f(mu_f), La(La), maxIntIter(maxIntIter), tolVar(tolVar), verbose(verbose),
               maxIter(maxIter), stoptest(stoptest), miniter(miniter), argsMC(args), showProgress(sp) {}
     };
-    /// Execution form: the fused two-pass iteration (default) or the reference's op sequence on the GPU.
-    enum Form { FUSED = 1, PLAIN = 0 };
+    /// Execution form: the fused iteration (default), the reference's op sequence on the GPU, or the fused iteration
+    /// with split-precision BF16-MFMA DFTs (H, W in {64, 128, 192, 256}).
+    enum Form { FUSED = 1, PLAIN = 0, SPLIT = 2 };
     void setForm(Form f) { form_ = f; }
 
     /// What the last launch computed: normU, lambda, mu0, iterations per stage, f per inner iteration.

@@ -121,6 +121,8 @@ int ocl_reduce(ocl_ctx* ctx, const void* x, const void* y, long n, int op, doubl
 /* ---- NESTA-TV: NestaUp::launch (src/processes/nesta/NestaUp.cpp:442-910) ---- */
 #define OCL_FORM_PLAIN 0   /* the reference's op sequence on the GPU, host stop test after each iteration */
 #define OCL_FORM_FUSED 1   /* fused row pass, scalar step, fused column pass; device-side stop test, no host sync */
+#define OCL_FORM_SPLIT 2   /* as FUSED, with the row and column DFTs as BF16 MFMA GEMMs (3-term hi/lo split product,
+                              FP32 accumulation; NESTA scalars and norms in FP64); H, W in {64, 128, 192, 256} */
 /* myNormest (NestaUp.cpp:291-439) for an F-frame image of N pixels, from a fresh POSIX-default drand48 stream. */
 int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
 int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
@@ -129,6 +131,9 @@ int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, 
 /* ---- batched NESTA-TV: many uneven slices advanced together, one launch per pass per iteration ---- */
 int ocl_batch_create(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
                      ocl_batch** out);
+/* form = OCL_FORM_FUSED (what ocl_batch_create uses) or OCL_FORM_SPLIT (every slice of the batch then shares H, W) */
+int ocl_batch_create_form(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm, int form,
+                          ocl_batch** out);
 int ocl_batch_setup(ocl_batch* b);                    /* E^H y, maxima, normU, state (per slice) */
 int ocl_batch_iterate(ocl_batch* b, int n_iters);     /* enqueue n_iters iterations (no host sync) */
 int ocl_batch_capture(ocl_batch* b, int n_iters);     /* record n_iters iterations into a hipGraph */

@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(_PKG), "include", "ocl_amd.h")
 
 OCL_FORM_PLAIN = 0
 OCL_FORM_FUSED = 1
+OCL_FORM_SPLIT = 2
 MAX_STAGES = 16
 TRACE = 4096
 RED = {"nrm2": 0, "dotc": 1, "sum": 2, "asum": 3, "maxabs": 4}
@@ -116,6 +117,7 @@ def lib():
         "ocl_nesta_run": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(NestaReport)],
                           I),
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
+        "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_batch_setup": ([P], I), "ocl_batch_iterate": ([P, I], I), "ocl_batch_capture": ([P, I], I),
         "ocl_batch_replay": ([P], I), "ocl_batch_active": ([P, ctypes.POINTER(I)], I),
         "ocl_batch_finalize": ([P], I), "ocl_batch_report": ([P, I, ctypes.POINTER(NestaReport)], I),
@@ -299,14 +301,14 @@ class Context:
                                    ctypes.byref(rep)))
         return rep.as_dict()
 
-    def batch(self, slices, prm: NestaParams | None = None) -> "Batch":
-        return Batch(self, slices, prm or nesta_params())
+    def batch(self, slices, prm: NestaParams | None = None, form=OCL_FORM_FUSED) -> "Batch":
+        return Batch(self, slices, prm or nesta_params(), form)
 
 
 class Batch:
     """Many uneven slices advanced together (ocl_batch_*).  `slices` = list of (y, S, M, x_out) DeviceArrays."""
 
-    def __init__(self, ctx: Context, slices, prm: NestaParams):
+    def __init__(self, ctx: Context, slices, prm: NestaParams, form=OCL_FORM_FUSED):
         self.ctx = ctx
         self.keep = list(slices)
         arr = (Slice * len(slices))()
@@ -315,7 +317,8 @@ class Batch:
             arr[i] = Slice(y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W)
         self.prm = prm
         h = ctypes.c_void_p()
-        _check(lib().ocl_batch_create(ctx.handle, len(slices), arr, ctypes.byref(prm), ctypes.byref(h)))
+        _check(lib().ocl_batch_create_form(ctx.handle, len(slices), arr, ctypes.byref(prm), int(form),
+                                           ctypes.byref(h)))
         self.handle = h.value
 
     def setup(self):

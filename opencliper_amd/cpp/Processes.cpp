@@ -213,7 +213,7 @@ void NestaUp::launch() {
     checkOcl(ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
                            (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), (int)k->getDynDimsTotalSize(),
                            (int)k->getNCoils(), (int)NDARRAYHEIGHT(a), (int)NDARRAYWIDTH(a), &p,
-                           form_ == FUSED ? OCL_FORM_FUSED : OCL_FORM_PLAIN, rep.get()),
+                           form_ == FUSED ? OCL_FORM_FUSED : form_ == SPLIT ? OCL_FORM_SPLIT : OCL_FORM_PLAIN, rep.get()),
              "NestaUp::launch");
     stopProfiling();
     lp->tolVar = (float)rep->final_tolVar;   // the reference overwrites the caller's tolVar

@@ -68,10 +68,12 @@ size_t lds_CA(const SliceDesc& d) {
 }
 
 // Step 1 of admission: plans, sampled-row lists (one small device round trip) and the exact device footprint.
-int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long total_cols, SliceRes* r) {
+int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long total_cols, int prec, SliceRes* r) {
     if (s.F < 1 || s.C < 1 || s.H < 1 || s.W < 1 || !s.y || !s.S || !s.M || !s.x_out)
         return fail(OCL_E_INVALID, "NESTA: slice has bad dimensions or null pointers");
     if (s.H > 512 || s.W > 1024) return fail(OCL_E_UNSUPPORTED, "NESTA: fused form supports H <= 512 and W <= 1024");
+    if (prec && (!mfma_size_supported(s.H) || !mfma_size_supported(s.W)))
+        return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports H, W in {64, 128, 192, 256}");
     r->c = c;
     r->in = s;
     r->prm = prm;
@@ -83,8 +85,9 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.fastH = fast_p(s.H);
     d.fastW = fast_p(s.W);
     choose_tiles(total_cols, s, &d.tile_cols, &d.rows_per_wg);
-    if (d.fastH) d.tile_cols = 16;
-    if (d.fastW) d.rows_per_wg = 16;
+    if (d.fastH || prec) d.tile_cols = 16;
+    if (d.fastW || prec) d.rows_per_wg = 16;
+    d.prec = prec;
     if (lds_B(d) > LDS_MAX || lds_CA(d) > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
     // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
     const size_t fb = (size_t)s.F * s.H;
@@ -115,6 +118,9 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
             }
     }
     d.nmax = nmax;
+    d.kp = (nmax + 31) / 32 * 32;
+    if (prec && d.kp > 64)
+        return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports at most 64 sampled rows per frame");
     d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
     d.wg_B = s.F * s.C * d.nrow_groups;
@@ -136,7 +142,10 @@ size_t slice_footprint(const SliceRes& r) {
     return 4 * rnd(xb) + rnd(sizeof(float2) * (size_t)d.F * d.C * d.nmax * d.W) + rnd(sizeof(int16_t) * r.rows.size()) +
            rnd(sizeof(int16_t) * r.rowpos.size()) + rnd(sizeof(int) * d.F) + rnd(sizeof(NState)) +
            rnd(sizeof(double) * d.wg_B) + rnd(sizeof(double) * 2 * d.wg_U) + rnd(sizeof(unsigned) * 4) +
-           rnd(sizeof(double) * 3 * OCL_TRACE) + rnd(sizeof(double) * 2 * d.wg_CA);
+           rnd(sizeof(double) * 3 * OCL_TRACE) + rnd(sizeof(double) * 2 * d.wg_CA) +
+           (d.prec ? 2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.W * d.W) +
+                         2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.F * d.H * d.kp)
+                   : 0);
 }
 
 // Step 2 of admission: device buffers and row tables.
@@ -163,6 +172,25 @@ int slice_alloc(SliceRes* r) {
     CHK(r->alloc(sizeof(unsigned) * 4, &p)); d.counter = (unsigned*)p;
     CHK(r->alloc(sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
     CHK(r->alloc(sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
+    if (d.prec) {
+        // DFT matrices of the split-precision passes (row: forward, inverse; column: inverse, forward per frame)
+        const size_t rowb = sizeof(uint16_t) * 4 * (size_t)d.W * d.W;
+        const size_t colb = sizeof(uint16_t) * 4 * (size_t)d.F * d.H * d.kp;
+        uint16_t* m[4];
+        CHK(r->alloc(rowb, &p)); m[0] = (uint16_t*)p;
+        CHK(r->alloc(rowb, &p)); m[1] = (uint16_t*)p;
+        CHK(r->alloc(colb, &p)); m[2] = (uint16_t*)p;
+        CHK(r->alloc(colb, &p)); m[3] = (uint16_t*)p;
+        for (int kind = 0; kind < 4; ++kind) {
+            const int L = kind < 2 ? d.W : d.H;
+            const long plane = kind < 2 ? (long)L * L : (long)L * d.kp;
+            const dim3 grid((unsigned)std::min<long>((plane + OCL_NT - 1) / OCL_NT, 1024), kind < 2 ? 1 : d.F);
+            hipLaunchKernelGGL(k_dft_split, grid, dim3(OCL_NT), 0, c->stream, m[kind], kind, L, d.kp, d.rows, d.nrows,
+                               d.nmax);
+            HIPCHK(hipGetLastError());
+        }
+        d.drowF = m[0]; d.drowI = m[1]; d.dcolI = m[2]; d.dcolF = m[3];
+    }
     HIPCHK(hipStreamSynchronize(c->stream));    // the host-side row tables are reused by later admissions
     r->allocated = true;
     return OCL_OK;
@@ -201,8 +229,14 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L = NestaLaunch{};
     L.maxe = 4;
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
+    L.prec = ns ? v[0]->d.prec : 0;
+    L.kj = 1;
     for (int i = 0; i < ns; ++i) {
         const SliceDesc& d = v[i]->d;
+        if (d.prec != L.prec) return fail(OCL_E_INVALID, "batch: slices of one table must share the precision form");
+        if (L.prec && (d.H != v[0]->d.H || d.W != v[0]->d.W))
+            return fail(OCL_E_UNSUPPORTED, "batch: split-precision slices of one table must share H and W");
+        L.kj = std::max(L.kj, d.kp / 32);
         hd[i] = d;
         oB[i + 1] = oB[i] + d.wg_B + d.wg_U;
         oC[i + 1] = oC[i] + d.wg_CA;
@@ -223,6 +257,11 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.totCA = oC[ns];
     L.PH = PH;
     L.PW = PW;
+    if (L.prec && ns) {
+        L.PH = v[0]->d.H / 16;
+        L.PW = v[0]->d.W / 16;
+        L.ldsB = L.ldsCA = 0;
+    }
     if (ns) {
         HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offB, oB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
@@ -343,11 +382,19 @@ struct ocl_batch {
     float2** d_outs = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int prec = 0;
     bool ready = false;
 };
 
 extern "C" int ocl_batch_create(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
                                 ocl_batch** out) {
+    return ocl_batch_create_form(c, nslices, slices, prm, OCL_FORM_FUSED, out);
+}
+
+extern "C" int ocl_batch_create_form(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
+                                     int form, ocl_batch** out) {
+    if (form != OCL_FORM_FUSED && form != OCL_FORM_SPLIT)
+        return fail(OCL_E_INVALID, "batch: form must be OCL_FORM_FUSED or OCL_FORM_SPLIT");
     if (!c) return fail(OCL_E_INVALID, "null context");
     if (nslices < 1 || !slices || !prm || !out) return fail(OCL_E_INVALID, "batch: bad arguments");
     if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER)
@@ -362,6 +409,7 @@ extern "C" int ocl_batch_create(ocl_ctx* c, int nslices, const ocl_slice* slices
     }
     ocl_batch* b = new ocl_batch();
     b->ctx = c;
+    b->prec = form == OCL_FORM_SPLIT ? 1 : 0;
     b->sl.assign(slices, slices + nslices);
     b->prm = *prm;
     *out = b;
@@ -388,7 +436,7 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
         for (const auto& s : b->sl) total_cols += (long)s.F * s.W;
         for (const auto& s : b->sl) {
             b->res.emplace_back(new SliceRes());
-            CHK(slice_plan(c, s, b->prm, total_cols, b->res.back().get()));
+            CHK(slice_plan(c, s, b->prm, total_cols, b->prec, b->res.back().get()));
             CHK(slice_alloc(b->res.back().get()));
             b->view.push_back(b->res.back().get());
         }
@@ -597,7 +645,7 @@ static void server_loop(ocl_server* sv) {
             }
             if (!r->res) {
                 r->res.reset(new SliceRes());
-                int rc = slice_plan(c, r->s, r->prm, (long)r->s.F * r->s.W, r->res.get());
+                int rc = slice_plan(c, r->s, r->prm, (long)r->s.F * r->s.W, 0, r->res.get());
                 if (rc == OCL_OK) r->footprint = slice_footprint(*r->res);
                 if (rc == OCL_OK && r->footprint > sv->budget) {
                     rc = OCL_E_NOMEM;

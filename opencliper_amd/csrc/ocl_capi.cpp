@@ -669,10 +669,10 @@ extern "C" int ocl_nesta_run(ocl_ctx* c, const void* y, const void* S, const uin
     if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER) return fail(OCL_E_INVALID, "NESTA: miniter out of range");
     HIPCHK(hipSetDevice(c->device));
     if (form == OCL_FORM_PLAIN) return nesta_plain(c, y, S, M, x_out, F, C, H, W, *prm, rep);
-    if (form != OCL_FORM_FUSED) return fail(OCL_E_INVALID, "NESTA: unknown form");
+    if (form != OCL_FORM_FUSED && form != OCL_FORM_SPLIT) return fail(OCL_E_INVALID, "NESTA: unknown form");
     ocl_slice s{y, S, M, x_out, F, C, H, W};
     ocl_batch* b = nullptr;
-    CHK(ocl_batch_create(c, 1, &s, prm, &b));
+    CHK(ocl_batch_create_form(c, 1, &s, prm, form, &b));
     int rc = ocl_batch_run(b, 16);
     if (rc == OCL_OK && rep) rc = ocl_batch_report(b, 0, rep);
     ocl_batch_destroy(b);

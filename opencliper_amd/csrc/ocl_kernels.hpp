@@ -63,6 +63,14 @@ struct SliceDesc {
     unsigned* counter;  // [4] spare device counters (zeroed at setup)
     double* trace;      // [trace_cap][3] : f, ||r||^2, l1 per inner iteration (may be null)
     double* maxpart;    // [wg_CA][2] partial maxima of |x_ref| and |U x_ref| (setup)
+    // split-precision form (prec = 1): DFT matrices as bf16 hi/lo planes (re_hi, re_lo, im_hi, im_lo), see
+    // ocl_nesta_mfma.hip for the index order of each
+    int prec;           // 0: FP32 FFT passes, 1: BF16-MFMA DFT passes with a 3-term split product
+    int kp;             // nmax rounded up to a multiple of 32 (the K extent of the inverse column DFT)
+    const uint16_t* drowF;  // [4][W][W]  forward row DFT
+    const uint16_t* drowI;  // [4][W][W]  inverse row DFT, K index permuted
+    const uint16_t* dcolI;  // [F][4][H][kp]  inverse column DFT from the sampled rows
+    const uint16_t* dcolF;  // [F][4][kp][H]  forward column DFT onto the sampled rows, K index permuted
 };
 
 // kernels (ocl_ops.hip)
@@ -102,10 +110,19 @@ struct NestaLaunch {
     int ns, totB, totCA;
     size_t ldsB, ldsCA;
     int maxe, PH, PW;
+    int prec;           // 1: every slice uses the split-precision passes (then PH = H/16, PW = W/16)
+    int kj;             // split-precision passes: largest kp / 32 of the table
 };
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
 void nesta_kernel_attributes(int lds_bytes);
+// split-precision passes (ocl_nesta_mfma.hip)
+hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
+hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
+bool mfma_size_supported(int n);      // H or W the split-precision passes are instantiated for
+// fills one DFT matrix of the split-precision passes (kind: 0 row forward, 1 row inverse, 2 column inverse,
+// 3 column forward); rows/nrows/nmax describe the sampled rows of each frame for the column kinds
+__global__ void k_dft_split(uint16_t* out, int kind, int L, int kp, const int16_t* rows, const int* nrows, int nmax);
 __global__ void k_setup_max(const SliceDesc* descs, const int* wg_off, int nslices, double* out);
 __global__ void k_finalize(const SliceDesc* descs, int nslices, float2* const* outs);

@@ -10,17 +10,8 @@
 // The last workgroup of CA folds the per-workgroup partial sums in index order (deterministic), evaluates
 // f = 1/2 ||r||^2 + lambda (Re<u,Ux> - mu/2 ||u||^2), runs the reference's stop test and advances the device state
 // (NestaUp.cpp:797-889), so no host round trip is needed between iterations.
-#include "ocl_kernels.hpp"
+#include "ocl_nesta_common.hpp"
 #include "ocl_nesta_fast.hpp"
-
-__device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, int b) {
-    int lo = 0, hi = ns;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= b) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 // Row flags for the sampled-row lists: flag[f][h] = 1 if the mask row or any coil's k-space row is non-zero.
 __global__ void k_rowflags(const float2* __restrict__ y, const uint8_t* __restrict__ M, int F, int C, int H, int W,
@@ -145,24 +136,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B_t(const SliceDesc* __restric
     if (local >= d.wg_B) {
         // ---- temporal-TV terms of x_k: sum |u|^2 and sum Re(conj(u) Ux), u = Ux / max(mu, |Ux|) ----
         if (mode != 0) return;
-        const int uid = local - d.wg_B;
-        const int per_f = (int)((d.N + d.u_chunk - 1) / d.u_chunk);
-        const int f = uid / per_f;
-        const long p0 = (long)(uid - f * per_f) * d.u_chunk;
-        const long p1 = min(d.N, p0 + d.u_chunk);
-        const int fm = f == 0 ? d.F - 1 : f - 1;
-        const float2* xk = st->cur ? d.x1 : d.x0;
-        const float mu = (float)st->mu;
-        double v[2] = {0.0, 0.0};
-        for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-            const float2 ux = c_sub(xk[(long)f * d.N + p], xk[(long)fm * d.N + p]);
-            const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
-            const float2 u = make_float2(ux.x / nf, ux.y / nf);
-            v[0] += (double)u.x * u.x + (double)u.y * u.y;
-            v[1] += (double)u.x * ux.x + (double)u.y * ux.y;
-        }
-        block_sum<2>(v, red);
-        if (threadIdx.x == 0) { d.partU[2 * uid] = v[0]; d.partU[2 * uid + 1] = v[1]; }
+        nesta_U_task(d, local - d.wg_B, red);
     } else if (PWU > 0 || d.fastW) {
         double acc[1];
         switch (PWU > 0 ? PWU : d.fastW) {
@@ -249,7 +223,6 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B_t(const SliceDesc* __restric
 // ------------------------------------------------------------------------------------------------------------------
 template <int MAXE>
 __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, int mode, float2* lds) {
-    NState* st = d.st;
     const int tile = local % d.ncol_tiles;
     const int f = local / d.ncol_tiles;
     const int F = d.F, C = d.C, H = d.H, W = d.W, TW = d.tile_cols, nmax = d.nmax;
@@ -306,43 +279,15 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
             }
         }
     } else {
-        const float2* xk = st->it_cur ? d.x1 : d.x0;
-        float2* xo = st->it_cur ? d.x0 : d.x1;
+        const UpdCoef u = upd_coef(d);
         const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
-        const float mu = (float)st->it_mu, lam = (float)st->lambda, L1 = (float)st->it_L1;
-        const float apk = (float)st->it_apk;
-        const float tau = (float)st->it_tau;
-        const int sstart = st->it_sstart;
 #pragma unroll
         for (int i = 0; i < MAXE; ++i) {
             const int e = threadIdx.x + i * OCL_NT;
             xn[i] = make_float2(0.f, 0.f);
             if (e < nel) {
                 const int h = e >> lsh, wl = e & (TW - 1);
-                if (w0 + wl < W) {
-                    const long p = (long)h * W + w0 + wl;
-                    const float2 xf = xk[(long)f * N + p], xm = xk[(long)fm * N + p], xp = xk[(long)fp * N + p];
-                    const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-                    const float nf = fmaxf(mu, sqrtf(c_abs2(uxf))), np = fmaxf(mu, sqrtf(c_abs2(uxp)));
-                    const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
-                    const float2 ares = c_scale(acc[i], s);
-                    const float2 df = make_float2(lam * (uf.x - up.x) + ares.x, lam * (uf.y - up.y) + ares.y);
-                    float2 xr, wo;
-                    if (sstart) {
-                        xr = xf;
-                        d.xref[(long)f * N + p] = xf;
-                        wo = make_float2(0.f, 0.f);
-                    } else {
-                        xr = d.xref[(long)f * N + p];
-                        wo = d.Wk[(long)f * N + p];
-                    }
-                    const float2 yk = make_float2(xf.x - L1 * df.x, xf.y - L1 * df.y);
-                    const float2 wn = make_float2(wo.x + apk * df.x, wo.y + apk * df.y);
-                    d.Wk[(long)f * N + p] = wn;
-                    const float2 z = make_float2(xr.x - L1 * wn.x, xr.y - L1 * wn.y);
-                    xn[i] = make_float2(tau * z.x + (1.0f - tau) * yk.x, tau * z.y + (1.0f - tau) * yk.y);
-                    xo[(long)f * N + p] = xn[i];
-                }
+                if (w0 + wl < W) xn[i] = nesta_update_px(d, u, f, fm, fp, (long)h * W + w0 + wl, c_scale(acc[i], s));
             }
         }
     }
@@ -439,6 +384,7 @@ static NestaKernel pick_CA(int PH, int maxe) {
 }
 
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode) {
+    if (L.prec) return launch_nesta_B_mfma(L, stream, mode);
     hipLaunchKernelGGL(pick_B(L.PW), dim3(L.totB), dim3(OCL_NT), L.ldsB, stream, L.dd, L.offB, L.ns, mode);
     return hipGetLastError();
 }
@@ -449,6 +395,7 @@ hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream) {
 }
 
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode) {
+    if (L.prec) return launch_nesta_CA_mfma(L, stream, mode);
     hipLaunchKernelGGL(pick_CA(L.PH, L.maxe), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 // Register four-step bodies of the fused NESTA passes for H = 16 P_H and W = 16 P_W (see ocl_rfft.hpp).
 // Included by ocl_nesta.hip only.
 #pragma once
-#include "ocl_kernels.hpp"
+#include "ocl_nesta_common.hpp"
 #include "ocl_rfft.hpp"
 
 // LDS footprint (float2 elements) of the fast passes
@@ -133,7 +133,6 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     const int w = tile * 16 + s;
     const bool wv = w < W;
     const int wc = wv ? w : 0;
-    NState* st = d.st;
     float2* tw = lds + P * FAST_CA_KS;
     int16_t* rp = (int16_t*)(tw + H);
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
@@ -193,39 +192,12 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             if (wv) d.x0[(long)f * N + (long)(t + 16 * n1) * W + w] = xn[n1];
         }
     } else {
-        const float2* xk = st->it_cur ? d.x1 : d.x0;
-        float2* xo = st->it_cur ? d.x0 : d.x1;
+        const UpdCoef u = upd_coef(d);
         const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
-        const float mu = (float)st->it_mu, lam = (float)st->lambda, L1 = (float)st->it_L1;
-        const float apk = (float)st->it_apk, tau = (float)st->it_tau;
-        const int sstart = st->it_sstart;
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
             xn[n1] = make_float2(0.f, 0.f);
-            if (wv) {
-                const long p = (long)(t + 16 * n1) * W + w;
-                const float2 xf = xk[(long)f * N + p], xm = xk[(long)fm * N + p], xp = xk[(long)fp * N + p];
-                const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-                const float nf = fmaxf(mu, sqrtf(c_abs2(uxf))), np = fmaxf(mu, sqrtf(c_abs2(uxp)));
-                const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
-                const float2 ares = c_scale(acc[n1], sc);
-                const float2 df = make_float2(lam * (uf.x - up.x) + ares.x, lam * (uf.y - up.y) + ares.y);
-                float2 xr, wo;
-                if (sstart) {
-                    xr = xf;
-                    d.xref[(long)f * N + p] = xf;
-                    wo = make_float2(0.f, 0.f);
-                } else {
-                    xr = d.xref[(long)f * N + p];
-                    wo = d.Wk[(long)f * N + p];
-                }
-                const float2 yk = make_float2(xf.x - L1 * df.x, xf.y - L1 * df.y);
-                const float2 wn = make_float2(wo.x + apk * df.x, wo.y + apk * df.y);
-                d.Wk[(long)f * N + p] = wn;
-                const float2 z = make_float2(xr.x - L1 * wn.x, xr.y - L1 * wn.y);
-                xn[n1] = make_float2(tau * z.x + (1.0f - tau) * yk.x, tau * z.y + (1.0f - tau) * yk.y);
-                xo[(long)f * N + p] = xn[n1];
-            }
+            if (wv) xn[n1] = nesta_update_px(d, u, f, fm, fp, (long)(t + 16 * n1) * W + w, c_scale(acc[n1], sc));
         }
     }
 

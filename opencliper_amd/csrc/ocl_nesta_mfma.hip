@@ -1,0 +1,386 @@
+// Split-precision NESTA-TV passes for gfx950: the row and column DFTs of E = M F S and its adjoint run as dense
+// complex GEMMs on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with a 3-term split product, accumulated in FP32.
+//
+// Every FP32 operand value v is split into hi = bf16(v) and lo = bf16(v - hi); a product is taken as
+// a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, ~2^-16 relative, is dropped).  A complex 16x16x32 block then
+// costs 12 MFMAs: Re += Ar Br - Ai Bi and Im += Ar Bi + Ai Br, three terms each.  The DFT matrices are the A operands
+// (precomputed per slice in double and split, k_dft_split); the data are the B operands, split in registers.
+//
+// The passes have the structure of the FP32 passes (pass B: rows, pass CA: columns + fused NESTA update; the scalar
+// step, the state machine and the temporal-TV reductions are shared), with one wavefront per workgroup:
+//   B  (frame f, coil c, 16 sampled rows): K = X D_W (forward row DFT), r = M K / sqrt(N) - y, V = r conj(D_W)
+//   CA (frame f, 16 columns): for each coil Z = D_inv(f) V (inverse column DFT from the sampled rows), A^H r =
+//      sum_c conj(S_c) Z / sqrt(N), NESTA update, then for each coil U = D_fwd(f) (S_c x_{k+1}) onto the sampled rows.
+// The output tile of the first GEMM of a pass leaves each lane holding 4 consecutive rows (m = 16 t + 4 g + i) of its
+// column; the second GEMM of the pass sums over that same index, so its K order is permuted to
+// k = 32 s + 8 g + e  <->  m = 16 (2 s + e / 4) + 4 g + e % 4, which makes its B operand exactly the lane's own
+// registers (no LDS exchange).  The matching A matrices (drowI, dcolF) are stored in that permuted K order.
+#include "ocl_nesta_common.hpp"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+struct Op8 {           // 8 complex values as bf16 hi/lo planes
+    bf16x8 rh, rl, ih, il;
+};
+
+__device__ __forceinline__ void split8(const float2 (&v)[8], Op8& o) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 hr = (__bf16)v[e].x, hi = (__bf16)v[e].y;
+        o.rh[e] = hr;
+        o.rl[e] = (__bf16)(v[e].x - (float)hr);
+        o.ih[e] = hi;
+        o.il[e] = (__bf16)(v[e].y - (float)hi);
+    }
+}
+
+__device__ __forceinline__ bf16x8 bneg(bf16x8 v) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+// A operand: 8 consecutive K entries of row m of a 4-plane matrix (plane stride ps), off = m * K + k0
+__device__ __forceinline__ Op8 loadA(const uint16_t* __restrict__ base, long ps, long off) {
+    Op8 a;
+    a.rh = *(const bf16x8*)(base + off);
+    a.rl = *(const bf16x8*)(base + ps + off);
+    a.ih = *(const bf16x8*)(base + 2 * ps + off);
+    a.il = *(const bf16x8*)(base + 3 * ps + off);
+    return a;
+}
+
+#define MF(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
+
+// (dr, di) += A B for one 16 x 16 x 32 complex block, 3-term split product
+__device__ __forceinline__ void cmac(f32x4& dr, f32x4& di, const Op8& a, const Op8& b) {
+    const bf16x8 nih = bneg(b.ih), nil = bneg(b.il);
+    dr = MF(a.rh, b.rh, dr);
+    di = MF(a.rh, b.ih, di);
+    dr = MF(a.rh, b.rl, dr);
+    di = MF(a.rh, b.il, di);
+    dr = MF(a.rl, b.rh, dr);
+    di = MF(a.rl, b.ih, di);
+    dr = MF(a.ih, nih, dr);
+    di = MF(a.ih, b.rh, di);
+    dr = MF(a.ih, nil, dr);
+    di = MF(a.ih, b.rl, di);
+    dr = MF(a.il, nih, dr);
+    di = MF(a.il, b.rh, di);
+}
+
+__device__ __forceinline__ int kperm(int k) {   // permuted K index -> row index of the first GEMM's output
+    const int s = k >> 5, r = k & 31, g = r >> 3, e = r & 7;
+    return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3);
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass B: 16 sampled rows of one (frame, coil); W = 16 MT
+// ---------------------------------------------------------------------------------------------------------------------
+template <int MT>
+__device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, int mode) {
+    constexpr int W = 16 * MT, KS = MT / 2;
+    constexpr long PS = (long)W * W;
+    const int l = threadIdx.x, n = l & 15, g = l >> 4;
+    const int G = d.nrow_groups;
+    const int grp = local % G, fc = local / G;
+    const int C = d.C, H = d.H, nmax = d.nmax;
+    const int f = fc / C, c = fc - f * C;
+    const int nr = d.nrows[f];
+    if (grp * 16 >= nr) return 0.0;
+    const int j = grp * 16 + n;
+    const bool valid = j < nr;
+    float2* kb = d.kbuf + (((long)f * C + c) * nmax + (valid ? j : 0)) * W;
+    const int ky = valid ? d.rows[(long)f * nmax + j] : 0;
+    const float2* yrow = d.y + (((long)f * C + c) * H + ky) * W;
+    const uint8_t* mrow = d.M + ((long)f * H + ky) * W;
+    f32x4 ar[MT], ai[MT];
+    double racc = 0.0;
+    if (mode == 0) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t) { ar[t] = f32x4{0.f, 0.f, 0.f, 0.f}; ai[t] = ar[t]; }
+        // forward row DFT: D[kx][row] = sum_w F_W[kx][w] X[row][w]
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            float2 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = valid ? kb[32 * s + 8 * g + e] : make_float2(0.f, 0.f);
+            Op8 b;
+            split8(v, b);
+#pragma unroll
+            for (int t = 0; t < MT; ++t) cmac(ar[t], ai[t], loadA(d.drowF, PS, (long)(16 * t + n) * W + 32 * s + 8 * g), b);
+        }
+        // residual r = M K / sqrt(N) - y at kx = 16 t + 4 g + i (NestaUp.cpp:716-732)
+        const float sc = d.inv_sqrtN;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kx = 16 * t + 4 * g + i;
+                float2 rv = make_float2(0.f, 0.f);
+                if (valid) {
+                    const float2 yv = yrow[kx];
+                    rv = mrow[kx] ? make_float2(ar[t][i] * sc - yv.x, ai[t][i] * sc - yv.y) : make_float2(-yv.x, -yv.y);
+                    racc += (double)rv.x * rv.x + (double)rv.y * rv.y;
+                }
+                ar[t][i] = rv.x;
+                ai[t][i] = rv.y;
+            }
+    } else {
+        // setup: the inverse row DFT of the y row itself
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 yv = valid ? yrow[16 * t + 4 * g + i] : make_float2(0.f, 0.f);
+                ar[t][i] = yv.x;
+                ai[t][i] = yv.y;
+            }
+    }
+    // inverse row DFT (unnormalised): V[w][row] = sum_kx conj(F_W)[w][kx] r[kx][row], K in the permuted order
+    Op8 rb[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        float2 v[8];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[e] = make_float2(ar[2 * s][e], ai[2 * s][e]);
+            v[e + 4] = make_float2(ar[2 * s + 1][e], ai[2 * s + 1][e]);
+        }
+        split8(v, rb[s]);
+    }
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        f32x4 orr = {0.f, 0.f, 0.f, 0.f}, oi = orr;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) cmac(orr, oi, loadA(d.drowI, PS, (long)(16 * t + n) * W + 32 * s + 8 * g), rb[s]);
+        if (valid) {
+            float4* o = (float4*)(kb + 16 * t + 4 * g);
+            o[0] = make_float4(orr[0], oi[0], orr[1], oi[1]);
+            o[1] = make_float4(orr[2], oi[2], orr[3], oi[3]);
+        }
+    }
+    return racc;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass CA: 16 columns of one frame; H = 16 MT, sampled-row capacity kp <= 32 KJ
+// ---------------------------------------------------------------------------------------------------------------------
+template <int MT, int KJ>
+__device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int mode) {
+    constexpr int H = 16 * MT, KS = MT / 2;
+    const int l = threadIdx.x, n = l & 15, g = l >> 4;
+    const int tile = local % d.ncol_tiles, f = local / d.ncol_tiles;
+    const int F = d.F, C = d.C, W = d.W, nmax = d.nmax, kp = d.kp;
+    const long N = d.N;
+    const int w = tile * 16 + n;
+    const int nr = d.nrows[f];
+    const long psI = (long)H * kp, psF = (long)kp * H;
+    const uint16_t* AI = d.dcolI + (long)f * 4 * psI;
+    const uint16_t* AF = d.dcolF + (long)f * 4 * psF;
+    f32x4 ar[MT], ai[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) { ar[t] = f32x4{0.f, 0.f, 0.f, 0.f}; ai[t] = ar[t]; }
+
+    // ---- A^H r: Z = D_inv(f) V per coil (K = sampled rows), conj(S_c) multiply, coil sum ----
+    for (int c = 0; c < C; ++c) {
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + w;
+        const float2* Sc = d.S + (long)c * N + w;
+        Op8 bj[KJ];
+#pragma unroll
+        for (int s = 0; s < KJ; ++s) {
+            float2 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int jj = 32 * s + 8 * g + e;
+                v[e] = jj < nr ? kb[(long)jj * W] : make_float2(0.f, 0.f);
+            }
+            split8(v, bj[s]);
+        }
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            f32x4 zr = {0.f, 0.f, 0.f, 0.f}, zi = zr;
+#pragma unroll
+            for (int s = 0; s < KJ; ++s)
+                if (32 * s < kp) cmac(zr, zi, loadA(AI, psI, (long)(16 * t + n) * kp + 32 * s + 8 * g), bj[s]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 sv = Sc[(long)(16 * t + 4 * g + i) * W];
+                ar[t][i] += zr[i] * sv.x + zi[i] * sv.y;
+                ai[t][i] += zi[i] * sv.x - zr[i] * sv.y;
+            }
+        }
+    }
+
+    // ---- NESTA update at h = 16 t + 4 g + i (shared with the FP32 passes); x_{k+1} replaces the sums ----
+    const float sc = d.inv_sqrtN;
+    if (mode == 1) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar[t][i] *= sc;
+                ai[t][i] *= sc;
+                d.x0[(long)f * N + (long)(16 * t + 4 * g + i) * W + w] = make_float2(ar[t][i], ai[t][i]);
+            }
+    } else {
+        const UpdCoef u = upd_coef(d);
+        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 xn = nesta_update_px(d, u, f, fm, fp, (long)(16 * t + 4 * g + i) * W + w,
+                                                  make_float2(ar[t][i] * sc, ai[t][i] * sc));
+                ar[t][i] = xn.x;
+                ai[t][i] = xn.y;
+            }
+    }
+
+    // ---- forward column DFT of S_c x_{k+1} onto the sampled rows, K (= h) in the permuted order ----
+    for (int c = 0; c < C; ++c) {
+        const float2* Sc = d.S + (long)c * N + w;
+        float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + w;
+        f32x4 orr[2 * KJ], oi[2 * KJ];
+#pragma unroll
+        for (int q = 0; q < 2 * KJ; ++q) { orr[q] = f32x4{0.f, 0.f, 0.f, 0.f}; oi[q] = orr[q]; }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            float2 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int t = 2 * s + (e >> 2), i = e & 3;
+                const float2 sv = Sc[(long)(16 * t + 4 * g + i) * W];
+                v[e] = make_float2(ar[t][i] * sv.x - ai[t][i] * sv.y, ar[t][i] * sv.y + ai[t][i] * sv.x);
+            }
+            Op8 b;
+            split8(v, b);
+#pragma unroll
+            for (int q = 0; q < 2 * KJ; ++q)
+                if (16 * q < kp) cmac(orr[q], oi[q], loadA(AF, psF, (long)(16 * q + n) * H + 32 * s + 8 * g), b);
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * KJ; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int jj = 16 * q + 4 * g + i;
+                if (jj < nr) kb[(long)jj * W] = make_float2(orr[q][i], oi[q][i]);
+            }
+    }
+}
+
+template <int MT>
+__global__ __launch_bounds__(64) void k_nesta_B_mfma(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
+                                                     int nslices, int mode) {
+    __shared__ double red[64];
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (mode == 0 && !d.st->active) return;
+    const int local = blockIdx.x - wg_off[si];
+    if (local >= d.wg_B) {
+        if (mode == 0) nesta_U_task(d, local - d.wg_B, red);
+        return;
+    }
+    const double acc = wave_sum(nesta_B_mfma<MT>(d, local, mode));
+    if (mode == 0 && threadIdx.x == 0) d.partB[local] = acc;
+}
+
+template <int MT, int KJ>
+__global__ __launch_bounds__(64) void k_nesta_CA_mfma(const SliceDesc* __restrict__ descs,
+                                                      const int* __restrict__ wg_off, int nslices, int mode) {
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (mode == 0 && !d.st->it_pending) return;
+    nesta_CA_mfma<MT, KJ>(d, blockIdx.x - wg_off[si], mode);
+}
+
+typedef void (*NestaKernel)(const SliceDesc*, const int*, int, int);
+
+NestaKernel pick_B_mfma(int MT) {
+    switch (MT) {
+        case 4: return k_nesta_B_mfma<4>;
+        case 8: return k_nesta_B_mfma<8>;
+        case 12: return k_nesta_B_mfma<12>;
+        case 16: return k_nesta_B_mfma<16>;
+        default: return nullptr;
+    }
+}
+
+NestaKernel pick_CA_mfma(int MT, int KJ) {
+    if (KJ == 1) {
+        switch (MT) {
+            case 4: return k_nesta_CA_mfma<4, 1>;
+            case 8: return k_nesta_CA_mfma<8, 1>;
+            case 12: return k_nesta_CA_mfma<12, 1>;
+            case 16: return k_nesta_CA_mfma<16, 1>;
+            default: return nullptr;
+        }
+    }
+    if (KJ == 2) {
+        switch (MT) {
+            case 4: return k_nesta_CA_mfma<4, 2>;
+            case 8: return k_nesta_CA_mfma<8, 2>;
+            case 12: return k_nesta_CA_mfma<12, 2>;
+            case 16: return k_nesta_CA_mfma<16, 2>;
+            default: return nullptr;
+        }
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+bool mfma_size_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256; }
+
+hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
+    NestaKernel k = pick_B_mfma(L.PW);
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(L.totB), dim3(64), 0, stream, L.dd, L.offB, L.ns, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
+    NestaKernel k = pick_CA_mfma(L.PH, L.kj);
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(L.totCA), dim3(64), 0, stream, L.dd, L.offCA, L.ns, mode);
+    return hipGetLastError();
+}
+
+// DFT matrices of the split-precision passes, computed in double and split into bf16 hi/lo planes
+// (plane order re_hi, re_lo, im_hi, im_lo; plane = M x K entries, row-major).  blockIdx.y = frame for the column kinds.
+//   kind 0  row forward   M = K = W : A[kx][w]    = exp(-2 pi i kx w / W)
+//   kind 1  row inverse   M = K = W : A[w][k]     = exp(+2 pi i kperm(k) w / W)
+//   kind 2  col inverse   M = H, K = kp : A[h][j] = exp(+2 pi i rows_j h / H)          (0 for j >= nrows_f)
+//   kind 3  col forward   M = kp, K = H : A[j][k] = exp(-2 pi i rows_j kperm(k) / H)   (0 for j >= nrows_f)
+__global__ void k_dft_split(uint16_t* out, int kind, int L, int kp, const int16_t* rows, const int* nrows, int nmax) {
+    const int f = blockIdx.y;
+    const int M = kind == 3 ? kp : L, K = kind == 2 ? kp : L;
+    const long plane = (long)M * K;
+    uint16_t* base = out + (kind >= 2 ? (long)f * 4 * plane : 0);
+    const int nr = kind >= 2 ? nrows[f] : 0;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < plane; idx += (long)gridDim.x * blockDim.x) {
+        const int m = (int)(idx / K), k = (int)(idx - (long)m * K);
+        long prod = 0;
+        double sign = 1.0;
+        bool zero = false;
+        switch (kind) {
+            case 0: prod = (long)m * k; sign = -1.0; break;
+            case 1: prod = (long)m * kperm(k); break;
+            case 2: if (k < nr) prod = (long)rows[(long)f * nmax + k] * m; else zero = true; break;
+            default: if (m < nr) prod = (long)rows[(long)f * nmax + m] * kperm(k); else zero = true; sign = -1.0; break;
+        }
+        double sn = 0.0, cs = 0.0;
+        if (!zero) sincospi(2.0 * (double)(prod % L) / (double)L, &sn, &cs);
+        const double re = cs, im = sign * sn;
+        const __bf16 rh = (__bf16)(float)re, ih = (__bf16)(float)im;
+        const __bf16 rl = (__bf16)(float)(re - (double)(float)rh), il = (__bf16)(float)(im - (double)(float)ih);
+        base[idx] = __builtin_bit_cast(uint16_t, rh);
+        base[plane + idx] = __builtin_bit_cast(uint16_t, rl);
+        base[2 * plane + idx] = __builtin_bit_cast(uint16_t, ih);
+        base[3 * plane + idx] = __builtin_bit_cast(uint16_t, il);
+    }
+}
