@@ -39,7 +39,29 @@ STAGES, ITERS = 5, 20
 HBM_PEAK_GBS = 8000.0
 UNEVEN_SLICES = 64
 SERVE_RATE = 50.0
+SPLIT_SHAPE = dict(H=192, W=192, C=16, F=25, accel=8.0)   # split-precision workload, per slice
+SPLIT_BATCH = 16
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_nesta_batched_16x20x8x256.json")
+
+
+def split_mfma_flops(F, C, H, W, nmax):
+    """MFMA flops the split-precision passes execute per inner iteration of one slice: every 16x16x32 complex block
+    is 12 MFMAs of 16*16*32*2 flops.  Pass B: per (frame, coil, 16-row group) a forward and an inverse row DFT of
+    (W/16)(W/32) blocks each; pass CA: per (frame, 16-column tile, coil) an inverse column DFT of (H/16)(kp/32) blocks
+    and a forward one of (H/32)(kp/16) blocks."""
+    blk = 12 * 16 * 16 * 32 * 2
+    kp = (nmax + 31) // 32 * 32
+    groups = (nmax + 15) // 16
+    b = F * C * groups * 2 * (W // 16) * (W // 32)
+    ca = F * (W // 16) * C * ((H // 16) * (kp // 32) + (H // 32) * (kp // 16))
+    return blk * (b + ca)
+
+
+def dense_dft_flops(F, C, H, W):
+    """Algorithmic flops of the dense DFTs of one iteration (forward and adjoint, rows and columns, 8 real flops per
+    complex multiply-add): 16 F C H W (H + W)."""
+    return 16 * F * C * H * W * (H + W)
 
 
 def b_alg(F, C, H, W):
@@ -68,6 +90,12 @@ def _gen_parts(job):
     return synth.make_image_parts(h, w, c, f, accel=ACCEL, slice_id=1000 + i)
 
 
+def _gen_split(i):
+    from opencliper_amd import synth
+    return synth.make_image_parts(SPLIT_SHAPE["H"], SPLIT_SHAPE["W"], SPLIT_SHAPE["C"], SPLIT_SHAPE["F"],
+                                  accel=SPLIT_SHAPE["accel"], slice_id=3000 + i)
+
+
 def gen_inputs(args, workers):
     from opencliper_amd import synth
     jobs3 = list(enumerate(synth.batch_sizes(UNEVEN_SLICES))) if ("uneven" in args.legs or "served" in args.legs) \
@@ -77,9 +105,10 @@ def gen_inputs(args, workers):
     with mp.get_context("fork").Pool(max(1, min(workers, args.batch + len(jobs3)))) as pool:
         r2 = pool.map_async(_gen_cine, range(args.batch))
         r3 = pool.map_async(_gen_parts, jobs3)
-        cine, parts = r2.get(), r3.get()
+        r4 = pool.map_async(_gen_split, range(SPLIT_BATCH if "split" in args.legs else 0))
+        cine, parts, split = r2.get(), r3.get(), r4.get()
     full = synth.make_fully_sampled(H=256, W=256, C=8, F=1, slice_id=5000) if "simple" in args.legs else None
-    return cine, parts, full
+    return cine, parts, full, split
 
 
 def pct(a, q):
@@ -123,10 +152,11 @@ def leg_single(ctx, capi, dev0, prm, form, steps):
     return {"recons_per_s": 1.0 / np.mean(lat), "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99)}, x0.get()
 
 
-def leg_batched(ctx, dev, prm, steps, warmup):
+def leg_batched(ctx, dev, prm, steps, warmup, form=None):
     """Whole schedule captured once into a hipGraph; each step = setup (x_ref, normU, mu0, state) + replay +
     finalize for every slice of the batch."""
-    bt = ctx.batch(dev, prm)
+    from opencliper_amd import capi
+    bt = ctx.batch(dev, prm, form=capi.OCL_FORM_FUSED if form is None else form)
     bt.setup()
     bt.capture(STAGES * ITERS)
     ctx.sync()
@@ -157,22 +187,56 @@ def leg_batched(ctx, dev, prm, steps, warmup):
                         "span_ms_per_iter": ms_tot / (STAGES * ITERS), "bytes_alg_per_iter": byts,
                         "achieved_GBs": byts / (it_ms * 1e-3) / 1e9,
                         "frac_hbm": byts / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    res["reports"] = [bt.report(0)]
     bt.close()
     return res
 
 
 def leg_uneven(ctx, synth, parts, prm, steps, warmup):
+    """64 uneven slices; the tile shape of each H x W comes from the autotuner (tuned at batch setup, outside the
+    timed steps)."""
+    ctx.set_autotune(True)
     dev = []
     for i, (x, S, M) in enumerate(parts):
         y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
         dev.append((y, dS, dM, ctx.empty(x.shape)))
     ctx.sync()
+    t0 = time.perf_counter()
+    tiles = {f"{h}x{h}": ctx.autotune_shape(h, h) for h in sorted({x.shape[1] for (x, _, _) in parts})}
+    tune_s = time.perf_counter() - t0
     res = leg_batched(ctx, dev, prm, steps, warmup)
+    res["autotune"] = {"tune_s": tune_s, "tiles": tiles}
     shapes = [y.shape for (y, _, _, _) in dev]
     res["slices"] = len(dev)
     res["shapes_FCHW"] = shapes
     res["coil_images"] = int(sum(f * c for (f, c, _, _) in shapes))
     return res, dev
+
+
+def leg_split(ctx, synth, parts, prm, steps, warmup):
+    """Split-precision form (BF16-MFMA DFTs, 3-term split) beside the FP32 fused form on the same batch."""
+    from opencliper_amd import capi
+    dev = []
+    for i, (x, S, M) in enumerate(parts):
+        y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 3000 + i)
+        dev.append((y, dS, dM, ctx.empty(x.shape)))
+    ctx.sync()
+    res = {"fp32_fused": leg_batched(ctx, dev, prm, steps, warmup, capi.OCL_FORM_FUSED)}
+    x32 = dev[0][3].get()
+    res["split"] = leg_batched(ctx, dev, prm, steps, warmup, capi.OCL_FORM_SPLIT)
+    xs = dev[0][3].get()
+    sh = SPLIT_SHAPE
+    nmax = int(round(sh["H"] / sh["accel"]))
+    fl = len(dev) * split_mfma_flops(sh["F"], sh["C"], sh["H"], sh["W"], nmax)
+    dense = len(dev) * dense_dft_flops(sh["F"], sh["C"], sh["H"], sh["W"])
+    it = res["split"]["iteration"]
+    it["mfma_flops_per_iter"] = fl
+    it["mfma_TFLOPs"] = fl / (it["ms_iter"] * 1e-3) / 1e12
+    it["frac_mfma"] = it["mfma_TFLOPs"] / MFMA_BF16_PEAK_TFLOPS
+    it["dense_dft_flops_per_iter"] = dense
+    it["dense_dft_TFLOPs_equiv"] = dense / (it["ms_iter"] * 1e-3) / 1e12
+    host = (dev[0][0].get(), dev[0][1].get(), dev[0][2].get())
+    return res, {"x_split": xs, "x_fp32": x32, "slice0": host}
 
 
 def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
@@ -265,7 +329,7 @@ def leg_served_burst(ctx, capi, pool, n_req, budget, chunk, seed=11):
             "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"]}
 
 
-def gpu_legs(args, cine, parts, full):
+def gpu_legs(args, cine, parts, full, split):
     from opencliper_amd import capi, synth
     ctx = capi.Context(0)
     out = {"device": ctx.info()["name"], "x": {}}
@@ -282,6 +346,11 @@ def gpu_legs(args, cine, parts, full):
         out["batched"] = leg_batched(ctx, dev, prm, args.steps, args.warmup)
         out["x"]["batched0"] = dev[0][3].get()
         log(f"batched: {out['batched']['recons_per_s']:.1f} recons/s, iteration {out['batched']['iteration']}")
+        if "split" in args.legs:
+            out["split"], out["x"]["split"] = leg_split(ctx, synth, split, prm, args.steps, args.warmup)
+            sp = out["split"]
+            log(f"split: {sp['split']['recons_per_s']:.1f} recons/s (FP32 fused on the same batch "
+                f"{sp['fp32_fused']['recons_per_s']:.1f}), iteration {sp['split']['iteration']}")
         dev3 = []
         if "uneven" in args.legs:
             out["uneven_batch"], dev3 = leg_uneven(ctx, synth, parts, prm, args.steps, args.warmup)
@@ -352,6 +421,19 @@ def cpu_leg(args, slice0, full, g):
         rl = {form: {"vs_fp64": O.rel_l2(gx[form], x64), "vs_fp32_restatement": O.rel_l2(gx[form], x32)}
               for form in ("plain", "fused", "batched0")}
         rl["fp32_restatement_vs_fp64"] = O.rel_l2(x32, x64)
+        if "split" in gx:
+            sx = gx["split"]
+            yy, SS, MM = sx["slice0"]
+            ref, r64 = O.nesta(yy, SS, MM, prm, double=True)
+            f64 = np.asarray(r64["trace_f"])
+            fs = np.asarray(g["split"]["split"]["reports"][0]["trace_f"])
+            f32 = np.asarray(g["split"]["fp32_fused"]["reports"][0]["trace_f"])
+            rl["split"] = {"shape_FCHW": list(yy.shape), "vs_fp64": O.rel_l2(sx["x_split"], ref),
+                           "vs_fp32_form": O.rel_l2(sx["x_split"], sx["x_fp32"]),
+                           "fp32_form_vs_fp64": O.rel_l2(sx["x_fp32"], ref),
+                           "per_iter_f_relerr_vs_fp64_max": float(np.max(np.abs(fs - f64) / np.abs(f64))),
+                           "per_iter_f_relerr_vs_fp32_max": float(np.max(np.abs(fs - f32) / np.abs(f32))),
+                           "per_iter_f_relerr_vs_fp64": [float(v) for v in np.abs(fs - f64) / np.abs(f64)]}
         if "uneven_small" in gx:
             i, yy, SS, MM, xx = gx["uneven_small"]
             ref, _ = O.nesta(yy, SS, MM, prm, double=True)
@@ -375,8 +457,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--legs", default="simple,uneven,served",
-                    help="comma list of extra legs beside the headline batch: simple, uneven, served")
+    ap.add_argument("--legs", default="simple,split,uneven,served",
+                    help="comma list of extra legs beside the headline batch: simple, split, uneven, served")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson trace (300 = 5 min)")
     ap.add_argument("--serve-budget", type=int, default=64, help="server HBM budget in GiB")
     ap.add_argument("--serve-chunk", type=int, default=5, help="server iterations between admission points")
@@ -391,9 +473,9 @@ def main():
         sys.exit(2)
     workers = min(16, os.cpu_count() or 1)
     t = time.perf_counter()
-    cine, parts, full = gen_inputs(args, workers)
+    cine, parts, full, split = gen_inputs(args, workers)
     log(f"generated inputs in {time.perf_counter() - t:.1f} s")
-    g = gpu_legs(args, cine, parts, full)
+    g = gpu_legs(args, cine, parts, full, split)
     c = {} if args.no_cpu else cpu_leg(args, cine[0], full, g)
     it = g["batched"]["iteration"]
     achieved = it["achieved_GBs"]
@@ -437,11 +519,25 @@ def main():
     if "uneven_batch" in g:
         u = dict(g["uneven_batch"])
         u.pop("shapes_FCHW", None)
+        u.pop("reports", None)
         line["uneven_batch"] = u
     if "served" in g:
         s = dict(g["served"])
         s["speedup_vs_plain"] = s["recons_per_s"] / g["plain"]["recons_per_s"]
         line["served"] = s
+    if "split" in g:
+        sp = g["split"]
+        line["split_precision"] = {
+            "workload": f"{SPLIT_BATCH} slices F={SPLIT_SHAPE['F']} C={SPLIT_SHAPE['C']} "
+                        f"{SPLIT_SHAPE['H']}x{SPLIT_SHAPE['W']} R={SPLIT_SHAPE['accel']:g}, {STAGES} stages x {ITERS} "
+                        f"iterations, hipGraph",
+            "recons_per_s": sp["split"]["recons_per_s"], "ms_per_step": sp["split"]["ms_per_step"],
+            "fp32_fused_recons_per_s": sp["fp32_fused"]["recons_per_s"],
+            "iteration": sp["split"]["iteration"], "fp32_fused_iteration": sp["fp32_fused"]["iteration"],
+            "roofline": {"bound": "mfma", "achieved": sp["split"]["iteration"]["mfma_TFLOPs"],
+                         "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": sp["split"]["iteration"]["frac_mfma"]},
+            "rel_l2": (c.get("rel_l2") or {}).get("split")}
     if "served_burst" in g:
         s = dict(g["served_burst"])
         s["note"] = "same request mix submitted at once: server capacity"

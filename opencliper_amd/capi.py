@@ -118,6 +118,10 @@ def lib():
                           I),
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
+        "ocl_ctx_set_autotune": ([P, I], I),
+        "ocl_ctx_set_tiles": ([P, I, I, I, I, I, I], I),
+        "ocl_autotune_shape": ([P, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
+                                ctypes.POINTER(D), ctypes.POINTER(D)], I),
         "ocl_batch_setup": ([P], I), "ocl_batch_iterate": ([P, I], I), "ocl_batch_capture": ([P, I], I),
         "ocl_batch_replay": ([P], I), "ocl_batch_active": ([P, ctypes.POINTER(I)], I),
         "ocl_batch_finalize": ([P], I), "ocl_batch_report": ([P, I, ctypes.POINTER(NestaReport)], I),
@@ -220,6 +224,24 @@ class Context:
         ms = ctypes.c_double()
         _check(lib().ocl_timer_end(self.handle, h, ctypes.byref(ms)))
         return ms.value
+
+    def set_autotune(self, on=True):
+        """With autotuning on, batches created on this context tune the tile shape of each new H x W at setup."""
+        _check(lib().ocl_ctx_set_autotune(self.handle, 1 if on else 0))
+
+    def set_tiles(self, H, W, ca_fast, tile_cols, b_fast, rows_per_wg):
+        """Pins the tile shape of the FP32 passes for H x W."""
+        _check(lib().ocl_ctx_set_tiles(self.handle, int(H), int(W), int(ca_fast), int(tile_cols), int(b_fast),
+                                       int(rows_per_wg)))
+
+    def autotune_shape(self, H, W):
+        """Tunes (or returns the cached) tile shape of the FP32 passes for H x W."""
+        v = [ctypes.c_int() for _ in range(4)]
+        t = [ctypes.c_double() for _ in range(2)]
+        _check(lib().ocl_autotune_shape(self.handle, int(H), int(W), *[ctypes.byref(x) for x in v],
+                                        *[ctypes.byref(x) for x in t]))
+        return dict(ca_fast=v[0].value, tile_cols=v[1].value, b_fast=v[2].value, rows_per_wg=v[3].value,
+                    ms_CA=t[0].value, ms_B=t[1].value)
 
     # --- memory ---
     def empty(self, shape, dtype=np.complex64) -> DeviceArray:

@@ -67,8 +67,16 @@ size_t lds_CA(const SliceDesc& d) {
                    : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
 }
 
+size_t lds_CA_generic(int H, int tc) { return sizeof(float2) * ((size_t)2 * H * (tc + 1) + H); }
+size_t lds_B_generic(int W, int rb) { return sizeof(float2) * ((size_t)2 * rb * (W + 1) + W); }
+
+int tune_shape(ocl_ctx* c, int H, int W);
+
 // Step 1 of admission: plans, sampled-row lists (one small device round trip) and the exact device footprint.
-int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long total_cols, int prec, SliceRes* r) {
+// force (tuning runs only) fixes the tile shape; otherwise the context's tuned shape for (H, W) is used when there is
+// one (tuned on first use when autotuning is on), else the heuristic of choose_tiles.
+int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long total_cols, int prec, SliceRes* r,
+               const TileCfg* force = nullptr) {
     if (s.F < 1 || s.C < 1 || s.H < 1 || s.W < 1 || !s.y || !s.S || !s.M || !s.x_out)
         return fail(OCL_E_INVALID, "NESTA: slice has bad dimensions or null pointers");
     if (s.H > 512 || s.W > 1024) return fail(OCL_E_UNSUPPORTED, "NESTA: fused form supports H <= 512 and W <= 1024");
@@ -85,10 +93,24 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.fastH = fast_p(s.H);
     d.fastW = fast_p(s.W);
     choose_tiles(total_cols, s, &d.tile_cols, &d.rows_per_wg);
+    if (!prec) {
+        const TileCfg* cfg = force;
+        if (!cfg) {
+            auto it = c->tiles.find({s.H, s.W});
+            if (it == c->tiles.end() && c->autotune) {
+                CHK(tune_shape(c, s.H, s.W));
+                it = c->tiles.find({s.H, s.W});
+            }
+            if (it != c->tiles.end()) cfg = &it->second;
+        }
+        if (cfg && !cfg->ca_fast) { d.fastH = 0; d.tile_cols = cfg->tile_cols; }
+        if (cfg && !cfg->b_fast) { d.fastW = 0; d.rows_per_wg = cfg->rows_per_wg; }
+    }
     if (d.fastH || prec) d.tile_cols = 16;
     if (d.fastW || prec) d.rows_per_wg = 16;
     d.prec = prec;
-    if (lds_B(d) > LDS_MAX || lds_CA(d) > LDS_MAX) return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
+    if (lds_B(d) > LDS_MAX_NESTA || lds_CA(d) > LDS_MAX_NESTA)
+        return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
     // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
     const size_t fb = (size_t)s.F * s.H;
     uint8_t* flags_d = nullptr;
@@ -332,6 +354,126 @@ int table_iterate(ocl_ctx* c, const Table& t, int n) {
     return OCL_OK;
 }
 
+// n iterations with HIP events around pass B (+ scalar step) and pass CA: average ms of each, and the total span.
+int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, double* ms_total) {
+    std::vector<hipEvent_t> ev(3 * (size_t)n);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = OCL_OK;
+    for (int it = 0; it < n && rc == OCL_OK; ++it) {
+        hipError_t e = hipEventRecord(ev[3 * it], c->stream);
+        if (e == hipSuccess) e = launch_nesta_B(t.L, c->stream, 0);
+        if (e == hipSuccess) e = launch_nesta_S(t.L, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 1], c->stream);
+        if (e == hipSuccess) e = launch_nesta_CA(t.L, c->stream, 0);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 2], c->stream);
+        if (e != hipSuccess) rc = fail(OCL_E_HIP, std::string("timing launch: ") + hipGetErrorString(e));
+    }
+    double tb = 0, tc = 0;
+    float tot = 0;
+    if (rc == OCL_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        for (int it = 0; it < n && e == hipSuccess; ++it) {
+            float a = 0, q = 0;
+            e = hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&q, ev[3 * it + 1], ev[3 * it + 2]);
+            tb += a;
+            tc += q;
+        }
+        if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)n - 1]);
+        if (e != hipSuccess) rc = fail(OCL_E_HIP, std::string("timing events: ") + hipGetErrorString(e));
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (rc != OCL_OK) return rc;
+    if (ms_B) *ms_B = tb / n;
+    if (ms_CA) *ms_CA = tc / n;
+    if (ms_total) *ms_total = tot;
+    return OCL_OK;
+}
+
+// Autotuner of the FP32 passes' tile shapes.  For one (H, W) it times every admissible pass-CA variant (register
+// four-step when H = 16 P, LDS Stockham with 4, 8 or 16 columns per workgroup) and every pass-B variant (four-step
+// when W = 16 P, Stockham with 4, 8 or 16 rows per workgroup) on a synthetic batch of 4 slices of 16 frames x 8 coils
+// with a ky-line mask at acceleration 6, and keeps the fastest of each (the passes are timed separately, so the two
+// choices are independent).  Results are cached per context.
+int tune_shape(ocl_ctx* c, int H, int W) {
+    std::vector<std::pair<int, int>> caC, bC;
+    if (fast_p(H) && fast_ca_lds(fast_p(H)) * sizeof(float2) <= LDS_MAX_NESTA) caC.push_back({1, 16});
+    for (int tc : {4, 8, 16})
+        if ((long)H * tc <= (long)OCL_MAXE * OCL_NT && lds_CA_generic(H, tc) <= LDS_MAX_NESTA) caC.push_back({0, tc});
+    if (fast_p(W) && fast_b_lds(fast_p(W)) * sizeof(float2) <= LDS_MAX_NESTA) bC.push_back({1, 16});
+    for (int rb : {4, 8, 16})
+        if (lds_B_generic(W, rb) <= LDS_MAX_NESTA) bC.push_back({0, rb});
+    if (caC.empty() || bC.empty()) return fail(OCL_E_UNSUPPORTED, "autotune: no tile shape fits");
+    const int F = 16, C = 8, NS = 4;
+    const long N = (long)H * W;
+    std::vector<void*> bufs;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(c->stream);
+        for (void* p : bufs) hipFree(p);
+    };
+    auto dalloc = [&](size_t n, void** p) -> int {
+        if (hipMalloc(p, n) != hipSuccess) { (void)hipGetLastError(); return fail(OCL_E_NOMEM, "autotune: out of memory"); }
+        bufs.push_back(*p);
+        return OCL_OK;
+    };
+    void *y, *S, *M;
+    std::vector<void*> xs(NS);
+    int rc = dalloc(sizeof(float2) * F * C * N, &y);
+    if (rc == OCL_OK) rc = dalloc(sizeof(float2) * C * N, &S);
+    if (rc == OCL_OK) rc = dalloc((size_t)F * N, &M);
+    for (int k = 0; k < NS && rc == OCL_OK; ++k) rc = dalloc(sizeof(float2) * F * N, &xs[k]);
+    if (rc != OCL_OK) { cleanup(); return rc; }
+    std::vector<uint8_t> hm((size_t)F * N, 0);
+    for (int f = 0; f < F; ++f)
+        for (int h = 0; h < H; ++h)
+            if (h < 8 || h >= H - 8 || (h + f) % 6 == 0) std::memset(&hm[((size_t)f * H + h) * W], 1, W);
+    hipError_t e = hipMemcpy(M, hm.data(), hm.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { cleanup(); return fail(OCL_E_HIP, std::string("autotune: ") + hipGetErrorString(e)); }
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(F * C * N)), dim3(OCL_NT), 0, c->stream, (float2*)y, (long)F * C * N,
+                       make_float2(0.f, 0.f));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(C * N)), dim3(OCL_NT), 0, c->stream, (float2*)S, (long)C * N,
+                       make_float2(1.0f / std::sqrt((float)C), 0.f));
+    hipLaunchKernelGGL(k_add_noise_masked, dim3(grid_for(F * C * N)), dim3(OCL_NT), 0, c->stream, (float2*)y,
+                       (const uint8_t*)M, F, C, N, 1.0f, 0x5EEDULL);
+    ocl_nesta_params prm{1e-2, 3e-4, 1.0, 1, 3e-4, 0, 1 << 20, 0, 7};
+    TileCfg best;
+    best.ms_CA = best.ms_B = 1e30;
+    const size_t n = std::max(caC.size(), bC.size());
+    for (size_t i = 0; i < n && rc == OCL_OK; ++i) {
+        TileCfg cfg;
+        const auto& a = caC[std::min(i, caC.size() - 1)];
+        const auto& b = bC[std::min(i, bC.size() - 1)];
+        cfg.ca_fast = a.first; cfg.tile_cols = a.second;
+        cfg.b_fast = b.first; cfg.rows_per_wg = b.second;
+        std::vector<std::unique_ptr<SliceRes>> res;
+        std::vector<SliceRes*> view;
+        for (int k = 0; k < NS && rc == OCL_OK; ++k) {
+            ocl_slice sl{y, S, (const uint8_t*)M, xs[k], F, C, H, W};
+            res.emplace_back(new SliceRes());
+            rc = slice_plan(c, sl, prm, (long)NS * F * W, 0, res.back().get(), &cfg);
+            if (rc == OCL_OK) rc = slice_alloc(res.back().get());
+            view.push_back(res.back().get());
+        }
+        Table t;
+        double ms_b = 0, ms_ca = 0;
+        if (rc == OCL_OK) rc = table_build(c, &t, view);
+        if (rc == OCL_OK) rc = slices_setup(c, t, view);
+        if (rc == OCL_OK) rc = table_iterate(c, t, 2);
+        if (rc == OCL_OK) rc = time_table(c, t, 6, &ms_b, &ms_ca, nullptr);
+        if (rc != OCL_OK) break;
+        if (i < caC.size() && ms_ca / NS < best.ms_CA) {
+            best.ca_fast = cfg.ca_fast; best.tile_cols = cfg.tile_cols; best.ms_CA = ms_ca / NS;
+        }
+        if (i < bC.size() && ms_b / NS < best.ms_B) {
+            best.b_fast = cfg.b_fast; best.rows_per_wg = cfg.rows_per_wg; best.ms_B = ms_b / NS;
+        }
+    }
+    cleanup();
+    if (rc != OCL_OK) return rc;
+    c->tiles[{H, W}] = best;
+    return OCL_OK;
+}
+
 int pull_states(ocl_ctx* c, const std::vector<SliceRes*>& v, NState* h) {
     for (size_t i = 0; i < v.size(); ++i)
         HIPCHK(hipMemcpyAsync(&h[i], v[i]->d.st, sizeof(NState), hipMemcpyDeviceToHost, c->stream));
@@ -462,33 +604,47 @@ extern "C" int ocl_batch_iterate(ocl_batch* b, int n_iters) {
 extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total) {
     if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
     if (n_iters < 1) return fail(OCL_E_INVALID, "batch_time: need at least one iteration");
-    ocl_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    return time_table(b->ctx, b->t, n_iters, ms_B, ms_CA, ms_total);
+}
+
+extern "C" int ocl_ctx_set_autotune(ocl_ctx* c, int on) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    c->autotune = on ? 1 : 0;
+    return OCL_OK;
+}
+
+extern "C" int ocl_ctx_set_tiles(ocl_ctx* c, int H, int W, int ca_fast, int tile_cols, int b_fast, int rows_per_wg) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (H < 2 || W < 2 || H > 512 || W > 1024) return fail(OCL_E_INVALID, "set_tiles: bad size");
+    if ((ca_fast && !fast_p(H)) || (b_fast && !fast_p(W)))
+        return fail(OCL_E_INVALID, "set_tiles: the four-step passes need H, W = 16 P with P even, P <= 20");
+    if (!ca_fast && (tile_cols < 1 || tile_cols > 16 || (tile_cols & (tile_cols - 1)) ||
+                     (long)H * tile_cols > (long)OCL_MAXE * OCL_NT || lds_CA_generic(H, tile_cols) > LDS_MAX_NESTA))
+        return fail(OCL_E_INVALID, "set_tiles: tile_cols must be a power of two <= 16 that fits the column pass");
+    if (!b_fast && (rows_per_wg < 1 || rows_per_wg > 16 || (rows_per_wg & (rows_per_wg - 1)) ||
+                    lds_B_generic(W, rows_per_wg) > LDS_MAX_NESTA))
+        return fail(OCL_E_INVALID, "set_tiles: rows_per_wg must be a power of two <= 16 that fits the row pass");
+    TileCfg t;
+    t.ca_fast = ca_fast ? 1 : 0; t.tile_cols = ca_fast ? 16 : tile_cols;
+    t.b_fast = b_fast ? 1 : 0; t.rows_per_wg = b_fast ? 16 : rows_per_wg;
+    c->tiles[{H, W}] = t;
+    return OCL_OK;
+}
+
+extern "C" int ocl_autotune_shape(ocl_ctx* c, int H, int W, int* ca_fast, int* tile_cols, int* b_fast,
+                                  int* rows_per_wg, double* ms_CA, double* ms_B) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (H < 2 || W < 2 || H > 512 || W > 1024) return fail(OCL_E_INVALID, "autotune: bad size");
     HIPCHK(hipSetDevice(c->device));
-    std::vector<hipEvent_t> ev(3 * (size_t)n_iters);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    for (int it = 0; it < n_iters; ++it) {
-        HIPCHK(hipEventRecord(ev[3 * it], c->stream));
-        HIPCHK(launch_nesta_B(b->t.L, c->stream, 0));
-        HIPCHK(launch_nesta_S(b->t.L, c->stream));
-        HIPCHK(hipEventRecord(ev[3 * it + 1], c->stream));
-        HIPCHK(launch_nesta_CA(b->t.L, c->stream, 0));
-        HIPCHK(hipEventRecord(ev[3 * it + 2], c->stream));
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    double tb = 0, tc = 0;
-    for (int it = 0; it < n_iters; ++it) {
-        float a = 0, q = 0;
-        HIPCHK(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
-        HIPCHK(hipEventElapsedTime(&q, ev[3 * it + 1], ev[3 * it + 2]));
-        tb += a;
-        tc += q;
-    }
-    float tot = 0;
-    HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[3 * (size_t)n_iters - 1]));
-    for (auto& e : ev) hipEventDestroy(e);
-    if (ms_B) *ms_B = tb / n_iters;
-    if (ms_CA) *ms_CA = tc / n_iters;
-    if (ms_total) *ms_total = tot;
+    if (!c->tiles.count({H, W})) CHK(tune_shape(c, H, W));
+    const TileCfg& t = c->tiles[{H, W}];
+    if (ca_fast) *ca_fast = t.ca_fast;
+    if (tile_cols) *tile_cols = t.tile_cols;
+    if (b_fast) *b_fast = t.b_fast;
+    if (rows_per_wg) *rows_per_wg = t.rows_per_wg;
+    if (ms_CA) *ms_CA = t.ms_CA;
+    if (ms_B) *ms_B = t.ms_B;
     return OCL_OK;
 }
 

@@ -32,6 +32,14 @@ int fail(int code, const std::string& msg);
         if (r_ != OCL_OK) return r_; \
     } while (0)
 
+// Tile shape of the fused FP32 NESTA passes for one (H, W): the register four-step kernels (fast) or the LDS
+// Stockham kernels with tile_cols columns (pass CA) / rows_per_wg rows (pass B) per workgroup.
+struct TileCfg {
+    int ca_fast = 1, tile_cols = 16;
+    int b_fast = 1, rows_per_wg = 16;
+    double ms_CA = 0.0, ms_B = 0.0;   // measured per slice-iteration of the tuning workload (autotuned entries)
+};
+
 // One HIP device + one in-order stream (the role CLapp plays in src/CLapp.cpp:139-569 of the reference).
 struct ocl_ctx {
     int device = 0;
@@ -43,10 +51,13 @@ struct ocl_ctx {
     std::map<std::pair<int, long>, std::pair<double, int>> normU_cache;
     std::mutex mu;
     int num_cus = 256;
+    int autotune = 0;                                       // 1: tune the tile shape of each new (H, W) on first use
+    std::map<std::pair<int, int>, TileCfg> tiles;           // tuned tile shapes by (H, W)
 };
 
 static const int RED_PARTS = 1024;
-static const size_t LDS_MAX = 64 * 1024;   // per-workgroup dynamic LDS budget of every launch
+static const size_t LDS_MAX = 64 * 1024;          // per-workgroup dynamic LDS budget of the standalone FFT launches
+static const size_t LDS_MAX_NESTA = 160 * 1024;   // per-workgroup LDS of the fused NESTA passes (gfx950: 160 KB)
 
 int grid_for(long n);
 int make_plan(ocl_ctx* c, int n, FftPlan* P);

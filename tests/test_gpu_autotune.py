@@ -1,0 +1,69 @@
+"""Tile-shape autotuner of the FP32 fused passes: every variant it can choose reconstructs within the parity bar,
+and batches on an autotuning context use the tuned shape."""
+import numpy as np
+import pytest
+
+from opencliper_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, sl, prm):
+    y, S, M = ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M)
+    x = ctx.empty((sl.F, sl.H, sl.W))
+    rep = ctx.nesta(y, S, M, x, prm, form=capi.OCL_FORM_FUSED)
+    return x.get(), rep
+
+
+# (ca_fast, tile_cols, b_fast, rows_per_wg) on a 160 x 96 slice: H = 16 * 10 and W = 16 * 6 allow every variant
+VARIANTS = [(1, 16, 1, 16), (0, 4, 0, 4), (0, 8, 1, 16), (1, 16, 0, 8), (0, 16, 0, 16)]
+
+
+@pytest.mark.parametrize("v", VARIANTS)
+def test_every_tile_variant_matches_oracle(ctx, oracle, v):
+    sl = synth.make_cine(H=160, W=96, C=3, F=5, accel=5.0, slice_id=51)
+    prm = dict(maxIntIter=3, maxIter=8, stoptest=0)
+    ref, _ = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    print("tile variant", v, flush=True)
+    ctx.set_tiles(160, 96, *v)
+    got, rep = run(ctx, sl, capi.nesta_params(**prm))
+    err = oracle.rel_l2(got, ref)
+    print(f"variant {v}: rel-L2 vs FP64 {err:.2e}", flush=True)
+    assert rep["iters"] == [8] * 3
+    assert err < 1e-4
+
+
+def test_autotune_picks_a_valid_shape_and_is_used(ctx, oracle):
+    c2 = capi.Context(0)
+    try:
+        t = c2.autotune_shape(192, 128)
+        print("tuned 192x128:", t, flush=True)
+        assert t["ms_CA"] > 0 and t["ms_B"] > 0
+        assert (t["ca_fast"], t["tile_cols"]) in {(1, 16), (0, 4), (0, 8), (0, 16)}
+        assert (t["b_fast"], t["rows_per_wg"]) in {(1, 16), (0, 4), (0, 8), (0, 16)}
+        assert c2.autotune_shape(192, 128) == t          # cached
+        c2.set_autotune(True)
+        sl = synth.make_cine(H=224, W=224, C=4, F=6, accel=6.0, slice_id=52)
+        prm = dict(maxIntIter=3, maxIter=6, stoptest=0)
+        ref, _ = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+        got, _ = run(c2, sl, capi.nesta_params(**prm))
+        assert oracle.rel_l2(got, ref) < 1e-4
+        t2 = c2.autotune_shape(224, 224)                  # tuned during the batch setup above
+        print("tuned 224x224:", t2, flush=True)
+        assert t2["ms_CA"] > 0
+    finally:
+        c2.close()
+
+
+def test_set_tiles_rejects_bad_shapes(ctx):
+    with pytest.raises(capi.OclError):
+        ctx.set_tiles(100, 96, 1, 16, 1, 16)      # 100 is not 16 P
+    with pytest.raises(capi.OclError):
+        ctx.set_tiles(160, 96, 0, 3, 1, 16)       # not a power of two
