@@ -148,13 +148,14 @@ int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, doubl
 int ocl_batch_destroy(ocl_batch* b);
 
 /* ---- tile-shape autotuner of the FP32 fused passes (no reference counterpart: clFFT bakes its own plans,
- *      src/processes/FFT.cpp:430).  For one H x W it times each admissible column-pass variant (register four-step,
- *      LDS Stockham with 4/8/16 columns per workgroup) and row-pass variant (four-step, Stockham with 4/8/16 rows)
+ *      src/processes/FFT.cpp:430).  For one H x W it times each admissible column-pass variant (register four-step
+ *      with direct and prefetched sampled-row reads, LDS Stockham with 4/8/16 columns per workgroup) and row-pass variant (four-step, Stockham with 4/8/16 rows)
  *      on a synthetic 4-slice batch and caches the fastest per context.  With autotuning on, batches created on the
  *      context tune every new H x W at setup; otherwise tuned shapes are used when present, else a heuristic. ---- */
 int ocl_ctx_set_autotune(ocl_ctx* ctx, int on);
-/* pins the tile shape of H x W (e.g. a saved tuning): ca_fast / b_fast select the four-step passes, else tile_cols /
- * rows_per_wg (powers of two <= 16) size the Stockham passes */
+/* pins the tile shape of H x W (e.g. a saved tuning): ca_fast 1 / 2 selects the four-step column pass that gathers
+ * the sampled rows from global memory / prefetches the next coil's rows and sensitivities, b_fast 1 the four-step
+ * row pass; 0 selects the Stockham passes sized by tile_cols / rows_per_wg (powers of two <= 16) */
 int ocl_ctx_set_tiles(ocl_ctx* ctx, int H, int W, int ca_fast, int tile_cols, int b_fast, int rows_per_wg);
 int ocl_autotune_shape(ocl_ctx* ctx, int H, int W, int* ca_fast, int* tile_cols, int* b_fast, int* rows_per_wg,
                        double* ms_CA_per_slice_iter, double* ms_B_per_slice_iter);

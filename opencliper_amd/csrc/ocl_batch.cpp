@@ -63,7 +63,7 @@ size_t lds_B(const SliceDesc& d) {
                    : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W);
 }
 size_t lds_CA(const SliceDesc& d) {
-    return d.fastH ? sizeof(float2) * fast_ca_lds(d.fastH)
+    return d.fastH ? sizeof(float2) * (fast_ca_lds(d.fastH) + (d.ca_stage ? 2 * 16 * (size_t)d.nmax : 0))
                    : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
 }
 
@@ -104,6 +104,8 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
             if (it != c->tiles.end()) cfg = &it->second;
         }
         if (cfg && !cfg->ca_fast) { d.fastH = 0; d.tile_cols = cfg->tile_cols; }
+        // the prefetching four-step column pass unless a tuned or pinned shape asks for the direct one
+        d.ca_stage = (d.fastH && (!cfg || cfg->ca_fast == 2)) ? 1 : 0;
         if (cfg && !cfg->b_fast) { d.fastW = 0; d.rows_per_wg = cfg->rows_per_wg; }
     }
     if (d.fastH || prec) d.tile_cols = 16;
@@ -140,6 +142,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
             }
     }
     d.nmax = nmax;
+    if (d.ca_stage && (nmax > 16 * FAST_CA_KQ || lds_CA(d) > LDS_MAX_NESTA)) d.ca_stage = 0;
     d.kp = (nmax + 31) / 32 * 32;
     if (prec && d.kp > 64)
         return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports at most 64 sampled rows per frame");
@@ -253,12 +256,14 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
     L.kj = 1;
+    L.stage = ns ? v[0]->d.ca_stage : 0;
     for (int i = 0; i < ns; ++i) {
         const SliceDesc& d = v[i]->d;
         if (d.prec != L.prec) return fail(OCL_E_INVALID, "batch: slices of one table must share the precision form");
         if (L.prec && (d.H != v[0]->d.H || d.W != v[0]->d.W))
             return fail(OCL_E_UNSUPPORTED, "batch: split-precision slices of one table must share H and W");
         L.kj = std::max(L.kj, d.kp / 32);
+        if (d.ca_stage != L.stage) L.stage = 2;
         hd[i] = d;
         oB[i + 1] = oB[i] + d.wg_B + d.wg_U;
         oC[i + 1] = oC[i] + d.wg_CA;
@@ -397,7 +402,10 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
 // choices are independent).  Results are cached per context.
 int tune_shape(ocl_ctx* c, int H, int W) {
     std::vector<std::pair<int, int>> caC, bC;
-    if (fast_p(H) && fast_ca_lds(fast_p(H)) * sizeof(float2) <= LDS_MAX_NESTA) caC.push_back({1, 16});
+    if (fast_p(H) && fast_ca_lds(fast_p(H)) * sizeof(float2) <= LDS_MAX_NESTA) {
+        caC.push_back({1, 16});
+        caC.push_back({2, 16});
+    }
     for (int tc : {4, 8, 16})
         if ((long)H * tc <= (long)OCL_MAXE * OCL_NT && lds_CA_generic(H, tc) <= LDS_MAX_NESTA) caC.push_back({0, tc});
     if (fast_p(W) && fast_b_lds(fast_p(W)) * sizeof(float2) <= LDS_MAX_NESTA) bC.push_back({1, 16});
@@ -625,8 +633,9 @@ extern "C" int ocl_ctx_set_tiles(ocl_ctx* c, int H, int W, int ca_fast, int tile
     if (!b_fast && (rows_per_wg < 1 || rows_per_wg > 16 || (rows_per_wg & (rows_per_wg - 1)) ||
                     lds_B_generic(W, rows_per_wg) > LDS_MAX_NESTA))
         return fail(OCL_E_INVALID, "set_tiles: rows_per_wg must be a power of two <= 16 that fits the row pass");
+    if (ca_fast < 0 || ca_fast > 2) return fail(OCL_E_INVALID, "set_tiles: ca_fast must be 0, 1 or 2");
     TileCfg t;
-    t.ca_fast = ca_fast ? 1 : 0; t.tile_cols = ca_fast ? 16 : tile_cols;
+    t.ca_fast = ca_fast; t.tile_cols = ca_fast ? 16 : tile_cols;
     t.b_fast = b_fast ? 1 : 0; t.rows_per_wg = b_fast ? 16 : rows_per_wg;
     c->tiles[{H, W}] = t;
     return OCL_OK;

@@ -33,9 +33,11 @@ int fail(int code, const std::string& msg);
     } while (0)
 
 // Tile shape of the fused FP32 NESTA passes for one (H, W): the register four-step kernels (fast) or the LDS
-// Stockham kernels with tile_cols columns (pass CA) / rows_per_wg rows (pass B) per workgroup.
+// Stockham kernels with tile_cols columns (pass CA) / rows_per_wg rows (pass B) per workgroup.  ca_fast: 0 Stockham,
+// 1 four-step gathering the sampled rows from global memory, 2 four-step prefetching the next coil's sampled rows
+// (through LDS) and sensitivities.
 struct TileCfg {
-    int ca_fast = 1, tile_cols = 16;
+    int ca_fast = 2, tile_cols = 16;
     int b_fast = 1, rows_per_wg = 16;
     double ms_CA = 0.0, ms_B = 0.0;   // measured per slice-iteration of the tuning workload (autotuned entries)
 };

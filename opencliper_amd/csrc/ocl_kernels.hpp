@@ -45,6 +45,7 @@ struct SliceDesc {
     int u_chunk;        // pixels per U workgroup (per frame)
     float inv_sqrtN;
     int fastH, fastW;   // P with H (W) = 16 P when the register four-step path applies, else 0
+    int ca_stage;       // four-step pass CA with the next coil's sampled rows and S prefetched (nmax <= 64)
     FftPlan ph, pw;     // column (length H) and row (length W) plans
     const float2* y;    // [F][C][H][W]
     const float2* S;    // [C][H][W]
@@ -112,6 +113,7 @@ struct NestaLaunch {
     int maxe, PH, PW;
     int prec;           // 1: every slice uses the split-precision passes (then PH = H/16, PW = W/16)
     int kj;             // split-precision passes: largest kp / 32 of the table
+    int stage;          // 0 / 1: every slice uses the direct / prefetching four-step pass CA, 2: mixed
 };
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);

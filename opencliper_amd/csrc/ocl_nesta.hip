@@ -315,7 +315,16 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
 
 }
 
-template <int PHU, int MAXE>
+// STG: 0 / 1 every slice of the table uses the direct / prefetching four-step column pass, 2 either (per slice)
+template <int P, int STG>
+__device__ __forceinline__ void nesta_CA_fast_any(const SliceDesc& d, int local, int mode, float2* lds) {
+    if constexpr (STG == 0) nesta_CA_fast<P, false>(d, local, mode, lds);
+    else if constexpr (STG == 1) nesta_CA_fast<P, true>(d, local, mode, lds);
+    else if (d.ca_stage) nesta_CA_fast<P, true>(d, local, mode, lds);
+    else nesta_CA_fast<P, false>(d, local, mode, lds);
+}
+
+template <int PHU, int MAXE, int STG>
 __device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
                                               int nslices, int mode) {
     extern __shared__ float2 lds[];
@@ -325,29 +334,29 @@ __device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ desc
     if (mode == 0 && !st->it_pending) return;
     const int local = blockIdx.x - wg_off[si];
     if constexpr (PHU > 0) {
-        nesta_CA_fast<PHU>(d, local, mode, lds);
+        nesta_CA_fast_any<PHU, STG>(d, local, mode, lds);
     } else if (d.fastH) {
         switch (d.fastH) {
-            case 2: nesta_CA_fast<2>(d, local, mode, lds); break;
-            case 4: nesta_CA_fast<4>(d, local, mode, lds); break;
-            case 6: nesta_CA_fast<6>(d, local, mode, lds); break;
-            case 8: nesta_CA_fast<8>(d, local, mode, lds); break;
-            case 10: nesta_CA_fast<10>(d, local, mode, lds); break;
-            case 12: nesta_CA_fast<12>(d, local, mode, lds); break;
-            case 14: nesta_CA_fast<14>(d, local, mode, lds); break;
-            case 16: nesta_CA_fast<16>(d, local, mode, lds); break;
-            case 18: nesta_CA_fast<18>(d, local, mode, lds); break;
-            default: nesta_CA_fast<20>(d, local, mode, lds); break;
+            case 2: nesta_CA_fast_any<2, STG>(d, local, mode, lds); break;
+            case 4: nesta_CA_fast_any<4, STG>(d, local, mode, lds); break;
+            case 6: nesta_CA_fast_any<6, STG>(d, local, mode, lds); break;
+            case 8: nesta_CA_fast_any<8, STG>(d, local, mode, lds); break;
+            case 10: nesta_CA_fast_any<10, STG>(d, local, mode, lds); break;
+            case 12: nesta_CA_fast_any<12, STG>(d, local, mode, lds); break;
+            case 14: nesta_CA_fast_any<14, STG>(d, local, mode, lds); break;
+            case 16: nesta_CA_fast_any<16, STG>(d, local, mode, lds); break;
+            case 18: nesta_CA_fast_any<18, STG>(d, local, mode, lds); break;
+            default: nesta_CA_fast_any<20, STG>(d, local, mode, lds); break;
         }
     } else {
         nesta_CA_generic<MAXE>(d, local, mode, lds);
     }
 }
 
-template <int PHU, int MAXE>
+template <int PHU, int MAXE, int STG>
 __global__ __launch_bounds__(OCL_NT) void k_nesta_CA_t(const SliceDesc* __restrict__ descs,
                                                        const int* __restrict__ wg_off, int nslices, int mode) {
-    nesta_CA_body<PHU, MAXE>(descs, wg_off, nslices, mode);
+    nesta_CA_body<PHU, MAXE, STG>(descs, wg_off, nslices, mode);
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -368,19 +377,16 @@ static NestaKernel pick_B(int PW) {
     }
 }
 
-static NestaKernel pick_CA(int PH, int maxe) {
+static NestaKernel pick_CA(int PH, int maxe, int stg) {
+#define OCL_CA_P(P)                                                                                   \
+    case P: return stg == 0 ? k_nesta_CA_t<P, 4, 0> : stg == 1 ? k_nesta_CA_t<P, 4, 1> : k_nesta_CA_t<P, 4, 2>;
     switch (PH) {
-        case 8: return k_nesta_CA_t<8, 4>;
-        case 10: return k_nesta_CA_t<10, 4>;
-        case 12: return k_nesta_CA_t<12, 4>;
-        case 14: return k_nesta_CA_t<14, 4>;
-        case 16: return k_nesta_CA_t<16, 4>;
-        case 18: return k_nesta_CA_t<18, 4>;
-        case 20: return k_nesta_CA_t<20, 4>;
+        OCL_CA_P(8) OCL_CA_P(10) OCL_CA_P(12) OCL_CA_P(14) OCL_CA_P(16) OCL_CA_P(18) OCL_CA_P(20)
         default: break;
     }
-    return maxe <= 4 ? k_nesta_CA_t<0, 4> : maxe <= 8 ? k_nesta_CA_t<0, 8> : maxe <= 16 ? k_nesta_CA_t<0, 16>
-                                                                                      : k_nesta_CA_t<0, 32>;
+#undef OCL_CA_P
+    return maxe <= 4 ? k_nesta_CA_t<0, 4, 2> : maxe <= 8 ? k_nesta_CA_t<0, 8, 2> : maxe <= 16 ? k_nesta_CA_t<0, 16, 2>
+                                                                                         : k_nesta_CA_t<0, 32, 2>;
 }
 
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode) {
@@ -396,7 +402,7 @@ hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream) {
 
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode) {
     if (L.prec) return launch_nesta_CA_mfma(L, stream, mode);
-    hipLaunchKernelGGL(pick_CA(L.PH, L.maxe), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
+    hipLaunchKernelGGL(pick_CA(L.PH, L.maxe, L.stage), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }
 
@@ -405,7 +411,9 @@ void nesta_kernel_attributes(int lds_bytes) {
     for (int p : ps) {
         (void)hipFuncSetAttribute((const void*)pick_B(p), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
         for (int e : {4, 8, 16, 32})
-            (void)hipFuncSetAttribute((const void*)pick_CA(p, e), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+            for (int g : {0, 1, 2})
+                (void)hipFuncSetAttribute((const void*)pick_CA(p, e, g), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          lds_bytes);
     }
     (void)hipGetLastError();
 }

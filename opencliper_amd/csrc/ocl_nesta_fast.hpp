@@ -6,6 +6,7 @@
 
 // LDS footprint (float2 elements) of the fast passes
 #define FAST_CA_KS 272                          // per-k1 stride of the column exchange: 16 roles x 16 columns + pad
+#define FAST_CA_KQ 4                            // staged k-space values per thread and coil (nmax <= 64)
 #define FAST_B_RS(P) ((P) * 17 + 1)             // per-row stride of the row exchange
 __host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + 16 * P + 4 * P; }
 __host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + 16 * P; }
@@ -122,10 +123,16 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
 // ---------------------------------------------------------------------------------------------------------------------
 // pass CA, columns of length H = 16 P: 16 columns per workgroup, 16 roles per column
 // ---------------------------------------------------------------------------------------------------------------------
-template <int P>
+// STAGE = true adds two prefetches that keep global-memory latency off each coil's critical path: the sampled
+// k-space rows of the next coil are loaded cooperatively (coalesced, at most FAST_CA_KQ values per thread) while the
+// current coil is transformed and parked in one of two LDS slabs, and the S values of the next coil's forward pass
+// are issued before the current coil's exchange barrier.  It needs nmax <= 16 FAST_CA_KQ and 2 * 16 * nmax more
+// float2 of LDS.  STAGE = false gathers the sampled rows straight from global memory.
+template <int P, bool STAGE>
 __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int mode, float2* lds) {
     constexpr int H = 16 * P;
     const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
+    float2* X = lds + s;
     const int tile = local % d.ncol_tiles;
     const int f = local / d.ncol_tiles;
     const int F = d.F, C = d.C, W = d.W, nmax = d.nmax;
@@ -135,9 +142,31 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     const int wc = wv ? w : 0;
     float2* tw = lds + P * FAST_CA_KS;
     int16_t* rp = (int16_t*)(tw + H);
+    float2* kst = tw + H + 4 * P;                      // STAGE: slabs [2][nmax][16]
+    const int nst = d.nrows[f] * 16;                   // staged values per coil
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
         tw[i] = d.ph.tw[i];
         rp[i] = d.rowpos[(long)f * H + i];
+    }
+    float2 pre[FAST_CA_KQ];
+    auto stage_load = [&](int c) {
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + tile * 16;
+#pragma unroll
+        for (int q = 0; q < FAST_CA_KQ; ++q) {
+            const int e = threadIdx.x + OCL_NT * q;
+            pre[q] = (e < nst && tile * 16 + (e & 15) < W) ? kb[(long)(e >> 4) * W + (e & 15)] : make_float2(0.f, 0.f);
+        }
+    };
+    auto stage_store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < FAST_CA_KQ; ++q) {
+            const int e = threadIdx.x + OCL_NT * q;
+            if (e < nst) kst[buf * nmax * 16 + e] = pre[q];
+        }
+    };
+    if constexpr (STAGE) {
+        stage_load(0);
+        stage_store(0);
     }
     __syncthreads();
 
@@ -150,7 +179,11 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     float2 sv[P];
     for (int c = 0; c < C; ++c) {
         const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        const float2* ks = kst + (c & 1) * nmax * 16 + s;
         const float2* Sc = d.S + (long)c * N + wc;
+        if constexpr (STAGE) {
+            if (c + 1 < C) stage_load(c + 1);
+        }
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) sv[n1] = Sc[(long)(t + 16 * n1) * W];
 #pragma unroll
@@ -161,18 +194,23 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
                     const int jj = rp[k1 + P * k2];
-                    u[k2] = (jj >= 0 && wv) ? c_conj(kb[(long)jj * W + w]) : make_float2(0.f, 0.f);
+                    if constexpr (STAGE) u[k2] = jj >= 0 ? c_conj(ks[jj * 16]) : make_float2(0.f, 0.f);
+                    else u[k2] = (jj >= 0 && wv) ? c_conj(kb[(long)jj * W + w]) : make_float2(0.f, 0.f);
                 }
                 rf::rdft<16>(u);
 #pragma unroll
                 for (int n2 = 0; n2 < 16; ++n2)
-                    lds[k1 * FAST_CA_KS + n2 * 16 + s] = (k1 == 0 || n2 == 0) ? u[n2] : c_mul(u[n2], tw[k1 * n2]);
+                    X[k1 * FAST_CA_KS + n2 * 16] = (k1 == 0 || n2 == 0) ? u[n2] : c_mul(u[n2], tw[k1 * n2]);
             }
         }
         __syncthreads();
         float2 v[P];
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) v[k1] = lds[k1 * FAST_CA_KS + t * 16 + s];
+        for (int k1 = 0; k1 < P; ++k1) v[k1] = X[k1 * FAST_CA_KS + t * 16];
+        if constexpr (STAGE) {
+            // the other slab was last read in the previous coil's first phase, which the barrier above has closed
+            if (c + 1 < C) stage_store((c + 1) & 1);
+        }
         __syncthreads();
         rf::rdft<P>(v);
 #pragma unroll
@@ -202,15 +240,28 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     }
 
     // ---- forward column DFTs of S_c x_{k+1} (FA), output pruned to the sampled rows ----
+    float2 sn[P];                                          // STAGE: S of the coil about to be transformed
+    if constexpr (STAGE) {
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) sn[n1] = d.S[(long)(t + 16 * n1) * W + wc];
+    }
     for (int c = 0; c < C; ++c) {
         const float2* Sc = d.S + (long)c * N + wc;
         float2 v[P];
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) v[n1] = wv ? c_mul(xn[n1], Sc[(long)(t + 16 * n1) * W]) : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < P; ++n1) {
+            const float2 sc1 = STAGE ? sn[n1] : Sc[(long)(t + 16 * n1) * W];
+            v[n1] = wv ? c_mul(xn[n1], sc1) : make_float2(0.f, 0.f);
+        }
         rf::rdft<P>(v);
 #pragma unroll
         for (int k1 = 0; k1 < P; ++k1)
-            lds[k1 * FAST_CA_KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+            X[k1 * FAST_CA_KS + t * 16] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+        if constexpr (STAGE) {
+            if (c + 1 < C)
+#pragma unroll
+                for (int n1 = 0; n1 < P; ++n1) sn[n1] = Sc[N + (long)(t + 16 * n1) * W];
+        }
         __syncthreads();
         float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
 #pragma unroll
@@ -219,7 +270,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             if (k1 < P) {
                 float2 u[16];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) u[i] = lds[k1 * FAST_CA_KS + i * 16 + s];
+                for (int i = 0; i < 16; ++i) u[i] = X[k1 * FAST_CA_KS + i * 16];
                 rf::rdft<16>(u);
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
