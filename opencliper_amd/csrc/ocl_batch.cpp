@@ -149,7 +149,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
     d.wg_B = s.F * s.C * d.nrow_groups;
-    d.wg_CA = s.F * d.ncol_tiles;
+    d.wg_CA = prec ? s.F * ((d.ncol_tiles + MF_CA_TILES - 1) / MF_CA_TILES) : s.F * d.ncol_tiles;
     d.u_chunk = (int)std::min<long>(d.N, 4096);
     d.wg_U = s.F * (int)((d.N + d.u_chunk - 1) / d.u_chunk);
     d.inv_sqrtN = (float)(1.0 / std::sqrt((double)d.N));
@@ -287,7 +287,10 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     if (L.prec && ns) {
         L.PH = v[0]->d.H / 16;
         L.PW = v[0]->d.W / 16;
-        L.ldsB = L.ldsCA = 0;
+        L.ldsB = 0;
+        L.ldsCA = 0;
+        for (int i = 0; i < ns; ++i) L.ldsCA = std::max(L.ldsCA, mf_ca_lds_bytes(v[i]->d.H, v[i]->d.kp));
+        if (L.ldsCA > LDS_MAX_NESTA) return fail(OCL_E_UNSUPPORTED, "batch: split-precision column tile exceeds LDS");
     }
     if (ns) {
         HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));

@@ -46,6 +46,7 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (e == hipSuccess) {
         for (const void* k : lds_kernels) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
         nesta_kernel_attributes((int)LDS_MAX_NESTA);
+        mfma_kernel_attributes((int)LDS_MAX_NESTA);
     }
     (void)hipGetLastError();
     if (e != hipSuccess) {

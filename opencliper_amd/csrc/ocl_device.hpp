@@ -18,6 +18,30 @@ struct FftPlan {
 };
 
 // ------------------------------------------------------------------------------------------------------------------
+// global-memory access at a 32-bit element index from a wave-uniform base: the pointer is taken as a global-segment
+// pointer and the index as an unsigned 32-bit offset, so the loads and stores use the scalar-base + 32-bit-offset
+// global forms instead of flat accesses with 64-bit address arithmetic.  Callers keep base + index inside one array
+// of less than 4 GiB (bases advance per frame / coil).
+// ------------------------------------------------------------------------------------------------------------------
+typedef float ocl_f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const char ocl_gcchar;
+typedef __attribute__((address_space(1))) char ocl_gchar;
+
+__device__ __forceinline__ float2 ldg2(const float2* base, unsigned i) {
+    const ocl_f2v v = *(__attribute__((address_space(1))) const ocl_f2v*)((ocl_gcchar*)base + i * 8u);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stg2(float2* base, unsigned i, float2 x) {
+    ocl_f2v v;
+    v.x = x.x;
+    v.y = x.y;
+    *(__attribute__((address_space(1))) ocl_f2v*)((ocl_gchar*)base + i * 8u) = v;
+}
+__device__ __forceinline__ uint8_t ldg1(const uint8_t* base, unsigned i) {
+    return *(__attribute__((address_space(1))) const uint8_t*)((ocl_gcchar*)base + i);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
 // complex helpers
 // ------------------------------------------------------------------------------------------------------------------
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }

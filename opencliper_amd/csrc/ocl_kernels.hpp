@@ -123,6 +123,14 @@ void nesta_kernel_attributes(int lds_bytes);
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
 bool mfma_size_supported(int n);      // H or W the split-precision passes are instantiated for
+void mfma_kernel_attributes(int lds_bytes);
+#define MF_CA_TILES 4                  // column tiles (one wavefront each) per workgroup of the split-precision pass CA
+// LDS row stride (bf16 elements) of a staged DFT matrix with K columns: the 8-element pad keeps the 16-byte fragment
+// reads of a quarter-wavefront on distinct banks
+__host__ __device__ constexpr int mf_lds_ld(int K) { return K + 8; }
+__host__ __device__ constexpr size_t mf_ca_lds_bytes(int H, int kp) {
+    return 2 * 4 * (size_t)((H * mf_lds_ld(kp)) > (kp * mf_lds_ld(H)) ? H * mf_lds_ld(kp) : kp * mf_lds_ld(H));
+}
 // fills one DFT matrix of the split-precision passes (kind: 0 row forward, 1 row inverse, 2 column inverse,
 // 3 column forward); rows/nrows/nmax describe the sampled rows of each frame for the column kinds
 __global__ void k_dft_split(uint16_t* out, int kind, int L, int kp, const int16_t* rows, const int* nrows, int nmax);

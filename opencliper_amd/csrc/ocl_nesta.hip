@@ -279,15 +279,14 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
             }
         }
     } else {
-        const UpdCoef u = upd_coef(d);
-        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+        const UpdRow u = upd_row(d, f);
 #pragma unroll
         for (int i = 0; i < MAXE; ++i) {
             const int e = threadIdx.x + i * OCL_NT;
             xn[i] = make_float2(0.f, 0.f);
             if (e < nel) {
                 const int h = e >> lsh, wl = e & (TW - 1);
-                if (w0 + wl < W) xn[i] = nesta_update_px(d, u, f, fm, fp, (long)h * W + w0 + wl, c_scale(acc[i], s));
+                if (w0 + wl < W) xn[i] = nesta_update_at(u, (unsigned)(h * W + w0 + wl), c_scale(acc[i], s));
             }
         }
     }

@@ -24,8 +24,10 @@ __device__ __forceinline__ void nesta_U_task(const SliceDesc& d, int uid, double
     const float2* xk = st->cur ? d.x1 : d.x0;
     const float mu = (float)st->mu;
     double v[2] = {0.0, 0.0};
+    const float2* xf = xk + (long)f * d.N;
+    const float2* xm = xk + (long)fm * d.N;
     for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        const float2 ux = c_sub(xk[(long)f * d.N + p], xk[(long)fm * d.N + p]);
+        const float2 ux = c_sub(ldg2(xf, (unsigned)p), ldg2(xm, (unsigned)p));
         const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
         const float2 u = make_float2(ux.x / nf, ux.y / nf);
         v[0] += (double)u.x * u.x + (double)u.y * u.y;
@@ -57,32 +59,61 @@ __device__ __forceinline__ UpdCoef upd_coef(const SliceDesc& d) {
     return u;
 }
 
-// NESTA update of one pixel p of frame f (NestaUp.cpp:696-875): df = lambda (u_f - u_{f+1}) + A^H r (the temporal-TV
-// gradient D^T u with the periodic wrap), y_k = x_k - df / L_mu, W_k += apk df, z_k = x_ref - W_k / L_mu,
+// The same coefficients with the frame-f rows of every image buffer resolved once per workgroup (wave-uniform bases for
+// 32-bit indexed global accesses).
+struct UpdRow {
+    const float2 *xf, *xm, *xp, *xref_in;
+    float2 *xref_out, *Wk, *xo;
+    float mu, lam, L1, apk, tau;
+    int sstart;
+};
+
+__device__ __forceinline__ UpdRow upd_row(const SliceDesc& d, int f) {
+    const UpdCoef u = upd_coef(d);
+    const int fm = f == 0 ? d.F - 1 : f - 1, fp = f == d.F - 1 ? 0 : f + 1;
+    UpdRow r;
+    r.xf = u.xk + (long)f * d.N;
+    r.xm = u.xk + (long)fm * d.N;
+    r.xp = u.xk + (long)fp * d.N;
+    r.xref_in = d.xref + (long)f * d.N;
+    r.xref_out = d.xref + (long)f * d.N;
+    r.Wk = d.Wk + (long)f * d.N;
+    r.xo = u.xo + (long)f * d.N;
+    r.mu = u.mu; r.lam = u.lam; r.L1 = u.L1; r.apk = u.apk; r.tau = u.tau;
+    r.sstart = u.sstart;
+    return r;
+}
+
+// NESTA update of pixel ub + p of the frame resolved in r0 (ub wave-uniform, p per lane) (NestaUp.cpp:696-875): df = lambda (u_f - u_{f+1}) + A^H r (the
+// temporal-TV gradient D^T u with the periodic wrap), y_k = x_k - df / L_mu, W_k += apk df, z_k = x_ref - W_k / L_mu,
 // x_{k+1} = tau z_k + (1 - tau) y_k.  The first iteration of a stage sets x_ref = x_k and W_k = 0.
 // Writes W_k, x_ref (stage start) and x_{k+1}; returns x_{k+1}.
-__device__ __forceinline__ float2 nesta_update_px(const SliceDesc& d, const UpdCoef& u, int f, int fm, int fp, long p,
-                                                  float2 ares) {
-    const long N = d.N;
-    const float2 xf = u.xk[(long)f * N + p], xm = u.xk[(long)fm * N + p], xp = u.xk[(long)fp * N + p];
+__device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, unsigned p, float2 ares) {
+    UpdRow r = r0;
+    r.xf += ub; r.xm += ub; r.xp += ub; r.xref_in += ub; r.xref_out += ub; r.Wk += ub; r.xo += ub;
+    const float2 xf = ldg2(r.xf, p), xm = ldg2(r.xm, p), xp = ldg2(r.xp, p);
     const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-    const float nf = fmaxf(u.mu, sqrtf(c_abs2(uxf))), np = fmaxf(u.mu, sqrtf(c_abs2(uxp)));
+    const float nf = fmaxf(r.mu, sqrtf(c_abs2(uxf))), np = fmaxf(r.mu, sqrtf(c_abs2(uxp)));
     const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
-    const float2 df = make_float2(u.lam * (uf.x - up.x) + ares.x, u.lam * (uf.y - up.y) + ares.y);
+    const float2 df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
     float2 xr, wo;
-    if (u.sstart) {
+    if (r.sstart) {
         xr = xf;
-        d.xref[(long)f * N + p] = xf;
+        stg2(r.xref_out, p, xf);
         wo = make_float2(0.f, 0.f);
     } else {
-        xr = d.xref[(long)f * N + p];
-        wo = d.Wk[(long)f * N + p];
+        xr = ldg2(r.xref_in, p);
+        wo = ldg2(r.Wk, p);
     }
-    const float2 yk = make_float2(xf.x - u.L1 * df.x, xf.y - u.L1 * df.y);
-    const float2 wn = make_float2(wo.x + u.apk * df.x, wo.y + u.apk * df.y);
-    d.Wk[(long)f * N + p] = wn;
-    const float2 z = make_float2(xr.x - u.L1 * wn.x, xr.y - u.L1 * wn.y);
-    const float2 xn = make_float2(u.tau * z.x + (1.0f - u.tau) * yk.x, u.tau * z.y + (1.0f - u.tau) * yk.y);
-    u.xo[(long)f * N + p] = xn;
+    const float2 yk = make_float2(xf.x - r.L1 * df.x, xf.y - r.L1 * df.y);
+    const float2 wn = make_float2(wo.x + r.apk * df.x, wo.y + r.apk * df.y);
+    stg2(r.Wk, p, wn);
+    const float2 z = make_float2(xr.x - r.L1 * wn.x, xr.y - r.L1 * wn.y);
+    const float2 xn = make_float2(r.tau * z.x + (1.0f - r.tau) * yk.x, r.tau * z.y + (1.0f - r.tau) * yk.y);
+    stg2(r.xo, p, xn);
     return xn;
+}
+
+__device__ __forceinline__ float2 nesta_update_at(const UpdRow& r, unsigned p, float2 ares) {
+    return nesta_update_at(r, 0, p, ares);
 }

@@ -29,11 +29,14 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     float2* tw = lds + 16 * RS;
     for (int i = threadIdx.x; i < W; i += blockDim.x) tw[i] = d.pw.tw[i];
     float2* X = lds + r * RS;
-    float2* kb = d.kbuf + (((long)f * C + c) * nmax + (valid ? j : 0)) * W;
+    // wave-uniform bases of this (frame, coil) and per-thread 32-bit row offsets
+    float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+    const unsigned kro = (unsigned)((valid ? j : 0) * W);
     const int ky = valid ? d.rows[(long)f * nmax + j] : 0;
-    const float2* yrow = d.y + (((long)f * C + c) * H + ky) * W;
+    const float2* yfc = d.y + ((long)f * C + c) * H * W;
+    const uint8_t* mf = d.M + (long)f * H * W;
+    const unsigned yro = (unsigned)(ky * W);
     double acc = 0.0;
-    const uint8_t* mrow = d.M + ((long)f * H + ky) * W;
     // y and mask at the residual positions kx = k1 + P k2 of this thread's roles, fetched ahead of the FFT
     constexpr int NQ = P > 16 ? 2 : 1;
     float2 yq[NQ][16];
@@ -45,8 +48,8 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
 #pragma unroll
             for (int k2 = 0; k2 < 16; ++k2) {
                 const bool ok = valid && k1 < P;
-                yq[q][k2] = ok ? yrow[k1 + P * k2] : make_float2(0.f, 0.f);
-                mq[q][k2] = ok ? mrow[k1 + P * k2] : (uint8_t)0;
+                yq[q][k2] = ok ? ldg2(yfc + P * k2, yro + k1) : make_float2(0.f, 0.f);
+                mq[q][k2] = ok ? ldg1(mf + P * k2, yro + k1) : (uint8_t)0;
             }
         }
     }
@@ -55,7 +58,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
 #pragma unroll
     for (int b = 0; b < P; ++b) {
         float2 x = make_float2(0.f, 0.f);
-        if (valid) x = (mode == 0) ? kb[a + 16 * b] : c_conj(yrow[a + 16 * b]);
+        if (valid) x = (mode == 0) ? ldg2(kb + 16 * b, kro + a) : c_conj(ldg2(yfc + 16 * b, yro + a));
         v[b] = x;
     }
     __syncthreads();   // twiddles in LDS
@@ -81,7 +84,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
             const int k1 = a + 16 * q;
             if (k1 < P && valid)
 #pragma unroll
-                for (int k2 = 0; k2 < 16; ++k2) kb[k1 + P * k2] = c_conj(u[q][k2]);
+                for (int k2 = 0; k2 < 16; ++k2) stg2(kb + P * k2, kro + k1, c_conj(u[q][k2]));
         }
         return 0.0;
     }
@@ -116,7 +119,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     rf::rdft<P>(v);
     if (valid)
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) kb[a + 16 * n1] = c_conj(v[n1]);
+        for (int n1 = 0; n1 < P; ++n1) stg2(kb + 16 * n1, kro + a, c_conj(v[n1]));
     return acc;
 }
 
@@ -140,6 +143,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     const int w = tile * 16 + s;
     const bool wv = w < W;
     const int wc = wv ? w : 0;
+    const unsigned lane = (unsigned)(t * W + w), lanec = (unsigned)(t * W + wc);   // offsets of row t, column w
     float2* tw = lds + P * FAST_CA_KS;
     int16_t* rp = (int16_t*)(tw + H);
     float2* kst = tw + H + 4 * P;                      // STAGE: slabs [2][nmax][16]
@@ -150,11 +154,11 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     }
     float2 pre[FAST_CA_KQ];
     auto stage_load = [&](int c) {
-        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + tile * 16;
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
 #pragma unroll
         for (int q = 0; q < FAST_CA_KQ; ++q) {
             const int e = threadIdx.x + OCL_NT * q;
-            pre[q] = (e < nst && tile * 16 + (e & 15) < W) ? kb[(long)(e >> 4) * W + (e & 15)] : make_float2(0.f, 0.f);
+            pre[q] = (e < nst && wv) ? ldg2(kb + (long)16 * q * W, lane) : make_float2(0.f, 0.f);
         }
     };
     auto stage_store = [&](int buf) {
@@ -180,12 +184,12 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     for (int c = 0; c < C; ++c) {
         const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
         const float2* ks = kst + (c & 1) * nmax * 16 + s;
-        const float2* Sc = d.S + (long)c * N + wc;
+        const float2* Sc = d.S + (long)c * N;
         if constexpr (STAGE) {
             if (c + 1 < C) stage_load(c + 1);
         }
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) sv[n1] = Sc[(long)(t + 16 * n1) * W];
+        for (int n1 = 0; n1 < P; ++n1) sv[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int k1 = t + 16 * q;
@@ -195,7 +199,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
                 for (int k2 = 0; k2 < 16; ++k2) {
                     const int jj = rp[k1 + P * k2];
                     if constexpr (STAGE) u[k2] = jj >= 0 ? c_conj(ks[jj * 16]) : make_float2(0.f, 0.f);
-                    else u[k2] = (jj >= 0 && wv) ? c_conj(kb[(long)jj * W + w]) : make_float2(0.f, 0.f);
+                    else u[k2] = (jj >= 0 && wv) ? c_conj(ldg2(kb, (unsigned)(jj * W + w))) : make_float2(0.f, 0.f);
                 }
                 rf::rdft<16>(u);
 #pragma unroll
@@ -227,15 +231,14 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
             xn[n1] = c_scale(acc[n1], sc);
-            if (wv) d.x0[(long)f * N + (long)(t + 16 * n1) * W + w] = xn[n1];
+            if (wv) stg2(d.x0 + (long)f * N + (long)16 * n1 * W, lane, xn[n1]);
         }
     } else {
-        const UpdCoef u = upd_coef(d);
-        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+        const UpdRow u = upd_row(d, f);
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
             xn[n1] = make_float2(0.f, 0.f);
-            if (wv) xn[n1] = nesta_update_px(d, u, f, fm, fp, (long)(t + 16 * n1) * W + w, c_scale(acc[n1], sc));
+            if (wv) xn[n1] = nesta_update_at(u, 16 * n1 * W, lane, c_scale(acc[n1], sc));
         }
     }
 
@@ -243,14 +246,14 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     float2 sn[P];                                          // STAGE: S of the coil about to be transformed
     if constexpr (STAGE) {
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) sn[n1] = d.S[(long)(t + 16 * n1) * W + wc];
+        for (int n1 = 0; n1 < P; ++n1) sn[n1] = ldg2(d.S + (long)16 * n1 * W, lanec);
     }
     for (int c = 0; c < C; ++c) {
-        const float2* Sc = d.S + (long)c * N + wc;
+        const float2* Sc = d.S + (long)c * N;
         float2 v[P];
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
-            const float2 sc1 = STAGE ? sn[n1] : Sc[(long)(t + 16 * n1) * W];
+            const float2 sc1 = STAGE ? sn[n1] : ldg2(Sc + (long)16 * n1 * W, lanec);
             v[n1] = wv ? c_mul(xn[n1], sc1) : make_float2(0.f, 0.f);
         }
         rf::rdft<P>(v);
@@ -260,7 +263,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
         if constexpr (STAGE) {
             if (c + 1 < C)
 #pragma unroll
-                for (int n1 = 0; n1 < P; ++n1) sn[n1] = Sc[N + (long)(t + 16 * n1) * W];
+                for (int n1 = 0; n1 < P; ++n1) sn[n1] = ldg2(Sc + N + (long)16 * n1 * W, lanec);
         }
         __syncthreads();
         float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
@@ -275,7 +278,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
                     const int jj = rp[k1 + P * k2];
-                    if (jj >= 0 && wv) kb[(long)jj * W + w] = u[k2];
+                    if (jj >= 0 && wv) stg2(kb, (unsigned)(jj * W + w), u[k2]);
                 }
             }
         }

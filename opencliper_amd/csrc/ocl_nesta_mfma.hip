@@ -56,6 +56,13 @@ __device__ __forceinline__ Op8 loadA(const uint16_t* __restrict__ base, long ps,
 
 #define MF(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
 
+// Keeps memory operations of the unrolled loops on their own side of this point (compiler-level: an empty asm with a
+// memory clobber, plus a machine-scheduling barrier), so the loads in flight, and their registers, stay bounded.
+__device__ __forceinline__ void pipeline_fence() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 // (dr, di) += A B for one 16 x 16 x 32 complex block, 3-term split product
 __device__ __forceinline__ void cmac(f32x4& dr, f32x4& di, const Op8& a, const Op8& b) {
     const bf16x8 nih = bneg(b.ih), nil = bneg(b.il);
@@ -168,28 +175,53 @@ __device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, in
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// pass CA: 16 columns of one frame; H = 16 MT, sampled-row capacity kp <= 32 KJ
+// pass CA: MF_CA_TILES column tiles of 16 columns of one frame per workgroup, one wavefront per tile; H = 16 MT,
+// sampled-row capacity kp <= 32 KJ.  The frame's DFT matrices do not depend on the coil: the workgroup stages the
+// inverse one in LDS for the whole coil loop of A^H r, then the forward one for the coil loop of the next iteration's
+// forward columns, so each A fragment leaves L2 once per workgroup instead of once per coil and wavefront.
 // ---------------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void stage_planes(uint16_t* dst, const uint16_t* src, int M, int K) {
+    const int ld = mf_lds_ld(K), cpr = K / 8, total = 4 * M * cpr;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int pm = i / cpr, ck = i - pm * cpr;          // pm = plane * M + row
+        *(u32x4*)(dst + pm * ld + ck * 8) = *(const u32x4*)(src + (long)pm * K + ck * 8);
+    }
+}
+
+// A operand from a staged matrix with M rows and row stride ld: 8 consecutive K entries of row m starting at k0
+__device__ __forceinline__ Op8 loadA_lds(const uint16_t* base, int M, int ld, int m, int k0) {
+    const int ps = M * ld, off = m * ld + k0;
+    Op8 a;
+    a.rh = *(const bf16x8*)(base + off);
+    a.rl = *(const bf16x8*)(base + ps + off);
+    a.ih = *(const bf16x8*)(base + 2 * ps + off);
+    a.il = *(const bf16x8*)(base + 3 * ps + off);
+    return a;
+}
+
 template <int MT, int KJ>
-__device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int mode) {
+__device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int mode, uint16_t* lds) {
     constexpr int H = 16 * MT, KS = MT / 2;
-    const int l = threadIdx.x, n = l & 15, g = l >> 4;
-    const int tile = local % d.ncol_tiles, f = local / d.ncol_tiles;
+    const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
+    const int ngrp = (d.ncol_tiles + MF_CA_TILES - 1) / MF_CA_TILES;
+    const int tg = local % ngrp, f = local / ngrp;
+    const int tile = tg * MF_CA_TILES + (threadIdx.x >> 6);
+    const bool act = tile < d.ncol_tiles;                  // the last group of a frame may have idle wavefronts
     const int F = d.F, C = d.C, W = d.W, nmax = d.nmax, kp = d.kp;
     const long N = d.N;
-    const int w = tile * 16 + n;
+    const int w = (act ? tile : 0) * 16 + n;
     const int nr = d.nrows[f];
-    const long psI = (long)H * kp, psF = (long)kp * H;
-    const uint16_t* AI = d.dcolI + (long)f * 4 * psI;
-    const uint16_t* AF = d.dcolF + (long)f * 4 * psF;
+    const int ldI = mf_lds_ld(kp), ldF = mf_lds_ld(H);
     f32x4 ar[MT], ai[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) { ar[t] = f32x4{0.f, 0.f, 0.f, 0.f}; ai[t] = ar[t]; }
 
     // ---- A^H r: Z = D_inv(f) V per coil (K = sampled rows), conj(S_c) multiply, coil sum ----
-    for (int c = 0; c < C; ++c) {
-        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + w;
-        const float2* Sc = d.S + (long)c * N + w;
+    stage_planes(lds, d.dcolI + (long)f * 4 * H * kp, H, kp);
+    __syncthreads();
+    for (int c = 0; c < C && act; ++c) {
+        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        const float2* Sc = d.S + (long)c * N;
         Op8 bj[KJ];
 #pragma unroll
         for (int s = 0; s < KJ; ++s) {
@@ -197,24 +229,47 @@ __device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int jj = 32 * s + 8 * g + e;
-                v[e] = jj < nr ? kb[(long)jj * W] : make_float2(0.f, 0.f);
+                v[e] = jj < nr ? ldg2(kb, (unsigned)(jj * W + w)) : make_float2(0.f, 0.f);
             }
             split8(v, bj[s]);
         }
+        // one m-tile in flight ahead: its A fragments (LDS) and S values (global) load during this tile's MFMAs
+        Op8 an[KJ];
+        float2 sn[4];
+#pragma unroll
+        for (int s = 0; s < KJ; ++s) an[s] = loadA_lds(lds, H, ldI, n, 32 * s + 8 * g);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sn[i] = ldg2(Sc + (long)(4 * g + i) * W, (unsigned)w);
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
+            Op8 a[KJ];
+            float2 sv[4];
+#pragma unroll
+            for (int s = 0; s < KJ; ++s) a[s] = an[s];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sv[i] = sn[i];
+            if (t + 1 < MT) {
+#pragma unroll
+                for (int s = 0; s < KJ; ++s) an[s] = loadA_lds(lds, H, ldI, 16 * (t + 1) + n, 32 * s + 8 * g);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sn[i] = ldg2(Sc + (long)(16 * (t + 1) + 4 * g + i) * W, (unsigned)w);
+            }
             f32x4 zr = {0.f, 0.f, 0.f, 0.f}, zi = zr;
 #pragma unroll
             for (int s = 0; s < KJ; ++s)
-                if (32 * s < kp) cmac(zr, zi, loadA(AI, psI, (long)(16 * t + n) * kp + 32 * s + 8 * g), bj[s]);
+                if (32 * s < kp) cmac(zr, zi, a[s], bj[s]);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float2 sv = Sc[(long)(16 * t + 4 * g + i) * W];
-                ar[t][i] += zr[i] * sv.x + zi[i] * sv.y;
-                ai[t][i] += zi[i] * sv.x - zr[i] * sv.y;
+                ar[t][i] += zr[i] * sv[i].x + zi[i] * sv[i].y;
+                ai[t][i] += zi[i] * sv[i].x - zr[i] * sv[i].y;
             }
+            pipeline_fence();
         }
     }
+    __syncthreads();
+    stage_planes(lds, d.dcolF + (long)f * 4 * kp * H, kp, H);
+    __syncthreads();
+    if (!act) return;
 
     // ---- NESTA update at h = 16 t + 4 g + i (shared with the FP32 passes); x_{k+1} replaces the sums ----
     const float sc = d.inv_sqrtN;
@@ -225,50 +280,60 @@ __device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int
             for (int i = 0; i < 4; ++i) {
                 ar[t][i] *= sc;
                 ai[t][i] *= sc;
-                d.x0[(long)f * N + (long)(16 * t + 4 * g + i) * W + w] = make_float2(ar[t][i], ai[t][i]);
+                stg2(d.x0 + (long)f * N + (long)(16 * t + 4 * g + i) * W, (unsigned)w, make_float2(ar[t][i], ai[t][i]));
             }
     } else {
-        const UpdCoef u = upd_coef(d);
-        const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+        const UpdRow u = upd_row(d, f);
 #pragma unroll
-        for (int t = 0; t < MT; ++t)
+        for (int t = 0; t < MT; ++t) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float2 xn = nesta_update_px(d, u, f, fm, fp, (long)(16 * t + 4 * g + i) * W + w,
+                const float2 xn = nesta_update_at(u, (long)(16 * t + 4 * g + i) * W, (unsigned)w,
                                                   make_float2(ar[t][i] * sc, ai[t][i] * sc));
                 ar[t][i] = xn.x;
                 ai[t][i] = xn.y;
             }
+            pipeline_fence();   // bounds the loads in flight to one m-tile's pixels
+        }
     }
 
     // ---- forward column DFT of S_c x_{k+1} onto the sampled rows, K (= h) in the permuted order ----
     for (int c = 0; c < C; ++c) {
-        const float2* Sc = d.S + (long)c * N + w;
-        float2* kb = d.kbuf + ((long)f * C + c) * nmax * W + w;
+        const float2* Sc = d.S + (long)c * N;
+        float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
         f32x4 orr[2 * KJ], oi[2 * KJ];
 #pragma unroll
         for (int q = 0; q < 2 * KJ; ++q) { orr[q] = f32x4{0.f, 0.f, 0.f, 0.f}; oi[q] = orr[q]; }
+        // one k-step in flight ahead: its S values load during this step's MFMAs
+        float2 sn[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sn[e] = ldg2(Sc + (long)(16 * (e >> 2) + 4 * g + (e & 3)) * W, (unsigned)w);
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             float2 v[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int t = 2 * s + (e >> 2), i = e & 3;
-                const float2 sv = Sc[(long)(16 * t + 4 * g + i) * W];
-                v[e] = make_float2(ar[t][i] * sv.x - ai[t][i] * sv.y, ar[t][i] * sv.y + ai[t][i] * sv.x);
+                v[e] = make_float2(ar[t][i] * sn[e].x - ai[t][i] * sn[e].y, ar[t][i] * sn[e].y + ai[t][i] * sn[e].x);
+            }
+            if (s + 1 < KS) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    sn[e] = ldg2(Sc + (long)(16 * (2 * s + 2 + (e >> 2)) + 4 * g + (e & 3)) * W, (unsigned)w);
             }
             Op8 b;
             split8(v, b);
 #pragma unroll
             for (int q = 0; q < 2 * KJ; ++q)
-                if (16 * q < kp) cmac(orr[q], oi[q], loadA(AF, psF, (long)(16 * q + n) * H + 32 * s + 8 * g), b);
+                if (16 * q < kp) cmac(orr[q], oi[q], loadA_lds(lds, kp, ldF, 16 * q + n, 32 * s + 8 * g), b);
+            pipeline_fence();
         }
 #pragma unroll
         for (int q = 0; q < 2 * KJ; ++q)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int jj = 16 * q + 4 * g + i;
-                if (jj < nr) kb[(long)jj * W] = make_float2(orr[q][i], oi[q][i]);
+                if (jj < nr) stg2(kb, (unsigned)(jj * W + w), make_float2(orr[q][i], oi[q][i]));
             }
     }
 }
@@ -290,12 +355,14 @@ __global__ __launch_bounds__(64) void k_nesta_B_mfma(const SliceDesc* __restrict
 }
 
 template <int MT, int KJ>
-__global__ __launch_bounds__(64) void k_nesta_CA_mfma(const SliceDesc* __restrict__ descs,
-                                                      const int* __restrict__ wg_off, int nslices, int mode) {
+__global__ __launch_bounds__(64 * MF_CA_TILES) void k_nesta_CA_mfma(const SliceDesc* __restrict__ descs,
+                                                                    const int* __restrict__ wg_off, int nslices,
+                                                                    int mode) {
+    extern __shared__ uint16_t mlds[];
     const int si = find_slice(wg_off, nslices, blockIdx.x);
     const SliceDesc& d = descs[si];
     if (mode == 0 && !d.st->it_pending) return;
-    nesta_CA_mfma<MT, KJ>(d, blockIdx.x - wg_off[si], mode);
+    nesta_CA_mfma<MT, KJ>(d, blockIdx.x - wg_off[si], mode, mlds);
 }
 
 typedef void (*NestaKernel)(const SliceDesc*, const int*, int, int);
@@ -336,6 +403,14 @@ NestaKernel pick_CA_mfma(int MT, int KJ) {
 
 bool mfma_size_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256; }
 
+void mfma_kernel_attributes(int lds_bytes) {
+    for (int mt : {4, 8, 12, 16})
+        for (int kj : {1, 2})
+            (void)hipFuncSetAttribute((const void*)pick_CA_mfma(mt, kj), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      lds_bytes);
+    (void)hipGetLastError();
+}
+
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
     NestaKernel k = pick_B_mfma(L.PW);
     if (!k) return hipErrorInvalidValue;
@@ -346,7 +421,7 @@ hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mod
 hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
     NestaKernel k = pick_CA_mfma(L.PH, L.kj);
     if (!k) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, dim3(L.totCA), dim3(64), 0, stream, L.dd, L.offCA, L.ns, mode);
+    hipLaunchKernelGGL(k, dim3(L.totCA), dim3(64 * MF_CA_TILES), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }
 
