@@ -148,7 +148,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
         return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports at most 64 sampled rows per frame");
     d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
-    d.wg_B = s.F * s.C * d.nrow_groups;
+    d.wg_B = prec ? (s.F * s.C * d.nrow_groups + MF_B_TASKS - 1) / MF_B_TASKS : s.F * s.C * d.nrow_groups;
     d.wg_CA = prec ? s.F * ((d.ncol_tiles + MF_CA_TILES - 1) / MF_CA_TILES) : s.F * d.ncol_tiles;
     d.u_chunk = (int)std::min<long>(d.N, 4096);
     d.wg_U = s.F * (int)((d.N + d.u_chunk - 1) / d.u_chunk);
@@ -289,6 +289,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         L.PW = v[0]->d.W / 16;
         L.ldsB = 0;
         L.ldsCA = 0;
+        for (int i = 0; i < ns; ++i) L.ldsB = std::max(L.ldsB, mf_b_lds_bytes(v[i]->d.W));
         for (int i = 0; i < ns; ++i) L.ldsCA = std::max(L.ldsCA, mf_ca_lds_bytes(v[i]->d.H, v[i]->d.kp));
         if (L.ldsCA > LDS_MAX_NESTA) return fail(OCL_E_UNSUPPORTED, "batch: split-precision column tile exceeds LDS");
     }

@@ -125,9 +125,11 @@ hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mo
 bool mfma_size_supported(int n);      // H or W the split-precision passes are instantiated for
 void mfma_kernel_attributes(int lds_bytes);
 #define MF_CA_TILES 4                  // column tiles (one wavefront each) per workgroup of the split-precision pass CA
+#define MF_B_TASKS 4                   // 16-row tasks (one wavefront each) per workgroup of the split-precision pass B
 // LDS row stride (bf16 elements) of a staged DFT matrix with K columns: the 8-element pad keeps the 16-byte fragment
 // reads of a quarter-wavefront on distinct banks
 __host__ __device__ constexpr int mf_lds_ld(int K) { return K + 8; }
+__host__ __device__ constexpr size_t mf_b_lds_bytes(int W) { return 2 * 4 * (size_t)W * mf_lds_ld(32); }
 __host__ __device__ constexpr size_t mf_ca_lds_bytes(int H, int kp) {
     return 2 * 4 * (size_t)((H * mf_lds_ld(kp)) > (kp * mf_lds_ld(H)) ? H * mf_lds_ld(kp) : kp * mf_lds_ld(H));
 }

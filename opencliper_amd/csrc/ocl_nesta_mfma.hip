@@ -85,41 +85,76 @@ __device__ __forceinline__ int kperm(int k) {   // permuted K index -> row index
     return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3);
 }
 
+__device__ __forceinline__ void stage_planes(uint16_t* dst, const uint16_t* src, int M, int K) {
+    const int ld = mf_lds_ld(K), cpr = K / 8, total = 4 * M * cpr;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int pm = i / cpr, ck = i - pm * cpr;          // pm = plane * M + row
+        *(u32x4*)(dst + pm * ld + ck * 8) = *(const u32x4*)(src + (long)pm * K + ck * 8);
+    }
+}
+
+// A operand from a staged matrix with M rows and row stride ld: 8 consecutive K entries of row m starting at k0
+__device__ __forceinline__ Op8 loadA_lds(const uint16_t* base, int M, int ld, int m, int k0) {
+    const int ps = M * ld, off = m * ld + k0;
+    Op8 a;
+    a.rh = *(const bf16x8*)(base + off);
+    a.rl = *(const bf16x8*)(base + ps + off);
+    a.ih = *(const bf16x8*)(base + 2 * ps + off);
+    a.il = *(const bf16x8*)(base + 3 * ps + off);
+    return a;
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
-// pass B: 16 sampled rows of one (frame, coil); W = 16 MT
+// pass B: MF_B_TASKS row tasks (16 sampled rows of one (frame, coil) each, one wavefront per task) per workgroup;
+// W = 16 MT.  Both row DFTs run k-step-outer: the workgroup stages the 32-column k-slab of the (shared) DFT matrix in
+// LDS and every wavefront multiplies its rows against it, so each A fragment leaves L2 once per workgroup.
 // ---------------------------------------------------------------------------------------------------------------------
+// stages columns [k0, k0 + 32) of a 4-plane [M][K] matrix into LDS rows of mf_lds_ld(32) elements
+__device__ __forceinline__ void stage_slab(uint16_t* dst, const uint16_t* src, int M, int K, int k0) {
+    const int ld = mf_lds_ld(32), total = 4 * M * 4;       // 4 chunks of 8 elements per row
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int pm = i >> 2, ck = i & 3;
+        *(u32x4*)(dst + pm * ld + ck * 8) = *(const u32x4*)(src + (long)pm * K + k0 + ck * 8);
+    }
+}
+
 template <int MT>
-__device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, int mode) {
+__device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, int mode, uint16_t* lds) {
     constexpr int W = 16 * MT, KS = MT / 2;
-    constexpr long PS = (long)W * W;
-    const int l = threadIdx.x, n = l & 15, g = l >> 4;
+    constexpr int LD = mf_lds_ld(32);
+    const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
     const int G = d.nrow_groups;
-    const int grp = local % G, fc = local / G;
+    const int task = local * MF_B_TASKS + (threadIdx.x >> 6);
     const int C = d.C, H = d.H, nmax = d.nmax;
+    const bool tv = task < d.F * C * G;                      // the last workgroup may hold idle wavefronts
+    const int grp = tv ? task % G : 0, fc = tv ? task / G : 0;
     const int f = fc / C, c = fc - f * C;
     const int nr = d.nrows[f];
-    if (grp * 16 >= nr) return 0.0;
+    const bool act = tv && grp * 16 < nr;
     const int j = grp * 16 + n;
-    const bool valid = j < nr;
+    const bool valid = act && j < nr;
     float2* kb = d.kbuf + (((long)f * C + c) * nmax + (valid ? j : 0)) * W;
     const int ky = valid ? d.rows[(long)f * nmax + j] : 0;
     const float2* yrow = d.y + (((long)f * C + c) * H + ky) * W;
     const uint8_t* mrow = d.M + ((long)f * H + ky) * W;
     f32x4 ar[MT], ai[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) { ar[t] = f32x4{0.f, 0.f, 0.f, 0.f}; ai[t] = ar[t]; }
     double racc = 0.0;
     if (mode == 0) {
-#pragma unroll
-        for (int t = 0; t < MT; ++t) { ar[t] = f32x4{0.f, 0.f, 0.f, 0.f}; ai[t] = ar[t]; }
         // forward row DFT: D[kx][row] = sum_w F_W[kx][w] X[row][w]
-#pragma unroll
         for (int s = 0; s < KS; ++s) {
+            __syncthreads();                                 // previous slab fully read
+            stage_slab(lds, d.drowF, W, W, 32 * s);
+            __syncthreads();
+            if (!act) continue;
             float2 v[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) v[e] = valid ? kb[32 * s + 8 * g + e] : make_float2(0.f, 0.f);
             Op8 b;
             split8(v, b);
 #pragma unroll
-            for (int t = 0; t < MT; ++t) cmac(ar[t], ai[t], loadA(d.drowF, PS, (long)(16 * t + n) * W + 32 * s + 8 * g), b);
+            for (int t = 0; t < MT; ++t) cmac(ar[t], ai[t], loadA_lds(lds, W, LD, 16 * t + n, 8 * g), b);
         }
         // residual r = M K / sqrt(N) - y at kx = 16 t + 4 g + i (NestaUp.cpp:716-732)
         const float sc = d.inv_sqrtN;
@@ -148,27 +183,35 @@ __device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, in
                 ai[t][i] = yv.y;
             }
     }
-    // inverse row DFT (unnormalised): V[w][row] = sum_kx conj(F_W)[w][kx] r[kx][row], K in the permuted order
-    Op8 rb[KS];
+    // inverse row DFT (unnormalised): V[w][row] = sum_kx conj(F_W)[w][kx] r[kx][row], K in the permuted order; k-step
+    // s consumes the residual of m-tiles 2 s and 2 s + 1
+    f32x4 orr[MT], oi[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) { orr[t] = f32x4{0.f, 0.f, 0.f, 0.f}; oi[t] = orr[t]; }
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        float2 v[8];
+        __syncthreads();
+        stage_slab(lds, d.drowI, W, W, 32 * s);
+        __syncthreads();
+        if (act) {
+            float2 v[8];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            v[e] = make_float2(ar[2 * s][e], ai[2 * s][e]);
-            v[e + 4] = make_float2(ar[2 * s + 1][e], ai[2 * s + 1][e]);
+            for (int e = 0; e < 4; ++e) {
+                v[e] = make_float2(ar[2 * s][e], ai[2 * s][e]);
+                v[e + 4] = make_float2(ar[2 * s + 1][e], ai[2 * s + 1][e]);
+            }
+            Op8 rb;
+            split8(v, rb);
+#pragma unroll
+            for (int t = 0; t < MT; ++t) cmac(orr[t], oi[t], loadA_lds(lds, W, LD, 16 * t + n, 8 * g), rb);
         }
-        split8(v, rb[s]);
     }
+    if (valid) {
 #pragma unroll
-    for (int t = 0; t < MT; ++t) {
-        f32x4 orr = {0.f, 0.f, 0.f, 0.f}, oi = orr;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) cmac(orr, oi, loadA(d.drowI, PS, (long)(16 * t + n) * W + 32 * s + 8 * g), rb[s]);
-        if (valid) {
+        for (int t = 0; t < MT; ++t) {
             float4* o = (float4*)(kb + 16 * t + 4 * g);
-            o[0] = make_float4(orr[0], oi[0], orr[1], oi[1]);
-            o[1] = make_float4(orr[2], oi[2], orr[3], oi[3]);
+            o[0] = make_float4(orr[t][0], oi[t][0], orr[t][1], oi[t][1]);
+            o[1] = make_float4(orr[t][2], oi[t][2], orr[t][3], oi[t][3]);
         }
     }
     return racc;
@@ -180,25 +223,6 @@ __device__ __forceinline__ double nesta_B_mfma(const SliceDesc& d, int local, in
 // inverse one in LDS for the whole coil loop of A^H r, then the forward one for the coil loop of the next iteration's
 // forward columns, so each A fragment leaves L2 once per workgroup instead of once per coil and wavefront.
 // ---------------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void stage_planes(uint16_t* dst, const uint16_t* src, int M, int K) {
-    const int ld = mf_lds_ld(K), cpr = K / 8, total = 4 * M * cpr;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const int pm = i / cpr, ck = i - pm * cpr;          // pm = plane * M + row
-        *(u32x4*)(dst + pm * ld + ck * 8) = *(const u32x4*)(src + (long)pm * K + ck * 8);
-    }
-}
-
-// A operand from a staged matrix with M rows and row stride ld: 8 consecutive K entries of row m starting at k0
-__device__ __forceinline__ Op8 loadA_lds(const uint16_t* base, int M, int ld, int m, int k0) {
-    const int ps = M * ld, off = m * ld + k0;
-    Op8 a;
-    a.rh = *(const bf16x8*)(base + off);
-    a.rl = *(const bf16x8*)(base + ps + off);
-    a.ih = *(const bf16x8*)(base + 2 * ps + off);
-    a.il = *(const bf16x8*)(base + 3 * ps + off);
-    return a;
-}
-
 template <int MT, int KJ>
 __device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int mode, uint16_t* lds) {
     constexpr int H = 16 * MT, KS = MT / 2;
@@ -339,8 +363,10 @@ __device__ __forceinline__ void nesta_CA_mfma(const SliceDesc& d, int local, int
 }
 
 template <int MT>
-__global__ __launch_bounds__(64) void k_nesta_B_mfma(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
-                                                     int nslices, int mode) {
+__global__ __launch_bounds__(64 * MF_B_TASKS) void k_nesta_B_mfma(const SliceDesc* __restrict__ descs,
+                                                                  const int* __restrict__ wg_off, int nslices,
+                                                                  int mode) {
+    extern __shared__ uint16_t mlds[];
     __shared__ double red[64];
     const int si = find_slice(wg_off, nslices, blockIdx.x);
     const SliceDesc& d = descs[si];
@@ -350,8 +376,10 @@ __global__ __launch_bounds__(64) void k_nesta_B_mfma(const SliceDesc* __restrict
         if (mode == 0) nesta_U_task(d, local - d.wg_B, red);
         return;
     }
-    const double acc = wave_sum(nesta_B_mfma<MT>(d, local, mode));
-    if (mode == 0 && threadIdx.x == 0) d.partB[local] = acc;
+    double acc[1] = {nesta_B_mfma<MT>(d, local, mode, mlds)};
+    if (mode != 0) return;
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) d.partB[local] = acc[0];
 }
 
 template <int MT, int KJ>
@@ -405,6 +433,8 @@ bool mfma_size_supported(int n) { return n == 64 || n == 128 || n == 192 || n ==
 
 void mfma_kernel_attributes(int lds_bytes) {
     for (int mt : {4, 8, 12, 16})
+        (void)hipFuncSetAttribute((const void*)pick_B_mfma(mt), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    for (int mt : {4, 8, 12, 16})
         for (int kj : {1, 2})
             (void)hipFuncSetAttribute((const void*)pick_CA_mfma(mt, kj), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       lds_bytes);
@@ -414,7 +444,7 @@ void mfma_kernel_attributes(int lds_bytes) {
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
     NestaKernel k = pick_B_mfma(L.PW);
     if (!k) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, dim3(L.totB), dim3(64), 0, stream, L.dd, L.offB, L.ns, mode);
+    hipLaunchKernelGGL(k, dim3(L.totB), dim3(64 * MF_B_TASKS), L.ldsB, stream, L.dd, L.offB, L.ns, mode);
     return hipGetLastError();
 }
 
