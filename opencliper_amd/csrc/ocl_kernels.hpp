@@ -42,7 +42,7 @@ struct SliceDesc {
     int nrow_groups;    // ceil(nmax / rows_per_wg)
     int wg_B, wg_CA;    // workgroups of this slice in passes B (k-space rows) and CA
     int wg_U;           // extra workgroups of pass B that reduce the temporal-TV terms of x_k
-    int u_chunk;        // pixels per U workgroup (per frame)
+    int u_chunk;        // pixels per U workgroup (each covers every frame of its pixels)
     float inv_sqrtN;
     int fastH, fastW;   // P with H (W) = 16 P when the register four-step path applies, else 0
     int ca_stage;       // four-step pass CA with the next coil's sampled rows and S prefetched (nmax <= 64)

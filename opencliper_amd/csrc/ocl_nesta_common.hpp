@@ -12,26 +12,28 @@ __device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, i
     return lo;
 }
 
-// Temporal-TV terms of x_k for one chunk of one frame (NestaUp.cpp:669-691): sum |u|^2 and sum Re(conj(u) Ux) with
-// Ux = x_f - x_{f-1} (periodic) and u = Ux / max(mu, |Ux|).  Written to partU[2 uid .. 2 uid + 1].
+// Temporal-TV terms of x_k over one chunk of pixels and every frame (NestaUp.cpp:669-691): sum |u|^2 and
+// sum Re(conj(u) Ux) with Ux = x_f - x_{f-1} (periodic in f) and u = Ux / max(mu, |Ux|).  Each thread walks the
+// frames of its pixels with the previous frame in registers, so x_k is read once.  Written to partU[2 uid + {0, 1}].
 __device__ __forceinline__ void nesta_U_task(const SliceDesc& d, int uid, double* red) {
     const NState* st = d.st;
-    const int per_f = (int)((d.N + d.u_chunk - 1) / d.u_chunk);
-    const int f = uid / per_f;
-    const long p0 = (long)(uid - f * per_f) * d.u_chunk;
+    const long p0 = (long)uid * d.u_chunk;
     const long p1 = min(d.N, p0 + d.u_chunk);
-    const int fm = f == 0 ? d.F - 1 : f - 1;
     const float2* xk = st->cur ? d.x1 : d.x0;
     const float mu = (float)st->mu;
     double v[2] = {0.0, 0.0};
-    const float2* xf = xk + (long)f * d.N;
-    const float2* xm = xk + (long)fm * d.N;
     for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        const float2 ux = c_sub(ldg2(xf, (unsigned)p), ldg2(xm, (unsigned)p));
-        const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
-        const float2 u = make_float2(ux.x / nf, ux.y / nf);
-        v[0] += (double)u.x * u.x + (double)u.y * u.y;
-        v[1] += (double)u.x * ux.x + (double)u.y * ux.y;
+        float2 prev = ldg2(xk + (long)(d.F - 1) * d.N, (unsigned)p);
+#pragma unroll 4
+        for (int f = 0; f < d.F; ++f) {
+            const float2 cur = ldg2(xk + (long)f * d.N, (unsigned)p);
+            const float2 ux = c_sub(cur, prev);
+            const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
+            const float2 u = make_float2(ux.x / nf, ux.y / nf);
+            v[0] += (double)u.x * u.x + (double)u.y * u.y;
+            v[1] += (double)u.x * ux.x + (double)u.y * ux.y;
+            prev = cur;
+        }
     }
     block_sum<2>(v, red);
     if (threadIdx.x == 0) { d.partU[2 * uid] = v[0]; d.partU[2 * uid + 1] = v[1]; }
