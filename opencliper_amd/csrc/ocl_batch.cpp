@@ -150,8 +150,8 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
     d.wg_B = prec ? (s.F * s.C * d.nrow_groups + MF_B_TASKS - 1) / MF_B_TASKS : s.F * s.C * d.nrow_groups;
     d.wg_CA = prec ? s.F * ((d.ncol_tiles + MF_CA_TILES - 1) / MF_CA_TILES) : s.F * d.ncol_tiles;
-    d.u_chunk = (int)std::min<long>(d.N, OCL_NT);           // pixels per temporal-TV workgroup (all frames)
-    d.wg_U = (int)((d.N + d.u_chunk - 1) / d.u_chunk);
+    d.u_chunk = (int)std::min<long>(d.N, 2 * OCL_NT);       // pixels per temporal-TV workgroup
+    d.wg_U = (int)((d.N + d.u_chunk - 1) / d.u_chunk) * ((s.F + OCL_U_FRAMES - 1) / OCL_U_FRAMES);
     d.inv_sqrtN = (float)(1.0 / std::sqrt((double)d.N));
     d.y = (const float2*)s.y;
     d.S = (const float2*)s.S;

@@ -8,6 +8,7 @@ enum { RED_NRM2 = 0, RED_DOTC = 1, RED_SUM = 2, RED_ASUM = 3, RED_MAXABS = 4 };
 
 #define OCL_MAX_MINITER 32
 #define OCL_MAX_NSTAGES 32
+#define OCL_U_FRAMES 32      // frames per temporal-TV reduction workgroup (one group covers F <= 32)
 #define OCL_MAXE 32          // most owned elements per thread in pass CA: H * tile_cols <= OCL_MAXE * OCL_NT
 
 // Device-resident NESTA state of one slice (NestaUp.cpp:570-889 scalars).  Scalars are kept in double.
@@ -42,7 +43,7 @@ struct SliceDesc {
     int nrow_groups;    // ceil(nmax / rows_per_wg)
     int wg_B, wg_CA;    // workgroups of this slice in passes B (k-space rows) and CA
     int wg_U;           // extra workgroups of pass B that reduce the temporal-TV terms of x_k
-    int u_chunk;        // pixels per U workgroup (each covers every frame of its pixels)
+    int u_chunk;        // pixels per U workgroup (each covers OCL_U_FRAMES frames of its pixels)
     float inv_sqrtN;
     int fastH, fastW;   // P with H (W) = 16 P when the register four-step path applies, else 0
     int ca_stage;       // four-step pass CA with the next coil's sampled rows and S prefetched (nmax <= 64)

@@ -12,20 +12,24 @@ __device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, i
     return lo;
 }
 
-// Temporal-TV terms of x_k over one chunk of pixels and every frame (NestaUp.cpp:669-691): sum |u|^2 and
-// sum Re(conj(u) Ux) with Ux = x_f - x_{f-1} (periodic in f) and u = Ux / max(mu, |Ux|).  Each thread walks the
-// frames of its pixels with the previous frame in registers, so x_k is read once.  Written to partU[2 uid + {0, 1}].
+// Temporal-TV terms of x_k over one chunk of pixels and one group of OCL_U_FRAMES frames (NestaUp.cpp:669-691):
+// sum |u|^2 and sum Re(conj(u) Ux) with Ux = x_f - x_{f-1} (periodic in f) and u = Ux / max(mu, |Ux|).  Each thread
+// walks the group's frames with the previous frame in registers, so x_k is read (1 + 1 / OCL_U_FRAMES) times.
+// Written to partU[2 uid + {0, 1}].
 __device__ __forceinline__ void nesta_U_task(const SliceDesc& d, int uid, double* red) {
     const NState* st = d.st;
-    const long p0 = (long)uid * d.u_chunk;
+    const int nchunks = (int)((d.N + d.u_chunk - 1) / d.u_chunk);
+    const int fg = uid / nchunks, pc = uid - fg * nchunks;
+    const int f0 = fg * OCL_U_FRAMES, f1 = min(d.F, f0 + OCL_U_FRAMES);
+    const long p0 = (long)pc * d.u_chunk;
     const long p1 = min(d.N, p0 + d.u_chunk);
     const float2* xk = st->cur ? d.x1 : d.x0;
     const float mu = (float)st->mu;
     double v[2] = {0.0, 0.0};
     for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        float2 prev = ldg2(xk + (long)(d.F - 1) * d.N, (unsigned)p);
+        float2 prev = ldg2(xk + (long)(f0 == 0 ? d.F - 1 : f0 - 1) * d.N, (unsigned)p);
 #pragma unroll 4
-        for (int f = 0; f < d.F; ++f) {
+        for (int f = f0; f < f1; ++f) {
             const float2 cur = ldg2(xk + (long)f * d.N, (unsigned)p);
             const float2 ux = c_sub(cur, prev);
             const float nf = fmaxf(mu, sqrtf(c_abs2(ux)));
