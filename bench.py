@@ -254,8 +254,12 @@ def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
     done = []
     meta = {}
     t0 = time.perf_counter()
+    t_log = t0
     try:
         for r in range(n_req):
+            if time.perf_counter() - t_log > 30.0:
+                t_log = time.perf_counter()
+                log(f"served trace: {t_log - t0:.0f} s, {r} submitted, {len(done)} done")
             while True:
                 now = time.perf_counter() - t0
                 for rid in [k for k, v in inflight.items() if srv.poll(k)[0] == capi.SERVE_DONE]:
