@@ -42,7 +42,7 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->red_out, sizeof(double) * 16);
     if (e == hipSuccess) e = hipHostMalloc(&c->red_host, sizeof(double) * 16, hipHostMallocDefault);
     const void* lds_kernels[] = {(const void*)k_fft_rows, (const void*)k_fft_cols, (const void*)k_fft_rows_fast,
-                                 (const void*)k_fft_cols_fast};
+                                 (const void*)k_fft_cols_fast, (const void*)k_simple_cols_fast};
     if (e == hipSuccess) {
         for (const void* k : lds_kernels) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
         nesta_kernel_attributes((int)LDS_MAX_NESTA);
@@ -427,9 +427,30 @@ extern "C" int ocl_fft2d(ocl_ctx* c, const void* in, void* out, long batch, int 
 extern "C" int ocl_simple_mri_recon(ocl_ctx* c, void* y, const void* S, void* x, int F, int C, int H, int W) {
     CTXCHK(c);
     const long N = (long)H * W;
+    const int PH = fast_p(H), PW = fast_p(W);
+    if (PH && PW && F >= 1 && C >= 1 && y && S && x && (long)F * ((W + 15) / 16) >= 128) {
+        // two passes: inverse row DFTs in place, then the fused column DFT + conj(S) + coil sum (enough column
+        // tiles x frames to fill the device; otherwise the coil-parallel column transform below)
+        FftPlan pw, ph;
+        CHK(make_plan(c, W, &pw));
+        CHK(make_plan(c, H, &ph));
+        const long nrows = (long)F * C * H;
+        const size_t ldsr = sizeof(float2) * ((size_t)16 * (PW * 17 + 1) + W);
+        hipLaunchKernelGGL(k_fft_rows_fast, dim3((unsigned)((nrows + 15) / 16)), dim3(OCL_NT), ldsr, c->stream,
+                           (const float2*)y, (float2*)y, nrows, PW, pw.tw, 1, 1.0f);
+        HIPCHK(hipGetLastError());
+        const size_t ldsc = sizeof(float2) * ((size_t)PH * FAST_COLS_KS + H);
+        hipLaunchKernelGGL(k_simple_cols_fast, dim3((unsigned)((W + 15) / 16), (unsigned)F), dim3(OCL_NT), ldsc,
+                           c->stream, (float2*)y, (const float2*)S, (float2*)x, C, W, PH, ph.tw,
+                           (float)(1.0 / ((double)H * (double)W)));
+        HIPCHK(hipGetLastError());
+        return OCL_OK;
+    }
     CHK(fft2d(c, y, y, (long)F * C, H, W, +1));
-    CHK(ocl_cep(c, y, S, y, F, C, N, 1, 1));
-    CHK(ocl_ximage_sum(c, y, x, F, C, N));
+    if (!S || !x || F < 1 || C < 1) return fail(OCL_E_INVALID, "SimpleMRIRecon: bad arguments");
+    hipLaunchKernelGGL(k_cep_sum, dim3(grid_for((long)F * N)), dim3(OCL_NT), 0, c->stream, (float2*)y,
+                       (const float2*)S, (float2*)x, F, C, N);
+    HIPCHK(hipGetLastError());
     return OCL_OK;
 }
 

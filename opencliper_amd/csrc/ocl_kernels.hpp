@@ -81,6 +81,7 @@ __global__ void k_vecnorm(const float2* in, float2* out, long n, float mu);
 __global__ void k_cep(const float2* in, const float2* S, float2* out, int F, int C, long N, int in_coils, int conj);
 __global__ void k_apply_mask(float2* d, const uint8_t* M, int F, int C, long N);
 __global__ void k_ximage_sum(const float2* in, float2* out, int F, int C, long N);
+__global__ void k_cep_sum(float2* y, const float2* S, float2* x, int F, int C, long N);
 __global__ void k_abs(const float2* in, float2* out, long n, int pow2);
 __global__ void k_copy(const float2* x, float2* y, long n);
 __global__ void k_scal(float2* x, long n, float ar, float ai);
@@ -101,6 +102,8 @@ __global__ void k_fft_rows_fast(const float2* in, float2* out, long nrows, int P
                                 float scale);
 __global__ void k_fft_cols_fast(const float2* in, float2* out, int W, int P, const float2* tw, int conj_io,
                                 float scale);
+__global__ void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P, const float2* tw,
+                                   float scale);
 
 // kernels (ocl_nesta.hip)
 __global__ void k_rowflags(const float2* y, const uint8_t* M, int F, int C, int H, int W, uint8_t* flags);

@@ -69,6 +69,25 @@ __global__ void k_ximage_sum(const float2* __restrict__ in, float2* __restrict__
     }
 }
 
+// ComplexElementProd with conj(S) in place followed by XImageSum (SimpleMRIRecon.cpp:59-101) in one pass:
+// y[f][c] <- y[f][c] conj(S_c), x[f] = sum_c y[f][c] (coils summed in order, as xImageSum.cl:42-61 does).
+__global__ void k_cep_sum(float2* __restrict__ y, const float2* __restrict__ S, float2* __restrict__ x, int F, int C,
+                          long N) {
+    const long total = (long)F * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int c = 0; c < C; ++c) {
+            const long k = ((long)f * C + c) * N + p;
+            const float2 v = c_mulconj(y[k], S[(long)c * N + p]);
+            y[k] = v;
+            acc = c_add(acc, v);
+        }
+        x[i] = acc;
+    }
+}
+
 // ComplexAbs / ComplexAbsPow2 (src/kernels/complexAbs.cl:43-70, complexAbsPow2.cl:43-71): out = (|z| or |z|^2, 0)
 __global__ void k_abs(const float2* __restrict__ in, float2* __restrict__ out, long n, int pow2) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -295,6 +314,88 @@ __device__ __forceinline__ void fft_cols_fast(const float2* __restrict__ in, flo
                     O[(long)(k1 + P * k2) * W + w] = c_scale(y, scale);
                 }
         }
+    }
+}
+
+// SimpleMRIRecon after its row transform (src/processes/examples/SimpleMRIRecon.cpp:59-101): for 16 columns of frame
+// f, every coil's inverse column DFT (register four-step, FA order), times conj(S_c), is written back in place (the
+// state the reference leaves in its input) and summed over the coils in registers, so x is written once.  scale is
+// the 1 / (H W) of the inverse 2-D transform.
+template <int P>
+__device__ __forceinline__ void simple_cols_fast(float2* __restrict__ y, const float2* __restrict__ S,
+                                                 float2* __restrict__ x, int C, int W, const float2* __restrict__ twg,
+                                                 float scale) {
+    constexpr int H = 16 * P, KS = FAST_COLS_KS;
+    constexpr int NQ = P > 16 ? 2 : 1;
+    extern __shared__ float2 lds[];
+    float2* tw = lds + P * KS;
+    const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int f = blockIdx.y;
+    const int w = blockIdx.x * 16 + s;
+    const bool wv = w < W;
+    const int wc = wv ? w : 0;
+    const long N = (long)H * W;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) tw[i] = twg[i];
+    float2 acc[NQ][16];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) acc[q][k2] = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; ++c) {
+        float2* I = y + ((long)f * C + c) * N;
+        const float2* Sc = S + (long)c * N;
+        float2 v[P];
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+            const float2 a = ldg2(I + (long)16 * b * W, (unsigned)(t * W + wc));
+            v[b] = wv ? make_float2(a.x, -a.y) : make_float2(0.f, 0.f);    // IFFT = conj(FFT(conj(.)))
+        }
+        __syncthreads();   // twiddles ready / the previous coil's exchange fully read
+        rf::rdft<P>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k1 = t + 16 * q;
+            if (k1 < P) {
+                float2 u[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) u[i] = lds[k1 * KS + i * 16 + s];
+                rf::rdft<16>(u);
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    const unsigned off = (unsigned)((k1 + P * k2) * W + wc);
+                    const float2 z = make_float2(u[k2].x * scale, -u[k2].y * scale);
+                    const float2 pz = c_mulconj(z, ldg2(Sc, off));
+                    acc[q][k2] = c_add(acc[q][k2], pz);
+                    if (wv) stg2(I, off, pz);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k1 = t + 16 * q;
+        if (k1 < P && wv)
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) stg2(x + (long)f * N, (unsigned)((k1 + P * k2) * W + w), acc[q][k2]);
+    }
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P,
+                                                             const float2* tw, float scale) {
+    switch (P) {
+        case 2: simple_cols_fast<2>(y, S, x, C, W, tw, scale); break;
+        case 4: simple_cols_fast<4>(y, S, x, C, W, tw, scale); break;
+        case 6: simple_cols_fast<6>(y, S, x, C, W, tw, scale); break;
+        case 8: simple_cols_fast<8>(y, S, x, C, W, tw, scale); break;
+        case 10: simple_cols_fast<10>(y, S, x, C, W, tw, scale); break;
+        case 12: simple_cols_fast<12>(y, S, x, C, W, tw, scale); break;
+        case 14: simple_cols_fast<14>(y, S, x, C, W, tw, scale); break;
+        case 16: simple_cols_fast<16>(y, S, x, C, W, tw, scale); break;
+        case 18: simple_cols_fast<18>(y, S, x, C, W, tw, scale); break;
+        default: simple_cols_fast<20>(y, S, x, C, W, tw, scale); break;
     }
 }
 

@@ -119,13 +119,17 @@ def test_normest_matches_oracle(ctx, oracle, F, N):
     assert abs(g - e) < 2e-6 * e
 
 
-def test_simple_mri_recon(ctx, oracle):
-    sl = synth.make_fully_sampled(H=256, W=256, C=8, F=1)
+@pytest.mark.parametrize("H,W,C,F", [(256, 256, 8, 1), (160, 96, 3, 2), (128, 256, 4, 9)])
+def test_simple_mri_recon(ctx, oracle, H, W, C, F):
+    sl = synth.make_fully_sampled(H=H, W=W, C=C, F=F)
     y = ctx.to_device(sl.y)
     S = ctx.to_device(sl.S)
-    x = ctx.empty((1, 256, 256))
+    x = ctx.empty((F, H, W))
     ctx.simple_mri_recon(y, S, x)
     ref64 = oracle.simple_recon(sl.y, sl.S, double=True)
     got = x.get()
     assert oracle.rel_l2(got, ref64) < 1e-6
     assert oracle.rel_l2(got, oracle.simple_recon(sl.y, sl.S, double=False)) < 1e-6
+    # like the reference, the input is left holding conj(S_c) * IFFT2(y_c) (SimpleMRIRecon.cpp:59-101)
+    post = np.fft.ifft2(sl.y.astype(np.complex128), axes=(-2, -1)) * np.conj(sl.S.astype(np.complex128))[None]
+    assert oracle.rel_l2(y.get(), post) < 1e-6
