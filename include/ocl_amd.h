@@ -105,6 +105,11 @@ int ocl_fft2d(ocl_ctx* ctx, const void* in, void* out, long batch, int H, int W,
  * Like the reference it transforms y in place. */
 int ocl_simple_mri_recon(ocl_ctx* ctx, void* y, const void* S, void* x, int F, int C, int H, int W);
 
+/* RSoS::launch (src/processes/RSoS.cpp, kernel src/kernels/rss.cl): out_f = (sqrt(sum_c |in_{f,c}|^2), 0) */
+int ocl_rsos(ocl_ctx* ctx, const void* in, void* out, int F, int C, long N);
+/* SimpleMRIReconSOS::launch (src/processes/examples/SimpleMRIReconSOS.cpp): inverse 2-D FFT of y in place, then RSoS */
+int ocl_simple_mri_recon_sos(ocl_ctx* ctx, void* y, void* x, int F, int C, int H, int W);
+
 /* Synthetic-input helper (no reference counterpart): y = M * (y + sigma * n) with complex Gaussian n whose real and
  * imaginary parts have variance 1/2, drawn from a counter-based hash of (seed, index).  Used to build large k-space
  * test inputs on the device. */

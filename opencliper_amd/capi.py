@@ -119,6 +119,7 @@ def lib():
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_ctx_set_autotune": ([P, I], I),
+        "ocl_rsos": ([P, P, P, I, I, LG], I), "ocl_simple_mri_recon_sos": ([P, P, P, I, I, I, I], I),
         "ocl_ctx_set_tiles": ([P, I, I, I, I, I, I], I),
         "ocl_autotune_shape": ([P, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
                                 ctypes.POINTER(D), ctypes.POINTER(D)], I),
@@ -288,6 +289,14 @@ class Context:
     def simple_mri_recon(self, y, S, x):
         F, C, H, W = y.shape
         _check(lib().ocl_simple_mri_recon(self.handle, y.ptr, S.ptr, x.ptr, F, C, H, W))
+
+    def rsos(self, k, out):
+        F, C = k.shape[:2]
+        _check(lib().ocl_rsos(self.handle, k.ptr, out.ptr, F, C, k.size // (F * C)))
+
+    def simple_mri_recon_sos(self, y, x):
+        F, C, H, W = y.shape
+        _check(lib().ocl_simple_mri_recon_sos(self.handle, y.ptr, x.ptr, F, C, H, W))
 
     def scal(self, x, a):
         a = complex(a)

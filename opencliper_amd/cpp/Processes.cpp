@@ -1,6 +1,6 @@
 // The Processes of the NESTA-TV path in the OpenCLIPER-compatible C++ API, each one call into the C ABI.
 // Reference Processes mirrored: TemporalTV, VectorNormalization, FFT, ComplexElementProd, ApplyMask, XImageSum,
-// ComplexAbs, ComplexAbsPow2, CopyDataGPU, SimpleMRIRecon, NestaUp (src/processes/**).
+// ComplexAbs, ComplexAbsPow2, CopyDataGPU, RSoS, SimpleMRIRecon, SimpleMRIReconSOS, NestaUp (src/processes/**).
 #include <OpenCLIPER/processes/ApplyMask.hpp>
 #include <OpenCLIPER/processes/ComplexAbs.hpp>
 #include <OpenCLIPER/processes/ComplexAbsPow2.hpp>
@@ -8,6 +8,8 @@
 #include <OpenCLIPER/processes/FFT.hpp>
 #include <OpenCLIPER/processes/XImageSum.hpp>
 #include <OpenCLIPER/processes/examples/SimpleMRIRecon.hpp>
+#include <OpenCLIPER/processes/examples/SimpleMRIReconSOS.hpp>
+#include <OpenCLIPER/processes/RSoS.hpp>
 #include <OpenCLIPER/processes/groupwiseRegistration/initialize/CopyDataGPU.hpp>
 #include <OpenCLIPER/processes/nesta/NestaUp.hpp>
 #include <OpenCLIPER/processes/nesta/TemporalTV.hpp>
@@ -144,6 +146,19 @@ void XImageSum::launch() {
     stopProfiling();
 }
 
+// ---- RSoS (src/processes/RSoS.cpp, src/kernels/rss.cl) ----
+void RSoS::launch() {
+    auto in = getInput(), out = getOutput();
+    needIO(in, out, "RSoS");
+    auto k = std::dynamic_pointer_cast<KData>(in);
+    if (!k) throw std::invalid_argument("RSoS::launch: input must be KData");
+    startProfiling();
+    checkOcl(ocl_rsos(ctxOf(*this), in->getDeviceBuffer(), out->getDeviceBuffer(), (int)k->getDynDimsTotalSize(),
+                      (int)k->getNCoils(), pixels(*in)),
+             "RSoS::launch");
+    stopProfiling();
+}
+
 // ---- ComplexAbs / ComplexAbsPow2 (src/processes/ComplexAbs.cpp:48-78, ComplexAbsPow2.cpp:48-78) ----
 void ComplexAbs::launch() {
     auto in = getInput(), out = getOutput();
@@ -185,6 +200,24 @@ void SimpleMRIRecon::launch() {
                                   out->getDeviceBuffer(), (int)k->getDynDimsTotalSize(), (int)k->getNCoils(),
                                   (int)NDARRAYHEIGHT(a), (int)NDARRAYWIDTH(a)),
              "SimpleMRIRecon::launch");
+    stopProfiling();
+}
+
+// ---- SimpleMRIReconSOS (src/processes/examples/SimpleMRIReconSOS.cpp) ----
+void SimpleMRIReconSOS::init() {
+    auto k = std::dynamic_pointer_cast<KData>(getInput());
+    if (!k || !getOutput()) throw std::invalid_argument("SimpleMRIReconSOS::init: KData input and XData output required");
+}
+void SimpleMRIReconSOS::launch() {
+    auto k = std::dynamic_pointer_cast<KData>(getInput());
+    auto out = getOutput();
+    needIO(k, out, "SimpleMRIReconSOS");
+    const NDArray* a = k->getNDArray(0);
+    startProfiling();
+    checkOcl(ocl_simple_mri_recon_sos(ctxOf(*this), k->getDeviceBuffer(), out->getDeviceBuffer(),
+                                      (int)k->getDynDimsTotalSize(), (int)k->getNCoils(), (int)NDARRAYHEIGHT(a),
+                                      (int)NDARRAYWIDTH(a)),
+             "SimpleMRIReconSOS::launch");
     stopProfiling();
 }
 

@@ -424,6 +424,22 @@ extern "C" int ocl_fft2d(ocl_ctx* c, const void* in, void* out, long batch, int 
     return fft2d(c, in, out, batch, H, W, dir);
 }
 
+extern "C" int ocl_rsos(ocl_ctx* c, const void* in, void* out, int F, int C, long N) {
+    CTXCHK(c);
+    if (!in || !out || F < 1 || C < 1 || N < 1) return fail(OCL_E_INVALID, "RSoS: bad arguments");
+    hipLaunchKernelGGL(k_rsos, dim3(grid_for((long)F * N)), dim3(OCL_NT), 0, c->stream, (const float2*)in,
+                       (float2*)out, F, C, N);
+    HIPCHK(hipGetLastError());
+    return OCL_OK;
+}
+
+extern "C" int ocl_simple_mri_recon_sos(ocl_ctx* c, void* y, void* x, int F, int C, int H, int W) {
+    CTXCHK(c);
+    if (!y || !x || F < 1 || C < 1) return fail(OCL_E_INVALID, "SimpleMRIReconSOS: bad arguments");
+    CHK(fft2d(c, y, y, (long)F * C, H, W, +1));
+    return ocl_rsos(c, y, x, F, C, (long)H * W);
+}
+
 extern "C" int ocl_simple_mri_recon(ocl_ctx* c, void* y, const void* S, void* x, int F, int C, int H, int W) {
     CTXCHK(c);
     const long N = (long)H * W;

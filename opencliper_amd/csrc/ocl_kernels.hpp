@@ -82,6 +82,7 @@ __global__ void k_cep(const float2* in, const float2* S, float2* out, int F, int
 __global__ void k_apply_mask(float2* d, const uint8_t* M, int F, int C, long N);
 __global__ void k_ximage_sum(const float2* in, float2* out, int F, int C, long N);
 __global__ void k_cep_sum(float2* y, const float2* S, float2* x, int F, int C, long N);
+__global__ void k_rsos(const float2* in, float2* out, int F, int C, long N);
 __global__ void k_abs(const float2* in, float2* out, long n, int pow2);
 __global__ void k_copy(const float2* x, float2* y, long n);
 __global__ void k_scal(float2* x, long n, float ar, float ai);

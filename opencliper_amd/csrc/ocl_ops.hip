@@ -88,6 +88,19 @@ __global__ void k_cep_sum(float2* __restrict__ y, const float2* __restrict__ S, 
     }
 }
 
+// RSoS (src/kernels/rss.cl): out_f = (sqrt(sum_c |in_{f,c}|^2), 0), the sum taken in coil order.  The reference
+// writes only the real part; the imaginary part is set to 0 here.
+__global__ void k_rsos(const float2* __restrict__ in, float2* __restrict__ out, int F, int C, long N) {
+    const long total = (long)F * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        float s = 0.f;
+        for (int c = 0; c < C; ++c) s += c_abs2(in[((long)f * C + c) * N + p]);
+        out[i] = make_float2(sqrtf(s), 0.f);
+    }
+}
+
 // ComplexAbs / ComplexAbsPow2 (src/kernels/complexAbs.cl:43-70, complexAbsPow2.cl:43-71): out = (|z| or |z|^2, 0)
 __global__ void k_abs(const float2* __restrict__ in, float2* __restrict__ out, long n, int pow2) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {

@@ -20,6 +20,8 @@
 //   normest .............. src/processes/nesta/NestaUp.cpp:291-439 (power method, drand48 restart)
 //   nesta ................ src/processes/nesta/NestaUp.cpp:442-910 (continuation NESTA, stop test 1)
 //   simple_recon ......... src/processes/examples/SimpleMRIRecon.cpp:59-101
+//   rsos ................. src/kernels/rss.cl (rss_kernel): sqrt(sum_c |in_c|^2) in the real part, float sum in coil order
+//   simple_recon_sos ..... src/processes/examples/SimpleMRIReconSOS.cpp (in-place inverse FFT, then RSoS)
 //
 // Pinning (each checked in tests/test_oracle.py): fft vs brute-force DFT and numpy.fft; ttv vs closed form on
 // integers and the adjoint identity; normest's LCG vs libc drand48 from the POSIX default state and its result vs
@@ -537,6 +539,21 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
 
 }  // namespace
 
+// rss.cl: out_f = (sqrt(sum_c |in_{f,c}|^2), 0), the sum taken in coil order in T
+template <class T> void rsos(const Cx<T>* in, Cx<T>* out, int F, int C, long N) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < N; ++p)
+        for (int f = 0; f < F; ++f) {
+            T s = 0;
+            for (int c = 0; c < C; ++c) {
+                const Cx<T> v = in[((long)f * C + c) * N + p];
+                s += v.re * v.re + v.im * v.im;
+            }
+            out[(long)f * N + p].re = std::sqrt(s);
+            out[(long)f * N + p].im = 0;
+        }
+}
+
 extern "C" {
 
 // Full NESTA-TV reconstruction.  y: [F][C][H][W], S: [C][H][W], M: [F][H][W] u8, x_out: [F][H][W].
@@ -566,6 +583,17 @@ int orc_simple_recon_f64(const double* y, const double* S, double* x, int F, int
     std::vector<Cx<double>> aux(nK);
     Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, nullptr};
     op_At(P, (const Cx<double>*)y, (Cx<double>*)x, aux.data());
+    return 0;
+}
+
+void orc_rsos_f32(const float* in, float* out, int F, int C, long N) {
+    rsos((const Cx<float>*)in, (Cx<float>*)out, F, C, N);
+}
+
+// SimpleMRIReconSOS.cpp: inverse 2-D FFT (scaled 1/N) of every coil image in place, then rss.cl
+int orc_simple_recon_sos_f64(double* y, double* x, int F, int C, int H, int W) {
+    fft2d((const Cx<double>*)y, (Cx<double>*)y, (long)F * C, H, W, +1);
+    rsos((const Cx<double>*)y, (Cx<double>*)x, F, C, (long)H * W);
     return 0;
 }
 

@@ -71,6 +71,7 @@ def lib():
             "orc_nesta_f32": ([P, P, P, P, I, I, I, I, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
             "orc_nesta_f64": ([P, P, P, P, I, I, I, I, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
             "orc_simple_recon_f32": ([P, P, P, I, I, I, I], I), "orc_simple_recon_f64": ([P, P, P, I, I, I, I], I),
+            "orc_rsos_f32": ([P, P, I, I, LG], None), "orc_simple_recon_sos_f64": ([P, P, I, I, I, I], I),
             "orc_num_threads": ([], I), "orc_set_threads": ([I], None),
         }
         for name, (a, r) in sig.items():
@@ -191,6 +192,24 @@ def simple_recon(y, S, double=True):
     x = np.empty((F, H, W), dtype=dt)
     f = lib().orc_simple_recon_f64 if double else lib().orc_simple_recon_f32
     f(_p(y), _p(S), _p(x), F, C, H, W)
+    return x
+
+
+def rsos(k):
+    """rss.cl: (sqrt(sum_c |k[f][c]|^2), 0) per pixel, FP32 with the coil sum in order."""
+    a = _c(k, np.complex64)
+    F, C = a.shape[:2]
+    out = np.empty((F,) + a.shape[2:], dtype=np.complex64)
+    lib().orc_rsos_f32(_p(a), _p(out), F, C, a.size // (F * C))
+    return out
+
+
+def simple_recon_sos(y):
+    """SimpleMRIReconSOS in FP64: inverse 2-D FFT of each coil image, then the root sum of squares."""
+    y = np.array(y, dtype=np.complex128)
+    F, C, H, W = y.shape
+    x = np.empty((F, H, W), dtype=np.complex128)
+    lib().orc_simple_recon_sos_f64(_p(y), _p(x), F, C, H, W)
     return x
 
 

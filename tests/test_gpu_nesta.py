@@ -139,3 +139,18 @@ def test_batch_graph_replay_matches_eager(ctx):
         ctx.sync()
         assert np.array_equal(dev[0][3].get(), ref), rep
     b.close()
+
+
+@pytest.mark.parametrize("form", [capi.OCL_FORM_FUSED, capi.OCL_FORM_SPLIT])
+def test_many_frames(ctx, oracle, form):
+    """F = 40: the temporal-TV reduction covers the frames in two groups and the periodic wrap crosses them."""
+    sl = synth.make_cine(H=64, W=64, C=2, F=40, accel=4.0, slice_id=13)
+    prm = dict(maxIntIter=3, maxIter=6, stoptest=0)
+    x64, r64 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    y, S, M, x = upload(ctx, sl.y, sl.S, sl.M)
+    rep = ctx.nesta(y, S, M, x, capi.nesta_params(**prm), form=form)
+    err = oracle.rel_l2(x.get(), x64)
+    print("F=40 form", form, "rel-L2 vs FP64", err, flush=True)
+    assert rep["iters"] == [6] * 3
+    np.testing.assert_allclose(rep["trace_f"], r64["trace_f"], rtol=1e-4)
+    assert err < 1e-4 if form == capi.OCL_FORM_FUSED else err < 1e-3

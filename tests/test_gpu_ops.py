@@ -133,3 +133,17 @@ def test_simple_mri_recon(ctx, oracle, H, W, C, F):
     # like the reference, the input is left holding conj(S_c) * IFFT2(y_c) (SimpleMRIRecon.cpp:59-101)
     post = np.fft.ifft2(sl.y.astype(np.complex128), axes=(-2, -1)) * np.conj(sl.S.astype(np.complex128))[None]
     assert oracle.rel_l2(y.get(), post) < 1e-6
+
+
+@pytest.mark.parametrize("H,W,C,F", [(256, 256, 8, 1), (96, 160, 5, 3)])
+def test_rsos_and_simple_recon_sos(ctx, oracle, H, W, C, F):
+    sl = synth.make_fully_sampled(H=H, W=W, C=C, F=F, slice_id=23)
+    k = ctx.to_device(sl.y)
+    out = ctx.empty((F, H, W))
+    ctx.rsos(k, out)
+    assert oracle.rel_l2(out.get(), oracle.rsos(sl.y)) < 1e-6
+    x = ctx.empty((F, H, W))
+    ctx.simple_mri_recon_sos(k, x)
+    got = x.get()
+    assert np.all(got.imag == 0)
+    assert oracle.rel_l2(got, oracle.simple_recon_sos(sl.y)) < 1e-6

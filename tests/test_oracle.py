@@ -141,3 +141,20 @@ def test_planted_mistake_is_caught(oracle):
     prm = oracle.params(**info["nesta"])
     x_bad, _ = oracle.nesta(d["y"] * np.sqrt(12 * 10), d["S"], d["M"], prm, double=True)
     assert oracle.rel_l2(x_bad, d["x"]) > 1e-2
+
+
+def test_rsos_closed_form(oracle):
+    rng = np.random.default_rng(21)
+    k = (rng.normal(size=(3, 5, 12, 10)) + 1j * rng.normal(size=(3, 5, 12, 10))).astype(np.complex64)
+    got = oracle.rsos(k)
+    ref = np.sqrt(np.sum(np.abs(k.astype(np.complex128)) ** 2, axis=1))
+    assert np.all(got.imag == 0)
+    np.testing.assert_allclose(got.real, ref, rtol=2e-6)
+
+
+def test_simple_recon_sos_recovers_magnitude(oracle):
+    """With RSS-normalised maps, the root sum of squares of the coil images S_c x is |x| exactly."""
+    sl = synth.make_fully_sampled(H=32, W=24, C=4, F=2, slice_id=22)
+    k = np.fft.fft2(sl.S[None].astype(np.complex128) * sl.x_true[:, None].astype(np.complex128), axes=(-2, -1))
+    got = oracle.simple_recon_sos(k)
+    np.testing.assert_allclose(got.real, np.abs(sl.x_true.astype(np.complex128)), atol=1e-6)
