@@ -19,6 +19,12 @@ class KData : public Data {
     /// copySamplingMasks share the source's maps / masks.
     KData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<KData>& sourceData, bool copyData = false,
           bool copySensitivityMaps = false, bool copySamplingMasks = false);
+    /// From a MAT v5 file holding Dims, KData and optionally SensitivityMaps and SamplingMasks (KData.cpp:292-435).
+    KData(const std::shared_ptr<CLapp>& pCLapp, const std::string& fileName);
+
+    /// Saves Dims, KData and (when attached) SensitivityMaps to a zlib-compressed MAT v5 file (KData.cpp:837-936);
+    /// saveSamplingMasks also writes the masks as a uint8 pixel mask [H, W, temporal...], which the reference leaves out.
+    void matlabSave(const std::string& fileName, bool saveSamplingMasks = false);
 
     numCoilsType getNCoils() const { return nCoils_; }
     std::shared_ptr<SensitivityMapsData> getSensitivityMapsData() const { return sens_; }

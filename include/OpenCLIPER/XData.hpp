@@ -15,6 +15,11 @@ class XData : public Data {
           bool copyDataToDevice = true);
     /// Uninitialised image with one frame per KData frame (XData.cpp:425-440).
     XData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<KData>& sourceData);
+    /// From a MAT v5 file holding Dims and XData (XData.cpp:500-556).
+    XData(const std::shared_ptr<CLapp>& pCLapp, const std::string& fileName);
+
+    /// Saves Dims and XData to a zlib-compressed MAT v5 file (XData.cpp:898-930).
+    void matlabSave(const std::string& fileName);
 };
 
 }  // namespace OpenCLIPER

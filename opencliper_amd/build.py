@@ -17,6 +17,8 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libocl_amd.so")
 LIBCPP = os.path.join(PKG, "libOpenCLIPER_amd.so")
 EXAMPLE = os.path.join(PKG, "examples", "mri_recon")
+DATA_IO = os.path.join(PKG, "examples", "data_io")
+MAT_TOOL = os.path.join(PKG, "examples", "mat_tool")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -58,13 +60,22 @@ def build(verbose: bool = False) -> str:
     cpp_srcs = sorted(os.path.join(CPP, f) for f in os.listdir(CPP) if f.endswith(".cpp"))
     tmp = LIBCPP + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INC}", "-o", tmp] + cpp_srcs +
-         [f"-L{PKG}", "-locl_amd", "-Wl,-rpath,$ORIGIN"])
+         [f"-L{PKG}", "-locl_amd", "-lz", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, LIBCPP)
     # example driver shaped like the reference's tests/MRIRecon.cpp
     tmp = EXAMPLE + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "mri_recon.cpp"),
           f"-L{PKG}", "-lOpenCLIPER_amd", "-locl_amd", "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, EXAMPLE)
+    # MAT-file I/O through KData / XData (device needed) and the host-only MAT v5 utility
+    tmp = DATA_IO + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "data_io.cpp"),
+          f"-L{PKG}", "-lOpenCLIPER_amd", "-locl_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, DATA_IO)
+    tmp = MAT_TOOL + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "mat_tool.cpp"),
+          os.path.join(CPP, "MatFile.cpp"), "-lz"])
+    os.replace(tmp, MAT_TOOL)
     return LIB
 
 
