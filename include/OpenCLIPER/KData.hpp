@@ -19,8 +19,14 @@ class KData : public Data {
     /// copySamplingMasks share the source's maps / masks.
     KData(const std::shared_ptr<CLapp>& pCLapp, const std::shared_ptr<KData>& sourceData, bool copyData = false,
           bool copySensitivityMaps = false, bool copySamplingMasks = false);
-    /// From a MAT v5 file holding Dims, KData and optionally SensitivityMaps and SamplingMasks (KData.cpp:292-435).
+    /// From a MAT v5 file holding Dims, KData and optionally SensitivityMaps and SamplingMasks (KData.cpp:292-435), or,
+    /// when the file is not a MAT file, from the CFL set <base>.cfl + <base>.hdr with optional
+    /// <base>_SensitivityMaps.cfl and <base>_SamplingMasks.u32 (row masks) or .u8 (pixel masks) (KData.cpp:246-345).
     KData(const std::shared_ptr<CLapp>& pCLapp, const std::string& fileName);
+
+    /// Writes <base>.cfl, <base>.hdr and, when attached, <base>_SensitivityMaps.cfl and <base>_SamplingMasks.u8
+    /// (KData.cpp:688-710).
+    void saveCFLData(const std::string& baseFileName);
 
     /// Saves Dims, KData and (when attached) SensitivityMaps to a zlib-compressed MAT v5 file (KData.cpp:837-936);
     /// saveSamplingMasks also writes the masks as a uint8 pixel mask [H, W, temporal...], which the reference leaves out.
@@ -35,6 +41,9 @@ class KData : public Data {
     void setSamplingMasksData(const std::shared_ptr<SamplingMasksData>& p) { masks_ = p; }
 
   private:
+    void loadMat(const std::string& fileName);
+    void loadCFL(const std::string& fileName);
+
     numCoilsType nCoils_ = 1;
     std::shared_ptr<SensitivityMapsData> sens_;
     std::shared_ptr<SamplingMasksData> masks_;

@@ -33,6 +33,10 @@ struct Var {
     std::vector<uint8_t> asUint8() const;
 };
 
+/// True when the file starts with a MAT v5 header (the reference tells MAT input from CFL input by whether libmatio
+/// opens it, KData.cpp:300-302).
+bool isMatFile(const std::string& fileName);
+
 /// Reads every numeric variable of a MAT v5 file (other variable kinds are skipped).  Throws std::invalid_argument
 /// when the file is not a little-endian MAT v5 file.
 std::map<std::string, Var> read(const std::string& fileName);

@@ -1,5 +1,5 @@
-// MAT v5 file constructors and matlabSave of KData / XData (KData.cpp:292-435,837-936, XData.cpp:500-556,898-930 of
-// the reference).  File layout, as the reference reads and writes it:
+// File constructors of KData / XData, matlabSave and KData::saveCFLData (KData.cpp:246-435,688-710,837-936,
+// XData.cpp:500-556,898-930 of the reference).  MAT v5 layout, as the reference reads and writes it:
 //   Dims            uint32 1 x n: [nSpatial, allSizesEqual, nCoils, nTemporal, temporal..., spatial...], spatial sizes
 //                   in OpenCLIPER order (width first) (MatVarDimsData.cpp:39-58,106-127)
 //   KData           single complex [H, W, (D...), nCoils, temporal...]; a file may leave the coil dimension out when
@@ -14,6 +14,7 @@
 #include <OpenCLIPER/MatFile.hpp>
 #include <OpenCLIPER/XData.hpp>
 
+#include <fstream>
 #include <sstream>
 
 namespace OpenCLIPER {
@@ -146,6 +147,12 @@ mat::Var complexVar(const std::string& name, const std::vector<uint64_t>& dims, 
 }  // namespace
 
 KData::KData(const std::shared_ptr<CLapp>& app, const std::string& fileName) : Data(app, TYPEID_COMPLEX) {
+    if (mat::isMatFile(fileName)) loadMat(fileName);
+    else loadCFL(fileName);
+}
+
+void KData::loadMat(const std::string& fileName) {
+    const std::shared_ptr<CLapp>& app = app_;
     const std::map<std::string, mat::Var> vars = mat::read(fileName);
     const FileDims fd = readDims(vars);
     if (fd.nCoils < 1) throw std::invalid_argument("KData: Dims variable holds no coils");
@@ -205,6 +212,142 @@ KData::KData(const std::shared_ptr<CLapp>& app, const std::string& fileName) : D
         auto* mdyn = new std::vector<dimIndexType>(dynDims_);
         masks_ = std::make_shared<SamplingMasksData>(app, masks, mdyn, fd.spatial[0]);
     }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// CFL file sets (KData.cpp:246-345,688-710, Data.cpp:1071-1178): raw little-endian NDArrays back to back in the device
+// order (width fastest, coil fastest across NDArrays), and a text header "<width> <height> <nCoils> <frames> 1" whose
+// first non-comment line is read.
+// ---------------------------------------------------------------------------------------------------------------------
+namespace {
+
+std::string baseName(const std::string& fileName) {
+    const size_t dot = fileName.find_last_of('.');
+    return dot == std::string::npos ? fileName : fileName.substr(0, dot);
+}
+
+bool fileExists(const std::string& name) { return std::ifstream(name).good(); }
+
+template <class T>
+std::vector<NDArray*>* readRawArrays(const std::string& name, const std::vector<dimIndexType>& dims, size_t n) {
+    std::ifstream f(name, std::ios::binary);
+    if (!f) throw std::invalid_argument(name + " cannot be read");
+    auto* arrays = new std::vector<NDArray*>();
+    try {
+        for (size_t i = 0; i < n; ++i) {
+            auto* d = new std::vector<dimIndexType>(dims);
+            auto* a = new ConcreteNDArray<T>(d);
+            arrays->push_back(a);
+            const std::streamsize bytes = (std::streamsize)(a->size() * sizeof(T));
+            if (!f.read((char*)a->getHostDataAsVoidPtr(), bytes))
+                throw std::invalid_argument(name + ": file too short for " + std::to_string(n) + " arrays");
+        }
+    } catch (...) {
+        for (NDArray* a : *arrays) delete a;
+        delete arrays;
+        throw;
+    }
+    return arrays;
+}
+
+void writeRaw(std::ofstream& f, const Data& d, size_t elemBytes) {
+    for (dimIndexType i = 0; i < d.getNumNDArrays(); ++i)
+        f.write((const char*)d.getHostBuffer(i), (std::streamsize)(d.getNDArray(i)->size() * elemBytes));
+}
+
+}  // namespace
+
+void KData::loadCFL(const std::string& fileName) {
+    const std::string base = baseName(fileName);
+    const std::string hdr = base + ".hdr";
+    std::ifstream h(hdr);
+    if (!h) throw std::invalid_argument(hdr + " cannot be read (neither a MAT v5 file nor a CFL set)");
+    dimIndexType W = 0, H = 0, C = 0, F = 0;
+    std::string line;
+    bool found = false;
+    while (std::getline(h, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        if (!(iss >> W >> H >> C >> F))
+            throw std::invalid_argument("Invalid line format: " + line +
+                                        " (should be <width> <depth> <numCoils> <numFrames>)");
+        found = true;
+        break;
+    }
+    if (!found || W == 0 || H == 0 || C == 0 || F == 0)
+        throw std::invalid_argument(hdr + ": no valid <width> <height> <numCoils> <numFrames> line");
+    nCoils_ = C;
+    dynDims_ = {F};
+    const std::vector<dimIndexType> spatial = {W, H};
+    std::vector<NDArray*>* arrays = readRawArrays<complexType>(fileName, spatial, (size_t)C * F);
+    setNDArrays(arrays, true);
+
+    const std::string sensName = base + "_SensitivityMaps.cfl";
+    if (fileExists(sensName)) {
+        std::vector<NDArray*>* maps = readRawArrays<complexType>(sensName, spatial, C);
+        sens_ = std::make_shared<SensitivityMapsData>(app_, maps, nCoils_);
+    }
+    const std::string rowName = base + "_SamplingMasks.u32", pixName = base + "_SamplingMasks.u8";
+    std::vector<NDArray*>* masks = nullptr;
+    if (fileExists(rowName)) {
+        // ROWMASK: H uint32 entries (0 / 1) per frame, expanded to whole rows (SamplingMasksData.cpp:389-441)
+        std::vector<NDArray*>* rows = readRawArrays<uint32_t>(rowName, {H}, F);
+        masks = new std::vector<NDArray*>();
+        for (NDArray* a : *rows) {
+            const uint32_t* r = (const uint32_t*)a->getHostDataAsVoidPtr();
+            auto* d = new std::vector<dimIndexType>({H});
+            auto* m = new std::vector<uint8_t>(H);
+            for (dimIndexType i = 0; i < H; ++i) {
+                if (r[i] > 1) {
+                    const uint32_t bad = r[i];
+                    for (NDArray* b : *rows) delete b;
+                    delete rows;
+                    for (NDArray* b : *masks) delete b;
+                    delete masks;
+                    delete d;
+                    delete m;
+                    throw std::invalid_argument("sampling maks data element must be 0 or 1 (it is " +
+                                                std::to_string(bad) + ")");
+                }
+                (*m)[i] = (uint8_t)r[i];
+            }
+            masks->push_back(new ConcreteNDArray<uint8_t>(d, m));
+        }
+        for (NDArray* a : *rows) delete a;
+        delete rows;
+    } else if (fileExists(pixName)) {
+        masks = readRawArrays<uint8_t>(pixName, spatial, F);
+    }
+    if (masks) {
+        auto* mdyn = new std::vector<dimIndexType>(dynDims_);
+        masks_ = std::make_shared<SamplingMasksData>(app_, masks, mdyn, W);
+    }
+}
+
+void KData::saveCFLData(const std::string& baseFileName) {
+    const std::vector<dimIndexType>& spatial = *getNDArray(0)->getDims();
+    if (!getAllSizesEqual() || spatial.size() != 2 || dynDims_.size() != 1)
+        throw std::invalid_argument("KData::saveCFLData: equal-size 2-D NDArrays with one temporal dimension only");
+    device2Host();
+    {
+        std::ofstream f(baseFileName + ".cfl", std::ios::binary | std::ios::trunc);
+        if (!f) throw std::invalid_argument(baseFileName + ".cfl cannot be created");
+        writeRaw(f, *this, sizeof(complexType));
+    }
+    if (sens_) {
+        sens_->device2Host();
+        std::ofstream f(baseFileName + "_SensitivityMaps.cfl", std::ios::binary | std::ios::trunc);
+        writeRaw(f, *sens_, sizeof(complexType));
+    }
+    if (masks_) {
+        masks_->device2Host();
+        std::ofstream f(baseFileName + "_SamplingMasks.u8", std::ios::binary | std::ios::trunc);
+        writeRaw(f, *masks_, 1);
+    }
+    std::ofstream h(baseFileName + ".hdr", std::ios::trunc);
+    if (!h) throw std::invalid_argument(baseFileName + ".hdr cannot be created");
+    for (dimIndexType d : spatial) h << d << " ";
+    h << nCoils_ << " " << dynDims_[0] << " 1\n";
 }
 
 void KData::matlabSave(const std::string& fileName, bool saveSamplingMasks) {

@@ -237,6 +237,13 @@ std::vector<uint8_t> Var::asUint8() const {
     return out;
 }
 
+bool isMatFile(const std::string& fileName) {
+    std::ifstream f(fileName, std::ios::binary);
+    uint8_t h[128];
+    if (!f.read((char*)h, sizeof(h))) return false;
+    return h[124] == 0x00 && h[125] == 0x01 && h[126] == 'I' && h[127] == 'M';
+}
+
 std::map<std::string, Var> read(const std::string& fileName) {
     std::ifstream f(fileName, std::ios::binary);
     if (!f) throw std::invalid_argument("MAT-file: cannot open " + fileName);

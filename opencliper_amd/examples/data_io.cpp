@@ -1,6 +1,7 @@
-// Loads KData or XData from a MAT v5 file through the OpenCLIPER-compatible API, prints what was loaded as one JSON
-// line and saves it again with matlabSave:
-//   data_io kdata <in.mat> <out.mat> [masks]   (masks: also write the sampling masks)
+// Loads KData or XData from a file through the OpenCLIPER-compatible API, prints what was loaded as one JSON line and
+// saves it again:
+//   data_io kdata <in.mat|in.cfl> <out.mat> [masks]   matlabSave (masks: also write the sampling masks)
+//   data_io kdata <in.mat|in.cfl> <out.cfl>           saveCFLData(<out>)
 //   data_io xdata <in.mat> <out.mat>
 #include <OpenCLIPER/KData.hpp>
 #include <OpenCLIPER/XData.hpp>
@@ -8,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <string>
 
 using namespace OpenCLIPER;
 
@@ -32,7 +34,11 @@ int main(int argc, char* argv[]) {
             printDims("temporal", *k->getDynDims());
             std::printf(", \"sens\": %s, \"masks\": %s}\n", k->getSensitivityMapsData() ? "true" : "false",
                         k->getSamplingMasksData() ? "true" : "false");
-            k->matlabSave(argv[3], argc > 4 && !std::strcmp(argv[4], "masks"));
+            const std::string out = argv[3];
+            if (out.size() > 4 && out.compare(out.size() - 4, 4, ".cfl") == 0)
+                k->saveCFLData(out.substr(0, out.size() - 4));
+            else
+                k->matlabSave(out, argc > 4 && !std::strcmp(argv[4], "masks"));
         } else if (!std::strcmp(argv[1], "xdata")) {
             auto x = std::make_shared<XData>(pCLapp, std::string(argv[2]));
             std::printf("{\"kind\": \"xdata\", \"narrays\": %u, ", x->getNumNDArrays());

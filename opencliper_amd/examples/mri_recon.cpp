@@ -3,9 +3,9 @@
 // parameters and launch it.  Input and output are raw little-endian files so the Python tests can drive it:
 //   mri_recon <in.raw> <out.raw> F C H W maxIntIter maxIter stoptest miniter form [profile]
 // in.raw holds y [F][C][H][W] complex64, S [C][H][W] complex64, M [F][H][W] uint8; out.raw receives x [F][H][W].
-// With MAT v5 files (KData with SensitivityMaps and SamplingMasks in, XData out, as the reference's KData / XData
-// file constructors and matlabSave lay them out):
-//   mri_recon <in.mat> <out.mat> maxIntIter maxIter stoptest miniter form [profile]
+// With files in the reference's formats (KData with SensitivityMaps and SamplingMasks in, from a MAT v5 file or a CFL
+// set; XData out as a MAT v5 file), as the reference's KData / XData file constructors and matlabSave lay them out:
+//   mri_recon <in.mat|in.cfl> <out.mat> maxIntIter maxIter stoptest miniter form [profile]
 // form: 0 plain, 1 fused, 2 split precision.
 #include <OpenCLIPER/processes/nesta/NestaUp.hpp>
 #include <OpenCLIPER/processes/nesta/TemporalTV.hpp>
@@ -59,16 +59,16 @@ static std::shared_ptr<KData> readRaw(const std::shared_ptr<CLapp>& pCLapp, char
     return pKData;
 }
 
-static bool isMat(const char* name) {
+static bool isDataFile(const char* name) {
     const size_t n = std::strlen(name);
-    return n > 4 && !std::strcmp(name + n - 4, ".mat");
+    return n > 4 && (!std::strcmp(name + n - 4, ".mat") || !std::strcmp(name + n - 4, ".cfl"));
 }
 
 int main(int argc, char* argv[]) {
-    const bool mat = argc > 1 && isMat(argv[1]);
+    const bool mat = argc > 1 && isDataFile(argv[1]);
     if (argc < (mat ? 8 : 12)) {
         std::cerr << "usage: mri_recon in.raw out.raw F C H W maxIntIter maxIter stoptest miniter form [profile]\n"
-                     "       mri_recon in.mat out.mat maxIntIter maxIter stoptest miniter form [profile]\n";
+                     "       mri_recon in.mat|in.cfl out.mat maxIntIter maxIter stoptest miniter form [profile]\n";
         return 2;
     }
     const int a0 = mat ? 3 : 7;   // first launch-parameter argument

@@ -1,5 +1,6 @@
-"""KData / XData MAT v5 file constructors and matlabSave (KData.cpp:292-435,837-936, XData.cpp:500-556,898-930 of the
-reference) through the C++ API, and the MRIRecon-shaped driver reading KData from and writing XData to .mat files.
+"""KData / XData file constructors, matlabSave and saveCFLData (KData.cpp:246-435,688-710,837-936,
+XData.cpp:500-556,898-930 of the reference) through the C++ API, and the MRIRecon-shaped driver reading KData from
+.mat files or CFL sets and writing XData to .mat files.
 Files are written and read with scipy.io in the reference's layout: Dims = [nSpatial, allSizesEqual, nCoils,
 nTemporal, temporal..., spatial (width first)...], arrays column-major [H, W, coils, frames]."""
 import json
@@ -163,5 +164,88 @@ def test_driver_mat_matches_oracle(oracle, tmp_path, form, rowmask):
         fh.write(np.ascontiguousarray(sl.S, np.complex64).tobytes())
         fh.write(np.ascontiguousarray(np.where(sl.M != 0, 255, 0), np.uint8).tobytes())
     r = _run([DRIVER, raw_in, raw_out, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "3", "12", "1", "7", str(form)])
+    assert r.returncode == 0
+    assert np.array_equal(x, np.fromfile(raw_out, dtype=np.complex64).reshape(sl.F, sl.H, sl.W))
+
+
+def write_cfl_set(base, y, S=None, M=None, rowmask=False, comment=True):
+    """The reference's CFL set (KData.cpp:246-345): raw NDArrays in device order and a "W H C F 1" header."""
+    F, C, H, W = y.shape
+    with open(base + ".hdr", "w") as fh:
+        if comment:
+            fh.write("# width height coils frames\n")
+        fh.write(f"{W} {H} {C} {F} 1\n")
+    np.ascontiguousarray(y, np.complex64).tofile(base + ".cfl")
+    if S is not None:
+        np.ascontiguousarray(S, np.complex64).tofile(base + "_SensitivityMaps.cfl")
+    if M is not None:
+        if rowmask:
+            np.ascontiguousarray(M[:, :, 0] != 0, np.uint32).tofile(base + "_SamplingMasks.u32")
+        else:
+            np.ascontiguousarray(M, np.uint8).tofile(base + "_SamplingMasks.u8")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rowmask", [True, False])
+def test_kdata_cfl_set_round_trip(tmp_path, rowmask):
+    y, S, M = _kdata_case(seed=10)
+    F, C, H, W = y.shape
+    base = str(tmp_path / "k")
+    write_cfl_set(base, y, S if rowmask else None, M, rowmask=rowmask)
+    r = _run([DATA_IO, "kdata", base + ".cfl", str(tmp_path / "k_out.mat"), "masks"])
+    assert r.returncode == 0
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info == {"kind": "kdata", "ncoils": C, "narrays": F * C, "spatial": [W, H], "temporal": [F],
+                    "sens": rowmask, "masks": True}
+    back = sio.loadmat(str(tmp_path / "k_out.mat"))
+    assert np.array_equal(back["KData"], np.transpose(y, (2, 3, 1, 0)))
+    assert np.array_equal(back["SamplingMasks"], np.transpose(M, (1, 2, 0)))
+    # and out again as a CFL set (KData.cpp:688-710): masks in the pixel-mask form
+    out = str(tmp_path / "k2")
+    r = _run([DATA_IO, "kdata", base + ".cfl", out + ".cfl"])
+    assert r.returncode == 0
+    assert open(out + ".hdr").read() == f"{W} {H} {C} {F} 1\n"
+    assert np.array_equal(np.fromfile(out + ".cfl", np.complex64), y.ravel())
+    assert np.array_equal(np.fromfile(out + "_SamplingMasks.u8", np.uint8), M.ravel())
+    assert os.path.exists(out + "_SensitivityMaps.cfl") == rowmask
+    if rowmask:
+        assert np.array_equal(np.fromfile(out + "_SensitivityMaps.cfl", np.complex64), S.ravel())
+
+
+@pytest.mark.gpu
+def test_cfl_errors(tmp_path):
+    y, S, M = _kdata_case(seed=11)
+    base = str(tmp_path / "k")
+    np.ascontiguousarray(y, np.complex64).tofile(base + ".cfl")
+    r = _run([DATA_IO, "kdata", base + ".cfl", str(tmp_path / "o.mat")])
+    assert r.returncode == 1 and "neither a MAT v5 file nor a CFL set" in r.stderr
+    write_cfl_set(base, y[:, :, :-1], comment=False)
+    F, C, H, W = y.shape
+    with open(base + ".hdr", "w") as fh:
+        fh.write(f"{W} {H} {C} {F} 1\n")
+    r = _run([DATA_IO, "kdata", base + ".cfl", str(tmp_path / "o.mat")])
+    assert r.returncode == 1 and "file too short" in r.stderr
+    write_cfl_set(base, y, M=M, rowmask=True)
+    bad = np.ascontiguousarray(M[:, :, 0] != 0, np.uint32)
+    bad[0, 0] = 3
+    bad.tofile(base + "_SamplingMasks.u32")
+    r = _run([DATA_IO, "kdata", base + ".cfl", str(tmp_path / "o.mat")])
+    assert r.returncode == 1 and "must be 0 or 1" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_cfl_input_equals_raw(tmp_path):
+    sl = synth.make_cine(H=64, W=96, C=3, F=6, accel=3.0, n_centre=4, slice_id=44)
+    base = str(tmp_path / "in")
+    write_cfl_set(base, sl.y, sl.S, sl.M, rowmask=True)
+    r = _run([DRIVER, base + ".cfl", str(tmp_path / "out.mat"), "3", "12", "1", "7", "1"])
+    assert r.returncode == 0
+    x = np.transpose(sio.loadmat(str(tmp_path / "out.mat"))["XData"], (2, 0, 1))
+    raw_in, raw_out = str(tmp_path / "in.raw"), str(tmp_path / "out.raw")
+    with open(raw_in, "wb") as fh:
+        fh.write(np.ascontiguousarray(sl.y, np.complex64).tobytes())
+        fh.write(np.ascontiguousarray(sl.S, np.complex64).tobytes())
+        fh.write(np.ascontiguousarray(np.where(sl.M != 0, 255, 0), np.uint8).tobytes())
+    r = _run([DRIVER, raw_in, raw_out, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "3", "12", "1", "7", "1"])
     assert r.returncode == 0
     assert np.array_equal(x, np.fromfile(raw_out, dtype=np.complex64).reshape(sl.F, sl.H, sl.W))
