@@ -133,6 +133,28 @@ int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
 int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
                   int W, const ocl_nesta_params* prm, int form, ocl_nesta_report* report);
 
+/* ---- motion compensation (ArgumentsMotionCompensation, include/OpenCLIPER/processes/GroupwiseRegistration.hpp:95-100;
+ * MotionCompensation.cpp, AdjointMotionCompensation.cpp, src/kernels/optimizer.cl:832-930) ----
+ * xn: device float [2][F][H][W], deformed sample positions, 1-based ([0] column, [1] row);
+ * x:  device int32 [2][W][H], the original mesh, 1-based, element [r + c*H] = (column + 1, row + 1);
+ * bound_box: 1-based, as the reference's Interpolator reads it: box rows bound_box[1] - m0 - 2 ..
+ * bound_box[3] + m0 and columns bound_box[0] - m1 - 2 .. bound_box[2] + m1 (0-based), m0 = H / 40, m1 = W / 40.
+ * A box that leaves the frame is OCL_E_INVALID.  Neighbours outside the frame read as zero. */
+typedef struct {
+    const float* xn;
+    const int* x;
+    unsigned bound_box[4];
+} ocl_mc_args;
+/* MotionCompensation::launch: out = in, then the box pixels resampled bilinearly at xn (in != out) */
+int ocl_motion_compensation(ocl_ctx* ctx, const void* in, void* out, int F, int H, int W, const ocl_mc_args* mc);
+/* AdjointMotionCompensation::launch: out = in with the box cleared, then the transpose scatter (float atomics) */
+int ocl_adjoint_motion_compensation(ocl_ctx* ctx, const void* in, void* out, int F, int H, int W,
+                                    const ocl_mc_args* mc);
+/* NestaUp::launch with argsMC != nullptr (NestaUp.cpp:201-278,483-491): U = tTV(MC(x)), U^T = MC^T(tTV^T(u)), and
+ * x_io holds x_ref on entry (the previous reconstruction) instead of A^H y.  form must be OCL_FORM_PLAIN. */
+int ocl_nesta_run_mc(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_io, int F, int C, int H,
+                     int W, const ocl_nesta_params* prm, int form, const ocl_mc_args* mc, ocl_nesta_report* report);
+
 /* ---- batched NESTA-TV: many uneven slices advanced together, one launch per pass per iteration ---- */
 int ocl_batch_create(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
                      ocl_batch** out);

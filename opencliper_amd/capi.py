@@ -67,6 +67,12 @@ class Slice(ctypes.Structure):
                 ("F", ctypes.c_int), ("C", ctypes.c_int), ("H", ctypes.c_int), ("W", ctypes.c_int)]
 
 
+class McArgs(ctypes.Structure):
+    """ocl_mc_args: device pointers to the deformed positions xn [2][F][H][W] float32 and the original mesh
+    x [2][W][H] int32 (both 1-based), and the 1-based bound box."""
+    _fields_ = [("xn", ctypes.c_void_p), ("x", ctypes.c_void_p), ("bound_box", ctypes.c_uint * 4)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_long), ("failed", ctypes.c_long), ("queued", ctypes.c_long),
                 ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
@@ -116,6 +122,10 @@ def lib():
         "ocl_normest": ([P, I, LG, ctypes.POINTER(D), ctypes.POINTER(I)], I),
         "ocl_nesta_run": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(NestaReport)],
                           I),
+        "ocl_motion_compensation": ([P, P, P, I, I, I, ctypes.POINTER(McArgs)], I),
+        "ocl_adjoint_motion_compensation": ([P, P, P, I, I, I, ctypes.POINTER(McArgs)], I),
+        "ocl_nesta_run_mc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(McArgs),
+                              ctypes.POINTER(NestaReport)], I),
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_ctx_set_autotune": ([P, I], I),
@@ -330,6 +340,29 @@ class Context:
         rep = NestaReport()
         _check(lib().ocl_nesta_run(self.handle, y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W, ctypes.byref(prm), form,
                                    ctypes.byref(rep)))
+        return rep.as_dict()
+
+    # --- motion compensation ---
+    @staticmethod
+    def mc_args(xn: DeviceArray, mesh: DeviceArray, bound_box) -> McArgs:
+        a = McArgs(xn.ptr, mesh.ptr)
+        for i, v in enumerate(bound_box):
+            a.bound_box[i] = int(v)
+        return a
+
+    def motion_compensation(self, x: DeviceArray, out: DeviceArray, mc: McArgs, adjoint=False):
+        F, H, W = x.shape
+        fn = lib().ocl_adjoint_motion_compensation if adjoint else lib().ocl_motion_compensation
+        _check(fn(self.handle, x.ptr, out.ptr, F, H, W, ctypes.byref(mc)))
+
+    def nesta_mc(self, y: DeviceArray, S: DeviceArray, M: DeviceArray, x: DeviceArray, mc: McArgs,
+                 prm: NestaParams | None = None, form=OCL_FORM_PLAIN):
+        """NESTA with motion compensation: x holds x_ref on entry (the previous reconstruction)."""
+        prm = prm or nesta_params()
+        F, C, H, W = y.shape
+        rep = NestaReport()
+        _check(lib().ocl_nesta_run_mc(self.handle, y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W, ctypes.byref(prm), form,
+                                      ctypes.byref(mc), ctypes.byref(rep)))
         return rep.as_dict()
 
     def batch(self, slices, prm: NestaParams | None = None, form=OCL_FORM_FUSED) -> "Batch":

@@ -471,16 +471,104 @@ extern "C" int ocl_simple_mri_recon(ocl_ctx* c, void* y, const void* S, void* x,
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
+// Motion compensation (MotionCompensation.cpp, AdjointMotionCompensation.cpp; kernels in ocl_mc.hip)
+// ---------------------------------------------------------------------------------------------------------------------
+struct McGeom {
+    int r0, nr, c0, nc;
+};
+
+// Box rows / columns as Interpolator.cpp:31-47 builds r1margin / r2margin from the 1-based bound box
+static int mc_geometry(int H, int W, const ocl_mc_args* mc, McGeom* g) {
+    if (!mc || !mc->xn || !mc->x) return fail(OCL_E_INVALID, "motion compensation: null deformation arrays");
+    const long m0 = H / 40, m1 = W / 40;
+    const long r0 = (long)mc->bound_box[1] - m0 - 2, nr = (long)mc->bound_box[3] - (long)mc->bound_box[1] + 2 * m0 + 3;
+    const long c0 = (long)mc->bound_box[0] - m1 - 2, nc = (long)mc->bound_box[2] - (long)mc->bound_box[0] + 2 * m1 + 3;
+    if (nr < 1 || nc < 1 || r0 < 0 || c0 < 0 || r0 + nr > H || c0 + nc > W)
+        return fail(OCL_E_INVALID, "motion compensation: the bound box with its margins leaves the frame");
+    *g = McGeom{(int)r0, (int)nr, (int)c0, (int)nc};
+    return OCL_OK;
+}
+
+static int grid_box(const McGeom& g, int F) { return grid_for((long)g.nr * g.nc * F); }
+
+static int mc_apply(ocl_ctx* c, const float2* in, float2* out, int F, int H, int W, const ocl_mc_args* mc,
+                    const McGeom& g, bool adjoint) {
+    const long n = (long)F * H * W;
+    if (!adjoint) {
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, in, out, n);
+        hipLaunchKernelGGL(k_mc_interp, dim3(grid_box(g, F)), dim3(OCL_NT), 0, c->stream, in, out, mc->xn, mc->x, F,
+                           H, W, g.r0, g.nr, g.c0, g.nc);
+    } else {
+        hipLaunchKernelGGL(k_mc_copy_clear_box, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, in, out, F, H, W, g.r0,
+                           g.nr, g.c0, g.nc);
+        hipLaunchKernelGGL(k_mc_scatter, dim3(grid_box(g, F)), dim3(OCL_NT), 0, c->stream, in, out, mc->xn, mc->x, F,
+                           H, W, g.r0, g.nr, g.c0, g.nc);
+    }
+    HIPCHK(hipGetLastError());
+    return OCL_OK;
+}
+
+static int mc_entry(ocl_ctx* c, const void* in, void* out, int F, int H, int W, const ocl_mc_args* mc, bool adjoint) {
+    CTXCHK(c);
+    if (!in || !out || F < 1 || H < 1 || W < 1) return fail(OCL_E_INVALID, "motion compensation: bad arguments");
+    if (in == out) return fail(OCL_E_INVALID, "motion compensation: in and out must not alias");
+    McGeom g;
+    CHK(mc_geometry(H, W, mc, &g));
+    HIPCHK(hipSetDevice(c->device));
+    return mc_apply(c, (const float2*)in, (float2*)out, F, H, W, mc, g, adjoint);
+}
+
+extern "C" int ocl_motion_compensation(ocl_ctx* c, const void* in, void* out, int F, int H, int W,
+                                       const ocl_mc_args* mc) {
+    return mc_entry(c, in, out, F, H, W, mc, false);
+}
+
+extern "C" int ocl_adjoint_motion_compensation(ocl_ctx* c, const void* in, void* out, int F, int H, int W,
+                                               const ocl_mc_args* mc) {
+    return mc_entry(c, in, out, F, H, W, mc, true);
+}
+
+// The sparsity operator and its adjoint (NestaUp::operatorU / operatorUt, NestaUp.cpp:201-278); mc == nullptr is
+// the plain temporal TV.  aux: one image-sized scratch buffer.
+struct SparsityOp {
+    ocl_ctx* c;
+    int F, H, W;
+    const ocl_mc_args* mc;
+    McGeom g;
+    float2* aux;
+    int U(const float2* x, float2* out) const {
+        const long N = (long)H * W;
+        if (!mc) return ocl_ttv(c, x, out, F, N, 1);
+        CHK(mc_apply(c, x, aux, F, H, W, mc, g, false));
+        return ocl_ttv(c, aux, out, F, N, 1);
+    }
+    // tTV^T into tmp, then MC^T (or a copy) into out
+    int Ut(const float2* u, float2* tmp, float2* out) const {
+        const long N = (long)H * W;
+        CHK(ocl_ttv(c, u, tmp, F, N, 0));
+        if (!mc) return ocl_copy(c, tmp, out, (long)F * N);
+        return mc_apply(c, tmp, out, F, H, W, mc, g, true);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------------------
 // myNormest (NestaUp.cpp:291-439) on the device, FP32 vectors and float scalars as in the reference
 // ---------------------------------------------------------------------------------------------------------------------
 static const unsigned long long DRAND48_X0 = 0x1234ABCD330EULL;
 
-static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters) {
+// op: the sparsity operator with motion compensation (null: plain temporal TV over F frames of N pixels)
+static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, const SparsityOp* op = nullptr) {
     const long n = (long)F * N;
     float2 *x = nullptr, *Sx = nullptr;
     HIPCHK(hipMalloc(&x, sizeof(float2) * n));
     if (hipMalloc(&Sx, sizeof(float2) * n) != hipSuccess) {
         hipFree(x);
+        return fail(OCL_E_NOMEM, "normest: out of device memory");
+    }
+    float2* tmp = nullptr;
+    if (op && hipMalloc(&tmp, sizeof(float2) * n) != hipSuccess) {
+        hipFree(x);
+        hipFree(Sx);
         return fail(OCL_E_NOMEM, "normest: out of device memory");
     }
     int rc = OCL_OK;
@@ -495,7 +583,7 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters) {
     if ((rc = ocl_scal(c, x, n, 1.0f / e, 0.f)) != OCL_OK) goto done;
     while ((std::abs(e - e0) > (tol * e)) && (cnt < 100u)) {
         e0 = e;
-        if ((rc = ocl_ttv(c, x, Sx, F, N, 1)) != OCL_OK) goto done;
+        if ((rc = op ? op->U(x, Sx) : ocl_ttv(c, x, Sx, F, N, 1)) != OCL_OK) goto done;
         if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_ASUM, r)) != OCL_OK) goto done;
         if ((float)r[0] == 0.f) {
             const long chunks = (n + 63) / 64;
@@ -505,7 +593,7 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters) {
         }
         if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
         e = (float)r[0];
-        if ((rc = ocl_ttv(c, Sx, x, F, N, 0)) != OCL_OK) goto done;
+        if ((rc = op ? op->Ut(Sx, tmp, x) : ocl_ttv(c, Sx, x, F, N, 0)) != OCL_OK) goto done;
         if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
         if ((rc = ocl_scal(c, x, n, 1.0f / (float)r[0], 0.f)) != OCL_OK) goto done;
         ++cnt;
@@ -516,6 +604,7 @@ done:
     hipStreamSynchronize(c->stream);
     hipFree(x);
     hipFree(Sx);
+    if (tmp) hipFree(tmp);
     return rc;
 }
 
@@ -544,8 +633,10 @@ extern "C" int ocl_normest(ocl_ctx* c, int F, long N, double* normU, int* iters)
 // ---------------------------------------------------------------------------------------------------------------------
 // NESTA — PLAIN form: the reference's op sequence, float scalars, three host reads per iteration
 // ---------------------------------------------------------------------------------------------------------------------
+// mc != nullptr: the argsMC branch; xref then holds x_ref on entry (the previous reconstruction) and U includes the
+// motion compensation.
 static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* M, void* xref, int F, int C, int H,
-                       int W, const ocl_nesta_params& prm, ocl_nesta_report* rep) {
+                       int W, const ocl_nesta_params& prm, ocl_nesta_report* rep, const ocl_mc_args* mc = nullptr) {
     const long N = (long)H * W, nX = (long)F * N, nK = (long)F * C * N;
     std::vector<void*> bufs;
     auto alloc = [&](long n) -> float2* {
@@ -565,10 +656,13 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     float2 *yS = alloc(nK), *aux = alloc(nK), *Res = alloc(nK), *AuxRes = alloc(nK);
     float2 *Ux = alloc(nX), *Wk = alloc(nX), *Xk = alloc(nX), *Yk = alloc(nX), *Zk = alloc(nX), *Uk = alloc(nX),
            *AuxFx = alloc(nX), *Df = alloc(nX), *ARes = alloc(nX), *tmp = alloc(nX);
+    float2* mcAux = mc ? alloc(nX) : nullptr;
     for (void* p : bufs)
         if (!p) return fail(OCL_E_NOMEM, "NESTA plain: out of device memory");
-    if (bufs.size() != 14) return fail(OCL_E_NOMEM, "NESTA plain: out of device memory");
+    if (bufs.size() != (mc ? 15u : 14u)) return fail(OCL_E_NOMEM, "NESTA plain: out of device memory");
     float2* X = (float2*)xref;
+    SparsityOp sp{c, F, H, W, mc, {}, mcAux};
+    if (mc) CHK(mc_geometry(H, W, mc, &sp.g));
 
     auto opAt = [&](const float2* r, float2* xo) -> int {            // NestaUp.cpp:117-151
         CHK(fft2d(c, r, aux, (long)F * C, H, W, +1));
@@ -585,8 +679,8 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     CHK(ocl_copy(c, y, yS, nK));                                      // :468-481
     const float sqrtN = std::sqrt((float)(H * W));
     CHK(ocl_scal(c, yS, nK, sqrtN, 0.f));
-    CHK(opAt(yS, X));                                                 // :484-486
-    CHK(ocl_ttv(c, X, Ux, F, N, 1));                                  // :491
+    if (!mc) CHK(opAt(yS, X));                                        // :483-486
+    CHK(sp.U(X, Ux));                                                 // :491
     CHK(ocl_reduce(c, X, nullptr, nX, RED_MAXABS, r));                // :521-545
     const float maxXref = (float)r[0];
     CHK(ocl_reduce(c, Ux, nullptr, nX, RED_MAXABS, r));               // :547-568
@@ -597,7 +691,8 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     float mu0 = (float)(0.9 * (double)maxUXref);
     double nU = 0;
     int ne_it = 0;
-    CHK(ocl_normest(c, F, N, &nU, &ne_it));
+    if (mc) CHK(normest_compute(c, F, N, &nU, &ne_it, &sp));
+    else CHK(ocl_normest(c, F, N, &nU, &ne_it));
     const float normU = (float)nU;
     const float muL = lambda / (float)prm.La;
     if (muL > mu0) mu0 = muL;
@@ -629,13 +724,12 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
         int done_k = 0;
         for (int k = 0; k < prm.maxIter; ++k) {
             ++done_k;
-            CHK(ocl_ttv(c, Xk, Uk, F, N, 1));                         // :669
+            CHK(sp.U(Xk, Uk));                                        // :669
             CHK(ocl_copy(c, Uk, AuxFx, nX));                          // :675
             CHK(ocl_vecnorm(c, Uk, Uk, nX, mu));                      // :680-683
             CHK(reduce_async(c, Uk, nullptr, nX, RED_NRM2, dres));    // :686
             CHK(reduce_async(c, Uk, AuxFx, nX, RED_DOTC, dres + 2));  // :691
-            CHK(ocl_ttv(c, Uk, tmp, F, N, 0));                        // :696
-            CHK(ocl_copy(c, tmp, Df, nX));
+            CHK(sp.Ut(Uk, tmp, Df));                                  // :696
             CHK(ocl_scal(c, Df, nX, -1.f, 0.f));                      // :698-704
             CHK(opA(Xk, Res));                                        // :710
             CHK(ocl_scal(c, Res, nK, invSqrtN, 0.f));                 // :712-718
@@ -715,4 +809,21 @@ extern "C" int ocl_nesta_run(ocl_ctx* c, const void* y, const void* S, const uin
     if (rc == OCL_OK && rep) rc = ocl_batch_report(b, 0, rep);
     ocl_batch_destroy(b);
     return rc;
+}
+
+extern "C" int ocl_nesta_run_mc(ocl_ctx* c, const void* y, const void* S, const uint8_t* M, void* x_io, int F, int C,
+                                int H, int W, const ocl_nesta_params* prm, int form, const ocl_mc_args* mc,
+                                ocl_nesta_report* rep) {
+    CTXCHK(c);
+    if (!prm) return fail(OCL_E_INVALID, "NESTA: null launch parameters");
+    if (!y || !x_io || F < 1 || C < 1 || H < 1 || W < 1) return fail(OCL_E_INVALID, "NESTA: bad input/output");
+    if (!S) return fail(OCL_E_INVALID, "NESTA: non-existing SensitivityMaps");
+    if (!M) return fail(OCL_E_INVALID, "NESTA: null sampling masks");
+    if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER) return fail(OCL_E_INVALID, "NESTA: miniter out of range");
+    if (form != OCL_FORM_PLAIN)
+        return fail(OCL_E_UNSUPPORTED, "NESTA with motion compensation runs in the plain form only");
+    McGeom g;
+    CHK(mc_geometry(H, W, mc, &g));
+    HIPCHK(hipSetDevice(c->device));
+    return nesta_plain(c, y, S, M, x_io, F, C, H, W, *prm, rep, mc);
 }

@@ -106,6 +106,13 @@ __global__ void k_fft_cols_fast(const float2* in, float2* out, int W, int P, con
 __global__ void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P, const float2* tw,
                                    float scale);
 
+// kernels (ocl_mc.hip): motion compensation over the box rows r0..r0+nr-1, columns c0..c0+nc-1 of every frame
+__global__ void k_mc_interp(const float2* in, float2* out, const float* xn, const int* x, int F, int H, int W, int r0,
+                            int nr, int c0, int nc);
+__global__ void k_mc_copy_clear_box(const float2* in, float2* out, int F, int H, int W, int r0, int nr, int c0, int nc);
+__global__ void k_mc_scatter(const float2* in, float2* out, const float* xn, const int* x, int F, int H, int W, int r0,
+                             int nr, int c0, int nc);
+
 // kernels (ocl_nesta.hip)
 __global__ void k_rowflags(const float2* y, const uint8_t* M, int F, int C, int H, int W, uint8_t* flags);
 // launch shape of the fused passes over one batch; PH / PW > 0 when every slice shares that fast column / row size

@@ -196,3 +196,32 @@ def device_kspace(ctx, x, S, M, noise=1e-3, seed=BASE_SEED):
     ctx.add_noise_masked(y, dM, noise * peak, seed)
     del dx
     return y, dS, dM
+
+
+def identity_mesh(H, W):
+    """Original mesh of GroupwiseRegistration.cpp:150-164: int32 [2][W][H], element [d][c][r] = (c + 1, r + 1)."""
+    mesh = np.empty((2, W, H), dtype=np.int32)
+    mesh[0] = (np.arange(W, dtype=np.int32) + 1)[:, None]
+    mesh[1] = (np.arange(H, dtype=np.int32) + 1)[None, :]
+    return mesh
+
+
+def motion_fields(H, W, F, amp=1.5, roi=0.6, seed=BASE_SEED, slice_id=0):
+    """Smooth per-frame deformations for motion compensation: sample positions xn float32 [2][F][H][W] (1-based;
+    [0] column, [1] row) displaced by at most `amp` pixels, the identity mesh, and a centred bound box (1-based,
+    [col0, row0, col1, row1] as the interpolator reads it) covering `roi` of each side, with room for the margins
+    (H / 40, W / 40) and the displacement inside the frame."""
+    rng = slice_rng(seed, slice_id ^ 0x3C)
+    r = np.arange(H, dtype=np.float64)[:, None]
+    c = np.arange(W, dtype=np.float64)[None, :]
+    xn = np.empty((2, F, H, W), dtype=np.float32)
+    ph = rng.uniform(0, 2 * np.pi, size=(F, 4))
+    for f in range(F):
+        t = 2 * np.pi * f / F
+        dx = amp * np.sin(2 * np.pi * r / H + ph[f, 0]) * np.cos(2 * np.pi * c / W + ph[f, 1]) * np.cos(t)
+        dy = amp * np.cos(2 * np.pi * r / H + ph[f, 2]) * np.sin(2 * np.pi * c / W + ph[f, 3]) * np.sin(t + 0.5)
+        xn[0, f] = (c + 1 + dx).astype(np.float32)
+        xn[1, f] = (r + 1 + dy).astype(np.float32)
+    hr, hc = int(roi * H / 2), int(roi * W / 2)
+    bb = np.array([W // 2 - hc + 1, H // 2 - hr + 1, W // 2 + hc, H // 2 + hr], dtype=np.uint32)
+    return xn, identity_mesh(H, W), bb

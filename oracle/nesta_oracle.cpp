@@ -22,11 +22,19 @@
 //   simple_recon ......... src/processes/examples/SimpleMRIRecon.cpp:59-101
 //   rsos ................. src/kernels/rss.cl (rss_kernel): sqrt(sum_c |in_c|^2) in the real part, float sum in coil order
 //   simple_recon_sos ..... src/processes/examples/SimpleMRIReconSOS.cpp (in-place inverse FFT, then RSoS)
+//   mc_fwd / mc_adj ...... motion compensation: src/processes/MotionCompensation.cpp (copy, then the interpolator
+//                          kernel, src/kernels/optimizer.cl:832-867) and src/processes/AdjointMotionCompensation.cpp
+//                          (copy, initZeroRect, initialize.cl:55-69, then adjointInterpolator, optimizer.cl:880-930);
+//                          box rows / columns as Interpolator.cpp:31-47 builds r1margin / r2margin
+//   nesta with MC ........ the argsMC != nullptr branch: U = tTV(MC(x)), U^T = MC^T(tTV^T(u)) (NestaUp.cpp:201-278),
+//                          x_ref taken from the output instead of A^H y (:483-486), normest over the same U
 //
 // Pinning (each checked in tests/test_oracle.py): fft vs brute-force DFT and numpy.fft; ttv vs closed form on
 // integers and the adjoint identity; normest's LCG vs libc drand48 from the POSIX default state and its result vs
 // the closed-form sigma_max of the periodic difference; vecnorm vs the Huber identity; op_A/op_At adjointness and
-// E^H E = I when fully sampled; nesta vs an independent numpy FP64 implementation (tests/golden/make_golden.py).
+// E^H E = I when fully sampled; nesta vs an independent numpy FP64 implementation (tests/golden/make_golden.py);
+// mc_fwd vs scipy.ndimage.map_coordinates (order 1), identity and integer-shift meshes, and <MC x, y> = <x, MC^T y>;
+// normest with MC vs the largest singular value of the explicit matrix of U; nesta with an identity MC vs nesta.
 
 #include <cmath>
 #include <complex>
@@ -306,6 +314,116 @@ template <class T> void op_At(const Prob<T>& P, const Cx<T>* r, Cx<T>* x, Cx<T>*
 }
 
 // ------------------------------------------------------------------------------------------------------------------
+// Motion compensation.  Box: rows r0 .. r0+nr-1 and columns c0 .. c0+nc-1 (0-based) from the 1-based bound box
+// [b0, b1, b2, b3] and margins m0 = H / 40, m1 = W / 40 (integer division): r0 = b1 - m0 - 2, nr = b3 - b1 + 2 m0 + 3,
+// c0 = b0 - m1 - 2, nc = b2 - b0 + 2 m1 + 3 (Interpolator.cpp:31-47).  For box pixel (r, c) and frame k the output
+// position is (hor, ver) = mesh x at [r + c H] minus 1, and the sample position (nhor, nver) = xn at
+// [c + r W + k H W] (and + H W F) minus 1.  The reference reads and writes outside the frame when a position leaves
+// it; here such neighbours read as zero and such writes are dropped.
+// ------------------------------------------------------------------------------------------------------------------
+struct McBox { long r0, nr, c0, nc; };
+
+static bool mc_box(int H, int W, const unsigned bb[4], McBox* b) {
+    const long m0 = H / 40, m1 = W / 40;
+    b->r0 = (long)bb[1] - m0 - 2;
+    b->nr = (long)bb[3] - (long)bb[1] + 2 * m0 + 3;
+    b->c0 = (long)bb[0] - m1 - 2;
+    b->nc = (long)bb[2] - (long)bb[0] + 2 * m1 + 3;
+    return b->nr > 0 && b->nc > 0 && b->r0 >= 0 && b->c0 >= 0 && b->r0 + b->nr <= H && b->c0 + b->nc <= W;
+}
+
+struct McOp {
+    const float* xn;   // [2][F][H][W]
+    const int* x;      // [2][W][H]
+    McBox box;
+};
+
+template <class T> struct McSample {
+    bool ok;           // output position inside the frame
+    long out;          // output index within the frame
+    long q[4];         // neighbour indices (-1 outside): left-up, right-up, left-down, right-down
+    T nhor, nver;
+    long prehor, prever;
+};
+
+template <class T> McSample<T> mc_sample(const McOp& m, int F, int H, int W, long r, long c, int k) {
+    const long N = (long)H * W;
+    McSample<T> s;
+    const long hor = (long)m.x[r + c * H] - 1, ver = (long)m.x[r + c * H + N] - 1;
+    s.ok = hor >= 0 && hor < W && ver >= 0 && ver < H;
+    s.out = hor + ver * W;
+    s.nhor = (T)m.xn[c + r * W + (long)k * N] - (T)1;
+    s.nver = (T)m.xn[c + r * W + (long)k * N + N * F] - (T)1;
+    s.prehor = s.prever = 0;
+    for (int i = 0; i < 4; ++i) s.q[i] = -1;
+    if (!(s.nhor > (T)-2 && s.nhor < (T)(W + 1) && s.nver > (T)-2 && s.nver < (T)(H + 1))) return s;
+    s.prehor = (long)std::floor(s.nhor);
+    s.prever = (long)std::floor(s.nver);
+    const long hs[2] = {s.prehor, s.prehor + 1}, vs[2] = {s.prever, s.prever + 1};
+    for (int v = 0; v < 2; ++v)
+        for (int h = 0; h < 2; ++h)
+            if (hs[h] >= 0 && hs[h] < W && vs[v] >= 0 && vs[v] < H) s.q[2 * v + h] = hs[h] + vs[v] * W;
+    return s;
+}
+
+template <class T> void mc_fwd(const McOp& m, const Cx<T>* in, Cx<T>* out, int F, int H, int W) {
+    const long N = (long)H * W;
+    std::memcpy(out, in, sizeof(Cx<T>) * N * F);
+    for (int k = 0; k < F; ++k)
+        for (long i = 0; i < m.box.nr; ++i)
+            for (long j = 0; j < m.box.nc; ++j) {
+                const McSample<T> s = mc_sample<T>(m, F, H, W, m.box.r0 + i, m.box.c0 + j, k);
+                if (!s.ok) continue;
+                const Cx<T>* fr = in + (long)k * N;
+                Cx<T> p[4];
+                for (int t = 0; t < 4; ++t) p[t] = s.q[t] >= 0 ? fr[s.q[t]] : Cx<T>{0, 0};
+                const T wl = (T)(s.prehor + 1) - s.nhor, wr = s.nhor - (T)s.prehor;    // (posthor - nhor), (nhor - prehor)
+                const T wu = (T)(s.prever + 1) - s.nver, wd = s.nver - (T)s.prever;
+                const Cx<T> a1{wl * p[0].re + wr * p[1].re, wl * p[0].im + wr * p[1].im};
+                const Cx<T> a2{wl * p[2].re + wr * p[3].re, wl * p[2].im + wr * p[3].im};
+                out[(long)k * N + s.out] = Cx<T>{wu * a1.re + wd * a2.re, wu * a1.im + wd * a2.im};
+            }
+}
+
+template <class T> void mc_adj(const McOp& m, const Cx<T>* in, Cx<T>* out, int F, int H, int W) {
+    const long N = (long)H * W;
+    std::memcpy(out, in, sizeof(Cx<T>) * N * F);
+    for (int k = 0; k < F; ++k)                                   // initZeroRect over the box
+        for (long i = 0; i < m.box.nr; ++i)
+            for (long j = 0; j < m.box.nc; ++j) out[(long)k * N + (m.box.c0 + j) + (m.box.r0 + i) * W] = Cx<T>{0, 0};
+    for (int k = 0; k < F; ++k)
+        for (long i = 0; i < m.box.nr; ++i)
+            for (long j = 0; j < m.box.nc; ++j) {
+                const McSample<T> s = mc_sample<T>(m, F, H, W, m.box.r0 + i, m.box.c0 + j, k);
+                if (!s.ok) continue;
+                const Cx<T> i0 = in[(long)k * N + s.out];
+                const T Whh = s.nhor - (T)s.prehor, Wvv = s.nver - (T)s.prever;
+                for (int t = 0; t < 4; ++t) {                     // weights as written at optimizer.cl:913-928
+                    if (s.q[t] < 0) continue;
+                    const T hw = (t & 1) ? Whh : (1 - Whh), vw = (t < 2) ? (1 - Wvv) : Wvv;
+                    Cx<T>& o = out[(long)k * N + s.q[t]];
+                    o.re += i0.re * hw * vw;
+                    o.im += i0.im * hw * vw;
+                }
+            }
+}
+
+// U = tTV o MC (or tTV) and U^T = MC^T o tTV^T (or tTV^T, then a copy): NestaUp.cpp:201-278
+template <class T> void op_U(const McOp* m, const Cx<T>* x, Cx<T>* out, Cx<T>* aux, int F, int H, int W) {
+    if (m) {
+        mc_fwd(*m, x, aux, F, H, W);
+        ttv_fwd(aux, out, F, (long)H * W);
+    } else {
+        ttv_fwd(x, out, F, (long)H * W);
+    }
+}
+template <class T> void op_Ut(const McOp* m, const Cx<T>* u, Cx<T>* out, Cx<T>* aux, int F, int H, int W) {
+    ttv_adj(u, aux, F, (long)H * W);
+    if (m) mc_adj(*m, aux, out, F, H, W);
+    else copy((long)F * H * W, aux, out);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
 // drand48 from the POSIX default state (X0 = 0x1234ABCD330E, a = 0x5DEECE66D, c = 0xB, modulus 2^48).
 // ------------------------------------------------------------------------------------------------------------------
 struct Drand48 {
@@ -316,27 +434,33 @@ struct Drand48 {
     }
 };
 
-// myNormest (NestaUp.cpp:291-439): power method for the spectral norm of U = tTV, FP-typed like the reference.
-template <class T> T normest(int F, long N, int* iters_out) {
+// myNormest (NestaUp.cpp:291-439): power method for the spectral norm of U = tTV (or tTV o MC with mc, frames of
+// H x W), FP-typed like the reference.
+template <class T> T normest(int F, long N, int* iters_out, const McOp* mc = nullptr, int H = 0, int W = 0) {
     const long n = (long)F * N;
     const T tol = (T)1e-3;
     const unsigned maxiter = 100;
     unsigned cnt = 0;
-    std::vector<Cx<T>> x(n, Cx<T>{(T)1, (T)0}), Sx(n), aux(n);
+    std::vector<Cx<T>> x(n, Cx<T>{(T)1, (T)0}), Sx(n), aux(n), auxMC(mc ? n : 0);
     T e = nrm2(x.data(), n);
     scal(n, (T)1 / e, x.data());
     T e0 = 0;
     Drand48 rng;
     while ((std::abs(e - e0) > (tol * e)) && (cnt < maxiter)) {
         e0 = e;
-        ttv_fwd(x.data(), Sx.data(), F, N);
+        if (mc) op_U(mc, x.data(), Sx.data(), auxMC.data(), F, H, W);
+        else ttv_fwd(x.data(), Sx.data(), F, N);
         T nnz = asum(Sx.data(), n);
         if (nnz == 0) {
             for (long i = 0; i < n; ++i) { Sx[i].re = (T)rng.next(); Sx[i].im = 0; }
         }
         e = nrm2(Sx.data(), n);
-        ttv_adj(Sx.data(), aux.data(), F, N);
-        copy(n, aux.data(), x.data());
+        if (mc) {
+            op_Ut(mc, Sx.data(), x.data(), aux.data(), F, H, W);
+        } else {
+            ttv_adj(Sx.data(), aux.data(), F, N);
+            copy(n, aux.data(), x.data());
+        }
         T nx = nrm2(x.data(), n);
         scal(n, (T)1 / nx, x.data());
         cnt = cnt + 1;
@@ -422,18 +546,22 @@ double orc_drand48_nth(long k) {  // value of the k-th (1-based) drand48() call 
 // ------------------------------------------------------------------------------------------------------------------
 namespace {
 
-template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& prm, orc_report* rep) {
+// mc == nullptr: the reference's argsMC == nullptr path (x_ref = A^H y_s).  With mc, x_ref is the caller's xref
+// (the previous reconstruction) and U, U^T, normest include the motion compensation.
+template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& prm, orc_report* rep,
+                             const McOp* mc = nullptr) {
     const int F = P.F, C = P.C;
     const long N = P.N, nX = (long)F * N, nK = (long)F * C * N;
     std::vector<Cx<T>> yS(nK), aux(nK), Res(nK), AuxRes(nK);
     std::vector<Cx<T>> Ux(nX), Wk(nX), Xk(nX), Yk(nX), Zk(nX), Uk(nX), AuxFx(nX), Df(nX), ARes(nX), tmp(nX);
+    std::vector<Cx<T>> auxMC(nX);
 
-    // y_s = sqrt(N) * y ; x_ref = A^H(y_s)                                              (:468-486)
+    // y_s = sqrt(N) * y ; x_ref = A^H(y_s) without motion compensation                    (:468-486)
     copy(nK, P.y, yS.data());
     const T sqrtN = (T)std::sqrt((T)(P.H * P.W));
     scal(nK, sqrtN, yS.data());
-    op_At(P, yS.data(), xref, aux.data());
-    ttv_fwd(xref, Ux.data(), F, N);                                                        // (:491)
+    if (!mc) op_At(P, yS.data(), xref, aux.data());
+    op_U(mc, xref, Ux.data(), auxMC.data(), F, P.H, P.W);                                  // (:491)
 
     const T maxXref = max_abs(xref, nX);                                                   // (:521-545)
     const T maxUXref = max_abs(Ux.data(), nX);                                             // (:547-568)
@@ -442,7 +570,7 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
     const T muf = (T)prm.mu_f * maxXref;
     T mu0 = (T)(0.9 * (double)maxUXref);                                                 // double product, as in C++
     int ne_it = 0;
-    const T normU = normest<T>(F, N, &ne_it);
+    const T normU = normest<T>(F, N, &ne_it, mc, P.H, P.W);
     const T muL = lambda / (T)prm.La;
     if (muL > mu0) mu0 = muL;
     const T gamma = (T)std::pow((double)(muf / mu0), 1.0 / prm.maxIntIter);
@@ -476,13 +604,12 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
         int done_k = 0;
         for (int k = 0; k < prm.maxIter; ++k) {
             ++done_k;
-            ttv_fwd(Xk.data(), Uk.data(), F, N);                                           // (:669)
+            op_U(mc, Xk.data(), Uk.data(), auxMC.data(), F, P.H, P.W);                     // (:669)
             copy(nX, Uk.data(), AuxFx.data());                                             // (:675)
             vecnorm(Uk.data(), Uk.data(), nX, mu);                                         // (:680-683)
             const T normUk = nrm2(Uk.data(), nX);                                          // (:686)
             T fx = dotc_re(Uk.data(), AuxFx.data(), nX);                                   // (:691)
-            ttv_adj(Uk.data(), tmp.data(), F, N);                                          // (:696) Ut = adj kernel + copy
-            copy(nX, tmp.data(), Df.data());
+            op_Ut(mc, Uk.data(), Df.data(), tmp.data(), F, P.H, P.W);                      // (:696)
             scal(nX, (T)-1, Df.data());                                                    // (:698-704)
             op_A(P, Xk.data(), Res.data(), aux.data());                                    // (:710)
             scal(nK, invSqrtN, Res.data());                                                // (:712-718)
@@ -584,6 +711,47 @@ int orc_simple_recon_f64(const double* y, const double* S, double* x, int F, int
     Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, nullptr};
     op_At(P, (const Cx<double>*)y, (Cx<double>*)x, aux.data());
     return 0;
+}
+
+// Motion compensation of F frames of H x W (adjoint = 0: MotionCompensation, 1: AdjointMotionCompensation).
+// xn: [2][F][H][W] float, x: [2][W][H] int, bb: bound box (1-based).  Returns -1 when the box leaves the frame.
+int orc_mc_f64(const double* in, double* out, const float* xn, const int* x, const unsigned* bb, int F, int H, int W,
+               int adjoint) {
+    McOp m{xn, x, {}};
+    if (!mc_box(H, W, bb, &m.box)) return -1;
+    if (adjoint) mc_adj(m, (const Cx<double>*)in, (Cx<double>*)out, F, H, W);
+    else mc_fwd(m, (const Cx<double>*)in, (Cx<double>*)out, F, H, W);
+    return 0;
+}
+int orc_mc_f32(const float* in, float* out, const float* xn, const int* x, const unsigned* bb, int F, int H, int W,
+               int adjoint) {
+    McOp m{xn, x, {}};
+    if (!mc_box(H, W, bb, &m.box)) return -1;
+    if (adjoint) mc_adj(m, (const Cx<float>*)in, (Cx<float>*)out, F, H, W);
+    else mc_fwd(m, (const Cx<float>*)in, (Cx<float>*)out, F, H, W);
+    return 0;
+}
+double orc_normest_mc_f64(const float* xn, const int* x, const unsigned* bb, int F, int H, int W, int* iters) {
+    McOp m{xn, x, {}};
+    if (!mc_box(H, W, bb, &m.box)) return -1.0;
+    return normest<double>(F, (long)H * W, iters, &m, H, W);
+}
+// NESTA with motion compensation: x_io holds x_ref (the previous reconstruction) on entry and the result on exit
+int orc_nesta_mc_f64(const double* y, const double* S, const uint8_t* M, double* x_io, int F, int C, int H, int W,
+                     const float* xn, const int* x, const unsigned* bb, const orc_params* prm, orc_report* rep) {
+    if (!prm || prm->miniter < 1 || F < 1 || C < 1) return -1;
+    McOp m{xn, x, {}};
+    if (!mc_box(H, W, bb, &m.box)) return -1;
+    Prob<double> P{F, C, H, W, (long)H * W, (const Cx<double>*)y, (const Cx<double>*)S, M};
+    return nesta(P, (Cx<double>*)x_io, *prm, rep, &m);
+}
+int orc_nesta_mc_f32(const float* y, const float* S, const uint8_t* M, float* x_io, int F, int C, int H, int W,
+                     const float* xn, const int* x, const unsigned* bb, const orc_params* prm, orc_report* rep) {
+    if (!prm || prm->miniter < 1 || F < 1 || C < 1) return -1;
+    McOp m{xn, x, {}};
+    if (!mc_box(H, W, bb, &m.box)) return -1;
+    Prob<float> P{F, C, H, W, (long)H * W, (const Cx<float>*)y, (const Cx<float>*)S, M};
+    return nesta(P, (Cx<float>*)x_io, *prm, rep, &m);
 }
 
 void orc_rsos_f32(const float* in, float* out, int F, int C, long N) {

@@ -73,6 +73,10 @@ def lib():
             "orc_simple_recon_f32": ([P, P, P, I, I, I, I], I), "orc_simple_recon_f64": ([P, P, P, I, I, I, I], I),
             "orc_rsos_f32": ([P, P, I, I, LG], None), "orc_simple_recon_sos_f64": ([P, P, I, I, I, I], I),
             "orc_num_threads": ([], I), "orc_set_threads": ([I], None),
+            "orc_mc_f64": ([P, P, P, P, P, I, I, I, I], I), "orc_mc_f32": ([P, P, P, P, P, I, I, I, I], I),
+            "orc_normest_mc_f64": ([P, P, P, I, I, I, ctypes.POINTER(I)], D),
+            "orc_nesta_mc_f64": ([P, P, P, P, I, I, I, I, P, P, P, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
+            "orc_nesta_mc_f32": ([P, P, P, P, I, I, I, I, P, P, P, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -211,6 +215,48 @@ def simple_recon_sos(y):
     x = np.empty((F, H, W), dtype=np.complex128)
     lib().orc_simple_recon_sos_f64(_p(y), _p(x), F, C, H, W)
     return x
+
+
+def mc(x, xn, mesh, bb, adjoint=False, double=True):
+    """MotionCompensation (adjoint=False) or AdjointMotionCompensation of x [F][H][W]; xn [2][F][H][W] float32
+    sample positions (1-based), mesh [2][W][H] int32 original mesh (1-based), bb the 1-based bound box."""
+    dt = np.complex128 if double else np.complex64
+    a = _c(x, dt)
+    F, H, W = a.shape
+    xn, mesh, bb = _c(xn, np.float32), _c(mesh, np.int32), _c(bb, np.uint32)
+    assert xn.shape == (2, F, H, W) and mesh.shape == (2, W, H) and bb.shape == (4,)
+    out = np.empty_like(a)
+    f = lib().orc_mc_f64 if double else lib().orc_mc_f32
+    if f(_p(a), _p(out), _p(xn), _p(mesh), _p(bb), F, H, W, int(adjoint)) != 0:
+        raise ValueError("motion compensation box leaves the frame")
+    return out
+
+
+def normest_mc(xn, mesh, bb):
+    """myNormest of U = tTV o MC in FP64."""
+    xn, mesh, bb = _c(xn, np.float32), _c(mesh, np.int32), _c(bb, np.uint32)
+    _, F, H, W = xn.shape
+    it = ctypes.c_int(0)
+    v = lib().orc_normest_mc_f64(_p(xn), _p(mesh), _p(bb), F, H, W, ctypes.byref(it))
+    if v < 0:
+        raise ValueError("motion compensation box leaves the frame")
+    return v, it.value
+
+
+def nesta_mc(y, S, M, x_ref, xn, mesh, bb, prm: Params | None = None, double=True):
+    """NESTA with motion compensation (argsMC != nullptr), starting from x_ref; returns (x, report dict)."""
+    prm = prm or params()
+    dt = np.complex128 if double else np.complex64
+    y, S, M = _c(y, dt), _c(S, dt), _c(M, np.uint8)
+    F, C, H, W = y.shape
+    x = np.array(x_ref, dtype=dt)
+    xn, mesh, bb = _c(xn, np.float32), _c(mesh, np.int32), _c(bb, np.uint32)
+    rep = Report()
+    f = lib().orc_nesta_mc_f64 if double else lib().orc_nesta_mc_f32
+    rc = f(_p(y), _p(S), _p(M), _p(x), F, C, H, W, _p(xn), _p(mesh), _p(bb), ctypes.byref(prm), ctypes.byref(rep))
+    if rc != 0:
+        raise RuntimeError(f"oracle nesta_mc failed ({rc})")
+    return x, rep.as_dict()
 
 
 def set_threads(n):

@@ -158,3 +158,130 @@ def test_simple_recon_sos_recovers_magnitude(oracle):
     k = np.fft.fft2(sl.S[None].astype(np.complex128) * sl.x_true[:, None].astype(np.complex128), axes=(-2, -1))
     got = oracle.simple_recon_sos(k)
     np.testing.assert_allclose(got.real, np.abs(sl.x_true.astype(np.complex128)), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Motion compensation (MotionCompensation / AdjointMotionCompensation, optimizer.cl:832-930) and NESTA with argsMC
+# ---------------------------------------------------------------------------------------------------------------------
+def _box(H, W, bb):
+    m0, m1 = H // 40, W // 40
+    r0, c0 = int(bb[1]) - m0 - 2, int(bb[0]) - m1 - 2
+    return r0, int(bb[3]) - int(bb[1]) + 2 * m0 + 3, c0, int(bb[2]) - int(bb[0]) + 2 * m1 + 3
+
+
+def _cplx(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _identity_xn(F, H, W):
+    xn = np.empty((2, F, H, W), np.float32)
+    xn[0] = np.arange(1, W + 1, dtype=np.float32)[None, None, :]
+    xn[1] = np.arange(1, H + 1, dtype=np.float32)[None, :, None]
+    return xn
+
+
+@pytest.mark.parametrize("double", [True, False])
+def test_mc_identity_is_exact(oracle, double):
+    H, W, F = 40, 48, 3
+    rng = np.random.default_rng(31)
+    x = _cplx(rng, F, H, W)
+    xn, mesh, bb = _identity_xn(F, H, W), synth.identity_mesh(H, W), np.array([10, 8, 37, 30], np.uint32)
+    for adjoint in (False, True):
+        got = oracle.mc(x, xn, mesh, bb, adjoint=adjoint, double=double)
+        assert np.array_equal(got, x.astype(got.dtype))
+
+
+def test_mc_integer_shift(oracle):
+    """Sampling one column to the right inside the box shifts the box contents left; outside the box x is kept."""
+    H, W, F = 24, 32, 2
+    rng = np.random.default_rng(32)
+    x = _cplx(rng, F, H, W)
+    xn, mesh, bb = _identity_xn(F, H, W), synth.identity_mesh(H, W), np.array([6, 5, 25, 18], np.uint32)
+    xn[0] += 1
+    got = oracle.mc(x, xn, mesh, bb)
+    r0, nr, c0, nc = _box(H, W, bb)
+    want = x.copy()
+    want[:, r0:r0 + nr, c0:c0 + nc] = x[:, r0:r0 + nr, c0 + 1:c0 + nc + 1]
+    assert np.array_equal(got, want)
+
+
+def test_mc_matches_scipy_bilinear(oracle):
+    """Inside the box MC is bilinear interpolation at the deformed positions: scipy.ndimage.map_coordinates with
+    order 1 (an independent implementation) gives the same values."""
+    from scipy import ndimage
+    H, W, F = 64, 56, 4
+    rng = np.random.default_rng(33)
+    x = _cplx(rng, F, H, W)
+    xn, mesh, bb = synth.motion_fields(H, W, F, amp=2.5, slice_id=5)
+    got = oracle.mc(x, xn, mesh, bb)
+    r0, nr, c0, nc = _box(H, W, bb)
+    want = x.copy()
+    for f in range(F):
+        coords = [xn[1, f, r0:r0 + nr, c0:c0 + nc].astype(np.float64) - 1,
+                  xn[0, f, r0:r0 + nr, c0:c0 + nc].astype(np.float64) - 1]
+        re = ndimage.map_coordinates(x[f].real, coords, order=1, mode="grid-constant", cval=0.0)
+        im = ndimage.map_coordinates(x[f].imag, coords, order=1, mode="grid-constant", cval=0.0)
+        want[f, r0:r0 + nr, c0:c0 + nc] = re + 1j * im
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    assert not np.allclose(got, x)
+
+
+def test_mc_adjoint_identity(oracle):
+    """<MC x, y> = <x, MC^T y> for a non-trivial deformation (copy + zero box + scatter is the exact adjoint)."""
+    H, W, F = 48, 40, 5
+    rng = np.random.default_rng(34)
+    x, y = _cplx(rng, F, H, W), _cplx(rng, F, H, W)
+    xn, mesh, bb = synth.motion_fields(H, W, F, amp=3.0, slice_id=6)
+    lhs = np.vdot(oracle.mc(x, xn, mesh, bb), y)
+    rhs = np.vdot(x, oracle.mc(y, xn, mesh, bb, adjoint=True))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_mc_rejects_box_outside_frame(oracle):
+    H, W, F = 16, 16, 1
+    x = np.zeros((F, H, W), np.complex128)
+    with pytest.raises(ValueError):
+        oracle.mc(x, _identity_xn(F, H, W), synth.identity_mesh(H, W), np.array([1, 1, 8, 8], np.uint32))
+
+
+def test_normest_mc_vs_explicit_matrix(oracle):
+    """myNormest over U = tTV o MC against sigma_max of U's explicit matrix (FP64 SVD)."""
+    H, W, F = 12, 12, 3
+    xn, mesh, bb = synth.motion_fields(H, W, F, amp=1.5, slice_id=7)
+    n = F * H * W
+    cols = []
+    for i in range(n):
+        e = np.zeros(n, np.complex128)
+        e[i] = 1
+        cols.append(oracle.ttv(oracle.mc(e.reshape(F, H, W), xn, mesh, bb)).ravel())
+    smax = np.linalg.norm(np.stack(cols, axis=1), 2)
+    est, iters = oracle.normest_mc(xn, mesh, bb)
+    assert 1 < iters < 100
+    assert abs(est - smax) < 2e-2 * smax
+    # and the plain tTV norm differs: the deformation is felt
+    assert abs(est - oracle.normest(F, H * W)[0]) > 1e-3
+
+
+def test_nesta_mc_with_identity_motion_equals_nesta(oracle):
+    """With an identity deformation and x_ref = A^H(sqrt(N) y), NESTA with argsMC reproduces NESTA bit for bit."""
+    sl = synth.make_cine(H=24, W=20, C=3, F=5, accel=3.0, n_centre=4, slice_id=35)
+    prm = oracle.params(maxIntIter=3, maxIter=15)
+    x_plain, rep_plain = oracle.nesta(sl.y, sl.S, sl.M, prm)
+    x0 = oracle.op_At(sl.y.astype(np.complex128) * np.sqrt(sl.H * sl.W), sl.S)
+    x_mc, rep_mc = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, _identity_xn(sl.F, sl.H, sl.W),
+                                   synth.identity_mesh(sl.H, sl.W), np.array([6, 6, 15, 19], np.uint32), prm)
+    assert np.array_equal(x_mc, x_plain)
+    assert rep_mc["iters"] == rep_plain["iters"] and rep_mc["normU"] == rep_plain["normU"]
+
+
+def test_nesta_mc_moves_the_solution(oracle):
+    """A real deformation changes U, hence normU, the iterates and the result; the result stays close to the
+    motion-free one (small deformation) and finite."""
+    sl = synth.make_cine(H=32, W=32, C=3, F=6, accel=3.0, n_centre=4, slice_id=36)
+    prm = oracle.params(maxIntIter=3, maxIter=15)
+    x_plain, _ = oracle.nesta(sl.y, sl.S, sl.M, prm)
+    xn, mesh, bb = synth.motion_fields(sl.H, sl.W, sl.F, amp=1.0, slice_id=8)
+    x_mc, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x_plain, xn, mesh, bb, prm)
+    assert np.all(np.isfinite(x_mc))
+    assert 1e-4 < oracle.rel_l2(x_mc, x_plain) < 0.5
+    assert rep["normU"] == pytest.approx(oracle.normest_mc(xn, mesh, bb)[0], rel=1e-12)
