@@ -13,6 +13,9 @@ Other legs in the same run (each reported under its own key of the JSON line):
                 launch table per iteration, captured in a hipGraph; k-space synthesised on the device
   served        Poisson arrivals (50 req/s) mixing the sizes above, 5 stages x 10..30 iterations per request,
                 admitted by HBM budget into the continuous-batching server; p50/p99 latency and recons/s
+  motion_compensated  NESTA-TV with argsMC (U = tTV o MC, fused and plain forms) on the headline slice from the motion-free
+                result, with a smooth synthetic deformation (up to 1.5 px) over a centred bound box, plus the
+                MotionCompensation / AdjointMotionCompensation kernels alone against their HBM roofline
   cpu           the FP32 CPU restatement on the host cores (cpu_baseline, bounded sample) and rel-L2 of the GPU
                 results against the FP64 and FP32 CPU oracles on full schedules
 
@@ -333,6 +336,59 @@ def leg_served_burst(ctx, capi, pool, n_req, budget, chunk, seed=11):
             "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"]}
 
 
+def mc_bytes(F, H, W, box_pixels, adjoint):
+    """Algorithmic HBM bytes of one MotionCompensation (copy 16 B per pixel-frame, then per box pixel-frame the
+    deformed position 8 B, one input sample 8 B and the output 8 B, plus the 8-B mesh per box pixel) or
+    AdjointMotionCompensation (copy with the box cleared 16 B per pixel-frame, then per box pixel-frame the position
+    8 B, the input 8 B and the read-modify-write of the scattered output 16 B, plus the mesh)."""
+    per_box = 32 if adjoint else 24
+    return 16 * F * H * W + per_box * F * box_pixels + 8 * box_pixels
+
+
+def leg_mc(ctx, synth, dev0, x_ref, prm, steps, reps=20):
+    y0, S0, M0, _ = dev0
+    Fh, Hh, Wh = x_ref.shape
+    xn, mesh, bb = synth.motion_fields(Hh, Wh, Fh, amp=1.5, slice_id=77)
+    dxn, dmesh = ctx.to_device(xn), ctx.to_device(mesh)
+    mc = ctx.mc_args(dxn, dmesh, bb)
+    m0, m1 = Hh // 40, Wh // 40
+    nr, nc = int(bb[3]) - int(bb[1]) + 2 * m0 + 3, int(bb[2]) - int(bb[0]) + 2 * m1 + 3
+    xin, out = ctx.to_device(x_ref), ctx.empty(x_ref.shape)
+    kern = {}
+    for adjoint in (False, True):
+        for _ in range(3):
+            ctx.motion_compensation(xin, out, mc, adjoint=adjoint)
+        ms = []
+        for _ in range(reps):
+            t = ctx.timer_begin()
+            ctx.motion_compensation(xin, out, mc, adjoint=adjoint)
+            ms.append(ctx.timer_end(t))
+        byts = mc_bytes(Fh, Hh, Wh, nr * nc, adjoint)
+        mean = float(np.mean(ms))
+        kern["adjoint" if adjoint else "forward"] = {
+            "ms_mean": mean, "bytes_alg": byts, "achieved_GBs": byts / (mean * 1e-3) / 1e9,
+            "frac_hbm": byts / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    from opencliper_amd import capi
+    x = ctx.empty(x_ref.shape)
+    forms, xs = {}, {}
+    for name, form, reps in (("plain", capi.OCL_FORM_PLAIN, steps), ("fused", capi.OCL_FORM_FUSED, 3 * steps)):
+        lat, rep = [], None
+        for r in range(1 + reps):
+            x.set(x_ref)
+            t = time.perf_counter()
+            rep = ctx.nesta_mc(y0, S0, M0, x, mc, prm, form=form)
+            if r:
+                lat.append(time.perf_counter() - t)
+        forms[name] = {"recons_per_s": 1.0 / np.mean(lat), "p50_ms": 1e3 * pct(lat, 50),
+                       "p99_ms": 1e3 * pct(lat, 99), "normU": rep["normU"]}
+        xs[name] = x.get()
+    return {"recons_per_s": forms["fused"]["recons_per_s"], "forms": forms, "box_rows_cols": [nr, nc],
+            "bound_box": [int(v) for v in bb], "kernels": kern,
+            "workload": f"headline slice F={Fh} C={C} {Hh}x{Wh}, {STAGES} stages x {ITERS} iterations, from the "
+                        f"motion-free reconstruction; deformation up to 1.5 px; fused form (plain beside it)"}, \
+        xs, (xn, mesh, bb)
+
+
 def gpu_legs(args, cine, parts, full, split):
     from opencliper_amd import capi, synth
     ctx = capi.Context(0)
@@ -345,6 +401,13 @@ def gpu_legs(args, cine, parts, full, split):
         dev = [(ctx.to_device(y), ctx.to_device(S), ctx.to_device(M), ctx.empty((F, H, W))) for (y, S, M) in cine]
         out["plain"], out["x"]["plain"] = leg_single(ctx, capi, dev[0], prm, capi.OCL_FORM_PLAIN, args.plain_steps)
         log(f"plain: {out['plain']['recons_per_s']:.2f} recons/s")
+        if "mc" in args.legs:
+            out["mc"], out["x"]["mc"], out["x"]["mc_fields"] = leg_mc(ctx, synth, dev[0], out["x"]["plain"], prm,
+                                                                      args.plain_steps)
+            k = out["mc"]["kernels"]
+            log(f"motion-compensated: fused {out['mc']['forms']['fused']['recons_per_s']:.2f} recons/s, plain "
+                f"{out['mc']['forms']['plain']['recons_per_s']:.2f}; MC kernel {k['forward']['achieved_GBs']:.0f} "
+                f"GB/s, adjoint {k['adjoint']['achieved_GBs']:.0f} GB/s")
         out["fused_single"], out["x"]["fused"] = leg_single(ctx, capi, dev[0], prm, capi.OCL_FORM_FUSED, args.steps)
         log(f"fused single: {out['fused_single']['recons_per_s']:.2f} recons/s")
         out["batched"] = leg_batched(ctx, dev, prm, args.steps, args.warmup)
@@ -438,6 +501,11 @@ def cpu_leg(args, slice0, full, g):
                            "per_iter_f_relerr_vs_fp64_max": float(np.max(np.abs(fs - f64) / np.abs(f64))),
                            "per_iter_f_relerr_vs_fp32_max": float(np.max(np.abs(fs - f32) / np.abs(f32))),
                            "per_iter_f_relerr_vs_fp64": [float(v) for v in np.abs(fs - f64) / np.abs(f64)]}
+        if "mc" in gx:
+            xn, mesh, bb = gx["mc_fields"]
+            ref, _ = O.nesta_mc(y, S, M, gx["plain"], xn, mesh, bb, prm, double=True)
+            rl["motion_compensated"] = {"fused_vs_fp64": O.rel_l2(gx["mc"]["fused"], ref),
+                                        "plain_vs_fp64": O.rel_l2(gx["mc"]["plain"], ref)}
         if "uneven_small" in gx:
             i, yy, SS, MM, xx = gx["uneven_small"]
             ref, _ = O.nesta(yy, SS, MM, prm, double=True)
@@ -461,8 +529,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--legs", default="simple,split,uneven,served",
-                    help="comma list of extra legs beside the headline batch: simple, split, uneven, served")
+    ap.add_argument("--legs", default="simple,mc,split,uneven,served",
+                    help="comma list of extra legs beside the headline batch: simple, mc, split, uneven, served")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson trace (300 = 5 min)")
     ap.add_argument("--serve-budget", type=int, default=64, help="server HBM budget in GiB")
     ap.add_argument("--serve-chunk", type=int, default=5, help="server iterations between admission points")
@@ -520,6 +588,13 @@ def main():
     }
     if "simple_recon" in g:
         line["simple_recon"] = dict(g["simple_recon"], **c.get("simple_recon", {}))
+    if "mc" in g:
+        m = dict(g["mc"])
+        m["rel_l2_vs_fp64"] = (c.get("rel_l2") or {}).get("motion_compensated")
+        m["roofline"] = {"bound": "hbm", "kernel": "k_copy + k_mc_interp (MotionCompensation)",
+                         "achieved": m["kernels"]["forward"]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": m["kernels"]["forward"]["frac_hbm"]}
+        line["motion_compensated"] = m
     if "uneven_batch" in g:
         u = dict(g["uneven_batch"])
         u.pop("shapes_FCHW", None)

@@ -25,6 +25,7 @@ class NDArray {
     static dimIndexType getElementSize(ElementDataType t) {
         if (t == TYPEID_COMPLEX) return sizeof(complexType);
         if (t == TYPEID_REAL) return sizeof(realType);
+        if (t == TYPEID_UINT32 || t == TYPEID_INT32) return 4;
         return 1;
     }
 

@@ -20,3 +20,5 @@ typedef std::type_index ElementDataType;
 #define TYPEID_COMPLEX std::type_index(typeid(OpenCLIPER::complexType))
 #define TYPEID_REAL std::type_index(typeid(OpenCLIPER::realType))
 #define TYPEID_INDEX std::type_index(typeid(uint8_t))   // sampling masks are stored as bytes (non-zero = sampled)
+#define TYPEID_UINT32 std::type_index(typeid(uint32_t)) // meshes and index arrays (the reference's dimIndexType)
+#define TYPEID_INT32 std::type_index(typeid(int32_t))

@@ -4,16 +4,16 @@
 #include <OpenCLIPER/KData.hpp>
 #include <OpenCLIPER/XData.hpp>
 #include <OpenCLIPER/Process.hpp>
+#include <OpenCLIPER/processes/GroupwiseRegistration.hpp>
 
 namespace OpenCLIPER {
-
-/// Motion-compensation arguments of the reference's MC-NESTA; this library accepts only nullptr.
-struct ArgumentsMotionCompensation;
 
 class NestaUp : public Process {
   public:
     /// Same fields and constructor argument order as the reference (NestaUp.hpp:67-83).  Like the reference,
-    /// launch() overwrites tolVar (NestaUp.cpp:583,609).
+    /// launch() overwrites tolVar (NestaUp.cpp:583,609).  With argsMC the sparsity operator is tTV o MC and the
+    /// output's current contents are the starting estimate (NestaUp.cpp:201-278,483-491); that case runs the fused
+    /// form (SPLIT falls back to it) when the mesh is GroupwiseRegistration's original one, else the PLAIN form.
     struct LaunchParameters : Process::LaunchParameters {
         float lambda_i;
         float mu_f;

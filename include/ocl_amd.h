@@ -151,7 +151,8 @@ int ocl_motion_compensation(ocl_ctx* ctx, const void* in, void* out, int F, int 
 int ocl_adjoint_motion_compensation(ocl_ctx* ctx, const void* in, void* out, int F, int H, int W,
                                     const ocl_mc_args* mc);
 /* NestaUp::launch with argsMC != nullptr (NestaUp.cpp:201-278,483-491): U = tTV(MC(x)), U^T = MC^T(tTV^T(u)), and
- * x_io holds x_ref on entry (the previous reconstruction) instead of A^H y.  form must be OCL_FORM_PLAIN. */
+ * x_io holds x_ref on entry (the previous reconstruction) instead of A^H y.  form: OCL_FORM_PLAIN (any mesh) or
+ * OCL_FORM_FUSED (the original, identity mesh that GroupwiseRegistration produces); OCL_FORM_SPLIT is unsupported. */
 int ocl_nesta_run_mc(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_io, int F, int C, int H,
                      int W, const ocl_nesta_params* prm, int form, const ocl_mc_args* mc, ocl_nesta_report* report);
 
@@ -161,6 +162,9 @@ int ocl_batch_create(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const o
 /* form = OCL_FORM_FUSED (what ocl_batch_create uses) or OCL_FORM_SPLIT (every slice of the batch then shares H, W) */
 int ocl_batch_create_form(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm, int form,
                           ocl_batch** out);
+/* fused batch with motion compensation, mcs[i] for slice i (identity mesh); each slice's x_out holds x_ref on entry */
+int ocl_batch_create_mc(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
+                        const ocl_mc_args* mcs, ocl_batch** out);
 int ocl_batch_setup(ocl_batch* b);                    /* E^H y, maxima, normU, state (per slice) */
 int ocl_batch_iterate(ocl_batch* b, int n_iters);     /* enqueue n_iters iterations (no host sync) */
 int ocl_batch_capture(ocl_batch* b, int n_iters);     /* record n_iters iterations into a hipGraph */

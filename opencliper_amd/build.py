@@ -18,6 +18,7 @@ LIB = os.path.join(PKG, "libocl_amd.so")
 LIBCPP = os.path.join(PKG, "libOpenCLIPER_amd.so")
 EXAMPLE = os.path.join(PKG, "examples", "mri_recon")
 DATA_IO = os.path.join(PKG, "examples", "data_io")
+MRI_RECON_MC = os.path.join(PKG, "examples", "mri_recon_mc")
 MAT_TOOL = os.path.join(PKG, "examples", "mat_tool")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -67,6 +68,11 @@ def build(verbose: bool = False) -> str:
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "mri_recon.cpp"),
           f"-L{PKG}", "-lOpenCLIPER_amd", "-locl_amd", "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, EXAMPLE)
+    # NESTA then NESTA with motion compensation, as the reference's tests/MRIRecon.cpp chains them
+    tmp = MRI_RECON_MC + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "mri_recon_mc.cpp"),
+          f"-L{PKG}", "-lOpenCLIPER_amd", "-locl_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, MRI_RECON_MC)
     # MAT-file I/O through KData / XData (device needed) and the host-only MAT v5 utility
     tmp = DATA_IO + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INC}", "-o", tmp, os.path.join(PKG, "examples", "data_io.cpp"),

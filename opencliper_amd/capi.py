@@ -127,6 +127,8 @@ def lib():
         "ocl_nesta_run_mc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(McArgs),
                               ctypes.POINTER(NestaReport)], I),
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
+        "ocl_batch_create_mc": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), ctypes.POINTER(McArgs), PP],
+                                I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_ctx_set_autotune": ([P, I], I),
         "ocl_rsos": ([P, P, P, I, I, LG], I), "ocl_simple_mri_recon_sos": ([P, P, P, I, I, I, I], I),
@@ -365,14 +367,15 @@ class Context:
                                       ctypes.byref(mc), ctypes.byref(rep)))
         return rep.as_dict()
 
-    def batch(self, slices, prm: NestaParams | None = None, form=OCL_FORM_FUSED) -> "Batch":
-        return Batch(self, slices, prm or nesta_params(), form)
+    def batch(self, slices, prm: NestaParams | None = None, form=OCL_FORM_FUSED, mcs=None) -> "Batch":
+        return Batch(self, slices, prm or nesta_params(), form, mcs)
 
 
 class Batch:
     """Many uneven slices advanced together (ocl_batch_*).  `slices` = list of (y, S, M, x_out) DeviceArrays."""
 
-    def __init__(self, ctx: Context, slices, prm: NestaParams, form=OCL_FORM_FUSED):
+    def __init__(self, ctx: Context, slices, prm: NestaParams, form=OCL_FORM_FUSED, mcs=None):
+        """mcs: one McArgs per slice for motion compensation (fused form; each x_out then holds x_ref)."""
         self.ctx = ctx
         self.keep = list(slices)
         arr = (Slice * len(slices))()
@@ -381,8 +384,12 @@ class Batch:
             arr[i] = Slice(y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W)
         self.prm = prm
         h = ctypes.c_void_p()
-        _check(lib().ocl_batch_create_form(ctx.handle, len(slices), arr, ctypes.byref(prm), int(form),
-                                           ctypes.byref(h)))
+        if mcs is None:
+            _check(lib().ocl_batch_create_form(ctx.handle, len(slices), arr, ctypes.byref(prm), int(form),
+                                               ctypes.byref(h)))
+        else:
+            marr = (McArgs * len(mcs))(*mcs)
+            _check(lib().ocl_batch_create_mc(ctx.handle, len(slices), arr, ctypes.byref(prm), marr, ctypes.byref(h)))
         self.handle = h.value
 
     def setup(self):

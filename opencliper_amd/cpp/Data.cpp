@@ -91,6 +91,8 @@ void Data::cloneShape(const Data& src, bool copyData) {
         auto* dims = new std::vector<dimIndexType>(*a->getDims());
         if (type_ == TYPEID_COMPLEX) arrays->push_back(new ConcreteNDArray<complexType>(dims));
         else if (type_ == TYPEID_REAL) arrays->push_back(new ConcreteNDArray<realType>(dims));
+        else if (type_ == TYPEID_UINT32) arrays->push_back(new ConcreteNDArray<uint32_t>(dims));
+        else if (type_ == TYPEID_INT32) arrays->push_back(new ConcreteNDArray<int32_t>(dims));
         else arrays->push_back(new ConcreteNDArray<uint8_t>(dims));
     }
     dynDims_ = src.dynDims_;
@@ -137,6 +139,36 @@ XData::XData(const std::shared_ptr<CLapp>& app, std::vector<NDArray*>*& arrays, 
     setNDArrays(arrays, true);
     if (getDynDimsTotalSize() != getNumNDArrays())
         throw std::invalid_argument("XData: temporal dimensions do not match the number of NDArrays");
+}
+
+namespace {
+bool supportedElementType(ElementDataType t) {
+    return t == TYPEID_COMPLEX || t == TYPEID_REAL || t == TYPEID_INDEX || t == TYPEID_UINT32 || t == TYPEID_INT32;
+}
+}  // namespace
+
+XData::XData(const std::shared_ptr<CLapp>& app, NDArray*& nd, ElementDataType t) : Data(app, t) {
+    if (!nd) throw std::invalid_argument("XData: null NDArray");
+    if (!supportedElementType(t)) throw std::invalid_argument("XData: unsupported element type");
+    auto* arrays = new std::vector<NDArray*>({nd});
+    nd = nullptr;
+    dynDims_ = {};
+    setNDArrays(arrays, true);
+}
+
+XData::XData(const std::shared_ptr<CLapp>& app, std::vector<dimIndexType>*& dims, ElementDataType t) : Data(app, t) {
+    if (!dims) throw std::invalid_argument("XData: null dims");
+    if (!supportedElementType(t)) throw std::invalid_argument("XData: unsupported element type");
+    NDArray* nd;
+    if (t == TYPEID_COMPLEX) nd = new ConcreteNDArray<complexType>(dims);
+    else if (t == TYPEID_REAL) nd = new ConcreteNDArray<realType>(dims);
+    else if (t == TYPEID_UINT32) nd = new ConcreteNDArray<uint32_t>(dims);
+    else if (t == TYPEID_INT32) nd = new ConcreteNDArray<int32_t>(dims);
+    else nd = new ConcreteNDArray<uint8_t>(dims);
+    auto* arrays = new std::vector<NDArray*>({nd});
+    dynDims_ = {};
+    setNDArrays(arrays, false);
+    checkOcl(ocl_memset_zero(app_->getOclContext(), dev_, bytes_), "XData zero-init");
 }
 
 XData::XData(const std::shared_ptr<CLapp>& app, const std::shared_ptr<XData>& src, bool copyData, bool)

@@ -1,7 +1,10 @@
 // The Processes of the NESTA-TV path in the OpenCLIPER-compatible C++ API, each one call into the C ABI.
 // Reference Processes mirrored: TemporalTV, VectorNormalization, FFT, ComplexElementProd, ApplyMask, XImageSum,
-// ComplexAbs, ComplexAbsPow2, CopyDataGPU, RSoS, SimpleMRIRecon, SimpleMRIReconSOS, NestaUp (src/processes/**).
+// ComplexAbs, ComplexAbsPow2, CopyDataGPU, RSoS, SimpleMRIRecon, SimpleMRIReconSOS, MotionCompensation,
+// AdjointMotionCompensation, NestaUp (src/processes/**).
+#include <OpenCLIPER/processes/AdjointMotionCompensation.hpp>
 #include <OpenCLIPER/processes/ApplyMask.hpp>
+#include <OpenCLIPER/processes/MotionCompensation.hpp>
 #include <OpenCLIPER/processes/ComplexAbs.hpp>
 #include <OpenCLIPER/processes/ComplexAbsPow2.hpp>
 #include <OpenCLIPER/processes/ComplexElementProd.hpp>
@@ -221,6 +224,56 @@ void SimpleMRIReconSOS::launch() {
     stopProfiling();
 }
 
+// ---- MotionCompensation / AdjointMotionCompensation (src/processes/MotionCompensation.cpp,
+//      AdjointMotionCompensation.cpp) ----
+namespace {
+// Device view of ArgumentsMotionCompensation for F frames of H x W, with the sizes of its Data checked
+ocl_mc_args mcArgs(const ArgumentsMotionCompensation* a, int F, int H, int W, const char* who) {
+    if (!a) throw std::invalid_argument(std::string(who) + ": null argsMC");
+    if (!a->xnData || !a->xData || !a->bound_box || a->bound_box->size() != 4)
+        throw std::invalid_argument(std::string(who) + ": argsMC needs xnData, xData and a 4-value bound_box");
+    const size_t N = (size_t)H * W;
+    if (a->xnData->getElementDataType() != TYPEID_REAL || a->xnData->getDeviceBufferBytes() < 2 * (size_t)F * N * 4)
+        throw std::invalid_argument(std::string(who) + ": xnData must hold 2*F*H*W floats");
+    const ElementDataType xt = a->xData->getElementDataType();
+    if ((xt != TYPEID_UINT32 && xt != TYPEID_INT32) || a->xData->getDeviceBufferBytes() < 2 * N * 4)
+        throw std::invalid_argument(std::string(who) + ": xData must hold 2*H*W 32-bit integers");
+    ocl_mc_args m;
+    m.xn = (const float*)a->xnData->getDeviceBuffer();
+    m.x = (const int*)a->xData->getDeviceBuffer();
+    for (int i = 0; i < 4; ++i) m.bound_box[i] = a->bound_box->at(i);
+    return m;
+}
+
+void launchMC(Process& p, const std::shared_ptr<Data>& in, const std::shared_ptr<Data>& out,
+              ArgumentsMotionCompensation* args, bool adjoint, const char* who) {
+    needIO(in, out, who);
+    const NDArray* a = in->getNDArray(0);
+    const int F = (int)in->getNumNDArrays(), H = (int)NDARRAYHEIGHT(a), W = (int)NDARRAYWIDTH(a);
+    const ocl_mc_args m = mcArgs(args, F, H, W, who);
+    checkOcl(adjoint ? ocl_adjoint_motion_compensation(ctxOf(p), in->getDeviceBuffer(), out->getDeviceBuffer(), F, H,
+                                                       W, &m)
+                     : ocl_motion_compensation(ctxOf(p), in->getDeviceBuffer(), out->getDeviceBuffer(), F, H, W, &m),
+             who);
+}
+}  // namespace
+
+void MotionCompensation::launch() {
+    auto lp = std::dynamic_pointer_cast<LaunchParameters>(pLaunchParameters);
+    if (!lp) throw std::invalid_argument("MotionCompensation::launch: LaunchParameters required");
+    startProfiling();
+    launchMC(*this, getInput(), getOutput(), lp->argsMC, false, "MotionCompensation::launch");
+    stopProfiling();
+}
+
+void AdjointMotionCompensation::launch() {
+    auto lp = std::dynamic_pointer_cast<LaunchParameters>(pLaunchParameters);
+    if (!lp) throw std::invalid_argument("AdjointMotionCompensation::launch: LaunchParameters required");
+    startProfiling();
+    launchMC(*this, getInput(), getOutput(), lp->argsMC, true, "AdjointMotionCompensation::launch");
+    stopProfiling();
+}
+
 // ---- NestaUp (src/processes/nesta/NestaUp.cpp:81-106,442-910) ----
 void NestaUp::init() {
     auto k = std::dynamic_pointer_cast<KData>(getInput());
@@ -233,7 +286,6 @@ void NestaUp::launch() {
     if (!k || !x) throw std::invalid_argument("NestaUp::launch: KData input and XData output required");
     auto lp = std::dynamic_pointer_cast<LaunchParameters>(pLaunchParameters);
     if (!lp) throw std::invalid_argument("NestaUp::launch: LaunchParameters required");
-    if (lp->argsMC) throw std::invalid_argument("NestaUp::launch: the motion-compensated operator is not provided");
     auto S = k->getSensitivityMapsData();
     auto M = k->getSamplingMasksData();
     if (!S) throw std::invalid_argument("NestaUp::launch: non-existing SensitivityMaps");
@@ -242,12 +294,28 @@ void NestaUp::launch() {
     ocl_nesta_params p{lp->lambda_i, lp->mu_f, lp->La, (int)lp->maxIntIter, lp->tolVar, lp->verbose,
                        (int)lp->maxIter, (int)lp->stoptest, (int)lp->miniter};
     auto rep = std::unique_ptr<ocl_nesta_report>(new ocl_nesta_report());
+    const int F = (int)k->getDynDimsTotalSize(), H = (int)NDARRAYHEIGHT(a), W = (int)NDARRAYWIDTH(a);
     startProfiling();
-    checkOcl(ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
-                           (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), (int)k->getDynDimsTotalSize(),
-                           (int)k->getNCoils(), (int)NDARRAYHEIGHT(a), (int)NDARRAYWIDTH(a), &p,
-                           form_ == FUSED ? OCL_FORM_FUSED : form_ == SPLIT ? OCL_FORM_SPLIT : OCL_FORM_PLAIN, rep.get()),
-             "NestaUp::launch");
+    if (lp->argsMC) {
+        // argsMC != nullptr: tTV o MC, starting from the output's contents (NestaUp.cpp:201-278,483-491)
+        const ocl_mc_args m = mcArgs(lp->argsMC, F, H, W, "NestaUp::launch");
+        // the fused engine needs GroupwiseRegistration's identity mesh; other meshes and PLAIN take the plain form
+        int form = form_ == PLAIN ? OCL_FORM_PLAIN : OCL_FORM_FUSED;
+        int rc = ocl_nesta_run_mc(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
+                                  (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), F, (int)k->getNCoils(), H,
+                                  W, &p, form, &m, rep.get());
+        if (rc == OCL_E_UNSUPPORTED && form == OCL_FORM_FUSED)
+            rc = ocl_nesta_run_mc(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
+                                  (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), F, (int)k->getNCoils(), H,
+                                  W, &p, OCL_FORM_PLAIN, &m, rep.get());
+        checkOcl(rc, "NestaUp::launch");
+    } else {
+        checkOcl(ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
+                               (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), F, (int)k->getNCoils(), H, W,
+                               &p, form_ == FUSED ? OCL_FORM_FUSED : form_ == SPLIT ? OCL_FORM_SPLIT : OCL_FORM_PLAIN,
+                               rep.get()),
+                 "NestaUp::launch");
+    }
     stopProfiling();
     lp->tolVar = (float)rep->final_tolVar;   // the reference overwrites the caller's tolVar
     report_.normU = rep->normU;

@@ -31,6 +31,8 @@ struct SliceRes {
     std::vector<void*> allocs;
     size_t bytes = 0;
     bool allocated = false;
+    bool has_mc = false;      // motion compensation (argsMC) for this slice
+    ocl_mc_args mc{};
 
     ~SliceRes() {
         if (!allocs.empty()) {
@@ -174,6 +176,21 @@ size_t slice_footprint(const SliceRes& r) {
 }
 
 // Step 2 of admission: device buffers and row tables.
+// Motion-compensation geometry and pass sizes of a slice (before slice_alloc)
+int slice_plan_mc(SliceRes* r) {
+    SliceDesc& d = r->d;
+    if (!r->has_mc) {
+        d.mc_xn = nullptr;
+        d.wg_P = d.wg_X = 0;
+        return OCL_OK;
+    }
+    CHK(mc_box_geometry(d.H, d.W, &r->mc, &d.mc_r0, &d.mc_nr, &d.mc_c0, &d.mc_nc));
+    d.mc_xn = r->mc.xn;
+    d.wg_P = (int)(((long)d.F * d.N + OCL_MC_CHUNK - 1) / OCL_MC_CHUNK);
+    d.wg_X = (int)(((long)d.F * d.mc_nr * d.mc_nc + OCL_MC_CHUNK - 1) / OCL_MC_CHUNK);
+    return OCL_OK;
+}
+
 int slice_alloc(SliceRes* r) {
     if (r->allocated) return OCL_OK;
     ocl_ctx* c = r->c;
@@ -197,6 +214,11 @@ int slice_alloc(SliceRes* r) {
     CHK(r->alloc(sizeof(unsigned) * 4, &p)); d.counter = (unsigned*)p;
     CHK(r->alloc(sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
     CHK(r->alloc(sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
+    d.mcz = d.mcg = nullptr;
+    if (d.mc_xn) {
+        CHK(r->alloc(xb, &p)); d.mcz = (float2*)p;
+        CHK(r->alloc(xb, &p)); d.mcg = (float2*)p;
+    }
     if (d.prec) {
         // DFT matrices of the split-precision passes (row: forward, inverse; column: inverse, forward per frame)
         const size_t rowb = sizeof(uint16_t) * 4 * (size_t)d.W * d.W;
@@ -227,12 +249,16 @@ struct Table {
     SliceDesc* dd = nullptr;
     int* offB = nullptr;
     int* offCA = nullptr;
+    int* offP = nullptr;
+    int* offX = nullptr;
     int cap = 0;
     NestaLaunch L{};
     ~Table() {
         if (dd) hipFree(dd);
         if (offB) hipFree(offB);
         if (offCA) hipFree(offCA);
+        if (offP) hipFree(offP);
+        if (offX) hipFree(offX);
     }
 };
 
@@ -240,16 +266,21 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     const int ns = (int)v.size();
     t->c = c;
     if (ns > t->cap) {
-        if (t->dd) { hipStreamSynchronize(c->stream); hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); }
-        t->dd = nullptr; t->offB = t->offCA = nullptr;
+        if (t->dd) {
+            hipStreamSynchronize(c->stream);
+            hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); hipFree(t->offP); hipFree(t->offX);
+        }
+        t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = nullptr;
         const int cap = std::max(ns, 2 * t->cap);
         HIPCHK(hipMalloc(&t->dd, sizeof(SliceDesc) * cap));
         HIPCHK(hipMalloc(&t->offB, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offCA, sizeof(int) * (cap + 1)));
+        HIPCHK(hipMalloc(&t->offP, sizeof(int) * (cap + 1)));
+        HIPCHK(hipMalloc(&t->offX, sizeof(int) * (cap + 1)));
         t->cap = cap;
     }
     std::vector<SliceDesc> hd(ns);
-    std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0);
+    std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0), oP(ns + 1, 0), oX(ns + 1, 0);
     NestaLaunch& L = t->L;
     L = NestaLaunch{};
     L.maxe = 4;
@@ -257,9 +288,14 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.prec = ns ? v[0]->d.prec : 0;
     L.kj = 1;
     L.stage = ns ? v[0]->d.ca_stage : 0;
+    L.mc = ns && v[0]->d.mc_xn ? 1 : 0;
     for (int i = 0; i < ns; ++i) {
         const SliceDesc& d = v[i]->d;
         if (d.prec != L.prec) return fail(OCL_E_INVALID, "batch: slices of one table must share the precision form");
+        if ((d.mc_xn ? 1 : 0) != L.mc)
+            return fail(OCL_E_INVALID, "batch: motion compensation must be on for every slice of a table or none");
+        oP[i + 1] = oP[i] + d.wg_P;
+        oX[i + 1] = oX[i] + d.wg_X;
         if (L.prec && (d.H != v[0]->d.H || d.W != v[0]->d.W))
             return fail(OCL_E_UNSUPPORTED, "batch: split-precision slices of one table must share H and W");
         L.kj = std::max(L.kj, d.kp / 32);
@@ -279,9 +315,13 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.dd = t->dd;
     L.offB = t->offB;
     L.offCA = t->offCA;
+    L.offP = t->offP;
+    L.offX = t->offX;
     L.ns = ns;
     L.totB = oB[ns];
     L.totCA = oC[ns];
+    L.totP = oP[ns];
+    L.totX = oX[ns];
     L.PH = PH;
     L.PW = PW;
     if (L.prec && ns) {
@@ -297,18 +337,30 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offB, oB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offCA, oC.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));   // hd / oB / oC are host temporaries
+        HIPCHK(hipMemcpyAsync(t->offP, oP.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(t->offX, oX.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // hd / oB / oC / oP / oX are host temporaries
     }
     return OCL_OK;
 }
 
 // Per-reconstruction setup (NestaUp.cpp:442-649): x_ref = A^H y, the forward columns of iteration 0, the maxima of
 // |x_ref| and |U x_ref|, normU, and the initial device state.  Scalars are kept in double.
+// With motion compensation (NestaUp.cpp:483-491) x_ref is the slice's output on entry (the previous reconstruction):
+// it is copied to x0, MC(x_ref) feeds max |U x_ref|, pass CA (mode 2) only takes the forward columns, and normU is
+// the norm of tTV o MC.
 int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
     if (v.empty()) return OCL_OK;
     for (SliceRes* r : v) CHK(ocl_memset_zero(c, r->d.counter, sizeof(unsigned) * 4));
-    HIPCHK(launch_nesta_B(t.L, c->stream, 1));
-    HIPCHK(launch_nesta_CA(t.L, c->stream, 1));
+    if (t.L.mc) {
+        for (SliceRes* r : v)
+            CHK(ocl_memcpy_d2d(c, r->d.x0, r->in.x_out, sizeof(float2) * (size_t)r->d.F * r->d.N));
+        HIPCHK(launch_nesta_mc_z(t.L, c->stream, 1));
+        HIPCHK(launch_nesta_CA(t.L, c->stream, 2));
+    } else {
+        HIPCHK(launch_nesta_B(t.L, c->stream, 1));
+        HIPCHK(launch_nesta_CA(t.L, c->stream, 1));
+    }
     hipLaunchKernelGGL(k_setup_max, dim3(t.L.totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)t.dd,
                        (const int*)t.offCA, t.L.ns, (double*)nullptr);
     HIPCHK(hipGetLastError());
@@ -325,7 +377,8 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         SliceSetup& u = r->setup;
         u.maxX = (float)m0;
         u.maxUX = (float)m1;
-        CHK(ocl_normest(c, d.F, d.N, &u.normU, &u.ne_iters));
+        if (r->has_mc) CHK(normest_mc(c, d.F, d.H, d.W, &r->mc, &u.normU, &u.ne_iters));
+        else CHK(ocl_normest(c, d.F, d.N, &u.normU, &u.ne_iters));
         const ocl_nesta_params& P = r->prm;
         u.lambda = P.lambda_i * u.maxX;
         u.muf = P.mu_f * u.maxX;
@@ -356,8 +409,10 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
 int table_iterate(ocl_ctx* c, const Table& t, int n) {
     if (!t.L.ns) return OCL_OK;
     for (int it = 0; it < n; ++it) {
+        HIPCHK(launch_nesta_mc_z(t.L, c->stream, 0));      // motion compensation only
         HIPCHK(launch_nesta_B(t.L, c->stream, 0));
         HIPCHK(launch_nesta_S(t.L, c->stream));
+        HIPCHK(launch_nesta_mc_g(t.L, c->stream));         // motion compensation only
         HIPCHK(launch_nesta_CA(t.L, c->stream, 0));
     }
     return OCL_OK;
@@ -370,9 +425,11 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
     int rc = OCL_OK;
     for (int it = 0; it < n && rc == OCL_OK; ++it) {
         hipError_t e = hipEventRecord(ev[3 * it], c->stream);
+        if (e == hipSuccess) e = launch_nesta_mc_z(t.L, c->stream, 0);
         if (e == hipSuccess) e = launch_nesta_B(t.L, c->stream, 0);
         if (e == hipSuccess) e = launch_nesta_S(t.L, c->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 1], c->stream);
+        if (e == hipSuccess) e = launch_nesta_mc_g(t.L, c->stream);
         if (e == hipSuccess) e = launch_nesta_CA(t.L, c->stream, 0);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 2], c->stream);
         if (e != hipSuccess) rc = fail(OCL_E_HIP, std::string("timing launch: ") + hipGetErrorString(e));
@@ -538,6 +595,7 @@ struct ocl_batch {
     hipGraphExec_t gexec = nullptr;
     int prec = 0;
     bool ready = false;
+    std::vector<ocl_mc_args> mcs;   // per slice, or empty (no motion compensation)
 };
 
 extern "C" int ocl_batch_create(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
@@ -570,6 +628,37 @@ extern "C" int ocl_batch_create_form(ocl_ctx* c, int nslices, const ocl_slice* s
     return OCL_OK;
 }
 
+// True when the mesh is GroupwiseRegistration's original mesh ([d][c][r] = (c + 1, r + 1)), which the fused passes
+// rely on (box pixel (r, c) writes to (r, c))
+static int mesh_is_identity(ocl_ctx* c, const int* x, int H, int W, bool* ok) {
+    std::vector<int> h((size_t)2 * H * W);
+    HIPCHK(hipMemcpy(h.data(), x, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+    *ok = true;
+    for (int cc = 0; cc < W && *ok; ++cc)
+        for (int r = 0; r < H; ++r)
+            if (h[r + (size_t)cc * H] != cc + 1 || h[r + (size_t)cc * H + (size_t)H * W] != r + 1) { *ok = false; break; }
+    return OCL_OK;
+}
+
+extern "C" int ocl_batch_create_mc(ocl_ctx* c, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
+                                   const ocl_mc_args* mcs, ocl_batch** out) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (!mcs || nslices < 1 || !slices || !out) return fail(OCL_E_INVALID, "batch: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    for (int i = 0; i < nslices; ++i) {
+        int r0, nr, c0, nc;
+        CHK(mc_box_geometry(slices[i].H, slices[i].W, &mcs[i], &r0, &nr, &c0, &nc));
+        bool ok = false;
+        CHK(mesh_is_identity(c, mcs[i].x, slices[i].H, slices[i].W, &ok));
+        if (!ok)
+            return fail(OCL_E_UNSUPPORTED, "batch: the fused motion-compensated form needs the original (identity) mesh; "
+                                           "use OCL_FORM_PLAIN for other meshes");
+    }
+    CHK(ocl_batch_create_form(c, nslices, slices, prm, OCL_FORM_FUSED, out));
+    (*out)->mcs.assign(mcs, mcs + nslices);
+    return OCL_OK;
+}
+
 extern "C" int ocl_batch_destroy(ocl_batch* b) {
     if (!b) return OCL_OK;
     hipSetDevice(b->ctx->device);
@@ -588,10 +677,17 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
     if (b->res.empty()) {
         long total_cols = 0;
         for (const auto& s : b->sl) total_cols += (long)s.F * s.W;
-        for (const auto& s : b->sl) {
+        for (size_t i = 0; i < b->sl.size(); ++i) {
+            const ocl_slice& s = b->sl[i];
             b->res.emplace_back(new SliceRes());
-            CHK(slice_plan(c, s, b->prm, total_cols, b->prec, b->res.back().get()));
-            CHK(slice_alloc(b->res.back().get()));
+            SliceRes* r = b->res.back().get();
+            CHK(slice_plan(c, s, b->prm, total_cols, b->prec, r));
+            if (!b->mcs.empty()) {
+                r->has_mc = true;
+                r->mc = b->mcs[i];
+            }
+            CHK(slice_plan_mc(r));
+            CHK(slice_alloc(r));
             b->view.push_back(b->res.back().get());
         }
         CHK(table_build(c, &b->t, b->view));

@@ -608,6 +608,25 @@ done:
     return rc;
 }
 
+int mc_box_geometry(int H, int W, const ocl_mc_args* mc, int* r0, int* nr, int* c0, int* nc) {
+    McGeom g;
+    CHK(mc_geometry(H, W, mc, &g));
+    *r0 = g.r0; *nr = g.nr; *c0 = g.c0; *nc = g.nc;
+    return OCL_OK;
+}
+
+int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters) {
+    const long n = (long)F * H * W;
+    float2* aux = nullptr;
+    if (hipMalloc(&aux, sizeof(float2) * n) != hipSuccess) return fail(OCL_E_NOMEM, "normest: out of device memory");
+    SparsityOp sp{c, F, H, W, mc, {}, aux};
+    int rc = mc_geometry(H, W, mc, &sp.g);
+    if (rc == OCL_OK) rc = normest_compute(c, F, (long)H * W, normU, iters, &sp);
+    hipStreamSynchronize(c->stream);
+    hipFree(aux);
+    return rc;
+}
+
 extern "C" int ocl_normest(ocl_ctx* c, int F, long N, double* normU, int* iters) {
     CTXCHK(c);
     if (F < 1 || N < 1 || !normU) return fail(OCL_E_INVALID, "normest: bad arguments");
@@ -820,10 +839,17 @@ extern "C" int ocl_nesta_run_mc(ocl_ctx* c, const void* y, const void* S, const 
     if (!S) return fail(OCL_E_INVALID, "NESTA: non-existing SensitivityMaps");
     if (!M) return fail(OCL_E_INVALID, "NESTA: null sampling masks");
     if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER) return fail(OCL_E_INVALID, "NESTA: miniter out of range");
-    if (form != OCL_FORM_PLAIN)
-        return fail(OCL_E_UNSUPPORTED, "NESTA with motion compensation runs in the plain form only");
+    if (form != OCL_FORM_PLAIN && form != OCL_FORM_FUSED)
+        return fail(OCL_E_UNSUPPORTED, "NESTA with motion compensation runs in the plain or the fused form");
     McGeom g;
     CHK(mc_geometry(H, W, mc, &g));
     HIPCHK(hipSetDevice(c->device));
-    return nesta_plain(c, y, S, M, x_io, F, C, H, W, *prm, rep, mc);
+    if (form == OCL_FORM_PLAIN) return nesta_plain(c, y, S, M, x_io, F, C, H, W, *prm, rep, mc);
+    ocl_slice s{y, S, M, x_io, F, C, H, W};
+    ocl_batch* b = nullptr;
+    CHK(ocl_batch_create_mc(c, 1, &s, prm, mc, &b));
+    int rc = ocl_batch_run(b, 16);
+    if (rc == OCL_OK && rep) rc = ocl_batch_report(b, 0, rep);
+    ocl_batch_destroy(b);
+    return rc;
 }

@@ -73,7 +73,17 @@ struct SliceDesc {
     const uint16_t* drowI;  // [4][W][W]  inverse row DFT, K index permuted
     const uint16_t* dcolI;  // [F][4][H][kp]  inverse column DFT from the sampled rows
     const uint16_t* dcolF;  // [F][4][kp][H]  forward column DFT onto the sampled rows, K index permuted
+    // motion compensation (null mc_xn: none).  The sparsity operator is then tTV o MC: pass Z writes mcz = MC(x_k)
+    // before pass B (whose temporal-TV terms read it), pass G writes mcg = MC^T(D^T u(mcz)) after the scalar step,
+    // and pass CA's update reads mcg in place of the local temporal-TV gradient.  The original mesh must be the
+    // identity (checked at batch creation), so box pixel (r, c) resamples at xn and writes to (r, c).
+    const float* mc_xn;     // [2][F][H][W] deformed sample positions, 1-based ([0] column, [1] row)
+    int mc_r0, mc_nr, mc_c0, mc_nc;   // box rows / columns (0-based)
+    float2* mcz;            // [F][H][W]
+    float2* mcg;            // [F][H][W]
+    int wg_P, wg_X;         // workgroups of the pixel passes (Z, G clear) and of the box scatter
 };
+#define OCL_MC_CHUNK 1024    // pixel-frames per workgroup of the motion-compensation passes
 
 // kernels (ocl_ops.hip)
 __global__ void k_ttv(const float2* in, float2* out, int F, long N, int dir);
@@ -126,10 +136,18 @@ struct NestaLaunch {
     int prec;           // 1: every slice uses the split-precision passes (then PH = H/16, PW = W/16)
     int kj;             // split-precision passes: largest kp / 32 of the table
     int stage;          // 0 / 1: every slice uses the direct / prefetching four-step pass CA, 2: mixed
+    int mc;             // 1: every slice uses motion compensation (passes Z and G run)
+    const int* offP;    // workgroup offsets of the motion-compensation pixel passes
+    const int* offX;    // workgroup offsets of the motion-compensation box scatter
+    int totP, totX;
 };
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);
+// mode 0: iteration, 1: setup (x_ref = A^H y), 2: setup from a given x_ref in x0 (forward columns only)
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
+// motion-compensation passes of the fused iteration (ocl_mc.hip): Z (mode 0: x_k, mode 1: setup from x0), G
+hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode);
+hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream);
 void nesta_kernel_attributes(int lds_bytes);
 // split-precision passes (ocl_nesta_mfma.hip)
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);

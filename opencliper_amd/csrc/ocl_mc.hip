@@ -12,7 +12,7 @@
 // Memory: one thread per (box pixel, frame) with the column index fastest, so xn, the input neighbourhoods and the
 // output rows are read and written along contiguous rows.  The scatter uses float atomics (the reference does too),
 // so its summation order, and the last bits of overlapping contributions, vary from run to run.
-#include "ocl_kernels.hpp"
+#include "ocl_nesta_common.hpp"
 
 namespace {
 
@@ -42,6 +42,54 @@ __device__ __forceinline__ float2 mc_pixel(const float2* __restrict__ fr, int H,
     return (h >= 0 && h < W && v >= 0 && v < H) ? fr[h + (long)v * W] : make_float2(0.f, 0.f);
 }
 
+// Bilinear sample of frame fr at the deformed position (nhor, nver) (0-based), with the interpolator's arithmetic
+__device__ __forceinline__ float2 mc_bilinear(const float2* __restrict__ fr, int H, int W, float nhor, float nver,
+                                              int ph, int pv) {
+    const float2 i1 = mc_pixel(fr, H, W, ph, pv), i2 = mc_pixel(fr, H, W, ph + 1, pv);
+    const float2 i3 = mc_pixel(fr, H, W, ph, pv + 1), i4 = mc_pixel(fr, H, W, ph + 1, pv + 1);
+    const float wl = (float)(ph + 1) - nhor, wr = nhor - (float)ph;
+    const float wu = (float)(pv + 1) - nver, wd = nver - (float)pv;
+    const float2 a1 = make_float2(wl * i1.x + wr * i2.x, wl * i1.y + wr * i2.y);
+    const float2 a2 = make_float2(wl * i3.x + wr * i4.x, wl * i3.y + wr * i4.y);
+    return make_float2(wu * a1.x + wd * a2.x, wu * a1.y + wd * a2.y);
+}
+
+// Scatter of v with the bilinear weights of (nhor, nver) into frame fr (the adjoint interpolator's arithmetic)
+__device__ __forceinline__ void mc_scatter4(float2* __restrict__ fr, int H, int W, float nhor, float nver, int ph,
+                                            int pv, float2 v) {
+    const float Whh = nhor - (float)ph, Wvv = nver - (float)pv;
+    float* f = (float*)fr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int h = ph + (q & 1), vv = pv + (q >> 1);
+        if (h < 0 || h >= W || vv < 0 || vv >= H) continue;
+        const float hw = (q & 1) ? Whh : 1.0f - Whh, vw = (q >> 1) ? Wvv : 1.0f - Wvv;
+        const long o = 2 * (h + (long)vv * W);
+        atomicAdd(f + o, v.x * hw * vw);
+        atomicAdd(f + o + 1, v.y * hw * vw);
+    }
+}
+
+// Sample position of pixel p of frame k from xn (0-based), and whether any neighbour can exist
+__device__ __forceinline__ bool mc_position(const float* __restrict__ xn, int F, int H, int W, long N, int k, long p,
+                                            float* nhor, float* nver, int* ph, int* pv) {
+    *nhor = xn[p + k * N] - 1.0f;
+    *nver = xn[p + k * N + N * F] - 1.0f;
+    const bool inside = *nhor > -2.0f && *nhor < (float)(W + 1) && *nver > -2.0f && *nver < (float)(H + 1);
+    *ph = inside ? (int)floorf(*nhor) : 0;
+    *pv = inside ? (int)floorf(*nver) : 0;
+    return inside;
+}
+
+// D^T u at pixel p of frame f for u = D z / max(mu, |D z|) (periodic in f): u_f - u_{f+1}
+__device__ __forceinline__ float2 mc_tv_grad(const float2* __restrict__ z, int F, long N, int f, long p, float mu) {
+    const int fm = f == 0 ? F - 1 : f - 1, fp = f == F - 1 ? 0 : f + 1;
+    const float2 zf = z[(long)f * N + p], zm = z[(long)fm * N + p], zp = z[(long)fp * N + p];
+    const float2 uxf = c_sub(zf, zm), uxp = c_sub(zp, zf);
+    const float nf = fmaxf(mu, sqrtf(c_abs2(uxf))), np = fmaxf(mu, sqrtf(c_abs2(uxp)));
+    return make_float2(uxf.x / nf - uxp.x / np, uxf.y / nf - uxp.y / np);
+}
+
 }  // namespace
 
 // out[box pixels] = bilinear sample of in at the deformed positions (out holds a copy of in elsewhere)
@@ -54,18 +102,7 @@ __global__ void k_mc_interp(const float2* __restrict__ in, float2* __restrict__ 
         const int i = (int)(b / nc), j = (int)(b - (long)i * nc);
         const McPoint p = mc_point(xn, x, F, H, W, r0 + i, c0 + j, k);
         if (p.out < 0) continue;
-        float2 v = make_float2(0.f, 0.f);
-        if (p.inside) {
-            const float2* fr = in + k * N;
-            const float2 i1 = mc_pixel(fr, H, W, p.ph, p.pv), i2 = mc_pixel(fr, H, W, p.ph + 1, p.pv);
-            const float2 i3 = mc_pixel(fr, H, W, p.ph, p.pv + 1), i4 = mc_pixel(fr, H, W, p.ph + 1, p.pv + 1);
-            const float wl = (float)(p.ph + 1) - p.nhor, wr = p.nhor - (float)p.ph;
-            const float wu = (float)(p.pv + 1) - p.nver, wd = p.nver - (float)p.pv;
-            const float2 a1 = make_float2(wl * i1.x + wr * i2.x, wl * i1.y + wr * i2.y);
-            const float2 a2 = make_float2(wl * i3.x + wr * i4.x, wl * i3.y + wr * i4.y);
-            v = make_float2(wu * a1.x + wd * a2.x, wu * a1.y + wd * a2.y);
-        }
-        out[k * N + p.out] = v;
+        out[k * N + p.out] = p.inside ? mc_bilinear(in + k * N, H, W, p.nhor, p.nver, p.ph, p.pv) : make_float2(0.f, 0.f);
     }
 }
 
@@ -91,17 +128,103 @@ __global__ void k_mc_scatter(const float2* __restrict__ in, float2* __restrict__
         const int i = (int)(b / nc), j = (int)(b - (long)i * nc);
         const McPoint p = mc_point(xn, x, F, H, W, r0 + i, c0 + j, k);
         if (p.out < 0 || !p.inside) continue;
-        const float2 i0 = in[k * N + p.out];
-        const float Whh = p.nhor - (float)p.ph, Wvv = p.nver - (float)p.pv;
-        float* fr = (float*)(out + k * N);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int h = p.ph + (q & 1), v = p.pv + (q >> 1);
-            if (h < 0 || h >= W || v < 0 || v >= H) continue;
-            const float hw = (q & 1) ? Whh : 1.0f - Whh, vw = (q >> 1) ? Wvv : 1.0f - Wvv;
-            const long o = 2 * (h + (long)v * W);
-            atomicAdd(fr + o, i0.x * hw * vw);
-            atomicAdd(fr + o + 1, i0.y * hw * vw);
-        }
+        mc_scatter4(out + k * N, H, W, p.nhor, p.nver, p.ph, p.pv, in[k * N + p.out]);
     }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Passes of the fused batched iteration with motion compensation (identity mesh: box pixel (r, c) writes to (r, c)).
+// Each workgroup covers OCL_MC_CHUNK consecutive pixel-frames of one slice (pixel passes) or of its box (scatter).
+// ---------------------------------------------------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ bool in_box(const SliceDesc& d, int r, int c) {
+    return r >= d.mc_r0 && r < d.mc_r0 + d.mc_nr && c >= d.mc_c0 && c < d.mc_c0 + d.mc_nc;
+}
+}  // namespace
+
+// Z: mcz = MC(x_k) (mode 0, x_k where the state points) or MC(x0) (mode 1, setup)
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_z(const SliceDesc* __restrict__ descs, const int* __restrict__ off,
+                                                       int ns, int mode) {
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (mode == 0 && !d.st->active) return;
+    const float2* xk = mode ? d.x0 : (d.st->cur ? d.x1 : d.x0);
+    const long N = d.N, total = (long)d.F * N;
+    const long base = (long)(blockIdx.x - off[si]) * OCL_MC_CHUNK;
+#pragma unroll
+    for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
+        const long i = base + q * OCL_NT + threadIdx.x;
+        if (i >= total) break;
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        const int r = (int)(p / d.W), c = (int)(p - (long)r * d.W);
+        float2 v = xk[i];
+        if (in_box(d, r, c)) {
+            float nhor, nver;
+            int ph, pv;
+            v = mc_position(d.mc_xn, d.F, d.H, d.W, N, f, p, &nhor, &nver, &ph, &pv)
+                    ? mc_bilinear(xk + (long)f * N, d.H, d.W, nhor, nver, ph, pv)
+                    : make_float2(0.f, 0.f);
+        }
+        d.mcz[i] = v;
+    }
+}
+
+// G, first half: mcg = D^T u(mcz) outside the box, 0 inside (copy + initZeroRect of the adjoint)
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gclear(const SliceDesc* __restrict__ descs,
+                                                            const int* __restrict__ off, int ns) {
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (!d.st->it_pending) return;
+    const float mu = (float)d.st->it_mu;
+    const long N = d.N, total = (long)d.F * N;
+    const long base = (long)(blockIdx.x - off[si]) * OCL_MC_CHUNK;
+#pragma unroll
+    for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
+        const long i = base + q * OCL_NT + threadIdx.x;
+        if (i >= total) break;
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        const int r = (int)(p / d.W), c = (int)(p - (long)r * d.W);
+        d.mcg[i] = in_box(d, r, c) ? make_float2(0.f, 0.f) : mc_tv_grad(d.mcz, d.F, N, f, p, mu);
+    }
+}
+
+// G, second half: the box pixels' D^T u(mcz) scattered into mcg with the bilinear weights
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gscatter(const SliceDesc* __restrict__ descs,
+                                                              const int* __restrict__ off, int ns) {
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (!d.st->it_pending) return;
+    const float mu = (float)d.st->it_mu;
+    const long N = d.N, per = (long)d.mc_nr * d.mc_nc, total = per * d.F;
+    const long base = (long)(blockIdx.x - off[si]) * OCL_MC_CHUNK;
+#pragma unroll
+    for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
+        const long i = base + q * OCL_NT + threadIdx.x;
+        if (i >= total) break;
+        const int f = (int)(i / per);
+        const long b = i - (long)f * per;
+        const int ii = (int)(b / d.mc_nc), jj = (int)(b - (long)ii * d.mc_nc);
+        const long p = (long)(d.mc_r0 + ii) * d.W + d.mc_c0 + jj;
+        float nhor, nver;
+        int ph, pv;
+        if (!mc_position(d.mc_xn, d.F, d.H, d.W, N, f, p, &nhor, &nver, &ph, &pv)) continue;
+        mc_scatter4(d.mcg + (long)f * N, d.H, d.W, nhor, nver, ph, pv, mc_tv_grad(d.mcz, d.F, N, f, p, mu));
+    }
+}
+
+hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode) {
+    if (!L.mc || !L.totP) return hipSuccess;
+    hipLaunchKernelGGL(k_nesta_mc_z, dim3(L.totP), dim3(OCL_NT), 0, stream, L.dd, L.offP, L.ns, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream) {
+    if (!L.mc || !L.totP) return hipSuccess;
+    hipLaunchKernelGGL(k_nesta_mc_gclear, dim3(L.totP), dim3(OCL_NT), 0, stream, L.dd, L.offP, L.ns);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !L.totX) return e;
+    hipLaunchKernelGGL(k_nesta_mc_gscatter, dim3(L.totX), dim3(OCL_NT), 0, stream, L.dd, L.offX, L.ns);
+    return hipGetLastError();
 }

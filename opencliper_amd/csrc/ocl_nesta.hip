@@ -219,9 +219,10 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_B_t(const SliceDesc* __restric
 
 // ------------------------------------------------------------------------------------------------------------------
 // pass CA.  mode 0: NESTA iteration k with the coefficients prepared by B(k), then the forward columns of x_{k+1}.
-// mode 1: setup (x_ref = A^H y, forward columns of iteration 0).
+// mode 1: setup (x_ref = A^H y, forward columns of iteration 0).  mode 2: setup from a given x_ref in x0 (motion
+// compensation starts from the previous reconstruction): forward columns only.
 // ------------------------------------------------------------------------------------------------------------------
-template <int MAXE>
+template <int MAXE, bool MC = false>
 __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, int mode, float2* lds) {
     const int tile = local % d.ncol_tiles;
     const int f = local / d.ncol_tiles;
@@ -242,8 +243,9 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) acc[i] = make_float2(0.f, 0.f);
 
-    // ---- A^H r: inverse column DFTs of the sampled rows, conj(S) multiply, coil sum ----
-    for (int c = 0; c < C; ++c) {
+    // ---- A^H r: inverse column DFTs of the sampled rows, conj(S) multiply, coil sum (not in mode 2) ----
+    const int Cinv = (MC && mode == 2) ? 0 : C;
+    for (int c = 0; c < Cinv; ++c) {
         for (int e = threadIdx.x; e < nel; e += blockDim.x) a[(e >> lsh) * ld + (e & (TW - 1))] = make_float2(0.f, 0.f);
         __syncthreads();
         const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
@@ -278,6 +280,16 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
                 if (w0 + wl < W) d.x0[(long)f * N + (long)h * W + w0 + wl] = xn[i];
             }
         }
+    } else if (MC && mode == 2) {                          // x_ref given in x0: forward columns only
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int e = threadIdx.x + i * OCL_NT;
+            xn[i] = make_float2(0.f, 0.f);
+            if (e < nel) {
+                const int h = e >> lsh, wl = e & (TW - 1);
+                if (w0 + wl < W) xn[i] = d.x0[(long)f * N + (long)h * W + w0 + wl];
+            }
+        }
     } else {
         const UpdRow u = upd_row(d, f);
 #pragma unroll
@@ -286,7 +298,7 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
             xn[i] = make_float2(0.f, 0.f);
             if (e < nel) {
                 const int h = e >> lsh, wl = e & (TW - 1);
-                if (w0 + wl < W) xn[i] = nesta_update_at(u, (unsigned)(h * W + w0 + wl), c_scale(acc[i], s));
+                if (w0 + wl < W) xn[i] = nesta_update_at<MC>(u, (unsigned)(h * W + w0 + wl), c_scale(acc[i], s));
             }
         }
     }
@@ -315,15 +327,15 @@ __device__ __forceinline__ void nesta_CA_generic(const SliceDesc& d, int local, 
 }
 
 // STG: 0 / 1 every slice of the table uses the direct / prefetching four-step column pass, 2 either (per slice)
-template <int P, int STG>
+template <int P, int STG, bool MC>
 __device__ __forceinline__ void nesta_CA_fast_any(const SliceDesc& d, int local, int mode, float2* lds) {
-    if constexpr (STG == 0) nesta_CA_fast<P, false>(d, local, mode, lds);
-    else if constexpr (STG == 1) nesta_CA_fast<P, true>(d, local, mode, lds);
-    else if (d.ca_stage) nesta_CA_fast<P, true>(d, local, mode, lds);
-    else nesta_CA_fast<P, false>(d, local, mode, lds);
+    if constexpr (STG == 0) nesta_CA_fast<P, false, MC>(d, local, mode, lds);
+    else if constexpr (STG == 1) nesta_CA_fast<P, true, MC>(d, local, mode, lds);
+    else if (d.ca_stage) nesta_CA_fast<P, true, MC>(d, local, mode, lds);
+    else nesta_CA_fast<P, false, MC>(d, local, mode, lds);
 }
 
-template <int PHU, int MAXE, int STG>
+template <int PHU, int MAXE, int STG, bool MC>
 __device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ descs, const int* __restrict__ wg_off,
                                               int nslices, int mode) {
     extern __shared__ float2 lds[];
@@ -333,29 +345,36 @@ __device__ __forceinline__ void nesta_CA_body(const SliceDesc* __restrict__ desc
     if (mode == 0 && !st->it_pending) return;
     const int local = blockIdx.x - wg_off[si];
     if constexpr (PHU > 0) {
-        nesta_CA_fast_any<PHU, STG>(d, local, mode, lds);
+        nesta_CA_fast_any<PHU, STG, MC>(d, local, mode, lds);
     } else if (d.fastH) {
         switch (d.fastH) {
-            case 2: nesta_CA_fast_any<2, STG>(d, local, mode, lds); break;
-            case 4: nesta_CA_fast_any<4, STG>(d, local, mode, lds); break;
-            case 6: nesta_CA_fast_any<6, STG>(d, local, mode, lds); break;
-            case 8: nesta_CA_fast_any<8, STG>(d, local, mode, lds); break;
-            case 10: nesta_CA_fast_any<10, STG>(d, local, mode, lds); break;
-            case 12: nesta_CA_fast_any<12, STG>(d, local, mode, lds); break;
-            case 14: nesta_CA_fast_any<14, STG>(d, local, mode, lds); break;
-            case 16: nesta_CA_fast_any<16, STG>(d, local, mode, lds); break;
-            case 18: nesta_CA_fast_any<18, STG>(d, local, mode, lds); break;
-            default: nesta_CA_fast_any<20, STG>(d, local, mode, lds); break;
+            case 2: nesta_CA_fast_any<2, STG, MC>(d, local, mode, lds); break;
+            case 4: nesta_CA_fast_any<4, STG, MC>(d, local, mode, lds); break;
+            case 6: nesta_CA_fast_any<6, STG, MC>(d, local, mode, lds); break;
+            case 8: nesta_CA_fast_any<8, STG, MC>(d, local, mode, lds); break;
+            case 10: nesta_CA_fast_any<10, STG, MC>(d, local, mode, lds); break;
+            case 12: nesta_CA_fast_any<12, STG, MC>(d, local, mode, lds); break;
+            case 14: nesta_CA_fast_any<14, STG, MC>(d, local, mode, lds); break;
+            case 16: nesta_CA_fast_any<16, STG, MC>(d, local, mode, lds); break;
+            case 18: nesta_CA_fast_any<18, STG, MC>(d, local, mode, lds); break;
+            default: nesta_CA_fast_any<20, STG, MC>(d, local, mode, lds); break;
         }
     } else {
-        nesta_CA_generic<MAXE>(d, local, mode, lds);
+        nesta_CA_generic<MAXE, MC>(d, local, mode, lds);
     }
 }
 
 template <int PHU, int MAXE, int STG>
 __global__ __launch_bounds__(OCL_NT) void k_nesta_CA_t(const SliceDesc* __restrict__ descs,
                                                        const int* __restrict__ wg_off, int nslices, int mode) {
-    nesta_CA_body<PHU, MAXE, STG>(descs, wg_off, nslices, mode);
+    nesta_CA_body<PHU, MAXE, STG, false>(descs, wg_off, nslices, mode);
+}
+
+// The same pass for tables with motion compensation (update gradient from mcg, mode 2 setup); STG is per slice.
+template <int PHU, int MAXE>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_CA_mc_t(const SliceDesc* __restrict__ descs,
+                                                          const int* __restrict__ wg_off, int nslices, int mode) {
+    nesta_CA_body<PHU, MAXE, 2, true>(descs, wg_off, nslices, mode);
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -374,6 +393,21 @@ static NestaKernel pick_B(int PW) {
         case 20: return k_nesta_B_t<20>;
         default: return k_nesta_B_t<0>;
     }
+}
+
+static NestaKernel pick_CA_mc(int PH, int maxe) {
+    switch (PH) {
+        case 8: return k_nesta_CA_mc_t<8, 4>;
+        case 10: return k_nesta_CA_mc_t<10, 4>;
+        case 12: return k_nesta_CA_mc_t<12, 4>;
+        case 14: return k_nesta_CA_mc_t<14, 4>;
+        case 16: return k_nesta_CA_mc_t<16, 4>;
+        case 18: return k_nesta_CA_mc_t<18, 4>;
+        case 20: return k_nesta_CA_mc_t<20, 4>;
+        default: break;
+    }
+    return maxe <= 4 ? k_nesta_CA_mc_t<0, 4> : maxe <= 8 ? k_nesta_CA_mc_t<0, 8> : maxe <= 16 ? k_nesta_CA_mc_t<0, 16>
+                                                                                     : k_nesta_CA_mc_t<0, 32>;
 }
 
 static NestaKernel pick_CA(int PH, int maxe, int stg) {
@@ -401,7 +435,8 @@ hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream) {
 
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode) {
     if (L.prec) return launch_nesta_CA_mfma(L, stream, mode);
-    hipLaunchKernelGGL(pick_CA(L.PH, L.maxe, L.stage), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
+    hipLaunchKernelGGL(L.mc ? pick_CA_mc(L.PH, L.maxe) : pick_CA(L.PH, L.maxe, L.stage), dim3(L.totCA), dim3(OCL_NT),
+                       L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }
 
@@ -409,10 +444,13 @@ void nesta_kernel_attributes(int lds_bytes) {
     const int ps[] = {0, 8, 10, 12, 14, 16, 18, 20};
     for (int p : ps) {
         (void)hipFuncSetAttribute((const void*)pick_B(p), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-        for (int e : {4, 8, 16, 32})
+        for (int e : {4, 8, 16, 32}) {
             for (int g : {0, 1, 2})
                 (void)hipFuncSetAttribute((const void*)pick_CA(p, e, g), hipFuncAttributeMaxDynamicSharedMemorySize,
                                           lds_bytes);
+            (void)hipFuncSetAttribute((const void*)pick_CA_mc(p, e), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      lds_bytes);
+        }
     }
     (void)hipGetLastError();
 }
@@ -428,13 +466,14 @@ __global__ void k_setup_max(const SliceDesc* __restrict__ descs, const int* __re
     const long chunk = (total + d.wg_CA - 1) / d.wg_CA;
     const long lo = (long)local * chunk, hi = min(total, lo + chunk);
     double m0 = -1.0, m1 = -1.0;
+    const float2* z = d.mcz ? d.mcz : d.x0;              // U x_ref = tTV(MC(x_ref)) with motion compensation
     for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const int f = (int)(i / d.N);
         const long p = i - (long)f * d.N;
         const int fm = f == 0 ? d.F - 1 : f - 1;
         const float2 x = d.x0[i];
         m0 = fmax(m0, (double)sqrtf(c_abs2(x)));
-        m1 = fmax(m1, (double)sqrtf(c_abs2(c_sub(x, d.x0[(long)fm * d.N + p]))));
+        m1 = fmax(m1, (double)sqrtf(c_abs2(c_sub(z[i], z[(long)fm * d.N + p]))));
     }
     m0 = block_max(m0, red);
     m1 = block_max(m1, red);

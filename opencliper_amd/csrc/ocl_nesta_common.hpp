@@ -12,7 +12,8 @@ __device__ __forceinline__ int find_slice(const int* __restrict__ off, int ns, i
     return lo;
 }
 
-// Temporal-TV terms of x_k over one chunk of pixels and one group of OCL_U_FRAMES frames (NestaUp.cpp:669-691):
+// Temporal-TV terms of x_k (of MC(x_k) with motion compensation) over one chunk of pixels and one group of
+// OCL_U_FRAMES frames (NestaUp.cpp:669-691):
 // sum |u|^2 and sum Re(conj(u) Ux) with Ux = x_f - x_{f-1} (periodic in f) and u = Ux / max(mu, |Ux|).  Each thread
 // walks the group's frames with the previous frame in registers, so x_k is read (1 + 1 / OCL_U_FRAMES) times.
 // Written to partU[2 uid + {0, 1}].
@@ -23,7 +24,7 @@ __device__ __forceinline__ void nesta_U_task(const SliceDesc& d, int uid, double
     const int f0 = fg * OCL_U_FRAMES, f1 = min(d.F, f0 + OCL_U_FRAMES);
     const long p0 = (long)pc * d.u_chunk;
     const long p1 = min(d.N, p0 + d.u_chunk);
-    const float2* xk = st->cur ? d.x1 : d.x0;
+    const float2* xk = d.mcz ? d.mcz : (st->cur ? d.x1 : d.x0);
     const float mu = (float)st->mu;
     double v[2] = {0.0, 0.0};
     for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
@@ -69,6 +70,7 @@ __device__ __forceinline__ UpdCoef upd_coef(const SliceDesc& d) {
 // 32-bit indexed global accesses).
 struct UpdRow {
     const float2 *xf, *xm, *xp, *xref_in;
+    const float2* g;      // motion compensation: MC^T(D^T u) of frame f, replacing the local temporal-TV gradient
     float2 *xref_out, *Wk, *xo;
     float mu, lam, L1, apk, tau;
     int sstart;
@@ -85,23 +87,34 @@ __device__ __forceinline__ UpdRow upd_row(const SliceDesc& d, int f) {
     r.xref_out = d.xref + (long)f * d.N;
     r.Wk = d.Wk + (long)f * d.N;
     r.xo = u.xo + (long)f * d.N;
+    r.g = d.mcg ? d.mcg + (long)f * d.N : nullptr;
     r.mu = u.mu; r.lam = u.lam; r.L1 = u.L1; r.apk = u.apk; r.tau = u.tau;
     r.sstart = u.sstart;
     return r;
 }
 
-// NESTA update of pixel ub + p of the frame resolved in r0 (ub wave-uniform, p per lane) (NestaUp.cpp:696-875): df = lambda (u_f - u_{f+1}) + A^H r (the
-// temporal-TV gradient D^T u with the periodic wrap), y_k = x_k - df / L_mu, W_k += apk df, z_k = x_ref - W_k / L_mu,
+// NESTA update of pixel ub + p of the frame resolved in r0 (ub wave-uniform, p per lane) (NestaUp.cpp:696-875):
+// df = lambda (u_f - u_{f+1}) + A^H r (the temporal-TV gradient D^T u with the periodic wrap; with motion
+// compensation lambda MC^T(D^T u) read from g), y_k = x_k - df / L_mu, W_k += apk df, z_k = x_ref - W_k / L_mu,
 // x_{k+1} = tau z_k + (1 - tau) y_k.  The first iteration of a stage sets x_ref = x_k and W_k = 0.
 // Writes W_k, x_ref (stage start) and x_{k+1}; returns x_{k+1}.
+// MC (compile time): the table uses motion compensation, so the gradient comes from r.g.
+template <bool MC = false>
 __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, unsigned p, float2 ares) {
     UpdRow r = r0;
     r.xf += ub; r.xm += ub; r.xp += ub; r.xref_in += ub; r.xref_out += ub; r.Wk += ub; r.xo += ub;
-    const float2 xf = ldg2(r.xf, p), xm = ldg2(r.xm, p), xp = ldg2(r.xp, p);
-    const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-    const float nf = fmaxf(r.mu, sqrtf(c_abs2(uxf))), np = fmaxf(r.mu, sqrtf(c_abs2(uxp)));
-    const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
-    const float2 df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
+    const float2 xf = ldg2(r.xf, p);
+    float2 df;
+    if constexpr (MC) {
+        const float2 gv = ldg2(r.g + ub, p);
+        df = make_float2(r.lam * gv.x + ares.x, r.lam * gv.y + ares.y);
+    } else {
+        const float2 xm = ldg2(r.xm, p), xp = ldg2(r.xp, p);
+        const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
+        const float nf = fmaxf(r.mu, sqrtf(c_abs2(uxf))), np = fmaxf(r.mu, sqrtf(c_abs2(uxp)));
+        const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
+        df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
+    }
     float2 xr, wo;
     if (r.sstart) {
         xr = xf;
@@ -120,6 +133,7 @@ __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, uns
     return xn;
 }
 
+template <bool MC = false>
 __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r, unsigned p, float2 ares) {
-    return nesta_update_at(r, 0, p, ares);
+    return nesta_update_at<MC>(r, 0, p, ares);
 }

@@ -131,7 +131,8 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
 // current coil is transformed and parked in one of two LDS slabs, and the S values of the next coil's forward pass
 // are issued before the current coil's exchange barrier.  It needs nmax <= 16 FAST_CA_KQ and 2 * 16 * nmax more
 // float2 of LDS.  STAGE = false gathers the sampled rows straight from global memory.
-template <int P, bool STAGE>
+// MC: the table uses motion compensation (update gradient from mcg; mode 2 = setup from a given x_ref)
+template <int P, bool STAGE, bool MC = false>
 __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int mode, float2* lds) {
     constexpr int H = 16 * P;
     const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
@@ -169,8 +170,10 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
         }
     };
     if constexpr (STAGE) {
-        stage_load(0);
-        stage_store(0);
+        if (!MC || mode != 2) {
+            stage_load(0);
+            stage_store(0);
+        }
     }
     __syncthreads();
 
@@ -181,7 +184,8 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     // ---- A^H r: per coil, inverse column DFT (FB on conj(V), input pruned to the sampled rows) ----
     constexpr int NQ = P > 16 ? 2 : 1;
     float2 sv[P];
-    for (int c = 0; c < C; ++c) {
+    const int Cinv = (MC && mode == 2) ? 0 : C;            // no A^H r when x_ref is given
+    for (int c = 0; c < Cinv; ++c) {
         const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
         const float2* ks = kst + (c & 1) * nmax * 16 + s;
         const float2* Sc = d.S + (long)c * N;
@@ -233,12 +237,16 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             xn[n1] = c_scale(acc[n1], sc);
             if (wv) stg2(d.x0 + (long)f * N + (long)16 * n1 * W, lane, xn[n1]);
         }
+    } else if (MC && mode == 2) {                          // x_ref given in x0: forward columns only
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1)
+            xn[n1] = wv ? ldg2(d.x0 + (long)f * N + (long)16 * n1 * W, lane) : make_float2(0.f, 0.f);
     } else {
         const UpdRow u = upd_row(d, f);
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
             xn[n1] = make_float2(0.f, 0.f);
-            if (wv) xn[n1] = nesta_update_at(u, 16 * n1 * W, lane, c_scale(acc[n1], sc));
+            if (wv) xn[n1] = nesta_update_at<MC>(u, 16 * n1 * W, lane, c_scale(acc[n1], sc));
         }
     }
 

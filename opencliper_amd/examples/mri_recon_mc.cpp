@@ -1,0 +1,130 @@
+// Motion-compensated NESTA-TV through the OpenCLIPER-compatible API, chained as the reference's tests/MRIRecon.cpp
+// does it (:232-272): a first NESTA without motion compensation, then NESTA with argsMC starting from that result.
+// The deformation comes from a file instead of GroupwiseRegistration:
+//   mri_recon_mc <in.raw> <motion.raw> <out.raw> F C H W maxIntIter maxIter stoptest miniter
+// in.raw: y [F][C][H][W] complex64, S [C][H][W] complex64, M [F][H][W] uint8 (as mri_recon reads it).
+// motion.raw: xn [2][F][H][W] float32, mesh [2][W][H] int32, bound box 4 x uint32 (1-based).
+// Writes out.raw (the motion-compensated reconstruction), out.raw.nomc (the first reconstruction), and
+// out.raw.mc / out.raw.mcadj (MotionCompensation / AdjointMotionCompensation of the first reconstruction).
+#include <OpenCLIPER/processes/AdjointMotionCompensation.hpp>
+#include <OpenCLIPER/processes/MotionCompensation.hpp>
+#include <OpenCLIPER/processes/nesta/NestaUp.hpp>
+
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+
+using namespace OpenCLIPER;
+
+template <class T> static std::vector<T>* readVec(std::ifstream& in, size_t n) {
+    auto* v = new std::vector<T>(n);
+    in.read((char*)v->data(), (std::streamsize)(sizeof(T) * n));
+    return v;
+}
+
+static void writeX(const std::shared_ptr<XData>& x, const std::string& name) {
+    x->device2Host();
+    std::ofstream out(name, std::ios::binary);
+    for (dimIndexType f = 0; f < x->getNumNDArrays(); ++f)
+        out.write((const char*)x->getHostBuffer(f), (std::streamsize)(sizeof(complexType) * x->getNDArray(f)->size()));
+}
+
+int main(int argc, char* argv[]) {
+    if (argc < 12) {
+        std::cerr << "usage: mri_recon_mc in.raw motion.raw out.raw F C H W maxIntIter maxIter stoptest miniter\n";
+        return 2;
+    }
+    const int F = atoi(argv[4]), C = atoi(argv[5]), H = atoi(argv[6]), W = atoi(argv[7]);
+    const dimIndexType uW = (dimIndexType)W, uH = (dimIndexType)H;
+    try {
+        auto pCLapp = CLapp::create();
+        std::ifstream in(argv[1], std::ios::binary);
+        if (!in) throw std::runtime_error("cannot open input");
+        auto* karr = new std::vector<NDArray*>();
+        for (int i = 0; i < F * C; ++i) {
+            auto* dims = new std::vector<dimIndexType>({uW, uH});
+            auto* data = readVec<complexType>(in, (size_t)H * W);
+            karr->push_back(new ConcreteNDArray<complexType>(dims, data));
+        }
+        auto* sarr = new std::vector<NDArray*>();
+        for (int c = 0; c < C; ++c) {
+            auto* dims = new std::vector<dimIndexType>({uW, uH});
+            auto* data = readVec<complexType>(in, (size_t)H * W);
+            sarr->push_back(new ConcreteNDArray<complexType>(dims, data));
+        }
+        auto* marr = new std::vector<NDArray*>();
+        for (int f = 0; f < F; ++f) {
+            auto* dims = new std::vector<dimIndexType>({uW, uH});
+            auto* data = readVec<uint8_t>(in, (size_t)H * W);
+            marr->push_back(new ConcreteNDArray<uint8_t>(dims, data));
+        }
+        if (!in) throw std::runtime_error("input file too short");
+        auto* dyn = new std::vector<dimIndexType>({(dimIndexType)F});
+        auto pKData = std::make_shared<KData>(pCLapp, karr, (numCoilsType)C, dyn);
+        SensitivityMapsData* pS = new SensitivityMapsData(pCLapp, sarr, (numCoilsType)C);
+        auto* dynm = new std::vector<dimIndexType>({(dimIndexType)F});
+        SamplingMasksData* pM = new SamplingMasksData(pCLapp, marr, dynm, uW);
+        pKData->setSensitivityMapsData(pS);
+        pKData->setSamplingMasksData(pM);
+        auto pOutputXData = std::make_shared<XData>(pCLapp, pKData);
+
+        // the deformation, in the Data forms GroupwiseRegistration.cpp:150-164,307-314 uses
+        std::ifstream mf(argv[2], std::ios::binary);
+        if (!mf) throw std::runtime_error("cannot open motion file");
+        auto* xnDims = new std::vector<dimIndexType>({uW, uH, (dimIndexType)(2 * F)});
+        auto* xnVals = readVec<float>(mf, (size_t)2 * F * H * W);
+        NDArray* xnArr = new ConcreteNDArray<float>(xnDims, xnVals);
+        auto* meshDims = new std::vector<dimIndexType>({uH, uW, 2});
+        auto* meshVals = readVec<int32_t>(mf, (size_t)2 * H * W);
+        NDArray* meshArr = new ConcreteNDArray<int32_t>(meshDims, meshVals);
+        std::vector<dimIndexType> bb(4);
+        mf.read((char*)bb.data(), sizeof(dimIndexType) * 4);
+        if (!mf) throw std::runtime_error("motion file too short");
+
+        ArgumentsMotionCompensation* argsMC = new ArgumentsMotionCompensation();
+        argsMC->xnData = std::make_shared<XData>(pCLapp, xnArr, TYPEID_REAL);
+        argsMC->xData = std::make_shared<XData>(pCLapp, meshArr, TYPEID_INT32);
+        argsMC->bound_box = new std::vector<dimIndexType>(bb);
+
+        const float lambda_i = 1e-2f, mu_f = 3e-4f, La = 1.f, tolVar = 3e-4f;
+        const uint32_t maxIntIter = (uint32_t)atoi(argv[8]), stoptest = (uint32_t)atoi(argv[10]),
+                       miniter = (uint32_t)atoi(argv[11]);
+        const int maxIter = atoi(argv[9]);
+        auto nesta = Process::create<NestaUp>(pCLapp);
+        nesta->setForm(NestaUp::FUSED);
+        nesta->setInput(pKData);
+        nesta->setOutput(pOutputXData);
+        nesta->init();
+        // first reconstruction (tests/MRIRecon.cpp:232-235)
+        nesta->setLaunchParameters(std::make_shared<NestaUp::LaunchParameters>(lambda_i, mu_f, La, maxIntIter, tolVar, 1,
+                                                                                maxIter, stoptest, miniter, nullptr));
+        nesta->launch();
+        writeX(pOutputXData, std::string(argv[3]) + ".nomc");
+
+        auto mcOut = std::make_shared<XData>(pCLapp, pOutputXData, false);
+        auto mc = Process::create<MotionCompensation>(pCLapp, pOutputXData, mcOut);
+        mc->setLaunchParameters(std::make_shared<MotionCompensation::LaunchParameters>(argsMC));
+        mc->launch();
+        writeX(mcOut, std::string(argv[3]) + ".mc");
+        auto amc = Process::create<AdjointMotionCompensation>(pCLapp, pOutputXData, mcOut);
+        amc->setLaunchParameters(std::make_shared<AdjointMotionCompensation::LaunchParameters>(argsMC));
+        amc->launch();
+        writeX(mcOut, std::string(argv[3]) + ".mcadj");
+
+        // reconstruction with motion compensation (tests/MRIRecon.cpp:265-270)
+        nesta->setLaunchParameters(std::make_shared<NestaUp::LaunchParameters>(lambda_i, mu_f, La, maxIntIter, tolVar, 1,
+                                                                                maxIter, stoptest, miniter, argsMC));
+        nesta->launch();
+        writeX(pOutputXData, argv[3]);
+        const auto& r = nesta->getReport();
+        std::cout << "normU " << r.normU << " iters";
+        for (int n : r.itersPerStage) std::cout << " " << n;
+        std::cout << "\n";
+        delete argsMC->bound_box;
+        delete argsMC;
+    } catch (std::exception& e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}

@@ -18,7 +18,10 @@ def test_api_library_exports_reference_classes():
                 "OpenCLIPER::FFT::launch", "OpenCLIPER::KData::KData", "OpenCLIPER::XData::XData",
                 "OpenCLIPER::SimpleMRIRecon::launch", "OpenCLIPER::VectorNormalization::launch",
                 "OpenCLIPER::ComplexElementProd::launch", "OpenCLIPER::ApplyMask::launch",
-                "OpenCLIPER::XImageSum::launch", "OpenCLIPER::CopyDataGPU::launch"):
+                "OpenCLIPER::XImageSum::launch", "OpenCLIPER::CopyDataGPU::launch",
+                "OpenCLIPER::MotionCompensation::launch", "OpenCLIPER::AdjointMotionCompensation::launch",
+                "OpenCLIPER::KData::matlabSave", "OpenCLIPER::XData::matlabSave", "OpenCLIPER::KData::saveCFLData",
+                "OpenCLIPER::RSoS::launch", "OpenCLIPER::SimpleMRIReconSOS::launch"):
         assert sym in out, sym
 
 
@@ -54,3 +57,40 @@ def test_driver_nesta_matches_oracle(oracle, tmp_path, form):
     v = np.fromfile(outp + ".ttv", dtype=np.complex64).reshape(sl.F, sl.H, sl.W)
     d = x - np.roll(x, 1, axis=0)
     assert np.array_equal(v, np.roll(d, -1, axis=0) - d)
+
+
+DRIVER_MC = os.path.join(PKG, "examples", "mri_recon_mc")
+
+
+def test_mc_driver_usage_error():
+    r = subprocess.run([DRIVER_MC], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_motion_compensated_nesta_matches_oracle(oracle, tmp_path):
+    """NESTA, then NESTA with argsMC from its result, through NestaUp / MotionCompensation /
+    AdjointMotionCompensation of the C++ API (tests/MRIRecon.cpp:232-272 of the reference)."""
+    sl = synth.make_cine(H=64, W=64, C=3, F=6, accel=3.0, n_centre=4, slice_id=45)
+    xn, mesh, bb = synth.motion_fields(sl.H, sl.W, sl.F, amp=1.2, slice_id=45)
+    inp, mot, outp = str(tmp_path / "in.raw"), str(tmp_path / "motion.raw"), str(tmp_path / "out.raw")
+    _write_input(inp, sl)
+    with open(mot, "wb") as fh:
+        fh.write(np.ascontiguousarray(xn, np.float32).tobytes())
+        fh.write(np.ascontiguousarray(mesh, np.int32).tobytes())
+        fh.write(np.ascontiguousarray(bb, np.uint32).tobytes())
+    args = [DRIVER_MC, inp, mot, outp, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "3", "12", "1", "7"]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    shape = (sl.F, sl.H, sl.W)
+    x0 = np.fromfile(outp + ".nomc", dtype=np.complex64).reshape(shape)
+    prm = oracle.params(maxIntIter=3, maxIter=12, stoptest=1, miniter=7)
+    assert oracle.rel_l2(oracle.mc(x0, xn, mesh, bb), np.fromfile(outp + ".mc", np.complex64).reshape(shape)) < 1e-6
+    assert oracle.rel_l2(oracle.mc(x0, xn, mesh, bb, adjoint=True),
+                         np.fromfile(outp + ".mcadj", np.complex64).reshape(shape)) < 1e-6
+    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True)
+    x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
+    assert oracle.rel_l2(x, x64) < 1e-3
+    assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout

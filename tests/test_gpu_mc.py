@@ -80,11 +80,13 @@ def test_mc_errors(ctx):
         ctx.motion_compensation(dx, out, nul)
 
 
-@pytest.mark.parametrize("H,W,C,F", [(48, 48, 4, 6), (64, 80, 3, 8)])
-def test_nesta_mc_matches_oracle(ctx, oracle, H, W, C, F):
+@pytest.mark.parametrize("form", [capi.OCL_FORM_PLAIN, capi.OCL_FORM_FUSED])
+@pytest.mark.parametrize("H,W,C,F", [(48, 48, 4, 6), (64, 80, 3, 8), (128, 128, 4, 5)])
+def test_nesta_mc_matches_oracle(ctx, oracle, H, W, C, F, form):
     """NESTA with motion compensation from the motion-free reconstruction, as tests/MRIRecon.cpp:240-272 chains
-    them: within 1e-3 of the FP64 oracle with the same stage lengths."""
-    print("nesta_mc", H, W, C, F, flush=True)
+    them: within 1e-3 of the FP64 oracle with the same stage lengths, in the plain and the fused form (128 x 128
+    takes the four-step column and row passes)."""
+    print("nesta_mc", H, W, C, F, "form", form, flush=True)
     sl = synth.make_cine(H=H, W=W, C=C, F=F, accel=3.0, n_centre=4, slice_id=50 + F)
     prm = dict(maxIntIter=3, maxIter=15, stoptest=1, miniter=7)
     y, S, M = ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M.astype(np.uint8))
@@ -92,7 +94,7 @@ def test_nesta_mc_matches_oracle(ctx, oracle, H, W, C, F):
     ctx.nesta(y, S, M, x, capi.nesta_params(**prm), form=capi.OCL_FORM_PLAIN)
     x_ref = x.get()
     (xn, mesh, bb), (dxn, dmesh, mc) = _fields(ctx, H, W, F, amp=1.2, slice_id=11 + F)
-    rep = ctx.nesta_mc(y, S, M, x, mc, capi.nesta_params(**prm))
+    rep = ctx.nesta_mc(y, S, M, x, mc, capi.nesta_params(**prm), form=form)
     x64, r64 = oracle.nesta_mc(sl.y, sl.S, sl.M, x_ref, xn, mesh, bb, oracle.params(**prm), double=True)
     err = oracle.rel_l2(x.get(), x64)
     print("iters", rep["iters"], r64["iters"], "normU", rep["normU"], r64["normU"], "rel", err, flush=True)
@@ -102,11 +104,45 @@ def test_nesta_mc_matches_oracle(ctx, oracle, H, W, C, F):
     assert oracle.rel_l2(x.get(), x_ref) > 1e-4      # the deformation changed the solution
 
 
-def test_nesta_mc_rejects_fused_form(ctx):
+def test_nesta_mc_form_errors(ctx):
+    """The split form has no motion compensation; the fused one needs the original (identity) mesh."""
     H, W, C, F = 32, 32, 2, 4
     sl = synth.make_cine(H=H, W=W, C=C, F=F, accel=3.0, n_centre=4, slice_id=60)
     y, S, M = ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M.astype(np.uint8))
-    x = ctx.zeros((F, H, W))
-    (_, _, _), (dxn, dmesh, mc) = _fields(ctx, H, W, F, amp=1.0, slice_id=12)
-    with pytest.raises(capi.OclError, match="plain form"):
-        ctx.nesta_mc(y, S, M, x, mc, capi.nesta_params(maxIntIter=2, maxIter=4), form=capi.OCL_FORM_FUSED)
+    x = ctx.to_device(sl.x_true.astype(np.complex64))
+    (xn, mesh, bb), (dxn, dmesh, mc) = _fields(ctx, H, W, F, amp=1.0, slice_id=12)
+    prm = capi.nesta_params(maxIntIter=2, maxIter=4)
+    with pytest.raises(capi.OclError, match="plain or the fused form"):
+        ctx.nesta_mc(y, S, M, x, mc, prm, form=capi.OCL_FORM_SPLIT)
+    swapped = ctx.to_device(np.ascontiguousarray(mesh[::-1]))
+    with pytest.raises(capi.OclError, match="identity"):
+        ctx.nesta_mc(y, S, M, x, ctx.mc_args(dxn, swapped, bb), prm, form=capi.OCL_FORM_FUSED)
+    ctx.nesta_mc(y, S, M, x, ctx.mc_args(dxn, swapped, bb), prm, form=capi.OCL_FORM_PLAIN)   # any mesh in plain
+
+
+def test_fused_mc_batch_uneven_equals_solo(ctx, oracle):
+    """Two slices of different shapes with their own deformations in one fused launch table (graph replay of the
+    whole schedule) give each slice's solo fused result.  The adjoint's scatter adds with float atomics, whose order
+    varies, so the comparison allows rounding-level differences."""
+    prm = dict(maxIntIter=2, maxIter=10, stoptest=0, miniter=7)
+    shapes = [(64, 64, 3, 6, 70), (96, 80, 2, 5, 71)]
+    solo, batch_in, mcs = [], [], []
+    for H, W, C, F, sid in shapes:
+        sl = synth.make_cine(H=H, W=W, C=C, F=F, accel=3.0, n_centre=4, slice_id=sid)
+        y, S, M = ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M.astype(np.uint8))
+        x0 = synth.make_image_parts(H, W, C, F, slice_id=sid)[0]
+        (_, _, _), (dxn, dmesh, mc) = _fields(ctx, H, W, F, amp=1.0, slice_id=sid)
+        xs = ctx.to_device(x0)
+        ctx.nesta_mc(y, S, M, xs, mc, capi.nesta_params(**prm), form=capi.OCL_FORM_FUSED)
+        solo.append(xs.get())
+        xb = ctx.to_device(x0)
+        batch_in.append((y, S, M, xb, dxn, dmesh))
+        mcs.append(mc)
+    bt = ctx.batch([b[:4] for b in batch_in], capi.nesta_params(**prm), mcs=mcs)
+    bt.setup()
+    bt.capture(prm["maxIntIter"] * prm["maxIter"])
+    bt.replay()
+    bt.finalize()
+    ctx.sync()
+    for i in range(len(shapes)):
+        assert oracle.rel_l2(batch_in[i][3].get(), solo[i]) < 1e-5, i
