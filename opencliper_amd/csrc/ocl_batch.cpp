@@ -181,13 +181,14 @@ int slice_plan_mc(SliceRes* r) {
     SliceDesc& d = r->d;
     if (!r->has_mc) {
         d.mc_xn = nullptr;
-        d.wg_P = d.wg_X = 0;
+        d.wg_P = d.wg_X = d.wg_G = 0;
         return OCL_OK;
     }
     CHK(mc_box_geometry(d.H, d.W, &r->mc, &d.mc_r0, &d.mc_nr, &d.mc_c0, &d.mc_nc));
     d.mc_xn = r->mc.xn;
     d.wg_P = (int)(((long)d.F * d.N + OCL_MC_CHUNK - 1) / OCL_MC_CHUNK);
     d.wg_X = (int)(((long)d.F * d.mc_nr * d.mc_nc + OCL_MC_CHUNK - 1) / OCL_MC_CHUNK);
+    d.wg_G = d.F * ((d.H + OCL_MC_TR - 1) / OCL_MC_TR) * ((d.W + OCL_MC_TC - 1) / OCL_MC_TC);
     return OCL_OK;
 }
 
@@ -215,9 +216,11 @@ int slice_alloc(SliceRes* r) {
     CHK(r->alloc(sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
     CHK(r->alloc(sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
     d.mcz = d.mcg = nullptr;
+    d.mc_dmax = nullptr;
     if (d.mc_xn) {
         CHK(r->alloc(xb, &p)); d.mcz = (float2*)p;
         CHK(r->alloc(xb, &p)); d.mcg = (float2*)p;
+        CHK(r->alloc(sizeof(unsigned), &p)); d.mc_dmax = (unsigned*)p;
     }
     if (d.prec) {
         // DFT matrices of the split-precision passes (row: forward, inverse; column: inverse, forward per frame)
@@ -251,6 +254,7 @@ struct Table {
     int* offCA = nullptr;
     int* offP = nullptr;
     int* offX = nullptr;
+    int* offG = nullptr;
     int cap = 0;
     NestaLaunch L{};
     ~Table() {
@@ -259,6 +263,7 @@ struct Table {
         if (offCA) hipFree(offCA);
         if (offP) hipFree(offP);
         if (offX) hipFree(offX);
+        if (offG) hipFree(offG);
     }
 };
 
@@ -268,19 +273,20 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     if (ns > t->cap) {
         if (t->dd) {
             hipStreamSynchronize(c->stream);
-            hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); hipFree(t->offP); hipFree(t->offX);
+            hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); hipFree(t->offP); hipFree(t->offX); hipFree(t->offG);
         }
-        t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = nullptr;
+        t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = t->offG = nullptr;
         const int cap = std::max(ns, 2 * t->cap);
         HIPCHK(hipMalloc(&t->dd, sizeof(SliceDesc) * cap));
         HIPCHK(hipMalloc(&t->offB, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offCA, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offP, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offX, sizeof(int) * (cap + 1)));
+        HIPCHK(hipMalloc(&t->offG, sizeof(int) * (cap + 1)));
         t->cap = cap;
     }
     std::vector<SliceDesc> hd(ns);
-    std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0), oP(ns + 1, 0), oX(ns + 1, 0);
+    std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0), oP(ns + 1, 0), oX(ns + 1, 0), oG(ns + 1, 0);
     NestaLaunch& L = t->L;
     L = NestaLaunch{};
     L.maxe = 4;
@@ -296,6 +302,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             return fail(OCL_E_INVALID, "batch: motion compensation must be on for every slice of a table or none");
         oP[i + 1] = oP[i] + d.wg_P;
         oX[i + 1] = oX[i] + d.wg_X;
+        oG[i + 1] = oG[i] + d.wg_G;
         if (L.prec && (d.H != v[0]->d.H || d.W != v[0]->d.W))
             return fail(OCL_E_UNSUPPORTED, "batch: split-precision slices of one table must share H and W");
         L.kj = std::max(L.kj, d.kp / 32);
@@ -317,11 +324,14 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.offCA = t->offCA;
     L.offP = t->offP;
     L.offX = t->offX;
+    L.offG = t->offG;
     L.ns = ns;
     L.totB = oB[ns];
     L.totCA = oC[ns];
     L.totP = oP[ns];
     L.totX = oX[ns];
+    L.totG = oG[ns];
+    L.mcR = 0;
     L.PH = PH;
     L.PW = PW;
     if (L.prec && ns) {
@@ -339,7 +349,8 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         HIPCHK(hipMemcpyAsync(t->offCA, oC.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offP, oP.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offX, oX.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));   // hd / oB / oC / oP / oX are host temporaries
+        HIPCHK(hipMemcpyAsync(t->offG, oG.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // hd and the offset vectors are host temporaries
     }
     return OCL_OK;
 }
@@ -353,8 +364,11 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
     if (v.empty()) return OCL_OK;
     for (SliceRes* r : v) CHK(ocl_memset_zero(c, r->d.counter, sizeof(unsigned) * 4));
     if (t.L.mc) {
-        for (SliceRes* r : v)
+        for (SliceRes* r : v) {
             CHK(ocl_memcpy_d2d(c, r->d.x0, r->in.x_out, sizeof(float2) * (size_t)r->d.F * r->d.N));
+            CHK(ocl_memset_zero(c, r->d.mc_dmax, sizeof(unsigned)));
+        }
+        HIPCHK(launch_nesta_mc_dmax(t.L, c->stream));
         HIPCHK(launch_nesta_mc_z(t.L, c->stream, 1));
         HIPCHK(launch_nesta_CA(t.L, c->stream, 2));
     } else {
@@ -366,6 +380,21 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
     HIPCHK(hipGetLastError());
     std::vector<double> mp;
     std::vector<NState> st(v.size());
+    if (t.L.mc) {
+        // gather radius of pass G: floor(largest displacement) + 1 over the table, or the scatter form beyond
+        // OCL_MC_GATHER_RMAX (the margin covers rounding of |nhor - c| near integers)
+        int R = 1;
+        for (SliceRes* r : v) {
+            unsigned bits = 0;
+            HIPCHK(hipMemcpyAsync(&bits, r->d.mc_dmax, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            float D;
+            std::memcpy(&D, &bits, sizeof(D));
+            const int Ri = (D >= 0.f && D < 1e6f) ? (int)std::floor(D + 1e-3f) + 1 : OCL_MC_GATHER_RMAX + 1;
+            R = std::max(R, Ri);
+        }
+        t.L.mcR = R <= OCL_MC_GATHER_RMAX ? R : 0;
+    }
     for (size_t i = 0; i < v.size(); ++i) {
         SliceRes* r = v[i];
         const SliceDesc& d = r->d;

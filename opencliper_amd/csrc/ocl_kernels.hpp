@@ -81,8 +81,12 @@ struct SliceDesc {
     int mc_r0, mc_nr, mc_c0, mc_nc;   // box rows / columns (0-based)
     float2* mcz;            // [F][H][W]
     float2* mcg;            // [F][H][W]
-    int wg_P, wg_X;         // workgroups of the pixel passes (Z, G clear) and of the box scatter
+    int wg_P, wg_X, wg_G;   // workgroups of the pixel passes, of the box passes and of the tiled gather
+    unsigned* mc_dmax;      // largest displacement |xn - position| over the box (float bits), set at setup
 };
+#define OCL_MC_GATHER_RMAX 4 // pass G gathers (deterministically) when floor(max displacement) + 1 <= this
+#define OCL_MC_TR 16         // tile of the gather form of pass G: rows x columns of one frame
+#define OCL_MC_TC 64
 #define OCL_MC_CHUNK 1024    // pixel-frames per workgroup of the motion-compensation passes
 
 // kernels (ocl_ops.hip)
@@ -139,7 +143,9 @@ struct NestaLaunch {
     int mc;             // 1: every slice uses motion compensation (passes Z and G run)
     const int* offP;    // workgroup offsets of the motion-compensation pixel passes
     const int* offX;    // workgroup offsets of the motion-compensation box scatter
-    int totP, totX;
+    const int* offG;    // workgroup offsets of the tiled gather form of pass G
+    int totP, totX, totG;
+    int mcR;            // gather radius of pass G (largest over the table), 0: scatter form; set at setup
 };
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);
@@ -148,6 +154,7 @@ hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
 // motion-compensation passes of the fused iteration (ocl_mc.hip): Z (mode 0: x_k, mode 1: setup from x0), G
 hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream);
+hipError_t launch_nesta_mc_dmax(const NestaLaunch& L, hipStream_t stream);
 void nesta_kernel_attributes(int lds_bytes);
 // split-precision passes (ocl_nesta_mfma.hip)
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);

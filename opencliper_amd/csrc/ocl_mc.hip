@@ -170,7 +170,107 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_z(const SliceDesc* __restri
     }
 }
 
-// G, first half: mcg = D^T u(mcz) outside the box, 0 inside (copy + initZeroRect of the adjoint)
+// Setup: mc_dmax = max over the box of max(|nhor - c|, |nver - r|) for positions with neighbours in the frame.  The
+// host turns it into the gather radius of pass G: box pixel p reaches target q only if |q - p| <= floor(D) + 1 per
+// axis (the bilinear footprint of a position within D of p spans p - ceil(D) .. p + floor(D) + 1).
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_dmax(const SliceDesc* __restrict__ descs,
+                                                          const int* __restrict__ off, int ns) {
+    __shared__ double red[32];
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    const long N = d.N, per = (long)d.mc_nr * d.mc_nc, total = per * d.F;
+    const long base = (long)(blockIdx.x - off[si]) * OCL_MC_CHUNK;
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
+        const long i = base + q * OCL_NT + threadIdx.x;
+        if (i >= total) break;
+        const int f = (int)(i / per);
+        const long b = i - (long)f * per;
+        const int ii = (int)(b / d.mc_nc), jj = (int)(b - (long)ii * d.mc_nc);
+        const int r = d.mc_r0 + ii, c = d.mc_c0 + jj;
+        float nhor, nver;
+        int ph, pv;
+        if (mc_position(d.mc_xn, d.F, d.H, d.W, N, f, (long)r * d.W + c, &nhor, &nver, &ph, &pv))
+            m = fmax(m, (double)fmaxf(fabsf(nhor - (float)c), fabsf(nver - (float)r)));
+    }
+    m = block_max(m, red);
+    if (threadIdx.x == 0) atomicMax(d.mc_dmax, __float_as_uint((float)m));   // non-negative floats order as uints
+}
+
+// G, gather form (deterministic): workgroup = one OCL_MC_TR x OCL_MC_TC tile of one frame.  The box pixels of the
+// tile and its R-halo are staged in LDS with their sample positions and D^T u(mcz); each target pixel then starts
+// from D^T u outside the box (0 inside: copy + initZeroRect of the adjoint) and adds, in row-major order of the
+// sources (the order the adjoint's scatter visits them), the bilinear weight of each source's footprint on it times
+// the source's D^T u.
+template <int R>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __restrict__ descs,
+                                                             const int* __restrict__ off, int ns) {
+    constexpr int SR = OCL_MC_TR + 2 * R, SC = OCL_MC_TC + 2 * R;
+    __shared__ float2 spos[SR * SC];   // (nhor, nver) of a source, nhor = NaN if not a source
+    __shared__ float2 sval[SR * SC];   // D^T u of a source
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (!d.st->it_pending) return;
+    const float mu = (float)d.st->it_mu;
+    const long N = d.N;
+    const int ntc = (d.W + OCL_MC_TC - 1) / OCL_MC_TC, ntr = (d.H + OCL_MC_TR - 1) / OCL_MC_TR;
+    const int local = blockIdx.x - off[si];
+    const int f = local / (ntr * ntc), tt = local - f * (ntr * ntc);
+    const int r0 = (tt / ntc) * OCL_MC_TR, c0 = (tt % ntc) * OCL_MC_TC;
+    const int br0 = d.mc_r0, br1 = d.mc_r0 + d.mc_nr, bc0 = d.mc_c0, bc1 = d.mc_c0 + d.mc_nc;
+    const bool near = r0 - R < br1 && r0 + OCL_MC_TR + R > br0 && c0 - R < bc1 && c0 + OCL_MC_TC + R > bc0;
+    if (near) {
+        for (int e = threadIdx.x; e < SR * SC; e += OCL_NT) {
+            const int pr = r0 - R + e / SC, pc = c0 - R + e % SC;
+            float2 pos = make_float2(__int_as_float(0x7fc00000), 0.f), val = make_float2(0.f, 0.f);
+            if (pr >= br0 && pr < br1 && pc >= bc0 && pc < bc1) {
+                const long pp = (long)pr * d.W + pc;
+                float nhor, nver;
+                int ph, pv;
+                if (mc_position(d.mc_xn, d.F, d.H, d.W, N, f, pp, &nhor, &nver, &ph, &pv)) {
+                    pos = make_float2(nhor, nver);
+                    val = mc_tv_grad(d.mcz, d.F, N, f, pp, mu);
+                }
+            }
+            spos[e] = pos;
+            sval[e] = val;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < OCL_MC_TR * OCL_MC_TC / OCL_NT; ++k) {
+        const int e = threadIdx.x + k * OCL_NT;
+        const int lr = e / OCL_MC_TC, lc = e % OCL_MC_TC;
+        const int r = r0 + lr, c = c0 + lc;
+        if (r >= d.H || c >= d.W) continue;
+        const long p = (long)r * d.W + c;
+        const bool inbox = r >= br0 && r < br1 && c >= bc0 && c < bc1;
+        float2 g = inbox ? make_float2(0.f, 0.f) : mc_tv_grad(d.mcz, d.F, N, f, p, mu);
+        if (near) {
+#pragma unroll
+            for (int dr = -R; dr <= R; ++dr)
+#pragma unroll
+                for (int dc = -R; dc <= R; ++dc) {
+                    const int cell = (lr + R + dr) * SC + (lc + R + dc);
+                    const float2 pos = spos[cell];
+                    if (pos.x != pos.x) continue;                 // not a source
+                    const float fh = floorf(pos.x), fv = floorf(pos.y);
+                    const float a = (float)c - fh, b = (float)r - fv;
+                    if (a != 0.f && a != 1.f) continue;
+                    if (b != 0.f && b != 1.f) continue;
+                    const float Whh = pos.x - fh, Wvv = pos.y - fv;
+                    const float hw = a != 0.f ? Whh : 1.0f - Whh, vw = b != 0.f ? Wvv : 1.0f - Wvv;
+                    const float2 t = sval[cell];
+                    g.x += t.x * hw * vw;
+                    g.y += t.y * hw * vw;
+                }
+        }
+        d.mcg[(long)f * N + p] = g;
+    }
+}
+
+// G, scatter form (displacements beyond the gather radius), first half: mcg = D^T u(mcz) outside the box, 0 inside
 __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gclear(const SliceDesc* __restrict__ descs,
                                                             const int* __restrict__ off, int ns) {
     const int si = find_slice(off, ns, blockIdx.x);
@@ -190,7 +290,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gclear(const SliceDesc* __r
     }
 }
 
-// G, second half: the box pixels' D^T u(mcz) scattered into mcg with the bilinear weights
+// G, scatter form, second half: the box pixels' D^T u(mcz) added into mcg with float atomics (order varies)
 __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gscatter(const SliceDesc* __restrict__ descs,
                                                               const int* __restrict__ off, int ns) {
     const int si = find_slice(off, ns, blockIdx.x);
@@ -214,6 +314,12 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gscatter(const SliceDesc* _
     }
 }
 
+hipError_t launch_nesta_mc_dmax(const NestaLaunch& L, hipStream_t stream) {
+    if (!L.mc || !L.totX) return hipSuccess;
+    hipLaunchKernelGGL(k_nesta_mc_dmax, dim3(L.totX), dim3(OCL_NT), 0, stream, L.dd, L.offX, L.ns);
+    return hipGetLastError();
+}
+
 hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode) {
     if (!L.mc || !L.totP) return hipSuccess;
     hipLaunchKernelGGL(k_nesta_mc_z, dim3(L.totP), dim3(OCL_NT), 0, stream, L.dd, L.offP, L.ns, mode);
@@ -222,6 +328,13 @@ hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode)
 
 hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream) {
     if (!L.mc || !L.totP) return hipSuccess;
+    if (L.mcR > 0) {
+        void (*k)(const SliceDesc*, const int*, int) =
+            L.mcR == 1 ? k_nesta_mc_ggather<1> : L.mcR == 2 ? k_nesta_mc_ggather<2> : L.mcR == 3
+                ? k_nesta_mc_ggather<3> : k_nesta_mc_ggather<4>;
+        hipLaunchKernelGGL(k, dim3(L.totG), dim3(OCL_NT), 0, stream, L.dd, L.offG, L.ns);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_nesta_mc_gclear, dim3(L.totP), dim3(OCL_NT), 0, stream, L.dd, L.offP, L.ns);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !L.totX) return e;
