@@ -154,7 +154,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_z(const SliceDesc* __restri
 #pragma unroll
     for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
         const long i = base + q * OCL_NT + threadIdx.x;
-        if (i >= total) break;
+        if (i >= total) continue;
         const int f = (int)(i / N);
         const long p = i - (long)f * N;
         const int r = (int)(p / d.W), c = (int)(p - (long)r * d.W);
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_dmax(const SliceDesc* __res
 #pragma unroll
     for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
         const long i = base + q * OCL_NT + threadIdx.x;
-        if (i >= total) break;
+        if (i >= total) continue;
         const int f = (int)(i / per);
         const long b = i - (long)f * per;
         const int ii = (int)(b / d.mc_nc), jj = (int)(b - (long)ii * d.mc_nc);
@@ -221,10 +221,13 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
     const int br0 = d.mc_r0, br1 = d.mc_r0 + d.mc_nr, bc0 = d.mc_c0, bc1 = d.mc_c0 + d.mc_nc;
     const bool near = r0 - R < br1 && r0 + OCL_MC_TR + R > br0 && c0 - R < bc1 && c0 + OCL_MC_TC + R > bc0;
     if (near) {
-        for (int e = threadIdx.x; e < SR * SC; e += OCL_NT) {
+        constexpr int NS = (SR * SC + OCL_NT - 1) / OCL_NT;   // staged cells per thread
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = threadIdx.x + k * OCL_NT;
             const int pr = r0 - R + e / SC, pc = c0 - R + e % SC;
             float2 pos = make_float2(__int_as_float(0x7fc00000), 0.f), val = make_float2(0.f, 0.f);
-            if (pr >= br0 && pr < br1 && pc >= bc0 && pc < bc1) {
+            if (e < SR * SC && pr >= br0 && pr < br1 && pc >= bc0 && pc < bc1) {
                 const long pp = (long)pr * d.W + pc;
                 float nhor, nver;
                 int ph, pv;
@@ -233,8 +236,10 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
                     val = mc_tv_grad(d.mcz, d.F, N, f, pp, mu);
                 }
             }
-            spos[e] = pos;
-            sval[e] = val;
+            if (e < SR * SC) {
+                spos[e] = pos;
+                sval[e] = val;
+            }
         }
         __syncthreads();
     }
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gclear(const SliceDesc* __r
 #pragma unroll
     for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
         const long i = base + q * OCL_NT + threadIdx.x;
-        if (i >= total) break;
+        if (i >= total) continue;
         const int f = (int)(i / N);
         const long p = i - (long)f * N;
         const int r = (int)(p / d.W), c = (int)(p - (long)r * d.W);
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_gscatter(const SliceDesc* _
 #pragma unroll
     for (int q = 0; q < OCL_MC_CHUNK / OCL_NT; ++q) {
         const long i = base + q * OCL_NT + threadIdx.x;
-        if (i >= total) break;
+        if (i >= total) continue;
         const int f = (int)(i / per);
         const long b = i - (long)f * per;
         const int ii = (int)(b / d.mc_nc), jj = (int)(b - (long)ii * d.mc_nc);
