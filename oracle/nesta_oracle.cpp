@@ -19,6 +19,7 @@
 //   op_A / op_At ......... src/processes/nesta/NestaUp.cpp:162-190 / :117-151
 //   normest .............. src/processes/nesta/NestaUp.cpp:291-439 (power method, drand48 restart)
 //   nesta ................ src/processes/nesta/NestaUp.cpp:442-910 (continuation NESTA, stop test 1)
+//   StopRule ............. NestaUp.cpp:645,649,799-824 (ring of objective values, armed / ending hits)
 //   simple_recon ......... src/processes/examples/SimpleMRIRecon.cpp:59-101
 //   rsos ................. src/kernels/rss.cl (rss_kernel): sqrt(sum_c |in_c|^2) in the real part, float sum in coil order
 //   simple_recon_sos ..... src/processes/examples/SimpleMRIReconSOS.cpp (in-place inverse FFT, then RSoS)
@@ -34,7 +35,8 @@
 // the closed-form sigma_max of the periodic difference; vecnorm vs the Huber identity; op_A/op_At adjointness and
 // E^H E = I when fully sampled; nesta vs an independent numpy FP64 implementation (tests/golden/make_golden.py);
 // mc_fwd vs scipy.ndimage.map_coordinates (order 1), identity and integer-shift meshes, and <MC x, y> = <x, MC^T y>;
-// normest with MC vs the largest singular value of the explicit matrix of U; nesta with an identity MC vs nesta.
+// normest with MC vs the largest singular value of the explicit matrix of U; nesta with an identity MC vs nesta;
+// the stop rule on scripted objective sequences whose outcomes follow by hand from NestaUp.cpp:799-824.
 
 #include <cmath>
 #include <complex>
@@ -546,6 +548,38 @@ double orc_drand48_nth(long k) {  // value of the k-th (1-based) drand48() call 
 // ------------------------------------------------------------------------------------------------------------------
 namespace {
 
+// Stop test 1 (NestaUp.cpp:799-824): a ring of the last miniter objective values, its slot miniter-1 reset to
+// LONG_MAX at each stage start while the other slots keep the previous stage's values (:645); from k = miniter on,
+// qp = (mean - f) / mean; the first qp <= tolVar of a stage arms the rule and skips that iteration's ring write, the
+// next one (armed) ends the stage after the current update.
+template <class T> struct StopRule {
+    std::vector<T> fmean;
+    int miniter;
+    bool qpDone = false;
+    explicit StopRule(int mi) : fmean(std::max(mi, 1), (T)0), miniter(mi) {}
+    void start_stage() {                                                                   // (:645,649)
+        fmean[miniter - 1] = (T)LONG_MAX;
+        qpDone = false;
+    }
+    // iteration k with objective fx; true when the stage ends after this iteration
+    bool step(int k, T fx, T tolVar) {
+        bool end = false, skip_write = false;
+        if (k >= miniter) {
+            T mean = 0;
+            for (int ne = 0; ne < miniter; ++ne) mean += fmean[miniter - ne - 1];
+            mean = mean / (T)miniter;
+            const T qp = (mean - fx) / mean;
+            if (qp <= tolVar) {
+                if (qpDone) end = true;
+                qpDone = true;
+                skip_write = true;
+            }
+        }
+        if (!skip_write) fmean[(miniter - 1) - ((k + 1) % miniter)] = fx;
+        return end;
+    }
+};
+
 // mc == nullptr: the reference's argsMC == nullptr path (x_ref = A^H y_s).  With mc, x_ref is the caller's xref
 // (the previous reconstruction) and U, U^T, normest include the motion compensation.
 template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& prm, orc_report* rep,
@@ -583,8 +617,7 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
         rep->normU = normU; rep->maxXref = maxXref; rep->maxUXref = maxUXref; rep->lambda = lambda;
         rep->muf = muf; rep->mu0 = mu0; rep->gamma = gamma; rep->gammat = gammat; rep->normest_iters = ne_it;
     }
-    std::vector<T> fmean(std::max(miniter, 1), (T)0);
-    T qp = 0;
+    StopRule<T> stop(miniter);
     const T invSqrtN = (T)(1.0 / std::sqrt((double)(P.H * P.W)));
 
     for (int nl = 1; nl <= prm.maxIntIter; ++nl) {
@@ -597,10 +630,9 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
         std::fill(Zk.begin(), Zk.end(), Cx<T>{0, 0});
         Ak = 0;                                                                            // (:643-649)
         Lmu = normU / mu;
-        fmean[miniter - 1] = (T)LONG_MAX;
+        stop.start_stage();
         Lmu = lambda * Lmu + (T)prm.La;
         Lmu1 = (T)1 / Lmu;
-        bool qpDone = false;
         int done_k = 0;
         for (int k = 0; k < prm.maxIter; ++k) {
             ++done_k;
@@ -629,23 +661,7 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
                 rep->trace_f[rep->ntrace] = fx; rep->trace_l2[rep->ntrace] = l2; rep->trace_l1[rep->ntrace] = l1;
                 rep->ntrace++;
             }
-            if (prm.stoptest == 1) {                                                       // (:799-824)
-                T fmeanMean = 0;
-                bool skip_write = false;
-                if (k >= miniter) {
-                    for (int ne = 0; ne < miniter; ++ne) fmeanMean += fmean[miniter - ne - 1];
-                    fmeanMean = fmeanMean / (T)miniter;
-                    qp = (fmeanMean - fx) / fmeanMean;
-                    if (qp <= tolVar) {
-                        if (qpDone) k = prm.maxIter;
-                        qpDone = true;
-                        skip_write = true;
-                    }
-                } else {
-                    qp = fmean[miniter - 1];
-                }
-                if (!skip_write) fmean[(miniter - 1) - ((k + 1) % miniter)] = fx;
-            }
+            if (prm.stoptest == 1 && stop.step(k, fx, tolVar)) k = prm.maxIter;            // (:799-824)
             const T apk = (T)0.5 * ((T)k + (T)1);                                          // (:829-831)
             Ak = Ak + apk;
             const T tauk = (T)2 / ((T)k + (T)3);
@@ -660,7 +676,6 @@ template <class T> int nesta(const Prob<T>& P, Cx<T>* xref, const orc_params& pr
         if (rep && nl <= ORC_MAX_STAGES) { rep->iters[nl - 1] = done_k; rep->stages = nl; }
     }
     if (rep) { rep->final_tolVar = tolVar; rep->final_mu = mu; }
-    (void)qp;
     return 0;
 }
 
@@ -711,6 +726,26 @@ int orc_simple_recon_f64(const double* y, const double* S, double* x, int F, int
     Prob<double> P{F, C, H, W, (long)H * W, nullptr, (const Cx<double>*)S, nullptr};
     op_At(P, (const Cx<double>*)y, (Cx<double>*)x, aux.data());
     return 0;
+}
+
+// The stop test on a scripted objective sequence: f[0..n) feeds nstages stages of at most maxIter iterations each
+// (tolVar fixed); iters[s] receives the iterations stage s ran.  Returns the number of values consumed, or -1 when f
+// runs out first.
+int orc_stop_rule_f64(const double* f, int n, int miniter, double tolVar, int maxIter, int nstages, int* iters) {
+    if (miniter < 1 || maxIter < 1) return -1;
+    StopRule<double> stop(miniter);
+    int used = 0;
+    for (int st = 0; st < nstages; ++st) {
+        stop.start_stage();
+        int done = 0;
+        for (int k = 0; k < maxIter; ++k) {
+            if (used >= n) return -1;
+            ++done;
+            if (stop.step(k, f[used++], tolVar)) k = maxIter;
+        }
+        iters[st] = done;
+    }
+    return used;
 }
 
 // Motion compensation of F frames of H x W (adjoint = 0: MotionCompensation, 1: AdjointMotionCompensation).

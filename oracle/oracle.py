@@ -73,6 +73,7 @@ def lib():
             "orc_simple_recon_f32": ([P, P, P, I, I, I, I], I), "orc_simple_recon_f64": ([P, P, P, I, I, I, I], I),
             "orc_rsos_f32": ([P, P, I, I, LG], None), "orc_simple_recon_sos_f64": ([P, P, I, I, I, I], I),
             "orc_num_threads": ([], I), "orc_set_threads": ([I], None),
+            "orc_stop_rule_f64": ([P, I, I, D, I, I, P], I),
             "orc_mc_f64": ([P, P, P, P, P, I, I, I, I], I), "orc_mc_f32": ([P, P, P, P, P, I, I, I, I], I),
             "orc_normest_mc_f64": ([P, P, P, I, I, I, ctypes.POINTER(I)], D),
             "orc_nesta_mc_f64": ([P, P, P, P, I, I, I, I, P, P, P, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
@@ -215,6 +216,16 @@ def simple_recon_sos(y):
     x = np.empty((F, H, W), dtype=np.complex128)
     lib().orc_simple_recon_sos_f64(_p(y), _p(x), F, C, H, W)
     return x
+
+
+def stop_rule(f, miniter, tolVar, maxIter, nstages=1):
+    """The stop test (stoptest = 1) on a scripted objective sequence; returns the iterations of each stage."""
+    a = _c(f, np.float64)
+    iters = np.zeros(nstages, np.int32)
+    used = lib().orc_stop_rule_f64(_p(a), a.size, miniter, tolVar, maxIter, nstages, _p(iters))
+    if used < 0:
+        raise ValueError("objective sequence ran out")
+    return [int(v) for v in iters]
 
 
 def mc(x, xn, mesh, bb, adjoint=False, double=True):

@@ -285,3 +285,47 @@ def test_nesta_mc_moves_the_solution(oracle):
     assert np.all(np.isfinite(x_mc))
     assert 1e-4 < oracle.rel_l2(x_mc, x_plain) < 0.5
     assert rep["normU"] == pytest.approx(oracle.normest_mc(xn, mesh, bb)[0], rel=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Stop test 1 of NestaUp.cpp:799-824 on scripted objective sequences.  Semantics: the ring holds the last miniter
+# written values (slot (miniter-1) - ((k+1) mod miniter) is written at iteration k, so iterations 0..miniter-1 fill
+# every slot); from k = miniter on, qp = (mean - f) / mean; the first qp <= tolVar of a stage arms the rule and its
+# value is NOT written; any later qp <= tolVar in the same stage ends the stage after that iteration; arming resets
+# per stage.  Each expectation below is worked out by hand from those rules.
+# ---------------------------------------------------------------------------------------------------------------------
+def test_stop_rule_constant_objective(oracle):
+    # miniter 3: k = 0, 1, 2 only write; k = 3: qp = 0 arms; k = 4: qp = 0 ends -> 5 iterations
+    assert oracle.stop_rule([7.0] * 10, miniter=3, tolVar=1e-3, maxIter=20) == [5]
+
+
+def test_stop_rule_fast_decrease_never_stops(oracle):
+    # f halves each iteration: qp >= 0.5 > tolVar at every k >= miniter -> the stage runs to maxIter
+    assert oracle.stop_rule([0.5 ** k for k in range(20)], miniter=3, tolVar=0.1, maxIter=20) == [20]
+
+
+def test_stop_rule_hits_need_not_be_consecutive(oracle):
+    # miniter 2, tolVar 0.1: ring {10, 10}; k2 f=9.5 qp=0.05 arms (not written); k3 f=5 qp=0.5 miss, written
+    # -> ring {10, 5}; k4 f=7.4 mean 7.5 qp=0.0133 ends -> 5 iterations
+    assert oracle.stop_rule([10, 10, 9.5, 5, 7.4, 7.4, 7.4], miniter=2, tolVar=0.1, maxIter=20) == [5]
+
+
+def test_stop_rule_arming_iteration_is_not_written(oracle):
+    # as above, but k3 f=8.9: with the arming value skipped the ring is {10, 10}, qp = 0.11 > 0.1 (miss; had 9.5 been
+    # written the mean would be 9.75 and qp = 0.087 would end the stage at 4 iterations); ring {10, 8.9}; k4 f=9.4:
+    # mean 9.45, qp = 0.0053 ends -> 5 iterations
+    assert oracle.stop_rule([10, 10, 9.5, 8.9, 9.4, 9.4], miniter=2, tolVar=0.1, maxIter=20) == [5]
+
+
+def test_stop_rule_boundary_is_inclusive(oracle):
+    # miniter 1: ring {2}; k1 f=1: qp = 0.5 <= 0.5 arms; k2 f=1: mean 2 (the arming value was not written), qp = 0.5
+    # ends -> 3 iterations.  With tolVar just below 0.5: k1 misses (ring {1}); k2 qp = 0 arms; k3 ends -> 4
+    assert oracle.stop_rule([2, 1, 1, 1, 1], miniter=1, tolVar=0.5, maxIter=20) == [3]
+    assert oracle.stop_rule([2, 1, 1, 1, 1], miniter=1, tolVar=0.4999, maxIter=20) == [4]
+
+
+def test_stop_rule_arming_resets_per_stage_and_maxiter_caps(oracle):
+    # two stages of a constant objective: each needs its own arming hit -> [5, 5] (10 values consumed)
+    assert oracle.stop_rule([3.0] * 10, miniter=3, tolVar=1e-3, maxIter=20, nstages=2) == [5, 5]
+    # maxIter 4: k = 3 arms, then the stage ends at maxIter anyway
+    assert oracle.stop_rule([3.0] * 10, miniter=3, tolVar=1e-3, maxIter=4) == [4]
