@@ -450,35 +450,75 @@ def gpu_legs(args, cine, parts, full, split):
 # ---------------------------------------------------------------------------------------------------------------------
 # CPU leg: oracle (the FP32 restatement as the baseline, FP64 and FP32 as the checkers)
 # ---------------------------------------------------------------------------------------------------------------------
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "os_cpu_count": os.cpu_count()}
+
+
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
 def cpu_leg(args, slice0, full, g):
+    """FP32 restatement timed on the host: config 2 as setup + 100 iterations from the median of 5 runs of 2 and of
+    4 inner iterations (after one warm-up), all threads; the 1-thread figure from single runs of 1 and 2
+    iterations; config 1 as the median of 5 (and of 3 on one thread)."""
     from oracle import oracle as O
     y, S, M = slice0
     threads = O.num_threads()
     res = {}
-    t = time.perf_counter()
-    O.nesta(y, S, M, O.params(maxIntIter=1, maxIter=2, stoptest=0), double=False)
-    ta = time.perf_counter() - t
-    t = time.perf_counter()
-    O.nesta(y, S, M, O.params(maxIntIter=1, maxIter=8, stoptest=0), double=False)
-    tb = time.perf_counter() - t
-    t_iter = (tb - ta) / 6.0
+    run = lambda n: O.nesta(y, S, M, O.params(maxIntIter=1, maxIter=n, stoptest=0), double=False)
+    run(1)                                                                    # warm-up
+    ta, tb = _median_time(lambda: run(2), 5), _median_time(lambda: run(4), 5)
+    t_iter = (tb - ta) / 2.0
     t_setup = max(0.0, ta - 2 * t_iter)
     t_recon = t_setup + STAGES * ITERS * t_iter
+    O.set_threads(1)
+    try:
+        t1a = _median_time(lambda: run(1), 1)
+        t1b = _median_time(lambda: run(2), 1)
+    finally:
+        O.set_threads(threads)
+    t1_iter = max(1e-9, t1b - t1a)
+    t1_recon = max(0.0, t1a - t1_iter) + STAGES * ITERS * t1_iter
     res["cpu_baseline"] = {"value": 1.0 / t_recon, "unit": "recons/s", "cores": threads, "kind": "port",
-                           "sample": f"FP32 restatement (oracle/nesta_oracle.cpp, OpenMP), one 20x8x256x256 slice, "
-                                     f"runs of 2 and 8 inner iterations ({ta + tb:.1f} s of CPU work); setup + 100 "
-                                     f"iterations extrapolated linearly ({t_setup:.2f} s setup + {t_iter:.3f} "
-                                     f"s/iteration)"}
+                           "sample": f"FP32 restatement (oracle/nesta_oracle.cpp, OpenMP, {threads} threads), one "
+                                     f"20x8x256x256 slice: medians of 5 runs of 2 and 4 inner iterations after a "
+                                     f"warm-up; setup + 100 iterations extrapolated linearly ({t_setup:.2f} s setup + "
+                                     f"{t_iter:.3f} s/iteration)",
+                           "one_thread_recons_per_s": 1.0 / t1_recon,
+                           "one_thread_sample": f"single runs of 1 and 2 iterations on 1 thread ({t1_iter:.3f} "
+                                                f"s/iteration)",
+                           "host": host_info()}
     gx = g["x"]
     if full is not None and "simple" in gx:
         O.simple_recon(full.y, full.S, double=False)
-        ts = []
-        for _ in range(5):
-            t = time.perf_counter()
-            O.simple_recon(full.y, full.S, double=False)
-            ts.append(time.perf_counter() - t)
+        t_med = _median_time(lambda: O.simple_recon(full.y, full.S, double=False), 5)
+        O.set_threads(1)
+        try:
+            t_one = _median_time(lambda: O.simple_recon(full.y, full.S, double=False), 3)
+        finally:
+            O.set_threads(threads)
         x64 = O.simple_recon(full.y, full.S, double=True)
-        res["simple_recon"] = {"cpu_recons_per_s": 1.0 / float(np.median(ts)), "cores": threads,
+        res["simple_recon"] = {"cpu_recons_per_s": 1.0 / t_med, "cpu_one_thread_recons_per_s": 1.0 / t_one,
+                               "cores": threads,
                                "rel_l2_vs_fp64": O.rel_l2(gx["simple"], x64),
                                "rel_l2_vs_truth": O.rel_l2(gx["simple"], full.x_true)}
     if not args.no_ref:
