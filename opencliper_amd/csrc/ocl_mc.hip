@@ -216,7 +216,9 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
     const long N = d.N;
     const int ntc = (d.W + OCL_MC_TC - 1) / OCL_MC_TC, ntr = (d.H + OCL_MC_TR - 1) / OCL_MC_TR;
     const int local = blockIdx.x - off[si];
-    const int f = local / (ntr * ntc), tt = local - f * (ntr * ntc);
+    // frames fastest: consecutive workgroups share a tile across adjacent frames, whose z rows (f-1, f, f+1) then
+    // meet in L2
+    const int tt = local / d.F, f = local - tt * d.F;
     const int r0 = (tt / ntc) * OCL_MC_TR, c0 = (tt % ntc) * OCL_MC_TC;
     const int br0 = d.mc_r0, br1 = d.mc_r0 + d.mc_nr, bc0 = d.mc_c0, bc1 = d.mc_c0 + d.mc_nc;
     const bool near = r0 - R < br1 && r0 + OCL_MC_TR + R > br0 && c0 - R < bc1 && c0 + OCL_MC_TC + R > bc0;
@@ -257,17 +259,16 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
             for (int dr = -R; dr <= R; ++dr)
 #pragma unroll
                 for (int dc = -R; dc <= R; ++dc) {
+                    // branch-free: a non-source (NaN position) or a footprint that misses (r, c) adds 0 * t
                     const int cell = (lr + R + dr) * SC + (lc + R + dc);
                     const float2 pos = spos[cell];
-                    if (pos.x != pos.x) continue;                 // not a source
+                    const float2 t = sval[cell];
                     const float fh = floorf(pos.x), fv = floorf(pos.y);
                     const float a = (float)c - fh, b = (float)r - fv;
-                    if (a != 0.f && a != 1.f) continue;
-                    if (b != 0.f && b != 1.f) continue;
                     const float Whh = pos.x - fh, Wvv = pos.y - fv;
-                    const float hw = a != 0.f ? Whh : 1.0f - Whh, vw = b != 0.f ? Wvv : 1.0f - Wvv;
-                    const float2 t = sval[cell];
-                    g.x += t.x * hw * vw;
+                    const float hw = a == 1.f ? Whh : (a == 0.f ? 1.0f - Whh : 0.f);
+                    const float vw = b == 1.f ? Wvv : (b == 0.f ? 1.0f - Wvv : 0.f);
+                    g.x += t.x * hw * vw;                          // the adjoint scatter's association
                     g.y += t.y * hw * vw;
                 }
         }
