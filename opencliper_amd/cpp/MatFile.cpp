@@ -86,6 +86,7 @@ Element nextElement(const uint8_t*& p, const uint8_t* end, bool pad) {
     if (w0 >> 16) {   // small data element: byte count in the upper half, data in the second word
         e.type = w0 & 0xFFFF;
         e.bytes = w0 >> 16;
+        if (e.bytes > 4) throw std::invalid_argument("MAT-file: small data element longer than 4 bytes");
         e.data = p + 4;
         p += 8;
         return e;
@@ -245,9 +246,12 @@ bool isMatFile(const std::string& fileName) {
 }
 
 std::map<std::string, Var> read(const std::string& fileName) {
-    std::ifstream f(fileName, std::ios::binary);
+    std::ifstream f(fileName, std::ios::binary | std::ios::ate);
     if (!f) throw std::invalid_argument("MAT-file: cannot open " + fileName);
-    std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const std::streamsize size = f.tellg();
+    std::vector<uint8_t> buf(size > 0 ? (size_t)size : 0);
+    f.seekg(0);
+    if (size > 0 && !f.read((char*)buf.data(), size)) throw std::invalid_argument("MAT-file: cannot read " + fileName);
     if (buf.size() < 128 || buf[126] != 'I' || buf[127] != 'M')
         throw std::invalid_argument("MAT-file: " + fileName + " is not a little-endian MAT v5 file");
     if (buf[124] != 0x00 || buf[125] != 0x01)

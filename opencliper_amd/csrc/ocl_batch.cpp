@@ -675,6 +675,8 @@ extern "C" int ocl_batch_create_mc(ocl_ctx* c, int nslices, const ocl_slice* sli
     if (!mcs || nslices < 1 || !slices || !out) return fail(OCL_E_INVALID, "batch: bad arguments");
     HIPCHK(hipSetDevice(c->device));
     for (int i = 0; i < nslices; ++i) {
+        if (slices[i].H < 1 || slices[i].W < 1 || slices[i].F < 1)
+            return fail(OCL_E_INVALID, "batch: slice " + std::to_string(i) + " has bad dimensions");
         int r0, nr, c0, nc;
         CHK(mc_box_geometry(slices[i].H, slices[i].W, &mcs[i], &r0, &nr, &c0, &nc));
         bool ok = false;
