@@ -292,6 +292,18 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.maxe = 4;
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
+    // A table whose column lengths differ runs every slice through one column kernel that holds all the four-step
+    // sizes, at lower occupancy than a single-size kernel; there the prefetching pass (STAGE) hides the latency the
+    // direct pass exposes, so every eligible four-step slice takes it whatever its shape was tuned to alone.
+    bool mixedH = false;
+    for (int i = 1; i < ns; ++i) mixedH |= v[i]->d.fastH != v[0]->d.fastH;
+    if (mixedH && !L.prec)
+        for (int i = 0; i < ns; ++i) {
+            SliceDesc& d = v[i]->d;
+            if (!d.fastH || d.ca_stage || d.nmax > 16 * FAST_CA_KQ) continue;
+            d.ca_stage = 1;
+            if (lds_CA(d) > LDS_MAX_NESTA) d.ca_stage = 0;
+        }
     L.kj = 1;
     L.stage = ns ? v[0]->d.ca_stage : 0;
     L.mc = ns && v[0]->d.mc_xn ? 1 : 0;

@@ -193,8 +193,6 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             if (c + 1 < C) stage_load(c + 1);
         }
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) sv[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
-#pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int k1 = t + 16 * q;
             if (k1 < P) {
@@ -211,6 +209,9 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
                     X[k1 * FAST_CA_KS + n2 * 16] = (k1 == 0 || n2 == 0) ? u[n2] : c_mul(u[n2], tw[k1 * n2]);
             }
         }
+        // S of this coil: issued here, after the first phase, so it is not live across that phase's DFTs
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) sv[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
         __syncthreads();
         float2 v[P];
 #pragma unroll
