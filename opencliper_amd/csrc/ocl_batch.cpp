@@ -5,7 +5,10 @@
 // advance every slice of the table by one iteration per three launches, whatever their sizes.  The batch API keeps a
 // fixed table (and can record it in a hipGraph); the server rebuilds its table whenever requests are admitted or
 // finish, which is what lets new reconstructions join while others are mid-solve (continuous batching).
+#include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <map>
 #include <condition_variable>
 #include <deque>
 #include <thread>
@@ -246,15 +249,24 @@ int slice_alloc(SliceRes* r) {
     return OCL_OK;
 }
 
+// A group of a mixed-size table: the slices sharing a (column, row) four-step size, with their own descriptor copy
+// and offsets (sub-ranges of the table's group arrays) so passes B and CA run that size's specialized kernels.
+struct LaunchGroup {
+    NestaLaunch L{};
+};
+
 // Launch table over a set of slices.
 struct Table {
     ocl_ctx* c = nullptr;
+    std::vector<LaunchGroup> groups;   // empty: passes B and CA run once over the whole table
     SliceDesc* dd = nullptr;
     int* offB = nullptr;
     int* offCA = nullptr;
     int* offP = nullptr;
     int* offX = nullptr;
     int* offG = nullptr;
+    SliceDesc* gdd = nullptr;   // [cap] descriptors in group order
+    int* goff = nullptr;        // [2 (cap + cap)] per group: B offsets then CA offsets (group size + 1 each)
     int cap = 0;
     NestaLaunch L{};
     ~Table() {
@@ -264,8 +276,29 @@ struct Table {
         if (offP) hipFree(offP);
         if (offX) hipFree(offX);
         if (offG) hipFree(offG);
+        if (gdd) hipFree(gdd);
+        if (goff) hipFree(goff);
     }
 };
+
+// Passes B and CA over a table: once per launch group of a mixed-size table, else once
+hipError_t table_launch_B(const Table& t, hipStream_t s, int mode) {
+    if (t.groups.empty()) return launch_nesta_B(t.L, s, mode);
+    for (const LaunchGroup& g : t.groups) {
+        const hipError_t e = launch_nesta_B(g.L, s, mode);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t table_launch_CA(const Table& t, hipStream_t s, int mode) {
+    if (t.groups.empty()) return launch_nesta_CA(t.L, s, mode);
+    for (const LaunchGroup& g : t.groups) {
+        const hipError_t e = launch_nesta_CA(g.L, s, mode);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     const int ns = (int)v.size();
@@ -274,8 +307,10 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         if (t->dd) {
             hipStreamSynchronize(c->stream);
             hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); hipFree(t->offP); hipFree(t->offX); hipFree(t->offG);
+            hipFree(t->gdd); hipFree(t->goff);
         }
-        t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = t->offG = nullptr;
+        t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = t->offG = t->goff = nullptr;
+        t->gdd = nullptr;
         const int cap = std::max(ns, 2 * t->cap);
         HIPCHK(hipMalloc(&t->dd, sizeof(SliceDesc) * cap));
         HIPCHK(hipMalloc(&t->offB, sizeof(int) * (cap + 1)));
@@ -283,6 +318,8 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         HIPCHK(hipMalloc(&t->offP, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offX, sizeof(int) * (cap + 1)));
         HIPCHK(hipMalloc(&t->offG, sizeof(int) * (cap + 1)));
+        HIPCHK(hipMalloc(&t->gdd, sizeof(SliceDesc) * cap));
+        HIPCHK(hipMalloc(&t->goff, sizeof(int) * 4 * cap));
         t->cap = cap;
     }
     std::vector<SliceDesc> hd(ns);
@@ -292,18 +329,41 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.maxe = 4;
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
-    // A table whose column lengths differ runs every slice through one column kernel that holds all the four-step
-    // sizes, at lower occupancy than a single-size kernel; there the prefetching pass (STAGE) hides the latency the
-    // direct pass exposes, so every eligible four-step slice takes it whatever its shape was tuned to alone.
-    bool mixedH = false;
-    for (int i = 1; i < ns; ++i) mixedH |= v[i]->d.fastH != v[0]->d.fastH;
-    if (mixedH && !L.prec)
-        for (int i = 0; i < ns; ++i) {
-            SliceDesc& d = v[i]->d;
-            if (!d.fastH || d.ca_stage || d.nmax > 16 * FAST_CA_KQ) continue;
-            d.ca_stage = 1;
-            if (lds_CA(d) > LDS_MAX_NESTA) d.ca_stage = 0;
+    // Launch groups.  In a mixed-size table, the slices of a (column, row) four-step size whose column pass has
+    // enough workgroups to fill the GPU on its own (OCL_GROUP_MIN_WG) run passes B and CA as one launch each with that
+    // size's specialized kernels; the other slices share one launch of the kernels that hold every size.  Those run
+    // at lower occupancy, where the prefetching column pass (STAGE) hides the latency the direct pass exposes, so the
+    // four-step slices of a shared mixed launch all take it, whatever their shape was tuned to alone.  (Few, small
+    // groups would serialize launches that each under-fill the GPU, as in a server with a handful of requests.)
+    constexpr int OCL_GROUP_MIN_WG = 1024;
+    std::vector<std::vector<int>> sets;
+    {
+        std::map<std::pair<int, int>, std::vector<int>> by_size;
+        for (int i = 0; i < ns; ++i) by_size[{v[i]->d.fastH, v[i]->d.fastW}].push_back(i);
+        std::vector<int> rest;
+        if (by_size.size() > 1 && !L.prec) {
+            for (const auto& kv : by_size) {
+                long wg = 0;
+                for (int i : kv.second) wg += v[i]->d.wg_CA;
+                if (wg >= OCL_GROUP_MIN_WG) sets.push_back(kv.second);
+                else rest.insert(rest.end(), kv.second.begin(), kv.second.end());
+            }
+        } else {
+            for (int i = 0; i < ns; ++i) rest.push_back(i);
         }
+        std::sort(rest.begin(), rest.end());
+        bool restMixedH = false;
+        for (size_t j = 1; j < rest.size(); ++j) restMixedH |= v[rest[j]]->d.fastH != v[rest[0]]->d.fastH;
+        if (restMixedH && !L.prec)
+            for (int i : rest) {
+                SliceDesc& d = v[i]->d;
+                if (!d.fastH || d.ca_stage || d.nmax > 16 * FAST_CA_KQ) continue;
+                d.ca_stage = 1;
+                if (lds_CA(d) > LDS_MAX_NESTA) d.ca_stage = 0;
+            }
+        if (!sets.empty() && !rest.empty()) sets.push_back(rest);
+    }
+    const bool grouped = sets.size() > 1;
     L.kj = 1;
     L.stage = ns ? v[0]->d.ca_stage : 0;
     L.mc = ns && v[0]->d.mc_xn ? 1 : 0;
@@ -355,6 +415,47 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         for (int i = 0; i < ns; ++i) L.ldsCA = std::max(L.ldsCA, mf_ca_lds_bytes(v[i]->d.H, v[i]->d.kp));
         if (L.ldsCA > LDS_MAX_NESTA) return fail(OCL_E_UNSUPPORTED, "batch: split-precision column tile exceeds LDS");
     }
+    // launch groups of a mixed-size table: descriptors in group order in gdd, per group its B and CA offsets in goff
+    t->groups.clear();
+    std::vector<SliceDesc> gd;
+    std::vector<int> go;
+    if (grouped) {
+        for (const std::vector<int>& idx : sets) {
+            const int gn = (int)idx.size();
+            const size_t d0 = gd.size(), o0 = go.size();
+            LaunchGroup g;
+            NestaLaunch& G = g.L;
+            G = L;
+            G.maxe = 4;
+            G.ldsB = G.ldsCA = 0;
+            G.PH = v[idx[0]]->d.fastH;
+            G.PW = v[idx[0]]->d.fastW;
+            G.stage = v[idx[0]]->d.ca_stage;
+            go.resize(o0 + 2 * (gn + 1), 0);
+            for (int j = 0; j < gn; ++j) {
+                const SliceDesc& d = v[idx[j]]->d;
+                gd.push_back(d);
+                go[o0 + j + 1] = go[o0 + j] + d.wg_B + d.wg_U;
+                go[o0 + gn + 1 + j + 1] = go[o0 + gn + 1 + j] + d.wg_CA;
+                G.ldsB = std::max(G.ldsB, lds_B(d));
+                G.ldsCA = std::max(G.ldsCA, lds_CA(d));
+                if (d.ca_stage != G.stage) G.stage = 2;
+                if (d.fastH != G.PH) G.PH = 0;
+                if (d.fastW != G.PW) G.PW = 0;
+                if (!d.fastH) {
+                    const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
+                    while (G.maxe < need) G.maxe *= 2;
+                }
+            }
+            G.dd = t->gdd + d0;
+            G.offB = t->goff + o0;
+            G.offCA = t->goff + o0 + gn + 1;
+            G.ns = gn;
+            G.totB = go[o0 + gn];
+            G.totCA = go[o0 + 2 * gn + 1];
+            t->groups.push_back(g);
+        }
+    }
     if (ns) {
         HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offB, oB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
@@ -362,7 +463,11 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         HIPCHK(hipMemcpyAsync(t->offP, oP.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offX, oX.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(t->offG, oG.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));   // hd and the offset vectors are host temporaries
+        if (!gd.empty()) {
+            HIPCHK(hipMemcpyAsync(t->gdd, gd.data(), sizeof(SliceDesc) * gd.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(t->goff, go.data(), sizeof(int) * go.size(), hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));   // hd, gd and the offset vectors are host temporaries
     }
     return OCL_OK;
 }
@@ -382,10 +487,10 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         }
         HIPCHK(launch_nesta_mc_dmax(t.L, c->stream));
         HIPCHK(launch_nesta_mc_z(t.L, c->stream, 1));
-        HIPCHK(launch_nesta_CA(t.L, c->stream, 2));
+        HIPCHK(table_launch_CA(t, c->stream, 2));
     } else {
-        HIPCHK(launch_nesta_B(t.L, c->stream, 1));
-        HIPCHK(launch_nesta_CA(t.L, c->stream, 1));
+        HIPCHK(table_launch_B(t, c->stream, 1));
+        HIPCHK(table_launch_CA(t, c->stream, 1));
     }
     hipLaunchKernelGGL(k_setup_max, dim3(t.L.totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)t.dd,
                        (const int*)t.offCA, t.L.ns, (double*)nullptr);
@@ -451,10 +556,10 @@ int table_iterate(ocl_ctx* c, const Table& t, int n) {
     if (!t.L.ns) return OCL_OK;
     for (int it = 0; it < n; ++it) {
         HIPCHK(launch_nesta_mc_z(t.L, c->stream, 0));      // motion compensation only
-        HIPCHK(launch_nesta_B(t.L, c->stream, 0));
+        HIPCHK(table_launch_B(t, c->stream, 0));
         HIPCHK(launch_nesta_S(t.L, c->stream));
         HIPCHK(launch_nesta_mc_g(t.L, c->stream));         // motion compensation only
-        HIPCHK(launch_nesta_CA(t.L, c->stream, 0));
+        HIPCHK(table_launch_CA(t, c->stream, 0));
     }
     return OCL_OK;
 }
@@ -467,11 +572,11 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
     for (int it = 0; it < n && rc == OCL_OK; ++it) {
         hipError_t e = hipEventRecord(ev[3 * it], c->stream);
         if (e == hipSuccess) e = launch_nesta_mc_z(t.L, c->stream, 0);
-        if (e == hipSuccess) e = launch_nesta_B(t.L, c->stream, 0);
+        if (e == hipSuccess) e = table_launch_B(t, c->stream, 0);
         if (e == hipSuccess) e = launch_nesta_S(t.L, c->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 1], c->stream);
         if (e == hipSuccess) e = launch_nesta_mc_g(t.L, c->stream);
-        if (e == hipSuccess) e = launch_nesta_CA(t.L, c->stream, 0);
+        if (e == hipSuccess) e = table_launch_CA(t, c->stream, 0);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 2], c->stream);
         if (e != hipSuccess) rc = fail(OCL_E_HIP, std::string("timing launch: ") + hipGetErrorString(e));
     }
@@ -499,7 +604,7 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
 
 // Autotuner of the FP32 passes' tile shapes.  For one (H, W) it times every admissible pass-CA variant (register
 // four-step when H = 16 P, LDS Stockham with 4, 8 or 16 columns per workgroup) and every pass-B variant (four-step
-// when W = 16 P, Stockham with 4, 8 or 16 rows per workgroup) on a synthetic batch of 4 slices of 16 frames x 8 coils
+// when W = 16 P, Stockham with 4, 8 or 16 rows per workgroup) on a synthetic batch of 16 slices of 16 frames x 8 coils
 // with a ky-line mask at acceleration 6, and keeps the fastest of each (the passes are timed separately, so the two
 // choices are independent).  Results are cached per context.
 int tune_shape(ocl_ctx* c, int H, int W) {
@@ -514,7 +619,9 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     for (int rb : {4, 8, 16})
         if (lds_B_generic(W, rb) <= LDS_MAX_NESTA) bC.push_back({0, rb});
     if (caC.empty() || bC.empty()) return fail(OCL_E_UNSUPPORTED, "autotune: no tile shape fits");
-    const int F = 16, C = 8, NS = 4;
+    // 16 slices: enough workgroups to fill the GPU as a real batch does (a smaller tuning batch favours the
+    // higher-occupancy direct column pass, which loses at batch scale)
+    const int F = 16, C = 8, NS = 16;
     const long N = (long)H * W;
     std::vector<void*> bufs;
     auto cleanup = [&]() {
@@ -548,6 +655,7 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     ocl_nesta_params prm{1e-2, 3e-4, 1.0, 1, 3e-4, 0, 1 << 20, 0, 7};
     TileCfg best;
     best.ms_CA = best.ms_B = 1e30;
+    double best_score = 1e30;
     const size_t n = std::max(caC.size(), bC.size());
     for (size_t i = 0; i < n && rc == OCL_OK; ++i) {
         TileCfg cfg;
@@ -571,8 +679,12 @@ int tune_shape(ocl_ctx* c, int H, int W) {
         if (rc == OCL_OK) rc = table_iterate(c, t, 2);
         if (rc == OCL_OK) rc = time_table(c, t, 6, &ms_b, &ms_ca, nullptr);
         if (rc != OCL_OK) break;
-        if (i < caC.size() && ms_ca / NS < best.ms_CA) {
+        // the direct four-step column pass must win by 5 %: on a tie the prefetching pass, whose latency hiding
+        // matters more in the fuller and mixed launches of real batches, is kept
+        const double score = ms_ca / NS * (cfg.ca_fast == 1 ? 1.05 : 1.0);
+        if (i < caC.size() && score < best_score) {
             best.ca_fast = cfg.ca_fast; best.tile_cols = cfg.tile_cols; best.ms_CA = ms_ca / NS;
+            best_score = score;
         }
         if (i < bC.size() && ms_b / NS < best.ms_B) {
             best.b_fast = cfg.b_fast; best.rows_per_wg = cfg.rows_per_wg; best.ms_B = ms_b / NS;
