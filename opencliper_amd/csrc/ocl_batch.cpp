@@ -68,7 +68,7 @@ size_t lds_B(const SliceDesc& d) {
                    : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W);
 }
 size_t lds_CA(const SliceDesc& d) {
-    return d.fastH ? sizeof(float2) * (fast_ca_lds(d.fastH) + (d.ca_stage ? 2 * 16 * (size_t)d.nmax : 0))
+    return d.fastH ? sizeof(float2) * (fast_ca_lds(d.fastH) + (d.ca_stage ? 2 * 16 * ((size_t)d.nmax + 1) : 0))
                    : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
 }
 

@@ -72,7 +72,7 @@ struct UpdRow {
     const float2 *xf, *xm, *xp, *xref_in;
     const float2* g;      // motion compensation: MC^T(D^T u) of frame f, replacing the local temporal-TV gradient
     float2 *xref_out, *Wk, *xo;
-    float mu, lam, L1, apk, tau;
+    float mu, imu, lam, L1, apk, tau;
     int sstart;
 };
 
@@ -88,7 +88,7 @@ __device__ __forceinline__ UpdRow upd_row(const SliceDesc& d, int f) {
     r.Wk = d.Wk + (long)f * d.N;
     r.xo = u.xo + (long)f * d.N;
     r.g = d.mcg ? d.mcg + (long)f * d.N : nullptr;
-    r.mu = u.mu; r.lam = u.lam; r.L1 = u.L1; r.apk = u.apk; r.tau = u.tau;
+    r.mu = u.mu; r.imu = 1.0f / u.mu; r.lam = u.lam; r.L1 = u.L1; r.apk = u.apk; r.tau = u.tau;
     r.sstart = u.sstart;
     return r;
 }
@@ -111,8 +111,10 @@ __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, uns
     } else {
         const float2 xm = ldg2(r.xm, p), xp = ldg2(r.xp, p);
         const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-        const float nf = fmaxf(r.mu, sqrtf(c_abs2(uxf))), np = fmaxf(r.mu, sqrtf(c_abs2(uxp)));
-        const float2 uf = make_float2(uxf.x / nf, uxf.y / nf), up = make_float2(uxp.x / np, uxp.y / np);
+        // u = Ux / max(mu, |Ux|) as Ux * min(1/mu, rsqrt(|Ux|^2)) (rsqrt(0) = inf)
+        const float inf_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxf)));
+        const float inp_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxp)));
+        const float2 uf = c_scale(uxf, inf_), up = c_scale(uxp, inp_);
         df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
     }
     float2 xr, wo;

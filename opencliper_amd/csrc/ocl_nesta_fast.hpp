@@ -8,7 +8,8 @@
 #define FAST_CA_KS 272                          // per-k1 stride of the column exchange: 16 roles x 16 columns + pad
 #define FAST_CA_KQ 4                            // staged k-space values per thread and coil (nmax <= 64)
 #define FAST_B_RS(P) ((P) * 17 + 1)             // per-row stride of the row exchange
-__host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + 16 * P + 4 * P; }
+// pass CA: column exchange, twiddles by k1 role [P][16] (and, for P != 16, by a role [16][P]), row positions [P][16]
+__host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + (P == 16 ? 16 : 32) * P + 8 * P; }
 __host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + 16 * P; }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -123,13 +124,23 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     return acc;
 }
 
+// 16 consecutive ints of LDS (16-byte aligned) in four 128-bit reads
+__device__ __forceinline__ void load_rows16(const int* src, int* dst) {
+    const int4* s4 = (const int4*)src;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int4 v = s4[i];
+        dst[4 * i] = v.x; dst[4 * i + 1] = v.y; dst[4 * i + 2] = v.z; dst[4 * i + 3] = v.w;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // pass CA, columns of length H = 16 P: 16 columns per workgroup, 16 roles per column
 // ---------------------------------------------------------------------------------------------------------------------
 // STAGE = true adds two prefetches that keep global-memory latency off each coil's critical path: the sampled
 // k-space rows of the next coil are loaded cooperatively (coalesced, at most FAST_CA_KQ values per thread) while the
 // current coil is transformed and parked in one of two LDS slabs, and the S values of the next coil's forward pass
-// are issued before the current coil's exchange barrier.  It needs nmax <= 16 FAST_CA_KQ and 2 * 16 * nmax more
+// are issued before the current coil's exchange barrier.  It needs nmax <= 16 FAST_CA_KQ and 2 * 16 * (nmax + 1) more
 // float2 of LDS.  STAGE = false gathers the sampled rows straight from global memory.
 // MC: the table uses motion compensation (update gradient from mcg; mode 2 = setup from a given x_ref)
 template <int P, bool STAGE, bool MC = false>
@@ -145,13 +156,29 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     const bool wv = w < W;
     const int wc = wv ? w : 0;
     const unsigned lane = (unsigned)(t * W + w), lanec = (unsigned)(t * W + wc);   // offsets of row t, column w
-    float2* tw = lds + P * FAST_CA_KS;
-    int16_t* rp = (int16_t*)(tw + H);
-    float2* kst = tw + H + 4 * P;                      // STAGE: slabs [2][nmax][16]
+    // Per-role tables, so a thread reads its 16 (or P) values at one base with immediate offsets, all at once:
+    //   twk[k1][n2] = W_H^{k1 n2} (FB phase 1), twa[a][k1] = W_H^{a k1} (FA phase 1; the same table when P = 16),
+    //   rpk[k1][k2] = k-space slot of row k1 + P k2 in this frame, or nmax when the row is not sampled.
+    float2* twk = lds + P * FAST_CA_KS;
+    float2* twa = P == 16 ? twk : twk + H;
+    int* rpk = (int*)(twk + (P == 16 ? H : 2 * H));
+    // STAGE: slabs [2][nmax + 1][16] of conjugated k-space rows; slot nmax stays zero, so an unsampled row reads zero
+    float2* kst = (float2*)(rpk + H);
+    const int kss = (nmax + 1) * 16;                   // slab stride
     const int nst = d.nrows[f] * 16;                   // staged values per coil
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
-        tw[i] = d.ph.tw[i];
-        rp[i] = d.rowpos[(long)f * H + i];
+        const int k1 = i >> 4, n2 = i & 15;            // i = 16 k1 + n2 (k1 < P)
+        twk[i] = d.ph.tw[k1 * n2];
+        if (P != 16) {
+            const int a = i / P, k = i - a * P;        // i = P a + k
+            twa[i] = d.ph.tw[a * k];
+        }
+        const int j = k1 + P * n2;                     // the row of role k1, index n2
+        const int slot = d.rowpos[(long)f * H + j];
+        rpk[i] = slot < 0 ? nmax : slot;
+    }
+    if constexpr (STAGE) {
+        if (threadIdx.x < 32) kst[(threadIdx.x >> 4) * kss + nmax * 16 + (threadIdx.x & 15)] = make_float2(0.f, 0.f);
     }
     float2 pre[FAST_CA_KQ];
     auto stage_load = [&](int c) {
@@ -166,7 +193,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
         for (int q = 0; q < FAST_CA_KQ; ++q) {
             const int e = threadIdx.x + OCL_NT * q;
-            if (e < nst) kst[buf * nmax * 16 + e] = pre[q];
+            if (e < nst) kst[buf * kss + e] = c_conj(pre[q]);
         }
     };
     if constexpr (STAGE) {
@@ -187,7 +214,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     const int Cinv = (MC && mode == 2) ? 0 : C;            // no A^H r when x_ref is given
     for (int c = 0; c < Cinv; ++c) {
         const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
-        const float2* ks = kst + (c & 1) * nmax * 16 + s;
+        const float2* ks = kst + (c & 1) * kss + s;
         const float2* Sc = d.S + (long)c * N;
         if constexpr (STAGE) {
             if (c + 1 < C) stage_load(c + 1);
@@ -196,17 +223,20 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
         for (int q = 0; q < NQ; ++q) {
             const int k1 = t + 16 * q;
             if (k1 < P) {
+                int jv[16];
+                load_rows16(rpk + 16 * k1, jv);
                 float2 u[16];
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
-                    const int jj = rp[k1 + P * k2];
-                    if constexpr (STAGE) u[k2] = jj >= 0 ? c_conj(ks[jj * 16]) : make_float2(0.f, 0.f);
-                    else u[k2] = (jj >= 0 && wv) ? c_conj(ldg2(kb, (unsigned)(jj * W + w))) : make_float2(0.f, 0.f);
+                    const int jj = jv[k2];
+                    if constexpr (STAGE) u[k2] = ks[jj * 16];          // conjugated when staged; zero row at nmax
+                    else u[k2] = (jj < nmax && wv) ? c_conj(ldg2(kb, (unsigned)(jj * W + w))) : make_float2(0.f, 0.f);
                 }
                 rf::rdft<16>(u);
+                const float2* tr = twk + 16 * k1;                // tr[0] = 1
+                X[k1 * FAST_CA_KS] = u[0];
 #pragma unroll
-                for (int n2 = 0; n2 < 16; ++n2)
-                    X[k1 * FAST_CA_KS + n2 * 16] = (k1 == 0 || n2 == 0) ? u[n2] : c_mul(u[n2], tw[k1 * n2]);
+                for (int n2 = 1; n2 < 16; ++n2) X[k1 * FAST_CA_KS + n2 * 16] = c_mul(u[n2], tr[n2]);
             }
         }
         // S of this coil: issued here, after the first phase, so it is not live across that phase's DFTs
@@ -266,9 +296,10 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             v[n1] = wv ? c_mul(xn[n1], sc1) : make_float2(0.f, 0.f);
         }
         rf::rdft<P>(v);
+        const float2* tr = twa + P * t;                          // tr[0] = 1
+        X[t * 16] = v[0];
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1)
-            X[k1 * FAST_CA_KS + t * 16] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+        for (int k1 = 1; k1 < P; ++k1) X[k1 * FAST_CA_KS + t * 16] = c_mul(v[k1], tr[k1]);
         if constexpr (STAGE) {
             if (c + 1 < C)
 #pragma unroll
@@ -280,14 +311,16 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
         for (int q = 0; q < NQ; ++q) {
             const int k1 = t + 16 * q;
             if (k1 < P) {
+                int jv[16];
+                load_rows16(rpk + 16 * k1, jv);
                 float2 u[16];
 #pragma unroll
                 for (int i = 0; i < 16; ++i) u[i] = X[k1 * FAST_CA_KS + i * 16];
                 rf::rdft<16>(u);
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
-                    const int jj = rp[k1 + P * k2];
-                    if (jj >= 0 && wv) stg2(kb, (unsigned)(jj * W + w), u[k2]);
+                    const int jj = jv[k2];
+                    if (jj < nmax && wv) stg2(kb, (unsigned)(__umul24(jj, W) + w), u[k2]);
                 }
             }
         }
