@@ -10,7 +10,8 @@
 #define FAST_B_RS(P) ((P) * 17 + 1)             // per-row stride of the row exchange
 // pass CA: column exchange, twiddles by k1 role [P][16] (and, for P != 16, by a role [16][P]), row positions [P][16]
 __host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + (P == 16 ? 16 : 32) * P + 8 * P; }
-__host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + 16 * P; }
+// pass B: row exchange, twiddles by a role [16][P] and (for P != 16) by k1 role [P][16]
+__host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + (P == 16 ? 16 : 32) * P; }
 
 // ---------------------------------------------------------------------------------------------------------------------
 // pass B, rows of length W = 16 P: 16 rows per workgroup, 16 threads per row
@@ -27,8 +28,15 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     const int nr = d.nrows[f];
     const int j = g * 16 + r;
     const bool valid = j < nr;
-    float2* tw = lds + 16 * RS;
-    for (int i = threadIdx.x; i < W; i += blockDim.x) tw[i] = d.pw.tw[i];
+    // per-role twiddle tables: twa[a][k1] = W_W^{a k1} (FA phase 1), twk[k1][n2] = W_W^{k1 n2} (FB phase 1; the same
+    // table when P = 16), so a thread reads its row at one base with immediate offsets
+    float2* twa = lds + 16 * RS;
+    float2* twk = P == 16 ? twa : twa + W;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) {
+        const int ra = i / P, k = i - ra * P;          // i = P a + k1
+        twa[i] = d.pw.tw[ra * k];
+        if (P != 16) twk[i] = d.pw.tw[(i >> 4) * (i & 15)];
+    }
     float2* X = lds + r * RS;
     // wave-uniform bases of this (frame, coil) and per-thread 32-bit row offsets
     float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
@@ -64,8 +72,12 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     }
     __syncthreads();   // twiddles in LDS
     rf::rdft<P>(v);
+    {
+        const float2* tr = twa + P * a;                // tr[0] = 1
+        X[a] = v[0];
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) X[k1 * 17 + a] = (k1 == 0 || a == 0) ? v[k1] : c_mul(v[k1], tw[a * k1]);
+        for (int k1 = 1; k1 < P; ++k1) X[k1 * 17 + a] = c_mul(v[k1], tr[k1]);
+    }
     __syncthreads();
     // ---- FA phase 2: role k1 in {a, a+16} ----
     float2 u[NQ][16];
@@ -108,9 +120,10 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
                 u[q][k2] = c_conj(rv);
             }
             rf::rdft<16>(u[q]);
+            const float2* tr = twk + 16 * k1;          // tr[0] = 1
+            X[k1 * 17] = u[q][0];
 #pragma unroll
-            for (int n2 = 0; n2 < 16; ++n2)
-                X[k1 * 17 + n2] = (k1 == 0 || n2 == 0) ? u[q][n2] : c_mul(u[q][n2], tw[k1 * n2]);
+            for (int n2 = 1; n2 < 16; ++n2) X[k1 * 17 + n2] = c_mul(u[q][n2], tr[n2]);
         }
     }
     __syncthreads();
