@@ -93,29 +93,46 @@ __device__ __forceinline__ UpdRow upd_row(const SliceDesc& d, int f) {
     return r;
 }
 
-// NESTA update of pixel ub + p of the frame resolved in r0 (ub wave-uniform, p per lane) (NestaUp.cpp:696-875):
-// df = lambda (u_f - u_{f+1}) + A^H r (the temporal-TV gradient D^T u with the periodic wrap; with motion
-// compensation lambda MC^T(D^T u) read from g), y_k = x_k - df / L_mu, W_k += apk df, z_k = x_ref - W_k / L_mu,
-// x_{k+1} = tau z_k + (1 - tau) y_k.  The first iteration of a stage sets x_ref = x_k and W_k = 0.
-// Writes W_k, x_ref (stage start) and x_{k+1}; returns x_{k+1}.
+// The arithmetic of the NESTA update of one pixel (NestaUp.cpp:696-875) from its loaded operands:
+// df = lambda (u_f - u_{f+1}) + A^H r (the temporal-TV gradient D^T u with the periodic wrap, u = Ux / max(mu, |Ux|);
+// with motion compensation lambda MC^T(D^T u), given in a), y_k = x_k - df / L_mu, W_k += apk df,
+// z_k = x_ref - W_k / L_mu, x_{k+1} = tau z_k + (1 - tau) y_k.
+// a, b: x_{f-1}, x_{f+1} (MC: a = MC^T(D^T u), b unused); xr, wo: x_ref and W_k (x_k and 0 at a stage start).
+// Returns x_{k+1}; W_{k+1} in wn.
+template <bool MC>
+__device__ __forceinline__ float2 nesta_update_math(const UpdRow& r, float2 xf, float2 a, float2 b, float2 xr, float2 wo,
+                                                    float2 ares, float2& wn) {
+    float2 df;
+    if constexpr (MC) {
+        df = make_float2(r.lam * a.x + ares.x, r.lam * a.y + ares.y);
+    } else {
+        const float2 uxf = c_sub(xf, a), uxp = c_sub(b, xf);
+        // u = Ux / max(mu, |Ux|) as Ux * min(1/mu, rsqrt(|Ux|^2)) (rsqrt(0) = inf)
+        const float inf_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxf)));
+        const float inp_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxp)));
+        const float2 uf = c_scale(uxf, inf_), up = c_scale(uxp, inp_);
+        df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
+    }
+    const float2 yk = make_float2(xf.x - r.L1 * df.x, xf.y - r.L1 * df.y);
+    wn = make_float2(wo.x + r.apk * df.x, wo.y + r.apk * df.y);
+    const float2 z = make_float2(xr.x - r.L1 * wn.x, xr.y - r.L1 * wn.y);
+    return make_float2(r.tau * z.x + (1.0f - r.tau) * yk.x, r.tau * z.y + (1.0f - r.tau) * yk.y);
+}
+
+// NESTA update of pixel ub + p of the frame resolved in r0 (ub wave-uniform, p per lane).  The first iteration of a
+// stage sets x_ref = x_k and W_k = 0.  Writes W_k, x_ref (stage start) and x_{k+1}; returns x_{k+1}.
 // MC (compile time): the table uses motion compensation, so the gradient comes from r.g.
 template <bool MC = false>
 __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, unsigned p, float2 ares) {
     UpdRow r = r0;
     r.xf += ub; r.xm += ub; r.xp += ub; r.xref_in += ub; r.xref_out += ub; r.Wk += ub; r.xo += ub;
     const float2 xf = ldg2(r.xf, p);
-    float2 df;
+    float2 a, b = make_float2(0.f, 0.f);
     if constexpr (MC) {
-        const float2 gv = ldg2(r.g + ub, p);
-        df = make_float2(r.lam * gv.x + ares.x, r.lam * gv.y + ares.y);
+        a = ldg2(r.g + ub, p);
     } else {
-        const float2 xm = ldg2(r.xm, p), xp = ldg2(r.xp, p);
-        const float2 uxf = c_sub(xf, xm), uxp = c_sub(xp, xf);
-        // u = Ux / max(mu, |Ux|) as Ux * min(1/mu, rsqrt(|Ux|^2)) (rsqrt(0) = inf)
-        const float inf_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxf)));
-        const float inp_ = fminf(r.imu, __builtin_amdgcn_rsqf(c_abs2(uxp)));
-        const float2 uf = c_scale(uxf, inf_), up = c_scale(uxp, inp_);
-        df = make_float2(r.lam * (uf.x - up.x) + ares.x, r.lam * (uf.y - up.y) + ares.y);
+        a = ldg2(r.xm, p);
+        b = ldg2(r.xp, p);
     }
     float2 xr, wo;
     if (r.sstart) {
@@ -126,11 +143,9 @@ __device__ __forceinline__ float2 nesta_update_at(const UpdRow& r0, long ub, uns
         xr = ldg2(r.xref_in, p);
         wo = ldg2(r.Wk, p);
     }
-    const float2 yk = make_float2(xf.x - r.L1 * df.x, xf.y - r.L1 * df.y);
-    const float2 wn = make_float2(wo.x + r.apk * df.x, wo.y + r.apk * df.y);
+    float2 wn;
+    const float2 xn = nesta_update_math<MC>(r, xf, a, b, xr, wo, ares, wn);
     stg2(r.Wk, p, wn);
-    const float2 z = make_float2(xr.x - r.L1 * wn.x, xr.y - r.L1 * wn.y);
-    const float2 xn = make_float2(r.tau * z.x + (1.0f - r.tau) * yk.x, r.tau * z.y + (1.0f - r.tau) * yk.y);
     stg2(r.xo, p, xn);
     return xn;
 }

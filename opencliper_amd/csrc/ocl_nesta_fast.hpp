@@ -147,6 +147,54 @@ __device__ __forceinline__ void load_rows16(const int* src, int* dst) {
     }
 }
 
+// NESTA update of the P pixels h = t + 16 n1 of this thread's column (lane offset p; wv: the column exists).  The
+// loads of four rows are issued together, unconditionally (p is clamped to a valid column), and only the stores are
+// predicated, so each group of four waits on memory once.  SST: the first iteration of a stage.
+template <int P, bool MC, bool SST>
+__device__ __forceinline__ void ca_update(const UpdRow& r, int W, unsigned p, bool wv, const float2* acc, float sc,
+                                          float2* xn) {
+    constexpr int CH = 4;
+#pragma unroll
+    for (int n0 = 0; n0 < P; n0 += CH) {
+        float2 xf[CH], a[CH], b[CH], xr[CH], wo[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (n0 + j >= P) break;
+            const long ub = (long)16 * (n0 + j) * W;
+            xf[j] = ldg2(r.xf + ub, p);
+            if constexpr (MC) {
+                a[j] = ldg2(r.g + ub, p);
+                b[j] = make_float2(0.f, 0.f);
+            } else {
+                a[j] = ldg2(r.xm + ub, p);
+                b[j] = ldg2(r.xp + ub, p);
+            }
+            if constexpr (!SST) {
+                xr[j] = ldg2(r.xref_in + ub, p);
+                wo[j] = ldg2(r.Wk + ub, p);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int n1 = n0 + j;
+            if (n1 >= P) break;
+            const long ub = (long)16 * n1 * W;
+            if constexpr (SST) {
+                xr[j] = xf[j];
+                wo[j] = make_float2(0.f, 0.f);
+            }
+            float2 wn;
+            const float2 x1 = nesta_update_math<MC>(r, xf[j], a[j], b[j], xr[j], wo[j], c_scale(acc[n1], sc), wn);
+            xn[n1] = wv ? x1 : make_float2(0.f, 0.f);
+            if (wv) {
+                if constexpr (SST) stg2(r.xref_out + ub, p, xf[j]);
+                stg2(r.Wk + ub, p, wn);
+                stg2(r.xo + ub, p, x1);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // pass CA, columns of length H = 16 P: 16 columns per workgroup, 16 roles per column
 // ---------------------------------------------------------------------------------------------------------------------
@@ -287,11 +335,8 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
             xn[n1] = wv ? ldg2(d.x0 + (long)f * N + (long)16 * n1 * W, lane) : make_float2(0.f, 0.f);
     } else {
         const UpdRow u = upd_row(d, f);
-#pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) {
-            xn[n1] = make_float2(0.f, 0.f);
-            if (wv) xn[n1] = nesta_update_at<MC>(u, 16 * n1 * W, lane, c_scale(acc[n1], sc));
-        }
+        if (u.sstart) ca_update<P, MC, true>(u, W, lanec, wv, acc, sc, xn);
+        else ca_update<P, MC, false>(u, W, lanec, wv, acc, sc, xn);
     }
 
     // ---- forward column DFTs of S_c x_{k+1} (FA), output pruned to the sampled rows ----
