@@ -57,7 +57,12 @@ template <int N> __device__ __forceinline__ void rdft_split(float2* v) {
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) {
             const int m = (a * k1) % N;
-            t[k1 * B + a] = (m == 0) ? u[k1] : c_mul(u[k1], make_float2(kTw<N>.re[m], kTw<N>.im[m]));
+            // exact twiddles -i, -1, +i as swaps and sign changes (a multiply by a literal 0 does not fold away)
+            if (m == 0) t[k1 * B + a] = u[k1];
+            else if (4 * m == N) t[k1 * B + a] = make_float2(u[k1].y, -u[k1].x);
+            else if (2 * m == N) t[k1 * B + a] = make_float2(-u[k1].x, -u[k1].y);
+            else if (4 * m == 3 * N) t[k1 * B + a] = make_float2(-u[k1].y, u[k1].x);
+            else t[k1 * B + a] = c_mul(u[k1], make_float2(kTw<N>.re[m], kTw<N>.im[m]));
         }
     }
 #pragma unroll
