@@ -242,6 +242,9 @@ def leg_split(ctx, synth, parts, prm, steps, warmup):
     return res, {"x_split": xs, "x_fp32": x32, "slice0": host}
 
 
+N_SOLO_CHECK = 4
+
+
 def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
     """Poisson trace against the continuous-batching server.  Inputs come from a pool of resident slices; each
     request gets its own output buffer (recycled once the request is done) and 5 stages x U{10..30} iterations."""
@@ -266,26 +269,29 @@ def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
             while True:
                 now = time.perf_counter() - t0
                 for rid in [k for k, v in inflight.items() if srv.poll(k)[0] == capi.SERVE_DONE]:
-                    i, xo = inflight.pop(rid)
-                    free[i].append(xo)
+                    i, xo, own = inflight.pop(rid)
+                    if not own:
+                        free[i].append(xo)
                     done.append(rid)
                 if now >= arrivals[r]:
                     break
                 time.sleep(min(0.002, arrivals[r] - now))
             i = int(pick[r])
             y, S, M = pool[i]
-            xo = free[i].pop() if free[i] else ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
+            # the last N_SOLO_CHECK requests, compared with solo runs afterwards, keep output buffers of their own
+            own = r >= n_req - N_SOLO_CHECK
+            xo = free[i].pop() if (free[i] and not own) else ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
             prm = capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0)
             rid = srv.submit(y, S, M, xo, prm)
-            inflight[rid] = (i, xo)
+            inflight[rid] = (i, xo, own)
             meta[rid] = (i, int(iters[r]), xo)
         for rid in list(inflight):
             srv.wait(rid, 600.0)
             done.append(rid)
         stats = srv.stats()
         times = {rid: srv.times(rid) for rid in done}
-        # served result == solo result for the last requests (their buffers were not recycled afterwards)
-        last = sorted(meta)[-4:]
+        # served result == solo result for the last requests (their output buffers were never recycled)
+        last = sorted(meta)[-N_SOLO_CHECK:]
         eq = []
         for rid in last:
             i, it, xo = meta[rid]

@@ -12,6 +12,8 @@
 #include <condition_variable>
 #include <deque>
 #include <thread>
+#include <tuple>
+#include <mutex>
 
 #include "ocl_internal.hpp"
 
@@ -77,6 +79,36 @@ size_t lds_B_generic(int W, int rb) { return sizeof(float2) * ((size_t)2 * rb * 
 
 int tune_shape(ocl_ctx* c, int H, int W);
 
+// Tuned tile shapes are shared by every autotuning context of the process on the same device: the first context to
+// tune a shape publishes it and later ones adopt it, so two contexts (a server and a solo caller, say) run the same
+// kernels for a shape and give bit-identical results.  (Two correct variants, e.g. the Stockham and the four-step
+// column pass, round differently, and NESTA's iterations amplify that to ~1e-4 over long schedules.)  Shapes pinned
+// with ocl_ctx_set_tiles stay per context.
+namespace {
+std::mutex g_tuned_mu;
+std::map<std::tuple<int, int, int>, TileCfg> g_tuned;    // by (device, H, W)
+}
+
+// The tile shape for (H, W): the context's own (pinned or adopted) entry, else, when `tune`, the device's published
+// one, tuned and published first if there is none.  *out = nullptr: the heuristic tiles apply.
+int tiles_for(ocl_ctx* c, int H, int W, bool tune, const TileCfg** out) {
+    *out = nullptr;
+    auto it = c->tiles.find({H, W});
+    if (it == c->tiles.end()) {
+        if (!tune) return OCL_OK;
+        std::lock_guard<std::mutex> lk(g_tuned_mu);         // one tuning run per shape and device
+        auto g = g_tuned.find(std::make_tuple(c->device, H, W));
+        if (g == g_tuned.end()) {
+            CHK(tune_shape(c, H, W));
+            g = g_tuned.emplace(std::make_tuple(c->device, H, W), c->tiles[{H, W}]).first;
+        }
+        c->tiles[{H, W}] = g->second;
+        it = c->tiles.find({H, W});
+    }
+    *out = &it->second;
+    return OCL_OK;
+}
+
 // Step 1 of admission: plans, sampled-row lists (one small device round trip) and the exact device footprint.
 // force (tuning runs only) fixes the tile shape; otherwise the context's tuned shape for (H, W) is used when there is
 // one (tuned on first use when autotuning is on), else the heuristic of choose_tiles.
@@ -100,14 +132,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     choose_tiles(total_cols, s, &d.tile_cols, &d.rows_per_wg);
     if (!prec) {
         const TileCfg* cfg = force;
-        if (!cfg) {
-            auto it = c->tiles.find({s.H, s.W});
-            if (it == c->tiles.end() && c->autotune) {
-                CHK(tune_shape(c, s.H, s.W));
-                it = c->tiles.find({s.H, s.W});
-            }
-            if (it != c->tiles.end()) cfg = &it->second;
-        }
+        if (!cfg) CHK(tiles_for(c, s.H, s.W, c->autotune != 0, &cfg));
         if (cfg && !cfg->ca_fast) { d.fastH = 0; d.tile_cols = cfg->tile_cols; }
         // the prefetching four-step column pass unless a tuned or pinned shape asks for the direct one
         d.ca_stage = (d.fastH && (!cfg || cfg->ca_fast == 2)) ? 1 : 0;
@@ -606,7 +631,7 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
 // four-step when H = 16 P, LDS Stockham with 4, 8 or 16 columns per workgroup) and every pass-B variant (four-step
 // when W = 16 P, Stockham with 4, 8 or 16 rows per workgroup) on a synthetic batch of 16 slices of 16 frames x 8 coils
 // with a ky-line mask at acceleration 6, and keeps the fastest of each (the passes are timed separately, so the two
-// choices are independent).  Results are cached per context.
+// choices are independent).  Results are cached per context and published to the device (tiles_for).
 int tune_shape(ocl_ctx* c, int H, int W) {
     std::vector<std::pair<int, int>> caC, bC;
     if (fast_p(H) && fast_ca_lds(fast_p(H)) * sizeof(float2) <= LDS_MAX_NESTA) {
@@ -901,8 +926,9 @@ extern "C" int ocl_autotune_shape(ocl_ctx* c, int H, int W, int* ca_fast, int* t
     if (!c) return fail(OCL_E_INVALID, "null context");
     if (H < 2 || W < 2 || H > 512 || W > 1024) return fail(OCL_E_INVALID, "autotune: bad size");
     HIPCHK(hipSetDevice(c->device));
-    if (!c->tiles.count({H, W})) CHK(tune_shape(c, H, W));
-    const TileCfg& t = c->tiles[{H, W}];
+    const TileCfg* tp = nullptr;
+    CHK(tiles_for(c, H, W, true, &tp));
+    const TileCfg& t = *tp;
     if (ca_fast) *ca_fast = t.ca_fast;
     if (tile_cols) *tile_cols = t.tile_cols;
     if (b_fast) *b_fast = t.b_fast;

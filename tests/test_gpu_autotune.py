@@ -63,6 +63,23 @@ def test_autotune_picks_a_valid_shape_and_is_used(ctx, oracle):
         c2.close()
 
 
+def test_tuned_shapes_are_shared_by_contexts_on_a_device():
+    """A second context adopts the shape the first one tuned (the same kernels, so bit-identical results), while a
+    shape pinned on one context stays on that context."""
+    a, b = capi.Context(0), capi.Context(0)
+    try:
+        ta = a.autotune_shape(160, 160)
+        assert b.autotune_shape(160, 160) == ta
+        b.set_tiles(128, 128, 0, 8, 0, 8)
+        tb = b.autotune_shape(128, 128)
+        assert (tb["ca_fast"], tb["tile_cols"], tb["b_fast"], tb["rows_per_wg"]) == (0, 8, 0, 8)
+        assert tb["ms_CA"] == 0.0                          # pinned, not timed
+        assert a.autotune_shape(128, 128)["ms_CA"] > 0     # a tuned entry, not b's pinned one
+    finally:
+        a.close()
+        b.close()
+
+
 def test_set_tiles_rejects_bad_shapes(ctx):
     with pytest.raises(capi.OclError):
         ctx.set_tiles(100, 96, 1, 16, 1, 16)      # 100 is not 16 P
