@@ -630,8 +630,10 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
 // Autotuner of the FP32 passes' tile shapes.  For one (H, W) it times every admissible pass-CA variant (register
 // four-step when H = 16 P, LDS Stockham with 4, 8 or 16 columns per workgroup) and every pass-B variant (four-step
 // when W = 16 P, Stockham with 4, 8 or 16 rows per workgroup) on a synthetic batch of 16 slices of 16 frames x 8 coils
-// with a ky-line mask at acceleration 6, and keeps the fastest of each (the passes are timed separately, so the two
-// choices are independent).  Results are cached per context and published to the device (tiles_for).
+// with a ky-line mask at acceleration 6 (H / 6 + 4 sampled rows per frame, within the prefetching column pass's
+// 64-row limit up to H = 320, so both four-step column variants are really compared), and keeps the fastest of each
+// (the passes are timed separately, so the two choices are independent).  Results are cached per context and
+// published to the device (tiles_for).
 int tune_shape(ocl_ctx* c, int H, int W) {
     std::vector<std::pair<int, int>> caC, bC;
     if (fast_p(H) && fast_ca_lds(fast_p(H)) * sizeof(float2) <= LDS_MAX_NESTA) {
@@ -668,7 +670,7 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     std::vector<uint8_t> hm((size_t)F * N, 0);
     for (int f = 0; f < F; ++f)
         for (int h = 0; h < H; ++h)
-            if (h < 8 || h >= H - 8 || (h + f) % 6 == 0) std::memset(&hm[((size_t)f * H + h) * W], 1, W);
+            if (h < 2 || h >= H - 2 || (h + f) % 6 == 0) std::memset(&hm[((size_t)f * H + h) * W], 1, W);
     hipError_t e = hipMemcpy(M, hm.data(), hm.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { cleanup(); return fail(OCL_E_HIP, std::string("autotune: ") + hipGetErrorString(e)); }
     hipLaunchKernelGGL(k_fill, dim3(grid_for(F * C * N)), dim3(OCL_NT), 0, c->stream, (float2*)y, (long)F * C * N,

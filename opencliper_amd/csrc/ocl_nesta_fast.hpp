@@ -290,8 +290,16 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
                     const int jj = jv[k2];
-                    if constexpr (STAGE) u[k2] = ks[jj * 16];          // conjugated when staged; zero row at nmax
-                    else u[k2] = (jj < nmax && wv) ? c_conj(ldg2(kb, (unsigned)(jj * W + w))) : make_float2(0.f, 0.f);
+                    if constexpr (STAGE) {
+                        u[k2] = ks[jj * 16];                             // conjugated when staged; zero row at nmax
+                    } else {
+                        // every load issued (an unsampled row reads row 0, a missing column column 0) and masked by a
+                        // multiply, so none waits in a branch of its own
+                        const bool ok = jj < nmax;
+                        const float m = (ok && wv) ? 1.f : 0.f;
+                        const float2 kv = ldg2(kb, (unsigned)(ok ? jj * W + wc : wc));
+                        u[k2] = make_float2(kv.x * m, -(kv.y * m));
+                    }
                 }
                 rf::rdft<16>(u);
                 const float2* tr = twk + 16 * k1;                // tr[0] = 1
