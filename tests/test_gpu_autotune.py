@@ -85,3 +85,28 @@ def test_set_tiles_rejects_bad_shapes(ctx):
         ctx.set_tiles(100, 96, 1, 16, 1, 16)      # 100 is not 16 P
     with pytest.raises(capi.OclError):
         ctx.set_tiles(160, 96, 0, 3, 1, 16)       # not a power of two
+
+
+@pytest.mark.parametrize("H,W", [(320, 96), (288, 64)])
+def test_many_sampled_rows_take_the_direct_column_pass(oracle, H, W):
+    """More than 64 sampled rows per frame (acceleration 2) exceed the prefetching pass's staging slabs, so the
+    four-step column pass gathers the sampled rows straight from global memory (P = 20: two DFT-16 rounds per thread);
+    the result stays within the parity bar of the FP64 oracle, and both four-step settings agree bit for bit."""
+    sl = synth.make_cine(H=H, W=W, C=2, F=3, accel=2.0, n_centre=8, slice_id=57)
+    assert sl.M.any(axis=2).sum(axis=1).max() > 64
+    prm = dict(maxIntIter=2, maxIter=6, stoptest=0)
+    ref, _ = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    outs = []
+    for ca_fast in (1, 2):
+        c = capi.Context(0)
+        try:
+            c.set_tiles(H, W, ca_fast, 16, 1, 16)
+            print("direct column pass", H, W, "ca_fast", ca_fast, flush=True)
+            got, _ = run(c, sl, capi.nesta_params(**prm))
+        finally:
+            c.close()
+        err = oracle.rel_l2(got, ref)
+        print(f"rel-L2 vs FP64 {err:.2e}", flush=True)
+        assert err < 1e-4
+        outs.append(got)
+    assert np.array_equal(outs[0], outs[1])
