@@ -7,11 +7,23 @@
 // LDS footprint (float2 elements) of the fast passes
 #define FAST_CA_KS 272                          // per-k1 stride of the column exchange: 16 roles x 16 columns + pad
 #define FAST_CA_KQ 4                            // staged k-space values per thread and coil (nmax <= 64)
-#define FAST_B_RS(P) ((P) * 17 + 1)             // per-row stride of the row exchange
-// pass CA: column exchange, twiddles by k1 role [P][16] (and, for P != 16, by a role [16][P]), row positions [P][16]
-__host__ __device__ constexpr int fast_ca_lds(int P) { return P * FAST_CA_KS + (P == 16 ? 16 : 32) * P + 8 * P; }
-// pass B: row exchange, twiddles by a role [16][P] and (for P != 16) by k1 role [P][16]
-__host__ __device__ constexpr int fast_b_lds(int P) { return 16 * FAST_B_RS(P) + (P == 16 ? 16 : 32) * P; }
+// per-row stride of the row exchange (k1 stride 17 within a row): the least RS >= 17 P with RS = 16 (mod 32) float2,
+// so the 4 rows of a wavefront alternate between complementary halves of the 64 LDS banks and every b64 access of the
+// exchange takes the 2-cycle minimum
+#define FAST_B_RS(P) (17 * (P) + (((16 - 17 * (P)) % 32) + 32) % 32)
+// Per-role twiddle tables: [P][TK] by k1 role and [16][TA] by a role (one table when P = 16).  Pass B, whose 16 roles
+// of a wavefront read 16 different rows, pads the rows (TK = 17, TA = P + 1) so they start in different LDS banks;
+// pass CA, whose wavefronts read 4 rows, keeps them unpadded (TK = 16, TA = P) for 128-bit reads.
+__host__ __device__ constexpr int fast_twk_len(int P, int TK) { return TK * P; }
+__host__ __device__ constexpr int fast_twa_len(int P, int TA) { return P == 16 ? 0 : 16 * TA; }
+// pass CA: column exchange, twiddle tables, row positions [P][16] (ints)
+__host__ __device__ constexpr int fast_ca_lds(int P) {
+    return P * FAST_CA_KS + fast_twk_len(P, 16) + fast_twa_len(P, P) + 8 * P;
+}
+// pass B: row exchange, twiddle tables
+__host__ __device__ constexpr int fast_b_lds(int P) {
+    return 16 * FAST_B_RS(P) + fast_twk_len(P, 17) + fast_twa_len(P, P + 1);
+}
 
 // ---------------------------------------------------------------------------------------------------------------------
 // pass B, rows of length W = 16 P: 16 rows per workgroup, 16 threads per row
@@ -28,14 +40,16 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     const int nr = d.nrows[f];
     const int j = g * 16 + r;
     const bool valid = j < nr;
-    // per-role twiddle tables: twa[a][k1] = W_W^{a k1} (FA phase 1), twk[k1][n2] = W_W^{k1 n2} (FB phase 1; the same
-    // table when P = 16), so a thread reads its row at one base with immediate offsets
-    float2* twa = lds + 16 * RS;
-    float2* twk = P == 16 ? twa : twa + W;
+    // per-role twiddle tables: twk[k1][n2] = W_W^{k1 n2} (FB phase 1, row stride 17), twa[a][k1] = W_W^{a k1} (FA phase
+    // 1, row stride P + 1; the same table when P = 16), so a thread reads its row at one base with immediate offsets
+    float2* twk = lds + 16 * RS;
+    float2* twa = P == 16 ? twk : twk + fast_twk_len(P, 17);
     for (int i = threadIdx.x; i < W; i += blockDim.x) {
-        const int ra = i / P, k = i - ra * P;          // i = P a + k1
-        twa[i] = d.pw.tw[ra * k];
-        if (P != 16) twk[i] = d.pw.tw[(i >> 4) * (i & 15)];
+        twk[(i >> 4) * 17 + (i & 15)] = d.pw.tw[(i >> 4) * (i & 15)];
+        if (P != 16) {
+            const int ra = i / P, k = i - ra * P;      // i = P a + k1
+            twa[ra * (P + 1) + k] = d.pw.tw[ra * k];
+        }
     }
     float2* X = lds + r * RS;
     // wave-uniform bases of this (frame, coil) and per-thread 32-bit row offsets
@@ -73,7 +87,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     __syncthreads();   // twiddles in LDS
     rf::rdft<P>(v);
     {
-        const float2* tr = twa + P * a;                // tr[0] = 1
+        const float2* tr = twa + (P + 1) * a;          // tr[0] = 1
         X[a] = v[0];
 #pragma unroll
         for (int k1 = 1; k1 < P; ++k1) X[k1 * 17 + a] = c_mul(v[k1], tr[k1]);
@@ -120,7 +134,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
                 u[q][k2] = c_conj(rv);
             }
             rf::rdft<16>(u[q]);
-            const float2* tr = twk + 16 * k1;          // tr[0] = 1
+            const float2* tr = twk + 17 * k1;          // tr[0] = 1
             X[k1 * 17] = u[q][0];
 #pragma unroll
             for (int n2 = 1; n2 < 16; ++n2) X[k1 * 17 + n2] = c_mul(u[q][n2], tr[n2]);
@@ -221,8 +235,8 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
     //   twk[k1][n2] = W_H^{k1 n2} (FB phase 1), twa[a][k1] = W_H^{a k1} (FA phase 1; the same table when P = 16),
     //   rpk[k1][k2] = k-space slot of row k1 + P k2 in this frame, or nmax when the row is not sampled.
     float2* twk = lds + P * FAST_CA_KS;
-    float2* twa = P == 16 ? twk : twk + H;
-    int* rpk = (int*)(twk + (P == 16 ? H : 2 * H));
+    float2* twa = P == 16 ? twk : twk + fast_twk_len(P, 16);
+    int* rpk = (int*)(twk + fast_twk_len(P, 16) + fast_twa_len(P, P));
     // STAGE: slabs [2][nmax + 1][16] of conjugated k-space rows; slot nmax stays zero, so an unsampled row reads zero
     float2* kst = (float2*)(rpk + H);
     const int kss = (nmax + 1) * 16;                   // slab stride
