@@ -198,6 +198,7 @@ size_t slice_footprint(const SliceRes& r) {
            rnd(sizeof(int16_t) * r.rowpos.size()) + rnd(sizeof(int) * d.F) + rnd(sizeof(NState)) +
            rnd(sizeof(double) * d.wg_B) + rnd(sizeof(double) * 2 * d.wg_U) + rnd(sizeof(unsigned) * 4) +
            rnd(sizeof(double) * 3 * OCL_TRACE) + rnd(sizeof(double) * 2 * d.wg_CA) +
+           (d.fastW && !d.prec ? rnd(sizeof(uint16_t) * (size_t)d.F * d.nmax * d.fastW) : 0) +
            (d.prec ? 2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.W * d.W) +
                          2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.F * d.H * d.kp)
                    : 0);
@@ -237,6 +238,14 @@ int slice_alloc(SliceRes* r) {
     HIPCHK(hipMemcpyAsync(p, r->rowpos.data(), sizeof(int16_t) * r->rowpos.size(), hipMemcpyHostToDevice, c->stream));
     CHK(r->alloc(sizeof(int) * d.F, &p)); d.nrows = (const int*)p;
     HIPCHK(hipMemcpyAsync(p, r->nrows.data(), sizeof(int) * d.F, hipMemcpyHostToDevice, c->stream));
+    d.mbits = nullptr;
+    if (d.fastW && !d.prec) {
+        const long nw = (long)d.F * d.nmax * d.fastW;
+        CHK(r->alloc(sizeof(uint16_t) * nw, &p)); d.mbits = (const uint16_t*)p;
+        hipLaunchKernelGGL(k_mask_bits, dim3((unsigned)((nw + OCL_NT - 1) / OCL_NT)), dim3(OCL_NT), 0, c->stream, d.M,
+                           d.rows, d.nrows, d.F, d.H, d.W, d.nmax, d.fastW, (uint16_t*)p);
+        HIPCHK(hipGetLastError());
+    }
     CHK(r->alloc(sizeof(NState), &p)); d.st = (NState*)p;
     CHK(r->alloc(sizeof(double) * d.wg_B, &p)); d.partB = (double*)p;
     CHK(r->alloc(sizeof(double) * 2 * d.wg_U, &p)); d.partU = (double*)p;

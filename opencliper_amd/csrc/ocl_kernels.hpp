@@ -59,6 +59,8 @@ struct SliceDesc {
     const int16_t* rows;// [F][nmax] sampled rows of each frame
     const int16_t* rowpos; // [F][H] slot of row h in kbuf, or -1
     const int* nrows;   // [F]
+    const uint16_t* mbits; // four-step pass B: [F][nmax][P_W] mask bits of the sampled rows, bit k2 of word k1 = M at
+                           // column k1 + P_W k2 (the 16 mask values one thread of the pass needs, in one load)
     NState* st;
     double* partB;      // [wg_B]        sum |r|^2 per row workgroup
     double* partU;      // [wg_U][2]     sum |u|^2, sum Re(conj(u) Ux) per U workgroup
@@ -129,6 +131,8 @@ __global__ void k_mc_scatter(const float2* in, float2* out, const float* xn, con
 
 // kernels (ocl_nesta.hip)
 __global__ void k_rowflags(const float2* y, const uint8_t* M, int F, int C, int H, int W, uint8_t* flags);
+__global__ void k_mask_bits(const uint8_t* M, const int16_t* rows, const int* nrows, int F, int H, int W, int nmax, int P,
+                            uint16_t* out);
 // launch shape of the fused passes over one batch; PH / PW > 0 when every slice shares that fast column / row size
 struct NestaLaunch {
     const SliceDesc* dd;

@@ -29,6 +29,24 @@ __global__ void k_rowflags(const float2* __restrict__ y, const uint8_t* __restri
     if (threadIdx.x == 0) flags[fh] = (uint8_t)(any != 0);
 }
 
+// Mask bits of the sampled rows for the four-step pass B (W = 16 P): word (f, j, k1) holds, in bit k2, whether the mask
+// of frame f at column k1 + P k2 of sampled row j is set (rows j >= nrows[f] get 0).
+__global__ void k_mask_bits(const uint8_t* __restrict__ M, const int16_t* __restrict__ rows,
+                            const int* __restrict__ nrows, int F, int H, int W, int nmax, int P,
+                            uint16_t* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)F * nmax * P) return;
+    const int k1 = (int)(i % P);
+    const long fj = i / P;
+    const int j = (int)(fj % nmax), f = (int)(fj / nmax);
+    unsigned w = 0;
+    if (j < nrows[f]) {
+        const uint8_t* m = M + ((long)f * H + rows[(long)f * nmax + j]) * W;
+        for (int k2 = 0; k2 < 16; ++k2) w |= (m[k1 + P * k2] != 0 ? 1u : 0u) << k2;
+    }
+    out[i] = (uint16_t)w;
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // scalar step (NestaUp.cpp:797-889), run by the last workgroup of pass B(k) once ||r_k||^2 and the temporal-TV
 // terms of x_k are complete.  It fixes the coefficients pass CA(k) applies and advances the state to k+1.

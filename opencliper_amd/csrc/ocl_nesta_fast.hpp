@@ -57,23 +57,21 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     const unsigned kro = (unsigned)((valid ? j : 0) * W);
     const int ky = valid ? d.rows[(long)f * nmax + j] : 0;
     const float2* yfc = d.y + ((long)f * C + c) * H * W;
-    const uint8_t* mf = d.M + (long)f * H * W;
     const unsigned yro = (unsigned)(ky * W);
     double acc = 0.0;
-    // y and mask at the residual positions kx = k1 + P k2 of this thread's roles, fetched ahead of the FFT
+    // y and mask at the residual positions kx = k1 + P k2 of this thread's roles, fetched ahead of the FFT (the 16
+    // mask values of a role are the bits of one precomputed word)
     constexpr int NQ = P > 16 ? 2 : 1;
     float2 yq[NQ][16];
-    uint8_t mq[NQ][16];
+    unsigned mw[NQ];
     if (mode == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int k1 = a + 16 * q;
+            const bool ok = valid && k1 < P;
+            mw[q] = ok ? (unsigned)d.mbits[((long)f * nmax + j) * P + k1] : 0u;
 #pragma unroll
-            for (int k2 = 0; k2 < 16; ++k2) {
-                const bool ok = valid && k1 < P;
-                yq[q][k2] = ok ? ldg2(yfc + P * k2, yro + k1) : make_float2(0.f, 0.f);
-                mq[q][k2] = ok ? ldg1(mf + P * k2, yro + k1) : (uint8_t)0;
-            }
+            for (int k2 = 0; k2 < 16; ++k2) yq[q][k2] = ok ? ldg2(yfc + P * k2, yro + k1) : make_float2(0.f, 0.f);
         }
     }
     float2 v[P];
@@ -128,7 +126,7 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
                 if (valid) {
                     const float2 yv = yq[q][k2];
                     const float2 K = u[q][k2];
-                    rv = mq[q][k2] ? make_float2(K.x * s - yv.x, K.y * s - yv.y) : make_float2(-yv.x, -yv.y);
+                    rv = ((mw[q] >> k2) & 1u) ? make_float2(K.x * s - yv.x, K.y * s - yv.y) : make_float2(-yv.x, -yv.y);
                     acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
                 }
                 u[q][k2] = c_conj(rv);
