@@ -105,11 +105,18 @@ def gen_inputs(args, workers):
         else []
     if "served" in args.legs and "uneven" not in args.legs:
         jobs3 = jobs3[:12]
-    with mp.get_context("fork").Pool(max(1, min(workers, args.batch + len(jobs3)))) as pool:
-        r2 = pool.map_async(_gen_cine, range(args.batch))
-        r3 = pool.map_async(_gen_parts, jobs3)
-        r4 = pool.map_async(_gen_split, range(SPLIT_BATCH if "split" in args.legs else 0))
-        cine, parts, split = r2.get(), r3.get(), r4.get()
+    n_split = SPLIT_BATCH if "split" in args.legs else 0
+    if workers <= 1:
+        # in-process (no fork): for runs under a profiler that has initialised the GPU before the program started
+        cine = [_gen_cine(i) for i in range(args.batch)]
+        parts = [_gen_parts(j) for j in jobs3]
+        split = [_gen_split(i) for i in range(n_split)]
+    else:
+        with mp.get_context("fork").Pool(max(1, min(workers, args.batch + len(jobs3)))) as pool:
+            r2 = pool.map_async(_gen_cine, range(args.batch))
+            r3 = pool.map_async(_gen_parts, jobs3)
+            r4 = pool.map_async(_gen_split, range(n_split))
+            cine, parts, split = r2.get(), r3.get(), r4.get()
     full = synth.make_fully_sampled(H=256, W=256, C=8, F=1, slice_id=5000) if "simple" in args.legs else None
     return cine, parts, full, split
 
@@ -583,13 +590,16 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=96, help="requests submitted at once for server capacity")
     ap.add_argument("--no-ref", action="store_true", help="skip the full-schedule CPU references (rel-L2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg entirely")
+    ap.add_argument("--gen-workers", type=int, default=None,
+                    help="processes generating the synthetic inputs (default: up to 16 forked workers; 1 = in-process, "
+                         "for runs under a profiler that initialises the GPU before the program starts)")
     args = ap.parse_args()
     args.legs = set(x for x in args.legs.split(",") if x)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(slices are independent, so N GPUs would each run their own batch).", file=sys.stderr)
         sys.exit(2)
-    workers = min(16, os.cpu_count() or 1)
+    workers = args.gen_workers if args.gen_workers is not None else min(16, os.cpu_count() or 1)
     t = time.perf_counter()
     cine, parts, full, split = gen_inputs(args, workers)
     log(f"generated inputs in {time.perf_counter() - t:.1f} s")
