@@ -371,7 +371,7 @@ __device__ __forceinline__ void nesta_CA_fast(const SliceDesc& d, int local, int
 #pragma unroll
         for (int n1 = 0; n1 < P; ++n1) {
             const float2 sc1 = STAGE ? sn[n1] : ldg2(Sc + (long)16 * n1 * W, lanec);
-            v[n1] = wv ? c_mul(xn[n1], sc1) : make_float2(0.f, 0.f);
+            v[n1] = c_mul(xn[n1], sc1);      // a missing column (!wv) stays in its own lanes and is never stored
         }
         rf::rdft<P>(v);
         const float2* tr = twa + P * t;                          // tr[0] = 1
