@@ -531,11 +531,13 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
     HIPCHK(hipGetLastError());
     std::vector<double> mp;
     std::vector<NState> st(v.size());
+    std::vector<int> gatherR(v.size(), 0);     // per slice: gather radius of the adjoint (0: scatter form)
     if (t.L.mc) {
         // gather radius of pass G: floor(largest displacement) + 1 over the table, or the scatter form beyond
         // OCL_MC_GATHER_RMAX (the margin covers rounding of |nhor - c| near integers)
         int R = 1;
-        for (SliceRes* r : v) {
+        for (size_t i = 0; i < v.size(); ++i) {
+            SliceRes* r = v[i];
             unsigned bits = 0;
             HIPCHK(hipMemcpyAsync(&bits, r->d.mc_dmax, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -543,6 +545,7 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
             std::memcpy(&D, &bits, sizeof(D));
             const int Ri = (D >= 0.f && D < 1e6f) ? (int)std::floor(D + 1e-3f) + 1 : OCL_MC_GATHER_RMAX + 1;
             R = std::max(R, Ri);
+            gatherR[i] = Ri <= OCL_MC_GATHER_RMAX ? Ri : 0;
         }
         t.L.mcR = R <= OCL_MC_GATHER_RMAX ? R : 0;
     }
@@ -557,7 +560,7 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         SliceSetup& u = r->setup;
         u.maxX = (float)m0;
         u.maxUX = (float)m1;
-        if (r->has_mc) CHK(normest_mc(c, d.F, d.H, d.W, &r->mc, &u.normU, &u.ne_iters));
+        if (r->has_mc) CHK(normest_mc(c, d.F, d.H, d.W, &r->mc, &u.normU, &u.ne_iters, gatherR[i]));
         else CHK(ocl_normest(c, d.F, d.N, &u.normU, &u.ne_iters));
         const ocl_nesta_params& P = r->prm;
         u.lambda = P.lambda_i * u.maxX;

@@ -529,13 +529,16 @@ extern "C" int ocl_adjoint_motion_compensation(ocl_ctx* c, const void* in, void*
 }
 
 // The sparsity operator and its adjoint (NestaUp::operatorU / operatorUt, NestaUp.cpp:201-278); mc == nullptr is
-// the plain temporal TV.  aux: one image-sized scratch buffer.
+// the plain temporal TV.  aux: one image-sized scratch buffer.  gatherR > 0: the mesh is the identity and every
+// displacement is below gatherR pixels, so the adjoint runs as one deterministic gather (tTV^T fused in) instead of
+// tTV^T, copy-and-clear and the atomic scatter.
 struct SparsityOp {
     ocl_ctx* c;
     int F, H, W;
     const ocl_mc_args* mc;
     McGeom g;
     float2* aux;
+    int gatherR = 0;
     int U(const float2* x, float2* out) const {
         const long N = (long)H * W;
         if (!mc) return ocl_ttv(c, x, out, F, N, 1);
@@ -545,6 +548,11 @@ struct SparsityOp {
     // tTV^T into tmp, then MC^T (or a copy) into out
     int Ut(const float2* u, float2* tmp, float2* out) const {
         const long N = (long)H * W;
+        if (mc && gatherR > 0) {
+            HIPCHK(launch_mc_adj_tv_gather((const float2*)u, out, mc->xn, F, H, W, g.r0, g.nr, g.c0, g.nc, gatherR,
+                                           c->stream));
+            return OCL_OK;
+        }
         CHK(ocl_ttv(c, u, tmp, F, N, 0));
         if (!mc) return ocl_copy(c, tmp, out, (long)F * N);
         return mc_apply(c, tmp, out, F, H, W, mc, g, true);
@@ -584,14 +592,16 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, c
     while ((std::abs(e - e0) > (tol * e)) && (cnt < 100u)) {
         e0 = e;
         if ((rc = op ? op->U(x, Sx) : ocl_ttv(c, x, Sx, F, N, 1)) != OCL_OK) goto done;
-        if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_ASUM, r)) != OCL_OK) goto done;
+        // the reference tests sum |Sx| for zero; the 2-norm (accumulated in double) is zero exactly when that sum
+        // is, so one reduction serves both unless the random refill runs
+        if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
         if ((float)r[0] == 0.f) {
             const long chunks = (n + 63) / 64;
             hipLaunchKernelGGL(k_drand48_fill, dim3((unsigned)((chunks + OCL_NT - 1) / OCL_NT)), dim3(OCL_NT), 0,
                                c->stream, Sx, n, DRAND48_X0, drand_pos);
             drand_pos += n;
+            if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
         }
-        if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
         e = (float)r[0];
         if ((rc = op ? op->Ut(Sx, tmp, x) : ocl_ttv(c, Sx, x, F, N, 0)) != OCL_OK) goto done;
         if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
@@ -615,11 +625,12 @@ int mc_box_geometry(int H, int W, const ocl_mc_args* mc, int* r0, int* nr, int* 
     return OCL_OK;
 }
 
-int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters) {
+int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters, int gatherR) {
     const long n = (long)F * H * W;
     float2* aux = nullptr;
     if (hipMalloc(&aux, sizeof(float2) * n) != hipSuccess) return fail(OCL_E_NOMEM, "normest: out of device memory");
     SparsityOp sp{c, F, H, W, mc, {}, aux};
+    sp.gatherR = gatherR;
     int rc = mc_geometry(H, W, mc, &sp.g);
     if (rc == OCL_OK) rc = normest_compute(c, F, (long)H * W, normU, iters, &sp);
     hipStreamSynchronize(c->stream);

@@ -69,4 +69,5 @@ int reduce_async(ocl_ctx* c, const void* x, const void* y, long n, int op, doubl
 // motion compensation (ocl_capi.cpp): box rows r0..r0+nr-1, columns c0..c0+nc-1 of a bound box (OCL_E_INVALID when
 // it leaves the frame), and myNormest over U = tTV o MC
 int mc_box_geometry(int H, int W, const ocl_mc_args* mc, int* r0, int* nr, int* c0, int* nc);
-int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters);
+// gatherR > 0: identity mesh with displacements below gatherR (<= 4) pixels, adjoint by gathering
+int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters, int gatherR = 0);

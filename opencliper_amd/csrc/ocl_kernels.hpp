@@ -159,6 +159,9 @@ hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream);
 hipError_t launch_nesta_mc_dmax(const NestaLaunch& L, hipStream_t stream);
+// out = MC^T(tTV^T y) by gathering, identity mesh, box rows r0..r0+nr-1 x columns c0..c0+nc-1, displacements < R <= 4
+hipError_t launch_mc_adj_tv_gather(const float2* y, float2* out, const float* xn, int F, int H, int W, int r0, int nr,
+                                   int c0, int nc, int R, hipStream_t stream);
 void nesta_kernel_attributes(int lds_bytes);
 // split-precision passes (ocl_nesta_mfma.hip)
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);

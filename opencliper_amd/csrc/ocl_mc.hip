@@ -198,29 +198,19 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_dmax(const SliceDesc* __res
     if (threadIdx.x == 0) atomicMax(d.mc_dmax, __float_as_uint((float)m));   // non-negative floats order as uints
 }
 
-// G, gather form (deterministic): workgroup = one OCL_MC_TR x OCL_MC_TC tile of one frame.  The box pixels of the
-// tile and its R-halo are staged in LDS with their sample positions and D^T u(mcz); each target pixel then starts
-// from D^T u outside the box (0 inside: copy + initZeroRect of the adjoint) and adds, in row-major order of the
-// sources (the order the adjoint's scatter visits them), the bilinear weight of each source's footprint on it times
-// the source's D^T u.
-template <int R>
-__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __restrict__ descs,
-                                                             const int* __restrict__ off, int ns) {
+namespace {
+// Adjoint motion compensation of one OCL_MC_TR x OCL_MC_TC tile of frame f by gathering (deterministic), identity mesh
+// (box pixel (r, c) writes to (r, c)): the box pixels of the tile and its R-halo are staged in LDS with their sample
+// positions and source values val(f, p); each target pixel then starts from val outside the box (0 inside: copy +
+// initZeroRect of the adjoint) and adds, in row-major order of the sources (the order the adjoint's scatter visits
+// them), the bilinear weight of each source's footprint on it times the source's value.  Sources must move by less
+// than R pixels.  outf: the output frame.
+template <int R, class Val>
+__device__ __forceinline__ void mc_gather_tile(const float* __restrict__ xn, int F, int H, int W, int f, int r0,
+                                               int c0, int br0, int br1, int bc0, int bc1, const Val& val,
+                                               float2* __restrict__ outf, float2* spos, float2* sval) {
     constexpr int SR = OCL_MC_TR + 2 * R, SC = OCL_MC_TC + 2 * R;
-    __shared__ float2 spos[SR * SC];   // (nhor, nver) of a source, nhor = NaN if not a source
-    __shared__ float2 sval[SR * SC];   // D^T u of a source
-    const int si = find_slice(off, ns, blockIdx.x);
-    const SliceDesc& d = descs[si];
-    if (!d.st->it_pending) return;
-    const float mu = (float)d.st->it_mu;
-    const long N = d.N;
-    const int ntc = (d.W + OCL_MC_TC - 1) / OCL_MC_TC, ntr = (d.H + OCL_MC_TR - 1) / OCL_MC_TR;
-    const int local = blockIdx.x - off[si];
-    // frames fastest: consecutive workgroups share a tile across adjacent frames, whose z rows (f-1, f, f+1) then
-    // meet in L2
-    const int tt = local / d.F, f = local - tt * d.F;
-    const int r0 = (tt / ntc) * OCL_MC_TR, c0 = (tt % ntc) * OCL_MC_TC;
-    const int br0 = d.mc_r0, br1 = d.mc_r0 + d.mc_nr, bc0 = d.mc_c0, bc1 = d.mc_c0 + d.mc_nc;
+    const long N = (long)H * W;
     const bool near = r0 - R < br1 && r0 + OCL_MC_TR + R > br0 && c0 - R < bc1 && c0 + OCL_MC_TC + R > bc0;
     if (near) {
         constexpr int NS = (SR * SC + OCL_NT - 1) / OCL_NT;   // staged cells per thread
@@ -228,19 +218,19 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
         for (int k = 0; k < NS; ++k) {
             const int e = threadIdx.x + k * OCL_NT;
             const int pr = r0 - R + e / SC, pc = c0 - R + e % SC;
-            float2 pos = make_float2(__int_as_float(0x7fc00000), 0.f), val = make_float2(0.f, 0.f);
+            float2 pos = make_float2(__int_as_float(0x7fc00000), 0.f), v = make_float2(0.f, 0.f);
             if (e < SR * SC && pr >= br0 && pr < br1 && pc >= bc0 && pc < bc1) {
-                const long pp = (long)pr * d.W + pc;
+                const long pp = (long)pr * W + pc;
                 float nhor, nver;
                 int ph, pv;
-                if (mc_position(d.mc_xn, d.F, d.H, d.W, N, f, pp, &nhor, &nver, &ph, &pv)) {
+                if (mc_position(xn, F, H, W, N, f, pp, &nhor, &nver, &ph, &pv)) {
                     pos = make_float2(nhor, nver);
-                    val = mc_tv_grad(d.mcz, d.F, N, f, pp, mu);
+                    v = val(f, pp);
                 }
             }
             if (e < SR * SC) {
                 spos[e] = pos;
-                sval[e] = val;
+                sval[e] = v;
             }
         }
         __syncthreads();
@@ -250,10 +240,10 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
         const int e = threadIdx.x + k * OCL_NT;
         const int lr = e / OCL_MC_TC, lc = e % OCL_MC_TC;
         const int r = r0 + lr, c = c0 + lc;
-        if (r >= d.H || c >= d.W) continue;
-        const long p = (long)r * d.W + c;
+        if (r >= H || c >= W) continue;
+        const long p = (long)r * W + c;
         const bool inbox = r >= br0 && r < br1 && c >= bc0 && c < bc1;
-        float2 g = inbox ? make_float2(0.f, 0.f) : mc_tv_grad(d.mcz, d.F, N, f, p, mu);
+        float2 g = inbox ? make_float2(0.f, 0.f) : val(f, p);
         if (near) {
 #pragma unroll
             for (int dr = -R; dr <= R; ++dr)
@@ -272,8 +262,66 @@ __global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __
                     g.y += t.y * hw * vw;
                 }
         }
-        d.mcg[(long)f * N + p] = g;
+        outf[p] = g;
     }
+}
+}  // namespace
+
+// G, gather form (deterministic): workgroup = one tile of one frame, mcg = MC^T(D^T u(mcz)).
+template <int R>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_mc_ggather(const SliceDesc* __restrict__ descs,
+                                                             const int* __restrict__ off, int ns) {
+    constexpr int SR = OCL_MC_TR + 2 * R, SC = OCL_MC_TC + 2 * R;
+    __shared__ float2 spos[SR * SC];   // (nhor, nver) of a source, nhor = NaN if not a source
+    __shared__ float2 sval[SR * SC];   // D^T u of a source
+    const int si = find_slice(off, ns, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (!d.st->it_pending) return;
+    const float mu = (float)d.st->it_mu;
+    const long N = d.N;
+    const int ntc = (d.W + OCL_MC_TC - 1) / OCL_MC_TC;
+    const int local = blockIdx.x - off[si];
+    // frames fastest: consecutive workgroups share a tile across adjacent frames, whose z rows (f-1, f, f+1) then
+    // meet in L2
+    const int tt = local / d.F, f = local - tt * d.F;
+    const int r0 = (tt / ntc) * OCL_MC_TR, c0 = (tt % ntc) * OCL_MC_TC;
+    const float2* z = d.mcz;
+    const int F = d.F;
+    auto val = [=](int ff, long p) { return mc_tv_grad(z, F, N, ff, p, mu); };
+    mc_gather_tile<R>(d.mc_xn, d.F, d.H, d.W, f, r0, c0, d.mc_r0, d.mc_r0 + d.mc_nr, d.mc_c0, d.mc_c0 + d.mc_nc,
+                      val, d.mcg + (long)f * N, spos, sval);
+}
+
+// out = MC^T(tTV^T y) for the identity mesh, by gathering: the adjoint sparsity operator of myNormest with motion
+// compensation (NestaUp.cpp:238-278), with the temporal-TV adjoint (y_{f+1} - y_f, periodic) evaluated at each
+// source.  Workgroup = one tile of one frame, frames fastest.
+template <int R>
+__global__ __launch_bounds__(OCL_NT) void k_mc_adj_tv_gather(const float2* __restrict__ y, float2* __restrict__ out,
+                                                             const float* __restrict__ xn, int F, int H, int W,
+                                                             int br0, int br1, int bc0, int bc1) {
+    constexpr int SR = OCL_MC_TR + 2 * R, SC = OCL_MC_TC + 2 * R;
+    __shared__ float2 spos[SR * SC];
+    __shared__ float2 sval[SR * SC];
+    const long N = (long)H * W;
+    const int ntc = (W + OCL_MC_TC - 1) / OCL_MC_TC;
+    const int tt = blockIdx.x / F, f = blockIdx.x - tt * F;
+    const int r0 = (tt / ntc) * OCL_MC_TR, c0 = (tt % ntc) * OCL_MC_TC;
+    auto val = [=](int ff, long p) {
+        const int fp = ff == F - 1 ? 0 : ff + 1;
+        return c_sub(y[(long)fp * N + p], y[(long)ff * N + p]);
+    };
+    mc_gather_tile<R>(xn, F, H, W, f, r0, c0, br0, br1, bc0, bc1, val, out + (long)f * N, spos, sval);
+}
+
+hipError_t launch_mc_adj_tv_gather(const float2* y, float2* out, const float* xn, int F, int H, int W, int r0, int nr,
+                                   int c0, int nc, int R, hipStream_t stream) {
+    const int tiles = ((H + OCL_MC_TR - 1) / OCL_MC_TR) * ((W + OCL_MC_TC - 1) / OCL_MC_TC);
+    void (*k)(const float2*, float2*, const float*, int, int, int, int, int, int, int) =
+        R == 1 ? k_mc_adj_tv_gather<1> : R == 2 ? k_mc_adj_tv_gather<2> : R == 3 ? k_mc_adj_tv_gather<3>
+                                                                                 : k_mc_adj_tv_gather<4>;
+    hipLaunchKernelGGL(k, dim3((unsigned)(tiles * F)), dim3(OCL_NT), 0, stream, y, out, xn, F, H, W, r0, r0 + nr, c0,
+                       c0 + nc);
+    return hipGetLastError();
 }
 
 // G, scatter form (displacements beyond the gather radius), first half: mcg = D^T u(mcz) outside the box, 0 inside
