@@ -530,8 +530,8 @@ extern "C" int ocl_adjoint_motion_compensation(ocl_ctx* c, const void* in, void*
 
 // The sparsity operator and its adjoint (NestaUp::operatorU / operatorUt, NestaUp.cpp:201-278); mc == nullptr is
 // the plain temporal TV.  aux: one image-sized scratch buffer.  gatherR > 0: the mesh is the identity and every
-// displacement is below gatherR pixels, so the adjoint runs as one deterministic gather (tTV^T fused in) instead of
-// tTV^T, copy-and-clear and the atomic scatter.
+// displacement is below gatherR pixels, so the operator runs as one pass (MC and tTV fused) and the adjoint as one
+// deterministic gather (tTV^T fused in) instead of tTV^T, copy-and-clear and the atomic scatter.
 struct SparsityOp {
     ocl_ctx* c;
     int F, H, W;
@@ -542,6 +542,10 @@ struct SparsityOp {
     int U(const float2* x, float2* out) const {
         const long N = (long)H * W;
         if (!mc) return ocl_ttv(c, x, out, F, N, 1);
+        if (gatherR > 0) {
+            HIPCHK(launch_mc_tv_fwd(x, out, mc->xn, F, H, W, g.r0, g.nr, g.c0, g.nc, c->stream));
+            return OCL_OK;
+        }
         CHK(mc_apply(c, x, aux, F, H, W, mc, g, false));
         return ocl_ttv(c, aux, out, F, N, 1);
     }
@@ -564,57 +568,104 @@ struct SparsityOp {
 // ---------------------------------------------------------------------------------------------------------------------
 static const unsigned long long DRAND48_X0 = 0x1234ABCD330EULL;
 
-// op: the sparsity operator with motion compensation (null: plain temporal TV over F frames of N pixels)
+// op: the sparsity operator with motion compensation (null: plain temporal TV over F frames of N pixels).
+// The first power iteration runs host-driven (U of the constant start vector is zero for the temporal TV, and the
+// reference then refills Sx with drand48 values).  Later iterations run in batches of NE_BATCH with the norms and the
+// normalisation kept on the device: one host round trip per batch, after which the host replays the stop test on the
+// recorded norms and takes the value at the iteration where the sequential loop stops (the batch's later iterations
+// only change the discarded vector).  An exact zero inside a batch (where the sequential loop would refill) restarts
+// the whole estimate host-driven, so the result is always that of the sequential loop.
 static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, const SparsityOp* op = nullptr) {
+    constexpr int NE_BATCH = 8;
     const long n = (long)F * N;
-    float2 *x = nullptr, *Sx = nullptr;
-    HIPCHK(hipMalloc(&x, sizeof(float2) * n));
-    if (hipMalloc(&Sx, sizeof(float2) * n) != hipSuccess) {
-        hipFree(x);
+    float2 *x = nullptr, *Sx = nullptr, *tmp = nullptr;
+    double* dn = nullptr;                                   // [NE_BATCH + 1][2] norms of a batch, then ||x||
+    auto release = [&]() {
+        hipStreamSynchronize(c->stream);
+        if (x) hipFree(x);
+        if (Sx) hipFree(Sx);
+        if (tmp) hipFree(tmp);
+        if (dn) hipFree(dn);
+    };
+    if (hipMalloc(&x, sizeof(float2) * n) != hipSuccess || hipMalloc(&Sx, sizeof(float2) * n) != hipSuccess ||
+        (op && hipMalloc(&tmp, sizeof(float2) * n) != hipSuccess) ||
+        hipMalloc(&dn, sizeof(double) * 2 * (NE_BATCH + 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
         return fail(OCL_E_NOMEM, "normest: out of device memory");
     }
-    float2* tmp = nullptr;
-    if (op && hipMalloc(&tmp, sizeof(float2) * n) != hipSuccess) {
-        hipFree(x);
-        hipFree(Sx);
-        return fail(OCL_E_NOMEM, "normest: out of device memory");
-    }
-    int rc = OCL_OK;
-    double r[2];
     const float tol = 1e-3f;
-    unsigned cnt = 0;
-    long drand_pos = 0;
-    float e = 0.f, e0 = 0.f;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, x, n, make_float2(1.f, 0.f));
-    if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
-    e = (float)r[0];
-    if ((rc = ocl_scal(c, x, n, 1.0f / e, 0.f)) != OCL_OK) goto done;
-    while ((std::abs(e - e0) > (tol * e)) && (cnt < 100u)) {
-        e0 = e;
-        if ((rc = op ? op->U(x, Sx) : ocl_ttv(c, x, Sx, F, N, 1)) != OCL_OK) goto done;
-        // the reference tests sum |Sx| for zero; the 2-norm (accumulated in double) is zero exactly when that sum
-        // is, so one reduction serves both unless the random refill runs
-        if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
-        if ((float)r[0] == 0.f) {
-            const long chunks = (n + 63) / 64;
-            hipLaunchKernelGGL(k_drand48_fill, dim3((unsigned)((chunks + OCL_NT - 1) / OCL_NT)), dim3(OCL_NT), 0,
-                               c->stream, Sx, n, DRAND48_X0, drand_pos);
-            drand_pos += n;
-            if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
-        }
+    auto U = [&]() { return op ? op->U(x, Sx) : ocl_ttv(c, x, Sx, F, N, 1); };
+    auto Ut = [&]() { return op ? op->Ut(Sx, tmp, x) : ocl_ttv(c, Sx, x, F, N, 0); };
+    int rc = OCL_OK;
+    for (int host_only = 0; host_only < 2; ++host_only) {
+        double r[2];
+        unsigned cnt = 0;
+        long drand_pos = 0;
+        float e = 0.f, e0 = 0.f;
+        bool restart = false;
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, x, n, make_float2(1.f, 0.f));
+        if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) break;
         e = (float)r[0];
-        if ((rc = op ? op->Ut(Sx, tmp, x) : ocl_ttv(c, Sx, x, F, N, 0)) != OCL_OK) goto done;
-        if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) goto done;
-        if ((rc = ocl_scal(c, x, n, 1.0f / (float)r[0], 0.f)) != OCL_OK) goto done;
-        ++cnt;
+        if ((rc = ocl_scal(c, x, n, 1.0f / e, 0.f)) != OCL_OK) break;
+        while ((std::abs(e - e0) > (tol * e)) && (cnt < 100u) && rc == OCL_OK) {
+            if (cnt == 0 || host_only) {
+                e0 = e;
+                if ((rc = U()) != OCL_OK) break;
+                // the reference tests sum |Sx| for zero; the 2-norm (accumulated in double) is zero exactly when
+                // that sum is, so one reduction serves both unless the random refill runs
+                if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) break;
+                if ((float)r[0] == 0.f) {
+                    const long chunks = (n + 63) / 64;
+                    hipLaunchKernelGGL(k_drand48_fill, dim3((unsigned)((chunks + OCL_NT - 1) / OCL_NT)), dim3(OCL_NT),
+                                       0, c->stream, Sx, n, DRAND48_X0, drand_pos);
+                    drand_pos += n;
+                    if ((rc = ocl_reduce(c, Sx, nullptr, n, RED_NRM2, r)) != OCL_OK) break;
+                }
+                e = (float)r[0];
+                if ((rc = Ut()) != OCL_OK) break;
+                if ((rc = ocl_reduce(c, x, nullptr, n, RED_NRM2, r)) != OCL_OK) break;
+                if ((rc = ocl_scal(c, x, n, 1.0f / (float)r[0], 0.f)) != OCL_OK) break;
+                ++cnt;
+                continue;
+            }
+            const int K = (int)std::min<unsigned>(NE_BATCH, 100u - cnt);
+            for (int j = 0; j < K && rc == OCL_OK; ++j) {
+                if ((rc = U()) != OCL_OK) break;
+                if ((rc = reduce_async(c, Sx, nullptr, n, RED_NRM2, dn + 2 * j)) != OCL_OK) break;
+                if ((rc = Ut()) != OCL_OK) break;
+                if ((rc = reduce_async(c, x, nullptr, n, RED_NRM2, dn + 2 * NE_BATCH)) != OCL_OK) break;
+                hipLaunchKernelGGL(k_scal_inv_norm, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, x, n,
+                                   (const double*)(dn + 2 * NE_BATCH));
+                if (hipGetLastError() != hipSuccess) rc = fail(OCL_E_HIP, "normest: launch failed");
+            }
+            if (rc != OCL_OK) break;
+            double h[2 * NE_BATCH];
+            if (hipMemcpyAsync(h, dn, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(OCL_E_HIP, "normest: copy failed");
+                break;
+            }
+            // replay the sequential loop on the recorded norms
+            for (int j = 0; j < K; ++j) {
+                if ((float)h[2 * j] == 0.f) {
+                    restart = true;
+                    break;
+                }
+                e0 = e;
+                e = (float)h[2 * j];
+                ++cnt;
+                if (!((std::abs(e - e0) > (tol * e)) && (cnt < 100u))) break;
+            }
+            if (restart) break;
+        }
+        if (rc != OCL_OK) break;
+        if (restart) continue;
+        *out = e;
+        *iters = (int)cnt;
+        break;
     }
-    *out = e;
-    *iters = (int)cnt;
-done:
-    hipStreamSynchronize(c->stream);
-    hipFree(x);
-    hipFree(Sx);
-    if (tmp) hipFree(tmp);
+    release();
     return rc;
 }
 

@@ -102,6 +102,7 @@ __global__ void k_rsos(const float2* in, float2* out, int F, int C, long N);
 __global__ void k_abs(const float2* in, float2* out, long n, int pow2);
 __global__ void k_copy(const float2* x, float2* y, long n);
 __global__ void k_scal(float2* x, long n, float ar, float ai);
+__global__ void k_scal_inv_norm(float2* x, long n, const double* nrm);
 __global__ void k_axpy(long n, float ar, float ai, const float2* x, float2* y);
 __global__ void k_fill(float2* x, long n, float2 v);
 __global__ void k_fill_bytes(uint8_t* x, long n, uint8_t v);
@@ -159,6 +160,9 @@ hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_mc_z(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_mc_g(const NestaLaunch& L, hipStream_t stream);
 hipError_t launch_nesta_mc_dmax(const NestaLaunch& L, hipStream_t stream);
+// out = tTV(MC(x)) in one pass, identity mesh, box rows r0..r0+nr-1 x columns c0..c0+nc-1
+hipError_t launch_mc_tv_fwd(const float2* x, float2* out, const float* xn, int F, int H, int W, int r0, int nr, int c0,
+                            int nc, hipStream_t stream);
 // out = MC^T(tTV^T y) by gathering, identity mesh, box rows r0..r0+nr-1 x columns c0..c0+nc-1, displacements < R <= 4
 hipError_t launch_mc_adj_tv_gather(const float2* y, float2* out, const float* xn, int F, int H, int W, int r0, int nr,
                                    int c0, int nc, int R, hipStream_t stream);

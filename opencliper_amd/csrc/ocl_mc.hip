@@ -313,6 +313,37 @@ __global__ __launch_bounds__(OCL_NT) void k_mc_adj_tv_gather(const float2* __res
     mc_gather_tile<R>(xn, F, H, W, f, r0, c0, br0, br1, bc0, bc1, val, out + (long)f * N, spos, sval);
 }
 
+// out = tTV(MC(x)) for the identity mesh in one pass: (MC x)_f - (MC x)_{f-1} (periodic), with MC x = x outside the
+// box and the bilinear resample inside it (the per-pixel arithmetic of k_mc_interp and k_ttv, so the values are those
+// of the two passes).  The sparsity operator of myNormest with motion compensation (NestaUp.cpp:201-237).
+__global__ void k_mc_tv_fwd(const float2* __restrict__ x, float2* __restrict__ out, const float* __restrict__ xn,
+                            int F, int H, int W, int br0, int br1, int bc0, int bc1) {
+    const long N = (long)H * W, total = (long)F * N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(i / N);
+        const long p = i - (long)f * N;
+        const int r = (int)(p / W), c = (int)(p - (long)r * W);
+        const bool inbox = r >= br0 && r < br1 && c >= bc0 && c < bc1;
+        auto mcv = [&](int ff) {
+            if (!inbox) return x[(long)ff * N + p];
+            float nhor, nver;
+            int ph, pv;
+            return mc_position(xn, F, H, W, N, ff, p, &nhor, &nver, &ph, &pv)
+                       ? mc_bilinear(x + (long)ff * N, H, W, nhor, nver, ph, pv)
+                       : make_float2(0.f, 0.f);
+        };
+        out[i] = c_sub(mcv(f), mcv(f == 0 ? F - 1 : f - 1));
+    }
+}
+
+hipError_t launch_mc_tv_fwd(const float2* x, float2* out, const float* xn, int F, int H, int W, int r0, int nr, int c0,
+                            int nc, hipStream_t stream) {
+    const long blocks = ((long)F * H * W + OCL_NT - 1) / OCL_NT;          // grid-stride beyond 8192 workgroups
+    hipLaunchKernelGGL(k_mc_tv_fwd, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(OCL_NT), 0, stream, x, out,
+                       xn, F, H, W, r0, r0 + nr, c0, c0 + nc);
+    return hipGetLastError();
+}
+
 hipError_t launch_mc_adj_tv_gather(const float2* y, float2* out, const float* xn, int F, int H, int W, int r0, int nr,
                                    int c0, int nc, int R, hipStream_t stream) {
     const int tiles = ((H + OCL_MC_TR - 1) / OCL_MC_TR) * ((W + OCL_MC_TC - 1) / OCL_MC_TC);

@@ -118,6 +118,12 @@ __global__ void k_scal(float2* __restrict__ x, long n, float ar, float ai) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         x[i] = c_mul(a, x[i]);
 }
+// x *= 1 / ||x|| with the norm read on the device (the factor 1.0f / (float)norm that the host computes otherwise)
+__global__ void k_scal_inv_norm(float2* __restrict__ x, long n, const double* __restrict__ nrm) {
+    const float2 a = make_float2(1.0f / (float)nrm[0], 0.f);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = c_mul(a, x[i]);
+}
 __global__ void k_axpy(long n, float ar, float ai, const float2* __restrict__ x, float2* __restrict__ y) {
     const float2 a = make_float2(ar, ai);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
