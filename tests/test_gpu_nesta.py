@@ -154,3 +154,29 @@ def test_many_frames(ctx, oracle, form):
     assert rep["iters"] == [6] * 3
     np.testing.assert_allclose(rep["trace_f"], r64["trace_f"], rtol=1e-4)
     assert err < 1e-4 if form == capi.OCL_FORM_FUSED else err < 1e-3
+
+
+@pytest.mark.parametrize("H,W,F", [(128, 160, 5), (96, 288, 4), (64, 320, 3)])
+def test_pixel_mask_within_sampled_rows(ctx, oracle, H, W, F):
+    """A sampling mask that is not constant along kx (OpenCLIPER's pixel masks; only ROWMASK masks are whole rows):
+    inside the sampled rows every pixel is kept with probability 0.7.  The fused row pass reads a thread's 16 mask
+    values (kx = k1 + P k2) as the bits of one precomputed word; W = 16 P with P = 10, 18 and 20 covers one and two
+    roles per thread.  Both forms stay within 1e-4 of the FP64 oracle."""
+    sl = synth.make_cine(H=H, W=W, C=3, F=F, accel=4.0, n_centre=8, slice_id=90 + P_of(W))
+    rng = np.random.default_rng(91)
+    M = (sl.M.astype(bool) & (rng.random(sl.M.shape) < 0.7)).astype(np.uint8)
+    assert not all((M[f].min(axis=1) == M[f].max(axis=1)).all() for f in range(F))
+    y = (sl.y * M[:, None]).astype(np.complex64)
+    prm = dict(maxIntIter=2, maxIter=8, stoptest=0)
+    x64, _ = oracle.nesta(y, sl.S, M, oracle.params(**prm), double=True)
+    for form in (capi.OCL_FORM_PLAIN, capi.OCL_FORM_FUSED):
+        print("pixel mask", H, W, F, "form", form, flush=True)
+        dy, dS, dM, x = upload(ctx, y, sl.S, M)
+        ctx.nesta(dy, dS, dM, x, capi.nesta_params(**prm), form=form)
+        err = oracle.rel_l2(x.get(), x64)
+        print(f"rel-L2 vs FP64 {err:.2e}", flush=True)
+        assert err < 1e-4
+
+
+def P_of(W):
+    return W // 16
