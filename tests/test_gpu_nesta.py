@@ -180,3 +180,19 @@ def test_pixel_mask_within_sampled_rows(ctx, oracle, H, W, F):
 
 def P_of(W):
     return W // 16
+
+
+def test_pixel_mask_split_precision(ctx, oracle):
+    """The same kind of pixel mask through the split-precision form (BF16-MFMA DFTs): within 1e-3 of FP64."""
+    H, W, F = 128, 128, 4
+    sl = synth.make_cine(H=H, W=W, C=3, F=F, accel=4.0, n_centre=8, slice_id=95)
+    rng = np.random.default_rng(96)
+    M = (sl.M.astype(bool) & (rng.random(sl.M.shape) < 0.7)).astype(np.uint8)
+    y = (sl.y * M[:, None]).astype(np.complex64)
+    prm = dict(maxIntIter=2, maxIter=8, stoptest=0)
+    x64, _ = oracle.nesta(y, sl.S, M, oracle.params(**prm), double=True)
+    dy, dS, dM, x = upload(ctx, y, sl.S, M)
+    ctx.nesta(dy, dS, dM, x, capi.nesta_params(**prm), form=capi.OCL_FORM_SPLIT)
+    err = oracle.rel_l2(x.get(), x64)
+    print(f"split form, pixel mask: rel-L2 vs FP64 {err:.2e}", flush=True)
+    assert err < 1e-3
