@@ -203,7 +203,8 @@ namespace {
 // (box pixel (r, c) writes to (r, c)): the box pixels of the tile and its R-halo are staged in LDS with their sample
 // positions and source values val(f, p); each target pixel then starts from val outside the box (0 inside: copy +
 // initZeroRect of the adjoint) and adds, in row-major order of the sources (the order the adjoint's scatter visits
-// them), the bilinear weight of each source's footprint on it times the source's value.  Sources must move by less
+// them), the bilinear weight of each source's footprint on it times the source's value.  The weights are the tent
+// form of the interpolator's (equal up to the rounding of 1 - (floor + 1 - nhor) against nhor - floor).  Sources must move by less
 // than R pixels.  outf: the output frame.
 template <int R, class Val>
 __device__ __forceinline__ void mc_gather_tile(const float* __restrict__ xn, int F, int H, int W, int f, int r0,
@@ -244,20 +245,20 @@ __device__ __forceinline__ void mc_gather_tile(const float* __restrict__ xn, int
         const long p = (long)r * W + c;
         const bool inbox = r >= br0 && r < br1 && c >= bc0 && c < bc1;
         float2 g = inbox ? make_float2(0.f, 0.f) : val(f, p);
+        const float cf = (float)c, rf = (float)r;
         if (near) {
 #pragma unroll
             for (int dr = -R; dr <= R; ++dr)
 #pragma unroll
                 for (int dc = -R; dc <= R; ++dc) {
-                    // branch-free: a non-source (NaN position) or a footprint that misses (r, c) adds 0 * t
+                    // Branch-free tent weights: the bilinear weight of target column c for a sample at nhor is
+                    // max(0, 1 - |c - nhor|) (1 - frac at floor(nhor), frac at floor(nhor) + 1, 0 elsewhere); a
+                    // non-source (NaN position) gets max(0, NaN) = 0.
                     const int cell = (lr + R + dr) * SC + (lc + R + dc);
                     const float2 pos = spos[cell];
                     const float2 t = sval[cell];
-                    const float fh = floorf(pos.x), fv = floorf(pos.y);
-                    const float a = (float)c - fh, b = (float)r - fv;
-                    const float Whh = pos.x - fh, Wvv = pos.y - fv;
-                    const float hw = a == 1.f ? Whh : (a == 0.f ? 1.0f - Whh : 0.f);
-                    const float vw = b == 1.f ? Wvv : (b == 0.f ? 1.0f - Wvv : 0.f);
+                    const float hw = fmaxf(0.f, 1.0f - fabsf(cf - pos.x));
+                    const float vw = fmaxf(0.f, 1.0f - fabsf(rf - pos.y));
                     g.x += t.x * hw * vw;                          // the adjoint scatter's association
                     g.y += t.y * hw * vw;
                 }
