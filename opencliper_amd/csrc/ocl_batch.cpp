@@ -55,6 +55,13 @@ struct SliceRes {
     }
 };
 
+// Column-pass workgroups from which one (column, row) size of a mixed-size table gets launches of its own.  Measured
+// (64 uneven slices / 20 s served trace): 512 gives served p99 203 ms, 1024 179 ms, 2048 148-152 ms with the uneven
+// batch at 133-134 recons/s for all three, 4096 132 recons/s.
+#ifndef OCL_GROUP_MIN_WG
+#define OCL_GROUP_MIN_WG 2048
+#endif
+
 // Tile shape of the generic (non four-step) passes; the four-step passes always use 16 x 16.
 void choose_tiles(long total_cols, const ocl_slice& s, int* tile_cols, int* rows_per_wg) {
     int tc = (total_cols / 16 >= 2048) ? 16 : 8;
@@ -364,12 +371,11 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
     // Launch groups.  In a mixed-size table, the slices of a (column, row) four-step size whose column pass has
-    // enough workgroups to fill the GPU on its own (OCL_GROUP_MIN_WG) run passes B and CA as one launch each with that
-    // size's specialized kernels; the other slices share one launch of the kernels that hold every size.  Those run
+    // enough workgroups to fill the GPU several times over (OCL_GROUP_MIN_WG) run passes B and CA as one launch each
+    // with that size's specialized kernels; the other slices share one launch of the kernels that hold every size.  Those run
     // at lower occupancy, where the prefetching column pass (STAGE) hides the latency the direct pass exposes, so the
     // four-step slices of a shared mixed launch all take it, whatever their shape was tuned to alone.  (Few, small
     // groups would serialize launches that each under-fill the GPU, as in a server with a handful of requests.)
-    constexpr int OCL_GROUP_MIN_WG = 1024;
     std::vector<std::vector<int>> sets;
     {
         std::map<std::pair<int, int>, std::vector<int>> by_size;
