@@ -58,6 +58,11 @@ struct SliceRes {
 // Column-pass workgroups from which one (column, row) size of a mixed-size table gets launches of its own.  Measured
 // (64 uneven slices / 20 s served trace): 512 gives served p99 203 ms, 1024 179 ms, 2048 148-152 ms with the uneven
 // batch at 133-134 recons/s for all three, 4096 132 recons/s.
+// Pixels per temporal-TV reduction workgroup of pass B (each covers OCL_U_FRAMES frames of them).  Measured on 16 x
+// (20, 8, 256^2): B+S 0.179 ms at 256, 0.171 at 512, 0.180 at 1024 per iteration.
+#ifndef OCL_U_CHUNK
+#define OCL_U_CHUNK 512
+#endif
 #ifndef OCL_GROUP_MIN_WG
 #define OCL_GROUP_MIN_WG 2048
 #endif
@@ -187,7 +192,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
     d.wg_B = prec ? (s.F * s.C * d.nrow_groups + MF_B_TASKS - 1) / MF_B_TASKS : s.F * s.C * d.nrow_groups;
     d.wg_CA = prec ? s.F * ((d.ncol_tiles + MF_CA_TILES - 1) / MF_CA_TILES) : s.F * d.ncol_tiles;
-    d.u_chunk = (int)std::min<long>(d.N, 2 * OCL_NT);       // pixels per temporal-TV workgroup
+    d.u_chunk = (int)std::min<long>(d.N, OCL_U_CHUNK);      // pixels per temporal-TV workgroup
     d.wg_U = (int)((d.N + d.u_chunk - 1) / d.u_chunk) * ((s.F + OCL_U_FRAMES - 1) / OCL_U_FRAMES);
     d.inv_sqrtN = (float)(1.0 / std::sqrt((double)d.N));
     d.y = (const float2*)s.y;
