@@ -94,3 +94,31 @@ def test_driver_motion_compensated_nesta_matches_oracle(oracle, tmp_path):
     x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
     assert oracle.rel_l2(x, x64) < 1e-3
     assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_motion_compensated_non_identity_mesh_falls_back_to_plain(oracle, tmp_path):
+    """NestaUp with argsMC asks for the fused form; a mesh that is not the identity (here its two components swapped,
+    so box pixels write to transposed positions) is outside that form, and the launch falls back to the plain form,
+    which takes any mesh: the result still matches the FP64 oracle run with the same mesh."""
+    sl = synth.make_cine(H=64, W=64, C=3, F=6, accel=3.0, n_centre=4, slice_id=46)
+    xn, mesh, bb = synth.motion_fields(sl.H, sl.W, sl.F, amp=1.2, slice_id=46)
+    mesh = np.ascontiguousarray(mesh[::-1])
+    inp, mot, outp = str(tmp_path / "in.raw"), str(tmp_path / "motion.raw"), str(tmp_path / "out.raw")
+    _write_input(inp, sl)
+    with open(mot, "wb") as fh:
+        fh.write(np.ascontiguousarray(xn, np.float32).tobytes())
+        fh.write(np.ascontiguousarray(mesh, np.int32).tobytes())
+        fh.write(np.ascontiguousarray(bb, np.uint32).tobytes())
+    args = [DRIVER_MC, inp, mot, outp, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "2", "10", "1", "7"]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    shape = (sl.F, sl.H, sl.W)
+    x0 = np.fromfile(outp + ".nomc", dtype=np.complex64).reshape(shape)
+    prm = oracle.params(maxIntIter=2, maxIter=10, stoptest=1, miniter=7)
+    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True)
+    x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
+    assert oracle.rel_l2(x, x64) < 1e-3
+    assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout
