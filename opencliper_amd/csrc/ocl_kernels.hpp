@@ -87,8 +87,12 @@ struct SliceDesc {
     unsigned* mc_dmax;      // largest displacement |xn - position| over the box (float bits), set at setup
 };
 #define OCL_MC_GATHER_RMAX 4 // pass G gathers (deterministically) when floor(max displacement) + 1 <= this
-#define OCL_MC_TR 16         // tile of the gather form of pass G: rows x columns of one frame
+#ifndef OCL_MC_TR
+#define OCL_MC_TR 16         // tile of the gather form of pass G: rows x columns of one frame (32 x 64 measured
+#endif                       // slower: 63 against 66 fused MC recons/s)
+#ifndef OCL_MC_TC
 #define OCL_MC_TC 64
+#endif
 #define OCL_MC_CHUNK 1024    // pixel-frames per workgroup of the motion-compensation passes
 
 // kernels (ocl_ops.hip)
