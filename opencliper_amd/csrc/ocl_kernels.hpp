@@ -8,7 +8,9 @@ enum { RED_NRM2 = 0, RED_DOTC = 1, RED_SUM = 2, RED_ASUM = 3, RED_MAXABS = 4 };
 
 #define OCL_MAX_MINITER 32
 #define OCL_MAX_NSTAGES 32
-#define OCL_U_FRAMES 32      // frames per temporal-TV reduction workgroup (one group covers F <= 32)
+#ifndef OCL_U_FRAMES
+#define OCL_U_FRAMES 32      // frames per temporal-TV reduction workgroup (one group covers F <= 32); B+S of
+#endif                       // 16 x (20, 8, 256^2): 0.172 ms at 32, 0.175 at 16, 0.184 at 8
 #define OCL_MAXE 32          // most owned elements per thread in pass CA: H * tile_cols <= OCL_MAXE * OCL_NT
 
 // Device-resident NESTA state of one slice (NestaUp.cpp:570-889 scalars).  Scalars are kept in double.
