@@ -31,7 +31,7 @@ typedef struct ocl_ctx ocl_ctx;
 typedef struct ocl_batch ocl_batch;
 
 /* NestaUp::LaunchParameters (include/OpenCLIPER/processes/nesta/NestaUp.hpp:67-83), same field order.
- * argsMC is not carried: the motion-compensated operator is outside this library. */
+ * argsMC is passed separately: ocl_nesta_run_mc and ocl_batch_create_mc take it as an ocl_mc_args. */
 typedef struct {
     double lambda_i, mu_f, La;
     int maxIntIter;
@@ -132,6 +132,20 @@ int ocl_reduce(ocl_ctx* ctx, const void* x, const void* y, long n, int op, doubl
 int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
 int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
                   int W, const ocl_nesta_params* prm, int form, ocl_nesta_report* report);
+
+/* No counterpart in the reference.  ocl_nesta_run (form OCL_FORM_FUSED or OCL_FORM_SPLIT) for callers that cannot
+ * allocate device memory: serving loops and harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or other
+ * allocating routine: every work buffer (slice state, sampled-row tables, launch table, normest vectors) is carved from
+ * a static device array of OCL_SCRATCH_BYTES that each GPU holds from the first ocl_ctx_create on it until the process
+ * ends.  Inputs and outputs are the caller's, as for ocl_nesta_run.  A reconstruction whose work buffers do not fit
+ * returns OCL_E_NOMEM with a message naming the limit (about 80 MiB for 20 frames x 8 coils x 256^2 at acceleration 6,
+ * including the first normest of a context).  Tile shapes are never tuned here: a context's tuned or pinned shape is
+ * used when it has one, else the heuristic.  All calls on one GPU share its array, which keeps the last call's
+ * contents: calls that overlap on one GPU (from two threads or two contexts) race and must be serialized by the
+ * caller.  The call is synchronous, like ocl_nesta_run. */
+#define OCL_SCRATCH_BYTES ((size_t)128 << 20)
+int ocl_nesta_run_noalloc(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C,
+                          int H, int W, const ocl_nesta_params* prm, int form, ocl_nesta_report* report);
 
 /* ---- motion compensation (ArgumentsMotionCompensation, include/OpenCLIPER/processes/GroupwiseRegistration.hpp:95-100;
  * MotionCompensation.cpp, AdjointMotionCompensation.cpp, src/kernels/optimizer.cl:832-930) ----

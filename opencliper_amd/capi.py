@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(_PKG), "include", "ocl_amd.h")
 OCL_FORM_PLAIN = 0
 OCL_FORM_FUSED = 1
 OCL_FORM_SPLIT = 2
+OCL_E_INVALID, OCL_E_HIP, OCL_E_UNSUPPORTED, OCL_E_NOMEM = -1, -2, -3, -4
 MAX_STAGES = 16
 TRACE = 4096
 RED = {"nrm2": 0, "dotc": 1, "sum": 2, "asum": 3, "maxabs": 4}
@@ -122,6 +123,8 @@ def lib():
         "ocl_normest": ([P, I, LG, ctypes.POINTER(D), ctypes.POINTER(I)], I),
         "ocl_nesta_run": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(NestaReport)],
                           I),
+        "ocl_nesta_run_noalloc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I,
+                                   ctypes.POINTER(NestaReport)], I),
         "ocl_motion_compensation": ([P, P, P, I, I, I, ctypes.POINTER(McArgs)], I),
         "ocl_adjoint_motion_compensation": ([P, P, P, I, I, I, ctypes.POINTER(McArgs)], I),
         "ocl_nesta_run_mc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(McArgs),
@@ -342,6 +345,16 @@ class Context:
         rep = NestaReport()
         _check(lib().ocl_nesta_run(self.handle, y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W, ctypes.byref(prm), form,
                                    ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def nesta_noalloc(self, y: DeviceArray, S: DeviceArray, M: DeviceArray, x: DeviceArray,
+                      prm: NestaParams | None = None, form=OCL_FORM_FUSED):
+        """ocl_nesta_run_noalloc: the same reconstruction with every work buffer in the device's static scratch."""
+        prm = prm or nesta_params()
+        F, C, H, W = y.shape
+        rep = NestaReport()
+        _check(lib().ocl_nesta_run_noalloc(self.handle, y.ptr, S.ptr, M.ptr, x.ptr, F, C, H, W, ctypes.byref(prm),
+                                           form, ctypes.byref(rep)))
         return rep.as_dict()
 
     # --- motion compensation ---

@@ -17,6 +17,68 @@
 
 #include "ocl_internal.hpp"
 
+// ---------------------------------------------------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point (ocl_nesta_run_noalloc).  Each device holds its own copy of the
+// array for the life of the process; its address on a device is looked up once, at the first context creation on
+// that device, and only that address is used by host code.  Calls of the allocation-free entry point on one device
+// share the array (they must not overlap); the array keeps the last call's contents, and every buffer carved from
+// it is written before it is read, as with hipMalloc'ed memory.
+// ---------------------------------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[OCL_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_ptr;                 // device -> address of that device's scratch array
+struct ScratchArena {
+    char* base = nullptr;
+    size_t used = 0;
+    bool overflow = false;                          // a request did not fit
+};
+thread_local ScratchArena* t_arena = nullptr;       // set while an allocation-free call runs on this thread
+
+char* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+}  // namespace
+
+int scratch_init_device(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return OCL_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(OCL_E_HIP, std::string("scratch address lookup: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (char*)ptr;
+    return OCL_OK;
+}
+
+bool dev_scratch_active() { return t_arena != nullptr; }
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+    if (!t_arena) return hipMalloc(p, bytes);
+    const size_t off = (t_arena->used + 255) & ~(size_t)255;
+    if (off > OCL_SCRATCH_BYTES || bytes > OCL_SCRATCH_BYTES - off) {
+        *p = nullptr;
+        t_arena->overflow = true;
+        return hipErrorOutOfMemory;
+    }
+    *p = t_arena->base + off;
+    t_arena->used = off + bytes;
+    return hipSuccess;
+}
+
+hipError_t dev_free(void* p) {
+    if (!p) return hipSuccess;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        for (const auto& kv : g_scratch_ptr)
+            if ((char*)p >= kv.second && (char*)p < kv.second + OCL_SCRATCH_BYTES) return hipSuccess;
+    }
+    return hipFree(p);
+}
+
 namespace {
 
 struct SliceSetup {
@@ -42,11 +104,11 @@ struct SliceRes {
     ~SliceRes() {
         if (!allocs.empty()) {
             hipStreamSynchronize(c->stream);
-            for (void* p : allocs) hipFree(p);
+            for (void* p : allocs) dev_free(p);
         }
     }
     int alloc(size_t n, void** p) {
-        hipError_t e = hipMalloc(p, std::max<size_t>(n, 16));
+        hipError_t e = dev_malloc(p, std::max<size_t>(n, 16));
         if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
         HIPCHK(e);
         allocs.push_back(*p);
@@ -144,7 +206,9 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     choose_tiles(total_cols, s, &d.tile_cols, &d.rows_per_wg);
     if (!prec) {
         const TileCfg* cfg = force;
-        if (!cfg) CHK(tiles_for(c, s.H, s.W, c->autotune != 0, &cfg));
+        // an allocation-free call never tunes (tuning allocates): it takes a tuned or pinned shape when the context
+        // has one, else the heuristic
+        if (!cfg) CHK(tiles_for(c, s.H, s.W, c->autotune != 0 && !dev_scratch_active(), &cfg));
         if (cfg && !cfg->ca_fast) { d.fastH = 0; d.tile_cols = cfg->tile_cols; }
         // the prefetching four-step column pass unless a tuned or pinned shape asks for the direct one
         d.ca_stage = (d.fastH && (!cfg || cfg->ca_fast == 2)) ? 1 : 0;
@@ -158,14 +222,14 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
     const size_t fb = (size_t)s.F * s.H;
     uint8_t* flags_d = nullptr;
-    HIPCHK(hipMalloc(&flags_d, fb));
+    HIPCHK(dev_malloc((void**)&flags_d, fb));
     hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F, s.C,
                        s.H, s.W, flags_d);
     std::vector<uint8_t> flags(fb);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), flags_d, fb, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(flags_d);
+    dev_free(flags_d);
     HIPCHK(e);
     r->nrows.assign(s.F, 0);
     int nmax = 1;
@@ -316,14 +380,14 @@ struct Table {
     int cap = 0;
     NestaLaunch L{};
     ~Table() {
-        if (dd) hipFree(dd);
-        if (offB) hipFree(offB);
-        if (offCA) hipFree(offCA);
-        if (offP) hipFree(offP);
-        if (offX) hipFree(offX);
-        if (offG) hipFree(offG);
-        if (gdd) hipFree(gdd);
-        if (goff) hipFree(goff);
+        if (dd) dev_free(dd);
+        if (offB) dev_free(offB);
+        if (offCA) dev_free(offCA);
+        if (offP) dev_free(offP);
+        if (offX) dev_free(offX);
+        if (offG) dev_free(offG);
+        if (gdd) dev_free(gdd);
+        if (goff) dev_free(goff);
     }
 };
 
@@ -352,20 +416,20 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     if (ns > t->cap) {
         if (t->dd) {
             hipStreamSynchronize(c->stream);
-            hipFree(t->dd); hipFree(t->offB); hipFree(t->offCA); hipFree(t->offP); hipFree(t->offX); hipFree(t->offG);
-            hipFree(t->gdd); hipFree(t->goff);
+            dev_free(t->dd); dev_free(t->offB); dev_free(t->offCA); dev_free(t->offP); dev_free(t->offX); dev_free(t->offG);
+            dev_free(t->gdd); dev_free(t->goff);
         }
         t->dd = nullptr; t->offB = t->offCA = t->offP = t->offX = t->offG = t->goff = nullptr;
         t->gdd = nullptr;
         const int cap = std::max(ns, 2 * t->cap);
-        HIPCHK(hipMalloc(&t->dd, sizeof(SliceDesc) * cap));
-        HIPCHK(hipMalloc(&t->offB, sizeof(int) * (cap + 1)));
-        HIPCHK(hipMalloc(&t->offCA, sizeof(int) * (cap + 1)));
-        HIPCHK(hipMalloc(&t->offP, sizeof(int) * (cap + 1)));
-        HIPCHK(hipMalloc(&t->offX, sizeof(int) * (cap + 1)));
-        HIPCHK(hipMalloc(&t->offG, sizeof(int) * (cap + 1)));
-        HIPCHK(hipMalloc(&t->gdd, sizeof(SliceDesc) * cap));
-        HIPCHK(hipMalloc(&t->goff, sizeof(int) * 4 * cap));
+        HIPCHK(dev_malloc((void**)&t->dd, sizeof(SliceDesc) * cap));
+        HIPCHK(dev_malloc((void**)&t->offB, sizeof(int) * (cap + 1)));
+        HIPCHK(dev_malloc((void**)&t->offCA, sizeof(int) * (cap + 1)));
+        HIPCHK(dev_malloc((void**)&t->offP, sizeof(int) * (cap + 1)));
+        HIPCHK(dev_malloc((void**)&t->offX, sizeof(int) * (cap + 1)));
+        HIPCHK(dev_malloc((void**)&t->offG, sizeof(int) * (cap + 1)));
+        HIPCHK(dev_malloc((void**)&t->gdd, sizeof(SliceDesc) * cap));
+        HIPCHK(dev_malloc((void**)&t->goff, sizeof(int) * 4 * cap));
         t->cap = cap;
     }
     std::vector<SliceDesc> hd(ns);
@@ -676,10 +740,10 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     std::vector<void*> bufs;
     auto cleanup = [&]() {
         hipStreamSynchronize(c->stream);
-        for (void* p : bufs) hipFree(p);
+        for (void* p : bufs) dev_free(p);
     };
     auto dalloc = [&](size_t n, void** p) -> int {
-        if (hipMalloc(p, n) != hipSuccess) { (void)hipGetLastError(); return fail(OCL_E_NOMEM, "autotune: out of memory"); }
+        if (dev_malloc(p, n) != hipSuccess) { (void)hipGetLastError(); return fail(OCL_E_NOMEM, "autotune: out of memory"); }
         bufs.push_back(*p);
         return OCL_OK;
     };
@@ -870,7 +934,7 @@ extern "C" int ocl_batch_destroy(ocl_batch* b) {
     hipStreamSynchronize(b->ctx->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
-    if (b->d_outs) hipFree(b->d_outs);
+    if (b->d_outs) dev_free(b->d_outs);
     delete b;
     return OCL_OK;
 }
@@ -899,7 +963,7 @@ extern "C" int ocl_batch_setup(ocl_batch* b) {
         b->h_st.resize(b->view.size());
         std::vector<float2*> outs;
         for (SliceRes* r : b->view) outs.push_back((float2*)r->in.x_out);
-        HIPCHK(hipMalloc((void**)&b->d_outs, sizeof(float2*) * outs.size()));
+        HIPCHK(dev_malloc((void**)&b->d_outs, sizeof(float2*) * outs.size()));
         HIPCHK(hipMemcpy(b->d_outs, outs.data(), sizeof(float2*) * outs.size(), hipMemcpyHostToDevice));
     }
     CHK(slices_setup(c, b->t, b->view));
@@ -1035,6 +1099,48 @@ extern "C" int ocl_batch_run(ocl_batch* b, int chunk) {
     CHK(ocl_batch_finalize(b));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return OCL_OK;
+}
+
+// NestaUp::launch (FUSED or SPLIT form) with every work buffer carved from the device's static scratch array: no
+// hipMalloc or hipFree on this path.  The same kernels and tile shapes as ocl_nesta_run, except that tile shapes are
+// never tuned here (a context's tuned or pinned shape is used when it has one, else the heuristic).
+extern "C" int ocl_nesta_run_noalloc(ocl_ctx* c, const void* y, const void* S, const uint8_t* M, void* x_out, int F,
+                                     int C, int H, int W, const ocl_nesta_params* prm, int form,
+                                     ocl_nesta_report* rep) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (!prm) return fail(OCL_E_INVALID, "NESTA: null launch parameters");
+    if (!y || !x_out || F < 1 || C < 1 || H < 1 || W < 1) return fail(OCL_E_INVALID, "NESTA: bad input/output");
+    if (!S) return fail(OCL_E_INVALID, "NESTA: non-existing SensitivityMaps");
+    if (!M) return fail(OCL_E_INVALID, "NESTA: null sampling masks");
+    if (prm->miniter < 1 || prm->miniter > OCL_MAX_MINITER) return fail(OCL_E_INVALID, "NESTA: miniter out of range");
+    if (form == OCL_FORM_PLAIN)
+        return fail(OCL_E_UNSUPPORTED, "ocl_nesta_run_noalloc: the plain form needs four coil-domain work arrays; "
+                                       "use OCL_FORM_FUSED or OCL_FORM_SPLIT");
+    if (form != OCL_FORM_FUSED && form != OCL_FORM_SPLIT) return fail(OCL_E_INVALID, "NESTA: unknown form");
+    if (t_arena) return fail(OCL_E_INVALID, "ocl_nesta_run_noalloc: nested call on one thread");
+    const std::string too_big = "ocl_nesta_run_noalloc: a reconstruction of " + std::to_string(F) + " x " +
+                                std::to_string(C) + " x " + std::to_string(H) + " x " + std::to_string(W) +
+                                " needs more than the " + std::to_string(OCL_SCRATCH_BYTES >> 20) +
+                                " MiB scratch array; use ocl_nesta_run";
+    // the four image-sized state arrays alone (x_k twice, x_ref, W_k) are a lower bound of the footprint
+    if ((double)F * H * W * sizeof(float2) * 4 > (double)OCL_SCRATCH_BYTES) return fail(OCL_E_NOMEM, too_big);
+    char* base = scratch_of(c->device);
+    if (!base) return fail(OCL_E_HIP, "ocl_nesta_run_noalloc: no scratch address for this device");
+    HIPCHK(hipSetDevice(c->device));
+    ScratchArena arena;
+    arena.base = base;
+    struct Scope {
+        explicit Scope(ScratchArena* a) { t_arena = a; }
+        ~Scope() { t_arena = nullptr; }
+    } scope(&arena);
+    ocl_slice s{y, S, M, x_out, F, C, H, W};
+    ocl_batch* b = nullptr;
+    CHK(ocl_batch_create_form(c, 1, &s, prm, form, &b));
+    int rc = ocl_batch_run(b, 16);
+    if (rc == OCL_OK && rep) rc = ocl_batch_report(b, 0, rep);
+    ocl_batch_destroy(b);
+    if (arena.overflow) return fail(OCL_E_NOMEM, too_big);
+    return rc;
 }
 
 // =====================================================================================================================

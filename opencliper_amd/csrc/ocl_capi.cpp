@@ -41,6 +41,11 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->red_part, sizeof(double) * 2 * RED_PARTS);
     if (e == hipSuccess) e = hipMalloc(&c->red_out, sizeof(double) * 16);
     if (e == hipSuccess) e = hipHostMalloc(&c->red_host, sizeof(double) * 16, hipHostMallocDefault);
+    if (e == hipSuccess && scratch_init_device(device) != OCL_OK) {
+        const std::string m = ocl_last_error();
+        ocl_ctx_destroy(c);
+        return fail(OCL_E_HIP, m);
+    }
     const void* lds_kernels[] = {(const void*)k_fft_rows, (const void*)k_fft_cols, (const void*)k_fft_rows_fast,
                                  (const void*)k_fft_cols_fast, (const void*)k_simple_cols_fast};
     if (e == hipSuccess) {
@@ -210,6 +215,13 @@ int make_plan(ocl_ctx* c, int n, FftPlan* P) {
             h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         float2* d = nullptr;
+        if (dev_scratch_active()) {
+            // an allocation-free call: the table lives in the scratch array for this call only and is not cached
+            HIPCHK(dev_malloc((void**)&d, sizeof(float2) * n));
+            HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+            P->tw = d;
+            return OCL_OK;
+        }
         HIPCHK(hipMalloc(&d, sizeof(float2) * n));
         HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
         it = c->tw.emplace(n, d).first;
@@ -582,14 +594,15 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, c
     double* dn = nullptr;                                   // [NE_BATCH + 1][2] norms of a batch, then ||x||
     auto release = [&]() {
         hipStreamSynchronize(c->stream);
-        if (x) hipFree(x);
-        if (Sx) hipFree(Sx);
-        if (tmp) hipFree(tmp);
-        if (dn) hipFree(dn);
+        if (x) dev_free(x);
+        if (Sx) dev_free(Sx);
+        if (tmp) dev_free(tmp);
+        if (dn) dev_free(dn);
     };
-    if (hipMalloc(&x, sizeof(float2) * n) != hipSuccess || hipMalloc(&Sx, sizeof(float2) * n) != hipSuccess ||
-        (op && hipMalloc(&tmp, sizeof(float2) * n) != hipSuccess) ||
-        hipMalloc(&dn, sizeof(double) * 2 * (NE_BATCH + 1)) != hipSuccess) {
+    if (dev_malloc((void**)&x, sizeof(float2) * n) != hipSuccess ||
+        dev_malloc((void**)&Sx, sizeof(float2) * n) != hipSuccess ||
+        (op && dev_malloc((void**)&tmp, sizeof(float2) * n) != hipSuccess) ||
+        dev_malloc((void**)&dn, sizeof(double) * 2 * (NE_BATCH + 1)) != hipSuccess) {
         (void)hipGetLastError();
         release();
         return fail(OCL_E_NOMEM, "normest: out of device memory");
@@ -679,13 +692,14 @@ int mc_box_geometry(int H, int W, const ocl_mc_args* mc, int* r0, int* nr, int* 
 int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* normU, int* iters, int gatherR) {
     const long n = (long)F * H * W;
     float2* aux = nullptr;
-    if (hipMalloc(&aux, sizeof(float2) * n) != hipSuccess) return fail(OCL_E_NOMEM, "normest: out of device memory");
+    if (dev_malloc((void**)&aux, sizeof(float2) * n) != hipSuccess)
+        return fail(OCL_E_NOMEM, "normest: out of device memory");
     SparsityOp sp{c, F, H, W, mc, {}, aux};
     sp.gatherR = gatherR;
     int rc = mc_geometry(H, W, mc, &sp.g);
     if (rc == OCL_OK) rc = normest_compute(c, F, (long)H * W, normU, iters, &sp);
     hipStreamSynchronize(c->stream);
-    hipFree(aux);
+    dev_free(aux);
     return rc;
 }
 
@@ -722,7 +736,7 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     std::vector<void*> bufs;
     auto alloc = [&](long n) -> float2* {
         void* p = nullptr;
-        if (hipMalloc(&p, sizeof(float2) * std::max(1L, n)) != hipSuccess) return nullptr;
+        if (dev_malloc(&p, sizeof(float2) * std::max(1L, n)) != hipSuccess) return nullptr;
         bufs.push_back(p);
         return (float2*)p;
     };
@@ -731,7 +745,7 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
         hipStream_t s;
         ~Cleanup() {
             hipStreamSynchronize(s);
-            for (void* p : *b) hipFree(p);
+            for (void* p : *b) dev_free(p);
         }
     } cleanup{&bufs, c->stream};
     float2 *yS = alloc(nK), *aux = alloc(nK), *Res = alloc(nK), *AuxRes = alloc(nK);

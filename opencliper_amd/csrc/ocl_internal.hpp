@@ -61,6 +61,15 @@ static const int RED_PARTS = 1024;
 static const size_t LDS_MAX = 64 * 1024;          // per-workgroup dynamic LDS budget of the standalone FFT launches
 static const size_t LDS_MAX_NESTA = 160 * 1024;   // per-workgroup LDS of the fused NESTA passes (gfx950: 160 KB)
 
+// Device memory of the library's own work buffers (slice state, launch tables, normest vectors, FFT twiddles).
+// Every such allocation goes through dev_malloc / dev_free.  While an allocation-free call runs on the calling
+// thread (ocl_nesta_run_noalloc), dev_malloc carves from that device's static scratch array instead of calling
+// hipMalloc, fails with hipErrorOutOfMemory when the array is full, and dev_free of scratch memory does nothing.
+hipError_t dev_malloc(void** p, size_t bytes);
+hipError_t dev_free(void* p);
+bool dev_scratch_active();                 // an allocation-free call is running on this thread
+int scratch_init_device(int device);       // records the device's scratch address (ocl_ctx_create, device current)
+
 int grid_for(long n);
 int make_plan(ocl_ctx* c, int n, FftPlan* P);
 int fast_p(int n);

@@ -41,3 +41,25 @@ def test_null_arguments_are_rejected_without_gpu():
 def test_context_creation_fails_loudly_without_gpu():
     with pytest.raises(capi.OclError):
         capi.Context(0)
+
+
+def test_work_buffers_allocate_only_through_dev_malloc():
+    """ocl_nesta_run_noalloc relies on every work-buffer allocation of the library going through dev_malloc (which
+    carves from the static scratch during such a call): hipMalloc may appear only in dev_malloc itself, in context
+    creation, in ocl_malloc and in the cached FFT twiddle tables (which that call bypasses)."""
+    import re
+    csrc = os.path.join(os.path.dirname(capi.LIB_PATH), "csrc")
+    allowed = {("ocl_batch.cpp", "dev_malloc"), ("ocl_capi.cpp", "ocl_ctx_create"), ("ocl_capi.cpp", "ocl_malloc"),
+               ("ocl_capi.cpp", "make_plan")}
+    found = set()
+    for fn in sorted(os.listdir(csrc)):
+        if not fn.endswith((".cpp", ".hip", ".hpp")):
+            continue
+        func = None
+        for line in open(os.path.join(csrc, fn)):
+            m = re.match(r"^(?:extern \"C\" )?(?:static )?[\w:<>*\s]+?\b(\w+)\(", line)
+            if m and not line.startswith((" ", "\t", "#", "//")):
+                func = m.group(1)
+            if re.search(r"\bhipMalloc\w*\(", line) and not line.lstrip().startswith("//"):
+                found.add((fn, func))
+    assert found <= allowed, found - allowed
