@@ -190,6 +190,9 @@ int ocl_batch_run(ocl_batch* b, int chunk);           /* setup + iterate in chun
 /* enqueue n_iters iterations with HIP events around each launch; average ms of pass B and pass CA and the total
  * span (synchronises).  Used to price the fused iteration against the HBM roofline. */
 int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* ms_CA, double* ms_total);
+/* launch shape of a set-up batch (no reference counterpart): the size groups its launch table is split into (0: passes
+ * B and CA run as one launch each over the whole table) and the workgroups of passes B and CA per iteration */
+int ocl_batch_info(ocl_batch* b, int* launch_groups, long* wg_B, long* wg_CA);
 int ocl_batch_destroy(ocl_batch* b);
 
 /* ---- tile-shape autotuner of the FP32 fused passes (no reference counterpart: clFFT bakes its own plans,
@@ -219,6 +222,7 @@ typedef struct {
     size_t used_bytes, peak_used_bytes, budget_bytes;
     long slice_iterations, launches;
     double busy_s, elapsed_s;
+    long budget_waits;   /* requests that reached the head of the queue and waited there for HBM budget */
 } ocl_server_stats;
 int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_iters, ocl_server** out);
 /* report may be NULL; when given it is filled when the request completes */

@@ -78,7 +78,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_long), ("failed", ctypes.c_long), ("queued", ctypes.c_long),
                 ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
                 ("budget_bytes", ctypes.c_size_t), ("slice_iterations", ctypes.c_long), ("launches", ctypes.c_long),
-                ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double)]
+                ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("budget_waits", ctypes.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -142,6 +142,7 @@ def lib():
         "ocl_batch_replay": ([P], I), "ocl_batch_active": ([P, ctypes.POINTER(I)], I),
         "ocl_batch_finalize": ([P], I), "ocl_batch_report": ([P, I, ctypes.POINTER(NestaReport)], I),
         "ocl_batch_run": ([P, I], I), "ocl_batch_destroy": ([P], I),
+        "ocl_batch_info": ([P, ctypes.POINTER(I), ctypes.POINTER(LG), ctypes.POINTER(LG)], I),
         "ocl_batch_time": ([P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)], I),
         "ocl_timer_begin": ([P, PP], I), "ocl_timer_end": ([P, P, ctypes.POINTER(D)], I),
         "ocl_add_noise_masked": ([P, P, P, I, I, LG, F32, ctypes.c_ulonglong], I),
@@ -438,6 +439,11 @@ class Batch:
         rep = NestaReport()
         _check(lib().ocl_batch_report(self.handle, i, ctypes.byref(rep)))
         return rep.as_dict()
+
+    def info(self):
+        g, b, c = ctypes.c_int(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().ocl_batch_info(self.handle, ctypes.byref(g), ctypes.byref(b), ctypes.byref(c)))
+        return {"launch_groups": g.value, "wg_B": b.value, "wg_CA": c.value}
 
     def close(self):
         if self.handle:

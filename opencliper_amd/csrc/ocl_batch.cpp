@@ -432,7 +432,10 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         HIPCHK(dev_malloc((void**)&t->goff, sizeof(int) * 4 * cap));
         t->cap = cap;
     }
+    // the table's own copy of every descriptor: a table may change a slice's column-pass variant (below) without
+    // touching the slice, so a slice runs the same kernels in every table it is not grouped differently in
     std::vector<SliceDesc> hd(ns);
+    for (int i = 0; i < ns; ++i) hd[i] = v[i]->d;
     std::vector<int> oB(ns + 1, 0), oC(ns + 1, 0), oP(ns + 1, 0), oX(ns + 1, 0), oG(ns + 1, 0);
     NestaLaunch& L = t->L;
     L = NestaLaunch{};
@@ -448,12 +451,12 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     std::vector<std::vector<int>> sets;
     {
         std::map<std::pair<int, int>, std::vector<int>> by_size;
-        for (int i = 0; i < ns; ++i) by_size[{v[i]->d.fastH, v[i]->d.fastW}].push_back(i);
+        for (int i = 0; i < ns; ++i) by_size[{hd[i].fastH, hd[i].fastW}].push_back(i);
         std::vector<int> rest;
         if (by_size.size() > 1 && !L.prec) {
             for (const auto& kv : by_size) {
                 long wg = 0;
-                for (int i : kv.second) wg += v[i]->d.wg_CA;
+                for (int i : kv.second) wg += hd[i].wg_CA;
                 if (wg >= OCL_GROUP_MIN_WG) sets.push_back(kv.second);
                 else rest.insert(rest.end(), kv.second.begin(), kv.second.end());
             }
@@ -462,10 +465,10 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         }
         std::sort(rest.begin(), rest.end());
         bool restMixedH = false;
-        for (size_t j = 1; j < rest.size(); ++j) restMixedH |= v[rest[j]]->d.fastH != v[rest[0]]->d.fastH;
+        for (size_t j = 1; j < rest.size(); ++j) restMixedH |= hd[rest[j]].fastH != hd[rest[0]].fastH;
         if (restMixedH && !L.prec)
             for (int i : rest) {
-                SliceDesc& d = v[i]->d;
+                SliceDesc& d = hd[i];
                 if (!d.fastH || d.ca_stage || d.nmax > 16 * FAST_CA_KQ) continue;
                 d.ca_stage = 1;
                 if (lds_CA(d) > LDS_MAX_NESTA) d.ca_stage = 0;
@@ -474,10 +477,10 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     }
     const bool grouped = sets.size() > 1;
     L.kj = 1;
-    L.stage = ns ? v[0]->d.ca_stage : 0;
+    L.stage = ns ? hd[0].ca_stage : 0;
     L.mc = ns && v[0]->d.mc_xn ? 1 : 0;
     for (int i = 0; i < ns; ++i) {
-        const SliceDesc& d = v[i]->d;
+        const SliceDesc& d = hd[i];
         if (d.prec != L.prec) return fail(OCL_E_INVALID, "batch: slices of one table must share the precision form");
         if ((d.mc_xn ? 1 : 0) != L.mc)
             return fail(OCL_E_INVALID, "batch: motion compensation must be on for every slice of a table or none");
@@ -488,7 +491,6 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             return fail(OCL_E_UNSUPPORTED, "batch: split-precision slices of one table must share H and W");
         L.kj = std::max(L.kj, d.kp / 32);
         if (d.ca_stage != L.stage) L.stage = 2;
-        hd[i] = d;
         oB[i + 1] = oB[i] + d.wg_B + d.wg_U;
         oC[i + 1] = oC[i] + d.wg_CA;
         L.ldsB = std::max(L.ldsB, lds_B(d));
@@ -537,12 +539,12 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             G = L;
             G.maxe = 4;
             G.ldsB = G.ldsCA = 0;
-            G.PH = v[idx[0]]->d.fastH;
-            G.PW = v[idx[0]]->d.fastW;
-            G.stage = v[idx[0]]->d.ca_stage;
+            G.PH = hd[idx[0]].fastH;
+            G.PW = hd[idx[0]].fastW;
+            G.stage = hd[idx[0]].ca_stage;
             go.resize(o0 + 2 * (gn + 1), 0);
             for (int j = 0; j < gn; ++j) {
-                const SliceDesc& d = v[idx[j]]->d;
+                const SliceDesc& d = hd[idx[j]];
                 gd.push_back(d);
                 go[o0 + j + 1] = go[o0 + j] + d.wg_B + d.wg_U;
                 go[o0 + gn + 1 + j + 1] = go[o0 + gn + 1 + j] + d.wg_CA;
@@ -1027,6 +1029,14 @@ extern "C" int ocl_autotune_shape(ocl_ctx* c, int H, int W, int* ca_fast, int* t
     return OCL_OK;
 }
 
+extern "C" int ocl_batch_info(ocl_batch* b, int* launch_groups, long* wg_B, long* wg_CA) {
+    if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
+    if (launch_groups) *launch_groups = (int)b->t.groups.size();
+    if (wg_B) *wg_B = b->t.L.totB;
+    if (wg_CA) *wg_CA = b->t.L.totCA;
+    return OCL_OK;
+}
+
 extern "C" int ocl_batch_capture(ocl_batch* b, int n_iters) {
     if (!b || !b->ready) return fail(OCL_E_INVALID, "batch: not set up");
     if (n_iters < 1) return fail(OCL_E_INVALID, "batch: capture needs at least one iteration");
@@ -1157,6 +1167,7 @@ struct ServeReq {
     size_t footprint = 0;
     std::string err;
     ocl_nesta_report* rep = nullptr;   // optional, filled at completion
+    bool waited = false;               // waited at the head of the queue for HBM budget
 };
 }  // namespace
 
@@ -1172,7 +1183,7 @@ struct ocl_server {
     std::map<long, std::unique_ptr<ServeReq>> reqs;
     long next_id = 1;
     size_t used = 0, peak_used = 0;
-    long completed = 0, failed = 0, iterations = 0, launches = 0, n_active = 0;
+    long completed = 0, failed = 0, iterations = 0, launches = 0, n_active = 0, budget_waits = 0;
     double busy_s = 0;
     std::chrono::steady_clock::time_point t0;
     double now() const {
@@ -1239,7 +1250,14 @@ static void server_loop(ocl_server* sv) {
                     continue;
                 }
             }
-            if (sv->used + r->footprint > sv->budget) break;   // wait for running requests to free memory
+            if (sv->used + r->footprint > sv->budget) {       // wait for running requests to free memory
+                if (!r->waited) {
+                    r->waited = true;
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    sv->budget_waits++;
+                }
+                break;
+            }
             int rc = slice_alloc(r->res.get());
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->queue.pop_front();
@@ -1414,6 +1432,7 @@ extern "C" int ocl_server_get_stats(ocl_server* sv, ocl_server_stats* st) {
     st->launches = sv->launches;
     st->busy_s = sv->busy_s;
     st->elapsed_s = sv->now();
+    st->budget_waits = sv->budget_waits;
     return OCL_OK;
 }
 
