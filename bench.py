@@ -10,16 +10,21 @@ Other legs in the same run (each reported under its own key of the JSON line):
   plain / fused_single   the plain single-instance FP32 form (reference op sequence, host stop test) and the
                 fused single-instance form on the headline slice
   uneven_batch  64 slices of uneven size (H=W in 128..320 step 32, C in 4..32, F in 8..30) in one batched
-                launch table per iteration, captured in a hipGraph; k-space synthesised on the device
-  served        Poisson arrivals (50 req/s) mixing the sizes above, 5 stages x 10..30 iterations per request,
-                admitted by HBM budget into the continuous-batching server; p50/p99 latency and recons/s
+                launch table per iteration, captured in a hipGraph; k-space synthesised on the device; the plain
+                form timed over the same 64 slices beside it
+  served        Poisson arrivals (50 req/s, 60 s) mixing the sizes above, 5 stages x 10..30 iterations per request;
+                each request's inputs are uploaded from host memory at its arrival (inside its latency) and it is
+                admitted under a binding HBM budget (1 GiB) into the continuous-batching server; p50/p99 latency and
+                recons/s, a 100-request trace whose every result is compared with its solo run, the capacity (96
+                requests at once) and a second trace at 80% of that capacity; the plain form timed on the same mix
   motion_compensated  NESTA-TV with argsMC (U = tTV o MC, fused and plain forms) on the headline slice from the motion-free
                 result, with a smooth synthetic deformation (up to 1.5 px) over a centred bound box, plus the
                 MotionCompensation / AdjointMotionCompensation kernels alone against their HBM roofline
   cpu           the FP32 CPU restatement on the host cores (cpu_baseline, bounded sample) and rel-L2 of the GPU
                 results against the FP64 and FP32 CPU oracles on full schedules
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--batch 16] [--serve-seconds 20] [--legs ...] [--no-ref]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--batch 16] [--serve-seconds 60] [--legs ...] [--no-ref]
+    python bench.py --calibrate     (HBM counter calibration for rocprofv3 --pmc runs)
 """
 from __future__ import annotations
 
@@ -249,104 +254,147 @@ def leg_split(ctx, synth, parts, prm, steps, warmup):
     return res, {"x_split": xs, "x_fp32": x32, "slice0": host}
 
 
-N_SOLO_CHECK = 4
+def _serve_trace(ctx, capi, host_pool, arrivals, pick, iters, budget, chunk, check_all=False):
+    """Runs one arrival trace against a fresh server.  Each request starts on the host: at its arrival its inputs
+    (y, S, M of the pool slice it draws) are uploaded into device buffers of its own (recycled once it is done), then
+    it is submitted with an output buffer of its own.  Latency is arrival -> done, so it includes the upload and any
+    wait for HBM budget.  check_all: every output buffer is kept and compared afterwards with a solo run."""
+    n_req = len(arrivals)
+    free_in = [[] for _ in host_pool]
+    free_out = [[] for _ in host_pool]
 
+    def take(i):
+        y, S, M = host_pool[i]
+        if free_in[i]:
+            dy, dS, dM = free_in[i].pop()
+        else:
+            dy, dS, dM = ctx.empty(y.shape), ctx.empty(S.shape), ctx.empty(M.shape, np.uint8)
+        dy.set(y)
+        dS.set(S)
+        dM.set(M)
+        xo = free_out[i].pop() if (free_out[i] and not check_all) else ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
+        return dy, dS, dM, xo
 
-def leg_served(ctx, capi, pool, seconds, rate, budget, chunk, seed=7):
-    """Poisson trace against the continuous-batching server.  Inputs come from a pool of resident slices; each
-    request gets its own output buffer (recycled once the request is done) and 5 stages x U{10..30} iterations."""
-    rng = np.random.default_rng(seed)
-    n_req = max(1, int(round(rate * seconds)))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    pick = rng.integers(0, len(pool), n_req)
-    iters = rng.integers(10, 31, n_req)
-    free = [[ctx.empty((y.shape[0], y.shape[2], y.shape[3])) for _ in range(6)] for (y, _, _) in pool]
-    ctx.sync()
     srv = capi.Server(0, budget_bytes=budget, chunk=chunk)
-    inflight = {}
-    done = []
-    meta = {}
+    inflight, meta, arrive_py, submit_py, upload_s = {}, {}, {}, {}, []
     t0 = time.perf_counter()
     t_log = t0
     try:
         for r in range(n_req):
-            if time.perf_counter() - t_log > 30.0:
+            if time.perf_counter() - t_log > 20.0:
                 t_log = time.perf_counter()
-                log(f"served trace: {t_log - t0:.0f} s, {r} submitted, {len(done)} done")
+                log(f"served trace: {t_log - t0:.0f} s, {r} of {n_req} submitted")
             while True:
+                for rid in [k for k in inflight if srv.poll(k)[0] == capi.SERVE_DONE]:
+                    i, bufs = inflight.pop(rid)
+                    free_in[i].append(bufs[:3])
+                    if not check_all:
+                        free_out[i].append(bufs[3])
                 now = time.perf_counter() - t0
-                for rid in [k for k, v in inflight.items() if srv.poll(k)[0] == capi.SERVE_DONE]:
-                    i, xo, own = inflight.pop(rid)
-                    if not own:
-                        free[i].append(xo)
-                    done.append(rid)
                 if now >= arrivals[r]:
                     break
                 time.sleep(min(0.002, arrivals[r] - now))
             i = int(pick[r])
-            y, S, M = pool[i]
-            # the last N_SOLO_CHECK requests, compared with solo runs afterwards, keep output buffers of their own
-            own = r >= n_req - N_SOLO_CHECK
-            xo = free[i].pop() if (free[i] and not own) else ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
+            tu = time.perf_counter()
+            bufs = take(i)
+            upload_s.append(time.perf_counter() - tu)
             prm = capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0)
-            rid = srv.submit(y, S, M, xo, prm)
-            inflight[rid] = (i, xo, own)
-            meta[rid] = (i, int(iters[r]), xo)
+            ts = time.perf_counter() - t0
+            rid = srv.submit(*bufs, prm)
+            inflight[rid] = (i, bufs)
+            meta[rid] = (i, int(iters[r]), bufs[3])
+            arrive_py[rid] = float(arrivals[r])
+            submit_py[rid] = ts
         for rid in list(inflight):
             srv.wait(rid, 600.0)
-            done.append(rid)
         stats = srv.stats()
-        times = {rid: srv.times(rid) for rid in done}
-        # served result == solo result for the last requests (their output buffers were never recycled)
-        last = sorted(meta)[-N_SOLO_CHECK:]
+        times = {rid: srv.times(rid) for rid in meta}
+    finally:
+        srv.close()
+    # arrival -> submit on the host clock (upload and the submit loop), submit -> done on the server's clock
+    lat = [(submit_py[k] - arrive_py[k]) + (times[k][2] - times[k][0]) for k in times]
+    qwait = [times[k][1] - times[k][0] for k in times]
+    t_first = min(v[0] for v in times.values())
+    t_last = max(v[2] for v in times.values())
+    res = {"requests": len(times), "recons_per_s": len(times) / (t_last - t_first),
+           "p50_ms": 1e3 * pct(lat, 50), "p99_ms": 1e3 * pct(lat, 99), "max_ms": 1e3 * max(lat),
+           "upload_ms_mean": 1e3 * float(np.mean(upload_s)), "queue_wait_p50_ms": 1e3 * pct(qwait, 50),
+           "queue_wait_p99_ms": 1e3 * pct(qwait, 99), "slice_iterations": sum(STAGES * meta[k][1] for k in times),
+           "budget_bytes": budget, "peak_used_bytes": stats["peak_used_bytes"],
+           "budget_waits": stats["budget_waits"], "launches": stats["launches"], "busy_s": stats["busy_s"],
+           "failed": stats["failed"]}
+    return res, meta
+
+
+def leg_served(ctx, capi, host_pool, seconds, rate, budget, chunk, seed=7, n_check=None):
+    """Poisson trace at `rate` requests/s for `seconds`, 5 stages x U{10..30} iterations per request, sizes drawn
+    from the pool (the headline cine slices and the uneven sizes).  With n_check, that many requests (spread over the
+    trace) keep their output buffers and are compared with solo runs afterwards."""
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(round(rate * seconds)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    pick = rng.integers(0, len(host_pool), n_req)
+    iters = rng.integers(10, 31, n_req)
+    res, meta = _serve_trace(ctx, capi, host_pool, arrivals, pick, iters, budget, chunk)
+    res.update({"offered_rate_per_s": rate, "trace_s": float(arrivals[-1]), "pool_slices": len(host_pool)})
+    if n_check:
+        # a second, shorter trace at the same rate whose every output is kept: each served result against its solo run
+        m = min(n_req, n_check)
+        chk, cmeta = _serve_trace(ctx, capi, host_pool, arrivals[:m], pick[:m], iters[:m], budget, chunk,
+                                  check_all=True)
         eq = []
-        for rid in last:
-            i, it, xo = meta[rid]
-            y, S, M = pool[i]
+        dev = {}
+        for rid, (i, it, xo) in cmeta.items():
+            if i not in dev:
+                y, S, M = host_pool[i]
+                dev[i] = (ctx.to_device(y), ctx.to_device(S), ctx.to_device(M))
+            y, S, M = dev[i]
             xs = ctx.empty((y.shape[0], y.shape[2], y.shape[3]))
             ctx.nesta(y, S, M, xs, capi.nesta_params(maxIntIter=STAGES, maxIter=it, stoptest=0),
                       form=capi.OCL_FORM_FUSED)
             a, b = xo.get(), xs.get()
             eq.append(float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())))
-    finally:
-        srv.close()
-    lat = [times[k][2] - times[k][0] for k in times]
-    qwait = [times[k][1] - times[k][0] for k in times]
-    t_first = min(v[0] for v in times.values())
-    t_last = max(v[2] for v in times.values())
-    slice_iters = sum(STAGES * meta[k][1] for k in times)
-    return {"requests": len(times), "offered_rate_per_s": rate, "trace_s": float(arrivals[-1]),
-            "recons_per_s": len(times) / (t_last - t_first), "p50_ms": 1e3 * pct(lat, 50),
-            "p99_ms": 1e3 * pct(lat, 99), "max_ms": 1e3 * max(lat), "queue_wait_p50_ms": 1e3 * pct(qwait, 50),
-            "queue_wait_p99_ms": 1e3 * pct(qwait, 99), "slice_iterations": slice_iters,
-            "budget_bytes": budget, "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"],
-            "busy_s": stats["busy_s"], "failed": stats["failed"], "pool_slices": len(pool),
-            "served_vs_solo_rel_l2_max": max(eq) if eq else None}
+        res["served_vs_solo"] = {"requests": len(eq), "rel_l2_max": max(eq), "bit_equal": int(sum(e == 0.0 for e in eq)),
+                                 "budget_waits": chk["budget_waits"]}
+    return res
 
 
-def leg_served_burst(ctx, capi, pool, n_req, budget, chunk, seed=11):
-    """Capacity of the server: n_req requests (same mix as the Poisson trace) submitted at once."""
+def leg_served_burst(ctx, capi, host_pool, n_req, budget, chunk, seed=11):
+    """Capacity of the server: n_req requests of the same mix arriving at once (each uploaded from the host)."""
     rng = np.random.default_rng(seed)
-    pick = rng.integers(0, len(pool), n_req)
+    pick = rng.integers(0, len(host_pool), n_req)
     iters = rng.integers(10, 31, n_req)
-    outs = [ctx.empty((pool[i][0].shape[0], pool[i][0].shape[2], pool[i][0].shape[3])) for i in pick]
+    res, _ = _serve_trace(ctx, capi, host_pool, np.zeros(n_req), pick, iters, budget, chunk)
+    return res
+
+
+def leg_plain_mix(ctx, capi, host_pool, n_req, seed=7):
+    """The plain single-instance FP32 form over the served request mix (same pool and iteration distribution as the
+    trace): recons/s with inputs resident, one request after another."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(host_pool), n_req)
+    iters = rng.integers(10, 31, n_req)
+    dev = {}
+    for i in sorted(set(int(v) for v in pick)):
+        y, S, M = host_pool[i]
+        dev[i] = (ctx.to_device(y), ctx.to_device(S), ctx.to_device(M),
+                  ctx.empty((y.shape[0], y.shape[2], y.shape[3])))
     ctx.sync()
-    srv = capi.Server(0, budget_bytes=budget, chunk=chunk)
-    try:
-        t0 = time.perf_counter()
-        rids = [srv.submit(*pool[i], outs[r], capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0))
-                for r, i in enumerate(pick)]
-        for rid in rids:
-            srv.wait(rid, 600.0)
-        wall = time.perf_counter() - t0
-        stats = srv.stats()
-        times = [srv.times(rid) for rid in rids]
-    finally:
-        srv.close()
-    lat = [b[2] - b[0] for b in times]
-    return {"requests": n_req, "recons_per_s": n_req / wall, "wall_s": wall, "p50_ms": 1e3 * pct(lat, 50),
-            "p99_ms": 1e3 * pct(lat, 99), "slice_iterations": int(STAGES * iters.sum()),
-            "peak_used_bytes": stats["peak_used_bytes"], "launches": stats["launches"]}
+    t = time.perf_counter()
+    for r in range(n_req):
+        ctx.nesta(*dev[int(pick[r])], capi.nesta_params(maxIntIter=STAGES, maxIter=int(iters[r]), stoptest=0),
+                  form=capi.OCL_FORM_PLAIN)
+    total = time.perf_counter() - t
+    return {"requests": n_req, "recons_per_s": n_req / total}
+
+
+def leg_plain_uneven(ctx, capi, dev3, prm):
+    """The plain single-instance FP32 form over the 64 uneven slices, one after another (inputs resident)."""
+    t = time.perf_counter()
+    for (y, S, M, x) in dev3:
+        ctx.nesta(y, S, M, x, prm, form=capi.OCL_FORM_PLAIN)
+    total = time.perf_counter() - t
+    return {"slices": len(dev3), "recons_per_s": len(dev3) / total, "s_total": total}
 
 
 def mc_bytes(F, H, W, box_pixels, adjoint):
@@ -439,22 +487,35 @@ def gpu_legs(args, cine, parts, full, split):
                                         dev3[small][3].get())
             u = out["uneven_batch"]
             log(f"uneven batch: {u['recons_per_s']:.1f} recons/s, iteration {u['iteration']}")
+        if "uneven" in args.legs and dev3:
+            out["plain_uneven"] = leg_plain_uneven(ctx, capi, dev3, prm)
+            log(f"plain over the uneven batch: {out['plain_uneven']['recons_per_s']:.2f} recons/s")
         if "served" in args.legs:
             if not dev3:
                 for i, (x, S, M) in enumerate(parts):
                     y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
                     dev3.append((y, dS, dM, None))
-            pool = [d[:3] for d in dev[:4]] + [d[:3] for d in dev3[:12]]
-            out["served"] = leg_served(ctx, capi, pool, args.serve_seconds, SERVE_RATE, args.serve_budget << 30,
-                                       args.serve_chunk)
+            host_pool = [(y.get(), S.get(), M.get()) for (y, S, M, _) in dev[:4] + dev3[:12]]
+            out["plain_served_mix"] = leg_plain_mix(ctx, capi, host_pool, 16)
+            log(f"plain over the served mix: {out['plain_served_mix']['recons_per_s']:.2f} recons/s")
+            budget = int(args.serve_budget * (1 << 30))
+            out["served"] = leg_served(ctx, capi, host_pool, args.serve_seconds, SERVE_RATE, budget,
+                                       args.serve_chunk, n_check=args.serve_check)
             s = out["served"]
             log(f"served: {s['requests']} requests, {s['recons_per_s']:.1f} recons/s, p50 {s['p50_ms']:.0f} ms, "
-                f"p99 {s['p99_ms']:.0f} ms")
+                f"p99 {s['p99_ms']:.0f} ms, budget waits {s['budget_waits']}")
             if args.serve_burst > 0:
-                out["served_burst"] = leg_served_burst(ctx, capi, pool, args.serve_burst, args.serve_budget << 30,
+                out["served_burst"] = leg_served_burst(ctx, capi, host_pool, args.serve_burst, budget,
                                                        args.serve_chunk)
                 s = out["served_burst"]
                 log(f"served burst: {s['requests']} requests, {s['recons_per_s']:.1f} recons/s")
+                if args.serve_load_seconds > 0:
+                    rate = 0.8 * s["recons_per_s"]
+                    out["served_80pct"] = leg_served(ctx, capi, host_pool, args.serve_load_seconds, rate, budget,
+                                                     args.serve_chunk, seed=13)
+                    s = out["served_80pct"]
+                    log(f"served at 80% of capacity ({rate:.1f} req/s): {s['recons_per_s']:.1f} recons/s, "
+                        f"p50 {s['p50_ms']:.0f} ms, p99 {s['p99_ms']:.0f} ms")
     finally:
         ctx.close()
     return out
@@ -490,19 +551,16 @@ def _median_time(fn, reps):
 
 
 def cpu_leg(args, slice0, full, g):
-    """FP32 restatement timed on the host: config 2 as setup + 100 iterations from the median of 5 runs of 2 and of
-    4 inner iterations (after one warm-up), all threads; the 1-thread figure from single runs of 1 and 2
-    iterations; config 1 as the median of 5 (and of 3 on one thread)."""
+    """FP32 restatement timed on the host: config 2's full schedule (setup + 5 stages x 20 iterations) on all threads,
+    the median of 3 runs after a one-iteration warm-up; the 1-thread figure extrapolated from single runs of 1 and 2
+    iterations (a full 1-thread schedule would take minutes); config 1 as the median of 5 (and of 3 on one thread)."""
     from oracle import oracle as O
     y, S, M = slice0
     threads = O.num_threads()
     res = {}
-    run = lambda n: O.nesta(y, S, M, O.params(maxIntIter=1, maxIter=n, stoptest=0), double=False)
+    run = lambda n, k=1: O.nesta(y, S, M, O.params(maxIntIter=k, maxIter=n, stoptest=0), double=False)
     run(1)                                                                    # warm-up
-    ta, tb = _median_time(lambda: run(2), 5), _median_time(lambda: run(4), 5)
-    t_iter = (tb - ta) / 2.0
-    t_setup = max(0.0, ta - 2 * t_iter)
-    t_recon = t_setup + STAGES * ITERS * t_iter
+    t_recon = _median_time(lambda: run(ITERS, STAGES), 3)
     O.set_threads(1)
     try:
         t1a = _median_time(lambda: run(1), 1)
@@ -513,12 +571,12 @@ def cpu_leg(args, slice0, full, g):
     t1_recon = max(0.0, t1a - t1_iter) + STAGES * ITERS * t1_iter
     res["cpu_baseline"] = {"value": 1.0 / t_recon, "unit": "recons/s", "cores": threads, "kind": "port",
                            "sample": f"FP32 restatement (oracle/nesta_oracle.cpp, OpenMP, {threads} threads), one "
-                                     f"20x8x256x256 slice: medians of 5 runs of 2 and 4 inner iterations after a "
-                                     f"warm-up; setup + 100 iterations extrapolated linearly ({t_setup:.2f} s setup + "
-                                     f"{t_iter:.3f} s/iteration)",
+                                     f"20x8x256x256 slice, the full schedule (setup + {STAGES} stages x {ITERS} "
+                                     f"iterations): median of 3 runs after a one-iteration warm-up, "
+                                     f"{t_recon:.2f} s per reconstruction",
                            "one_thread_recons_per_s": 1.0 / t1_recon,
-                           "one_thread_sample": f"single runs of 1 and 2 iterations on 1 thread ({t1_iter:.3f} "
-                                                f"s/iteration)",
+                           "one_thread_sample": f"extrapolated from single runs of 1 and 2 iterations on 1 thread "
+                                                f"({t1_iter:.3f} s/iteration)",
                            "host": host_info()}
     gx = g["x"]
     if full is not None and "simple" in gx:
@@ -567,6 +625,27 @@ def cpu_leg(args, slice0, full, g):
     return res
 
 
+CALIB_BYTES = 256 << 20
+
+
+def calibrate():
+    """HBM byte-counter calibration for rocprofv3 --pmc runs: the library's copy kernel over 256 MiB, 5 dispatches.
+    Each dispatch reads and writes 256 MiB, so its FETCH_SIZE / WRITE_SIZE (KiB per dispatch) give the factors that
+    turn the same counters of the library's other kernels into bytes."""
+    from opencliper_amd import capi
+    ctx = capi.Context(0)
+    try:
+        n = CALIB_BYTES // 8
+        a = ctx.zeros((n,), np.complex64)
+        b = ctx.empty((n,), np.complex64)
+        for _ in range(5):
+            ctx.copy(a, b)
+        ctx.sync()
+        print(f"copy kernel: {CALIB_BYTES} bytes read + {CALIB_BYTES} bytes written per dispatch, 5 dispatches")
+    finally:
+        ctx.close()
+
+
 def load_traffic():
     try:
         with open(TRAFFIC_FILE) as fh:
@@ -584,10 +663,18 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--legs", default="simple,mc,split,uneven,served",
                     help="comma list of extra legs beside the headline batch: simple, mc, split, uneven, served")
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson trace (300 = 5 min)")
-    ap.add_argument("--serve-budget", type=int, default=64, help="server HBM budget in GiB")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the Poisson trace (300 = 5 min)")
+    ap.add_argument("--serve-budget", type=float, default=1.0,
+                    help="server HBM budget in GiB (the default binds: requests wait for memory)")
     ap.add_argument("--serve-chunk", type=int, default=5, help="server iterations between admission points")
     ap.add_argument("--serve-burst", type=int, default=96, help="requests submitted at once for server capacity")
+    ap.add_argument("--serve-load-seconds", type=float, default=30.0,
+                    help="length of a second Poisson trace at 80%% of the measured capacity (0: skip)")
+    ap.add_argument("--serve-check", type=int, default=100,
+                    help="requests of a trace whose every result is compared with its solo run (0: skip)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the HBM counter calibration (a 256 MiB device copy, 5 dispatches) for rocprofv3 "
+                         "--pmc runs, then exit")
     ap.add_argument("--no-ref", action="store_true", help="skip the full-schedule CPU references (rel-L2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg entirely")
     ap.add_argument("--gen-workers", type=int, default=None,
@@ -595,6 +682,9 @@ def main():
                          "for runs under a profiler that initialises the GPU before the program starts)")
     args = ap.parse_args()
     args.legs = set(x for x in args.legs.split(",") if x)
+    if args.calibrate:
+        calibrate()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(slices are independent, so N GPUs would each run their own batch).", file=sys.stderr)
@@ -655,11 +745,22 @@ def main():
         u = dict(g["uneven_batch"])
         u.pop("shapes_FCHW", None)
         u.pop("reports", None)
+        if "plain_uneven" in g:
+            u["plain_fp32_recons_per_s"] = g["plain_uneven"]["recons_per_s"]
+            u["speedup_vs_plain"] = u["recons_per_s"] / g["plain_uneven"]["recons_per_s"]
         line["uneven_batch"] = u
     if "served" in g:
         s = dict(g["served"])
-        s["speedup_vs_plain"] = s["recons_per_s"] / g["plain"]["recons_per_s"]
+        pm = g.get("plain_served_mix")
+        if pm:
+            s["plain_fp32_recons_per_s_on_mix"] = pm["recons_per_s"]
+            s["speedup_vs_plain"] = s["recons_per_s"] / pm["recons_per_s"]
         line["served"] = s
+        if "served_80pct" in g:
+            s8 = dict(g["served_80pct"])
+            if pm:
+                s8["speedup_vs_plain"] = s8["recons_per_s"] / pm["recons_per_s"]
+            line["served_80pct_capacity"] = s8
     if "split" in g:
         sp = g["split"]
         line["split_precision"] = {
@@ -676,6 +777,8 @@ def main():
     if "served_burst" in g:
         s = dict(g["served_burst"])
         s["note"] = "same request mix submitted at once: server capacity"
+        if g.get("plain_served_mix"):
+            s["speedup_vs_plain"] = s["recons_per_s"] / g["plain_served_mix"]["recons_per_s"]
         line["served_burst"] = s
     print(json.dumps(line, default=float))
 
