@@ -180,7 +180,11 @@ int ocl_batch_create_form(ocl_ctx* ctx, int nslices, const ocl_slice* slices, co
 int ocl_batch_create_mc(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,
                         const ocl_mc_args* mcs, ocl_batch** out);
 int ocl_batch_setup(ocl_batch* b);                    /* E^H y, maxima, normU, state (per slice) */
-int ocl_batch_iterate(ocl_batch* b, int n_iters);     /* enqueue n_iters iterations (no host sync) */
+/* enqueue n_iters iterations (no host sync).  With line masks (a mask constant along every k-space row) the fused FP32
+ * form runs one column pass per iteration (the Gram form); there a stage that the stop test ends takes one more
+ * iteration slot, so a stop-test schedule can need up to maxIntIter slots beyond maxIntIter x maxIter: iterate until
+ * ocl_batch_active reports 0 (ocl_batch_run does). */
+int ocl_batch_iterate(ocl_batch* b, int n_iters);
 int ocl_batch_capture(ocl_batch* b, int n_iters);     /* record n_iters iterations into a hipGraph */
 int ocl_batch_replay(ocl_batch* b);                   /* launch the recorded graph */
 int ocl_batch_active(ocl_batch* b, int* n_active);    /* synchronises; number of slices still iterating */

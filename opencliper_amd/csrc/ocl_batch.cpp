@@ -95,6 +95,7 @@ struct SliceRes {
     SliceSetup setup{};
     std::vector<int16_t> rows, rowpos;
     std::vector<int> nrows;
+    std::vector<uint8_t> rowmask;     // [F][nmax] carried row is a mask row (Gram form)
     std::vector<void*> allocs;
     size_t bytes = 0;
     bool allocated = false;
@@ -148,6 +149,8 @@ size_t lds_CA(const SliceDesc& d) {
                    : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
 }
 
+size_t lds_G(const SliceDesc& d) { return sizeof(float2) * (size_t)gram_lds(d.fastH, d.nmax); }
+
 size_t lds_CA_generic(int H, int tc) { return sizeof(float2) * ((size_t)2 * H * (tc + 1) + H); }
 size_t lds_B_generic(int W, int rb) { return sizeof(float2) * ((size_t)2 * rb * (W + 1) + W); }
 
@@ -199,6 +202,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     SliceDesc& d = r->d;
     d.F = s.F; d.C = s.C; d.H = s.H; d.W = s.W;
     d.N = (long)s.H * s.W;
+    bool want_gram = false;
     CHK(make_plan(c, s.H, &d.ph));
     CHK(make_plan(c, s.W, &d.pw));
     d.fastH = fast_p(s.H);
@@ -210,8 +214,10 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
         // has one, else the heuristic
         if (!cfg) CHK(tiles_for(c, s.H, s.W, c->autotune != 0 && !dev_scratch_active(), &cfg));
         if (cfg && !cfg->ca_fast) { d.fastH = 0; d.tile_cols = cfg->tile_cols; }
-        // the prefetching four-step column pass unless a tuned or pinned shape asks for the direct one
-        d.ca_stage = (d.fastH && (!cfg || cfg->ca_fast == 2)) ? 1 : 0;
+        // the Gram form when the mask allows it and no tuned or pinned shape asks otherwise; else (and for masks
+        // that vary along a row) the prefetching four-step column pass unless the shape asks for the direct one
+        want_gram = d.fastH && (!cfg || cfg->ca_fast == 3);
+        d.ca_stage = (d.fastH && (!cfg || cfg->ca_fast >= 2)) ? 1 : 0;
         if (cfg && !cfg->b_fast) { d.fastW = 0; d.rows_per_wg = cfg->rows_per_wg; }
     }
     if (d.fastH || prec) d.tile_cols = 16;
@@ -233,22 +239,30 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     HIPCHK(e);
     r->nrows.assign(s.F, 0);
     int nmax = 1;
+    bool line_mask = true;               // every mask row constant along the row
     for (int f = 0; f < s.F; ++f) {
-        for (int h = 0; h < s.H; ++h) r->nrows[f] += flags[(size_t)f * s.H + h];
+        for (int h = 0; h < s.H; ++h) {
+            r->nrows[f] += flags[(size_t)f * s.H + h] & 1;
+            line_mask &= !(flags[(size_t)f * s.H + h] & 2);
+        }
         nmax = std::max(nmax, r->nrows[f]);
     }
     r->rows.assign((size_t)s.F * nmax, 0);
     r->rowpos.assign((size_t)s.F * s.H, -1);
+    r->rowmask.assign((size_t)s.F * nmax, 0);
     for (int f = 0; f < s.F; ++f) {
         int j = 0;
         for (int h = 0; h < s.H; ++h)
-            if (flags[(size_t)f * s.H + h]) {
+            if (flags[(size_t)f * s.H + h] & 1) {
                 r->rowpos[(size_t)f * s.H + h] = (int16_t)j;
+                r->rowmask[(size_t)f * nmax + j] = (flags[(size_t)f * s.H + h] >> 2) & 1;
                 r->rows[(size_t)f * nmax + j++] = (int16_t)h;
             }
     }
     d.nmax = nmax;
     if (d.ca_stage && (nmax > 16 * FAST_CA_KQ || lds_CA(d) > LDS_MAX_NESTA)) d.ca_stage = 0;
+    d.gram_ok = (want_gram && !prec && line_mask && nmax <= 16 * FAST_CA_KQ &&
+                 sizeof(float2) * (size_t)gram_lds(d.fastH, nmax) <= LDS_MAX_NESTA) ? 1 : 0;
     d.kp = (nmax + 31) / 32 * 32;
     if (prec && d.kp > 64)
         return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports at most 64 sampled rows per frame");
@@ -277,7 +291,8 @@ size_t slice_footprint(const SliceRes& r) {
            (d.fastW && !d.prec ? rnd(sizeof(uint16_t) * (size_t)d.F * d.nmax * d.fastW) : 0) +
            (d.prec ? 2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.W * d.W) +
                          2 * rnd(sizeof(uint16_t) * 4 * (size_t)d.F * d.H * d.kp)
-                   : 0);
+                   : 0) +
+           (d.gram_ok ? rnd(xb) + rnd(r.rowmask.size()) + rnd(sizeof(double) * 3 * d.wg_CA) : 0);
 }
 
 // Step 2 of admission: device buffers and row tables.
@@ -328,6 +343,15 @@ int slice_alloc(SliceRes* r) {
     CHK(r->alloc(sizeof(unsigned) * 4, &p)); d.counter = (unsigned*)p;
     CHK(r->alloc(sizeof(double) * 3 * OCL_TRACE, &p)); d.trace = (double*)p;
     CHK(r->alloc(sizeof(double) * 2 * d.wg_CA, &p)); d.maxpart = (double*)p;
+    d.Wk2 = nullptr;
+    d.rowmask = nullptr;
+    d.partG = nullptr;
+    if (d.gram_ok) {
+        CHK(r->alloc(xb, &p)); d.Wk2 = (float2*)p;
+        CHK(r->alloc(r->rowmask.size(), &p)); d.rowmask = (const uint8_t*)p;
+        HIPCHK(hipMemcpyAsync(p, r->rowmask.data(), r->rowmask.size(), hipMemcpyHostToDevice, c->stream));
+        CHK(r->alloc(sizeof(double) * 3 * d.wg_CA, &p)); d.partG = (double*)p;
+    }
     d.mcz = d.mcg = nullptr;
     d.mc_dmax = nullptr;
     if (d.mc_xn) {
@@ -379,6 +403,12 @@ struct Table {
     int* goff = nullptr;        // [2 (cap + cap)] per group: B offsets then CA offsets (group size + 1 each)
     int cap = 0;
     NestaLaunch L{};
+    // slices of each iteration form: passes B, S, CA (n_ng slices) and the Gram form's G, S2 (n_g slices).  A table
+    // holding both runs them as separate launch groups; Lng / Lgr address the descriptors of each form for its
+    // scalar step (the non-Gram ones first in group order)
+    int n_ng = 0, n_g = 0;
+    NestaLaunch Lng{}, Lgr{};
+    std::vector<char> isg;             // per slice (table order): runs the Gram form
     ~Table() {
         if (dd) dev_free(dd);
         if (offB) dev_free(offB);
@@ -392,9 +422,12 @@ struct Table {
 };
 
 // Passes B and CA over a table: once per launch group of a mixed-size table, else once
+// Passes B and CA (G) over a table: once per launch group of its form, else once.  Pass B in setup mode (1) runs for
+// both forms (the Gram form keeps its output, V = W F_x^{-1} y); in iteration mode only for the B/S/CA form.
 hipError_t table_launch_B(const Table& t, hipStream_t s, int mode) {
-    if (t.groups.empty()) return launch_nesta_B(t.L, s, mode);
+    if (t.groups.empty()) return (mode == 0 && t.L.gram) ? hipSuccess : launch_nesta_B(t.L, s, mode);
     for (const LaunchGroup& g : t.groups) {
+        if (mode == 0 && g.L.gram) continue;
         const hipError_t e = launch_nesta_B(g.L, s, mode);
         if (e != hipSuccess) return e;
     }
@@ -402,9 +435,20 @@ hipError_t table_launch_B(const Table& t, hipStream_t s, int mode) {
 }
 
 hipError_t table_launch_CA(const Table& t, hipStream_t s, int mode) {
-    if (t.groups.empty()) return launch_nesta_CA(t.L, s, mode);
+    if (t.groups.empty()) return t.L.gram ? hipSuccess : launch_nesta_CA(t.L, s, mode);
     for (const LaunchGroup& g : t.groups) {
+        if (g.L.gram) continue;
         const hipError_t e = launch_nesta_CA(g.L, s, mode);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t table_launch_G(const Table& t, hipStream_t s, int mode) {
+    if (t.groups.empty()) return t.L.gram ? launch_nesta_G(t.L, s, mode) : hipSuccess;
+    for (const LaunchGroup& g : t.groups) {
+        if (!g.L.gram) continue;
+        const hipError_t e = launch_nesta_G(g.L, s, mode);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -442,38 +486,63 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.maxe = 4;
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
+    // iteration form of each slice: the Gram form where the slice allows it (split precision and motion compensation
+    // keep passes B, S, CA); a table holding both forms runs each as launch groups of its own
+    std::vector<char>& isg = t->isg;
+    isg.assign(ns, 0);
+    int n_g = 0;
+    for (int i = 0; i < ns; ++i) {
+        isg[i] = (!L.prec && hd[i].gram_ok && !hd[i].mc_xn) ? 1 : 0;
+        n_g += isg[i];
+    }
+    const bool mixed_forms = n_g > 0 && n_g < ns;
+    L.gram = (ns > 0 && n_g == ns) ? 1 : 0;
+    t->n_g = n_g;
+    t->n_ng = ns - n_g;
+    auto lds_col = [&](int i) { return isg[i] ? lds_G(hd[i]) : lds_CA(hd[i]); };
     // Launch groups.  In a mixed-size table, the slices of a (column, row) four-step size whose column pass has
     // enough workgroups to fill the GPU several times over (OCL_GROUP_MIN_WG) run passes B and CA as one launch each
     // with that size's specialized kernels; the other slices share one launch of the kernels that hold every size.  Those run
     // at lower occupancy, where the prefetching column pass (STAGE) hides the latency the direct pass exposes, so the
     // four-step slices of a shared mixed launch all take it, whatever their shape was tuned to alone.  (Few, small
     // groups would serialize launches that each under-fill the GPU, as in a server with a handful of requests.)
+    // The Gram form's pass G has no row transform, so its slices are grouped by column size alone.
     std::vector<std::vector<int>> sets;
-    {
+    std::vector<char> set_gram;
+    for (int form = 0; form < 2; ++form) {
+        std::vector<int> members;
+        for (int i = 0; i < ns; ++i)
+            if (isg[i] == form) members.push_back(i);
+        if (members.empty()) continue;
+        std::vector<std::vector<int>> fsets;
         std::map<std::pair<int, int>, std::vector<int>> by_size;
-        for (int i = 0; i < ns; ++i) by_size[{hd[i].fastH, hd[i].fastW}].push_back(i);
+        for (int i : members) by_size[{hd[i].fastH, form ? 0 : hd[i].fastW}].push_back(i);
         std::vector<int> rest;
         if (by_size.size() > 1 && !L.prec) {
             for (const auto& kv : by_size) {
                 long wg = 0;
                 for (int i : kv.second) wg += hd[i].wg_CA;
-                if (wg >= OCL_GROUP_MIN_WG) sets.push_back(kv.second);
+                if (wg >= OCL_GROUP_MIN_WG) fsets.push_back(kv.second);
                 else rest.insert(rest.end(), kv.second.begin(), kv.second.end());
             }
         } else {
-            for (int i = 0; i < ns; ++i) rest.push_back(i);
+            rest = members;
         }
         std::sort(rest.begin(), rest.end());
         bool restMixedH = false;
         for (size_t j = 1; j < rest.size(); ++j) restMixedH |= hd[rest[j]].fastH != hd[rest[0]].fastH;
-        if (restMixedH && !L.prec)
+        if (restMixedH && !L.prec && !form)
             for (int i : rest) {
                 SliceDesc& d = hd[i];
                 if (!d.fastH || d.ca_stage || d.nmax > 16 * FAST_CA_KQ) continue;
                 d.ca_stage = 1;
                 if (lds_CA(d) > LDS_MAX_NESTA) d.ca_stage = 0;
             }
-        if (!sets.empty() && !rest.empty()) sets.push_back(rest);
+        if ((!fsets.empty() || mixed_forms) && !rest.empty()) fsets.push_back(rest);
+        for (auto& fs : fsets) {
+            sets.push_back(fs);
+            set_gram.push_back((char)form);
+        }
     }
     const bool grouped = sets.size() > 1;
     L.kj = 1;
@@ -494,7 +563,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         oB[i + 1] = oB[i] + d.wg_B + d.wg_U;
         oC[i + 1] = oC[i] + d.wg_CA;
         L.ldsB = std::max(L.ldsB, lds_B(d));
-        L.ldsCA = std::max(L.ldsCA, lds_CA(d));
+        L.ldsCA = std::max(L.ldsCA, lds_col(i));
         if (!d.fastH) {
             const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
             while (L.maxe < need) L.maxe *= 2;
@@ -531,12 +600,14 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     std::vector<SliceDesc> gd;
     std::vector<int> go;
     if (grouped) {
-        for (const std::vector<int>& idx : sets) {
+        for (size_t si = 0; si < sets.size(); ++si) {
+            const std::vector<int>& idx = sets[si];
             const int gn = (int)idx.size();
             const size_t d0 = gd.size(), o0 = go.size();
             LaunchGroup g;
             NestaLaunch& G = g.L;
             G = L;
+            G.gram = set_gram[si];
             G.maxe = 4;
             G.ldsB = G.ldsCA = 0;
             G.PH = hd[idx[0]].fastH;
@@ -549,7 +620,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
                 go[o0 + j + 1] = go[o0 + j] + d.wg_B + d.wg_U;
                 go[o0 + gn + 1 + j + 1] = go[o0 + gn + 1 + j] + d.wg_CA;
                 G.ldsB = std::max(G.ldsB, lds_B(d));
-                G.ldsCA = std::max(G.ldsCA, lds_CA(d));
+                G.ldsCA = std::max(G.ldsCA, lds_col(idx[j]));
                 if (d.ca_stage != G.stage) G.stage = 2;
                 if (d.fastH != G.PH) G.PH = 0;
                 if (d.fastW != G.PW) G.PW = 0;
@@ -566,6 +637,17 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             G.totCA = go[o0 + 2 * gn + 1];
             t->groups.push_back(g);
         }
+    }
+    // descriptors addressed by each form's scalar step
+    t->Lng = NestaLaunch{};
+    t->Lgr = NestaLaunch{};
+    if (grouped) {
+        t->Lng.dd = t->gdd;
+        t->Lng.ns = t->n_ng;
+        t->Lgr.dd = t->gdd + t->n_ng;
+        t->Lgr.ns = t->n_g;
+    } else {
+        (L.gram ? t->Lgr : t->Lng) = L;
     }
     if (ns) {
         HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
@@ -600,8 +682,10 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         HIPCHK(launch_nesta_mc_z(t.L, c->stream, 1));
         HIPCHK(table_launch_CA(t, c->stream, 2));
     } else {
+        // kbuf = V = W F_x^{-1} y at the carried rows (the Gram form keeps it throughout); x0 = x_ref = A^H sqrt(N) y
         HIPCHK(table_launch_B(t, c->stream, 1));
         HIPCHK(table_launch_CA(t, c->stream, 1));
+        HIPCHK(table_launch_G(t, c->stream, 1));
     }
     hipLaunchKernelGGL(k_setup_max, dim3(t.L.totCA), dim3(OCL_NT), 0, c->stream, (const SliceDesc*)t.dd,
                        (const int*)t.offCA, t.L.ns, (double*)nullptr);
@@ -660,6 +744,12 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         S.Lmu = u.lambda * (u.normU / S.mu) + P.La;
         S.Lmu1 = 1.0 / S.Lmu;
         S.fmean[P.miniter - 1] = 9223372036854775807.0;
+        if (t.isg[i]) {
+            // the Gram form's first pass G takes iteration 0's coefficients from here (later ones from its scalar step)
+            S.it_mu = S.mu; S.it_L1 = S.Lmu1; S.it_apk = 0.5; S.it_tau = 2.0 / 3.0;
+            S.it_cur = 0; S.it_wcur = 0; S.it_sstart = 1; S.it_redo = 0;
+            S.it_pending = S.active;
+        }
         HIPCHK(hipMemcpyAsync(d.st, &S, sizeof(NState), hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -669,11 +759,17 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
 int table_iterate(ocl_ctx* c, const Table& t, int n) {
     if (!t.L.ns) return OCL_OK;
     for (int it = 0; it < n; ++it) {
-        HIPCHK(launch_nesta_mc_z(t.L, c->stream, 0));      // motion compensation only
-        HIPCHK(table_launch_B(t, c->stream, 0));
-        HIPCHK(launch_nesta_S(t.L, c->stream));
-        HIPCHK(launch_nesta_mc_g(t.L, c->stream));         // motion compensation only
-        HIPCHK(table_launch_CA(t, c->stream, 0));
+        if (t.n_ng) {                                      // passes B, S, CA
+            HIPCHK(launch_nesta_mc_z(t.L, c->stream, 0));  // motion compensation only
+            HIPCHK(table_launch_B(t, c->stream, 0));
+            HIPCHK(launch_nesta_S(t.Lng, c->stream));
+            HIPCHK(launch_nesta_mc_g(t.L, c->stream));     // motion compensation only
+            HIPCHK(table_launch_CA(t, c->stream, 0));
+        }
+        if (t.n_g) {                                       // the Gram form: pass G, then its scalar step
+            HIPCHK(table_launch_G(t, c->stream, 0));
+            HIPCHK(launch_nesta_S2(t.Lgr, c->stream));
+        }
     }
     return OCL_OK;
 }
@@ -683,14 +779,23 @@ int time_table(ocl_ctx* c, const Table& t, int n, double* ms_B, double* ms_CA, d
     std::vector<hipEvent_t> ev(3 * (size_t)n);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     int rc = OCL_OK;
+    // [ev 3it] passes B + S [ev 3it+1] pass CA, then the Gram form's G + S2 [ev 3it+2]
     for (int it = 0; it < n && rc == OCL_OK; ++it) {
         hipError_t e = hipEventRecord(ev[3 * it], c->stream);
-        if (e == hipSuccess) e = launch_nesta_mc_z(t.L, c->stream, 0);
-        if (e == hipSuccess) e = table_launch_B(t, c->stream, 0);
-        if (e == hipSuccess) e = launch_nesta_S(t.L, c->stream);
+        if (t.n_ng) {
+            if (e == hipSuccess) e = launch_nesta_mc_z(t.L, c->stream, 0);
+            if (e == hipSuccess) e = table_launch_B(t, c->stream, 0);
+            if (e == hipSuccess) e = launch_nesta_S(t.Lng, c->stream);
+        }
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 1], c->stream);
-        if (e == hipSuccess) e = launch_nesta_mc_g(t.L, c->stream);
-        if (e == hipSuccess) e = table_launch_CA(t, c->stream, 0);
+        if (t.n_ng) {
+            if (e == hipSuccess) e = launch_nesta_mc_g(t.L, c->stream);
+            if (e == hipSuccess) e = table_launch_CA(t, c->stream, 0);
+        }
+        if (t.n_g) {
+            if (e == hipSuccess) e = table_launch_G(t, c->stream, 0);
+            if (e == hipSuccess) e = launch_nesta_S2(t.Lgr, c->stream);
+        }
         if (e == hipSuccess) e = hipEventRecord(ev[3 * it + 2], c->stream);
         if (e != hipSuccess) rc = fail(OCL_E_HIP, std::string("timing launch: ") + hipGetErrorString(e));
     }
@@ -772,10 +877,16 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     TileCfg best;
     best.ms_CA = best.ms_B = 1e30;
     double best_score = 1e30;
+    // runs pair the i-th column-pass variant with the i-th row-pass variant; one more run times the Gram form (its
+    // pass G against the column-pass variants plus the fastest pass B, since it runs no pass B per iteration)
+    const bool try_gram = fast_p(H) && sizeof(float2) * (size_t)gram_lds(fast_p(H), H / 6 + 4) <= LDS_MAX_NESTA;
     const size_t n = std::max(caC.size(), bC.size());
-    for (size_t i = 0; i < n && rc == OCL_OK; ++i) {
+    double gram_ms = -1.0;
+    std::vector<double> ca_ms(caC.size(), 0.0);
+    for (size_t i = 0; i < n + (try_gram ? 1 : 0) && rc == OCL_OK; ++i) {
         TileCfg cfg;
-        const auto& a = caC[std::min(i, caC.size() - 1)];
+        const bool gram_run = i == n;
+        const auto& a = gram_run ? std::pair<int, int>(3, 16) : caC[std::min(i, caC.size() - 1)];
         const auto& b = bC[std::min(i, bC.size() - 1)];
         cfg.ca_fast = a.first; cfg.tile_cols = a.second;
         cfg.b_fast = b.first; cfg.rows_per_wg = b.second;
@@ -795,19 +906,30 @@ int tune_shape(ocl_ctx* c, int H, int W) {
         if (rc == OCL_OK) rc = table_iterate(c, t, 2);
         if (rc == OCL_OK) rc = time_table(c, t, 6, &ms_b, &ms_ca, nullptr);
         if (rc != OCL_OK) break;
-        // the direct four-step column pass must win by 5 %: on a tie the prefetching pass, whose latency hiding
-        // matters more in the fuller and mixed launches of real batches, is kept
-        const double score = ms_ca / NS * (cfg.ca_fast == 1 ? 1.05 : 1.0);
-        if (i < caC.size() && score < best_score) {
-            best.ca_fast = cfg.ca_fast; best.tile_cols = cfg.tile_cols; best.ms_CA = ms_ca / NS;
-            best_score = score;
+        if (gram_run) {
+            if (t.L.gram) gram_ms = (ms_ca + ms_b) / NS;     // pass G + its scalar step
+            continue;
         }
+        if (i < caC.size()) ca_ms[i] = ms_ca / NS;
         if (i < bC.size() && ms_b / NS < best.ms_B) {
             best.b_fast = cfg.b_fast; best.rows_per_wg = cfg.rows_per_wg; best.ms_B = ms_b / NS;
         }
     }
     cleanup();
     if (rc != OCL_OK) return rc;
+    // the direct four-step column pass must win by 5 %: on a tie the prefetching pass, whose latency hiding matters
+    // more in the fuller and mixed launches of real batches, is kept.  The Gram form competes with a column pass plus
+    // the fastest row pass, the iteration it replaces.
+    for (size_t i = 0; i < caC.size(); ++i) {
+        const double score = ca_ms[i] * (caC[i].first == 1 ? 1.05 : 1.0);
+        if (score < best_score) {
+            best.ca_fast = caC[i].first; best.tile_cols = caC[i].second; best.ms_CA = ca_ms[i];
+            best_score = score;
+        }
+    }
+    if (gram_ms > 0 && gram_ms < best_score + best.ms_B) {
+        best.ca_fast = 3; best.tile_cols = 16; best.ms_CA = gram_ms; best.ms_B = 0.0;
+    }
     c->tiles[{H, W}] = best;
     return OCL_OK;
 }
@@ -1004,7 +1126,7 @@ extern "C" int ocl_ctx_set_tiles(ocl_ctx* c, int H, int W, int ca_fast, int tile
     if (!b_fast && (rows_per_wg < 1 || rows_per_wg > 16 || (rows_per_wg & (rows_per_wg - 1)) ||
                     lds_B_generic(W, rows_per_wg) > LDS_MAX_NESTA))
         return fail(OCL_E_INVALID, "set_tiles: rows_per_wg must be a power of two <= 16 that fits the row pass");
-    if (ca_fast < 0 || ca_fast > 2) return fail(OCL_E_INVALID, "set_tiles: ca_fast must be 0, 1 or 2");
+    if (ca_fast < 0 || ca_fast > 3) return fail(OCL_E_INVALID, "set_tiles: ca_fast must be 0, 1, 2 or 3");
     TileCfg t;
     t.ca_fast = ca_fast; t.tile_cols = ca_fast ? 16 : tile_cols;
     t.b_fast = b_fast ? 1 : 0; t.rows_per_wg = b_fast ? 16 : rows_per_wg;
@@ -1096,11 +1218,13 @@ extern "C" int ocl_batch_run(ocl_batch* b, int chunk) {
     if (b->prm.stoptest != 1) {
         CHK(table_iterate(b->ctx, b->t, (int)max_total));
     } else {
+        // a stage that the stop test ends takes one more pass in the Gram form (the recompute of its last iteration)
         if (chunk < 1) chunk = 16;
         long done = 0;
         int n_active = 1;
-        while (done < max_total && n_active > 0) {
-            const int step = (int)std::min<long>(chunk, max_total - done);
+        const long max_passes = max_total + std::max(0, b->prm.maxIntIter);
+        while (done < max_passes && n_active > 0) {
+            const int step = (int)std::min<long>(chunk, max_passes - done);
             CHK(table_iterate(b->ctx, b->t, step));
             done += step;
             CHK(ocl_batch_active(b, &n_active));
@@ -1319,7 +1443,7 @@ static void server_loop(ocl_server* sv) {
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->busy_s += sv->now() - tb;
             sv->iterations += (long)sv->chunk * (long)view.size();
-            sv->launches += 3L * sv->chunk;
+            sv->launches += (table.L.gram ? 2L : 3L) * sv->chunk;
             sv->n_active = (long)view.size();
         }
         // ---- completions ----

@@ -30,6 +30,10 @@ struct NState {
     int it_pending;    // 1 when the scalar step has prepared an iteration for pass CA
     int it_cur, it_sstart;
     double it_mu, it_L1, it_apk, it_tau;
+    // Gram form (line masks): W_k is ping-ponged too, and an iteration that the stop test ends is recomputed
+    int wcur;          // which W buffer holds W_k
+    int it_wcur;       // W_k buffer of the prepared iteration
+    int it_redo;       // 1: the prepared iteration recomputes iteration k with the stage-ending coefficients
     double fmean[OCL_MAX_MINITER];
     int iters[OCL_MAX_NSTAGES];
 };
@@ -87,6 +91,12 @@ struct SliceDesc {
     float2* mcg;            // [F][H][W]
     int wg_P, wg_X, wg_G;   // workgroups of the pixel passes, of the box passes and of the tiled gather
     unsigned* mc_dmax;      // largest displacement |xn - position| over the box (float bits), set at setup
+    // Gram form of the iteration (line masks, four-step column size, nmax <= 64): one pass per iteration over column
+    // tiles (k_nesta_G_t) and a scalar step after it; kbuf holds V = W F_x^{-1} y of the carried rows throughout
+    int gram_ok;            // the slice can run the Gram form (mask constant along every row, sizes supported)
+    float2* Wk2;            // second W buffer [F][H][W] (W_k is ping-ponged with Wk)
+    const uint8_t* rowmask; // [F][nmax] 1 when the carried row is a sampled (mask) row
+    double* partG;          // [wg_CA][3] per column tile: ||r||^2, sum |u|^2, sum Re(conj(u) Ux)
 };
 #define OCL_MC_GATHER_RMAX 4 // pass G gathers (deterministically) when floor(max displacement) + 1 <= this
 #ifndef OCL_MC_TR
@@ -157,8 +167,12 @@ struct NestaLaunch {
     const int* offG;    // workgroup offsets of the tiled gather form of pass G
     int totP, totX, totG;
     int mcR;            // gather radius of pass G (largest over the table), 0: scatter form; set at setup
+    int gram;           // 1: every slice runs the Gram form (pass G over column tiles + scalar step S2)
 };
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
+// Gram form: pass G (mode 0 iteration, 1 setup x_ref = A^H sqrt(N) y into x0) and the scalar step after it
+hipError_t launch_nesta_G(const NestaLaunch& L, hipStream_t stream, int mode);
+hipError_t launch_nesta_S2(const NestaLaunch& L, hipStream_t stream);
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream);
 // mode 0: iteration, 1: setup (x_ref = A^H y), 2: setup from a given x_ref in x0 (forward columns only)
 hipError_t launch_nesta_CA(const NestaLaunch& L, hipStream_t stream, int mode);

@@ -12,21 +12,30 @@
 // (NestaUp.cpp:797-889), so no host round trip is needed between iterations.
 #include "ocl_nesta_common.hpp"
 #include "ocl_nesta_fast.hpp"
+#include "ocl_nesta_gram.hpp"
 
-// Row flags for the sampled-row lists: flag[f][h] = 1 if the mask row or any coil's k-space row is non-zero.
+// Row flags for the sampled-row lists, flag[f][h]: bit 0 if the mask row or any coil's k-space row is non-zero,
+// bit 1 if the mask row is not constant along the row, bit 2 if its first mask value is set.
 __global__ void k_rowflags(const float2* __restrict__ y, const uint8_t* __restrict__ M, int F, int C, int H, int W,
                            uint8_t* __restrict__ flags) {
     const int fh = blockIdx.x;
     const int f = fh / H, h = fh - f * H;
-    int any = 0;
-    for (int i = threadIdx.x; i < W; i += blockDim.x) any |= (M[((long)f * H + h) * W + i] != 0);
+    const uint8_t* m = M + ((long)f * H + h) * W;
+    const int m0 = m[0] != 0;
+    int any = 0, varies = 0;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) {
+        const int mi = m[i] != 0;
+        any |= mi;
+        varies |= mi != m0;
+    }
+    varies = __syncthreads_or(varies);
     for (int c = 0; c < C && !__syncthreads_or(any); ++c)
         for (int i = threadIdx.x; i < W; i += blockDim.x) {
             const float2 v = y[(((long)f * C + c) * H + h) * W + i];
             any |= (v.x != 0.f || v.y != 0.f);
         }
     any = __syncthreads_or(any);
-    if (threadIdx.x == 0) flags[fh] = (uint8_t)(any != 0);
+    if (threadIdx.x == 0) flags[fh] = (uint8_t)((any != 0) | ((varies != 0) << 1) | (m0 << 2));
 }
 
 // Mask bits of the sampled rows for the four-step pass B (W = 16 P): word (f, j, k1) holds, in bit k2, whether the mask
@@ -126,6 +135,137 @@ __device__ void nesta_scalar_step(const SliceDesc& d, double* red) {
         }
     }
     st->k = k;
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// scalar step of the Gram form, after pass G(k): the objective of x_k from G's partial sums, the stop test
+// (NestaUp.cpp:797-831), then either the commit of iteration k with the coefficients of iteration k+1 for the next
+// pass G, or, when the stop test ends the stage at k, a recompute of iteration k with apk and tau taken at
+// k = maxIter (G read x_k and W_k and wrote x_{k+1}, W_{k+1} to the other buffers, so both inputs are still there).
+// ------------------------------------------------------------------------------------------------------------------
+__device__ void nesta_gram_step(const SliceDesc& d, double* red) {
+    NState* st = d.st;
+    if (!st->active || !st->it_pending) {
+        if (threadIdx.x == 0) st->it_pending = 0;
+        return;
+    }
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < d.wg_CA; i += blockDim.x) {
+        s[0] += d.partG[3 * i];
+        s[1] += d.partG[3 * i + 1];
+        s[2] += d.partG[3 * i + 2];
+    }
+    block_sum<3>(s, red);
+    if (threadIdx.x != 0) return;
+    int k = st->k;
+    const int mi = st->miniter;
+    bool end_stage = false;
+    if (st->it_redo) {
+        st->it_redo = 0;                 // the recompute of a stage's last iteration: commit it, end the stage
+        end_stage = true;
+    } else {
+        const double l2 = s[0], u2 = s[1], fxs = s[2];
+        const double l1 = fxs - (st->mu / 2.0) * u2;
+        const double fv = 0.5 * l2 + st->lambda * l1;
+        st->last_f = fv;
+        if (d.trace && st->ntrace < st->trace_cap) {
+            d.trace[3 * st->ntrace] = fv;
+            d.trace[3 * st->ntrace + 1] = l2;
+            d.trace[3 * st->ntrace + 2] = l1;
+            st->ntrace++;
+        }
+        if (st->stoptest == 1) {
+            bool skip_write = false;
+            if (k >= mi) {
+                double mean = 0.0;
+                for (int ne = 0; ne < mi; ++ne) mean += st->fmean[mi - ne - 1];
+                mean = mean / mi;
+                const double qp = (mean - fv) / mean;
+                st->last_qp = qp;
+                if (qp <= st->tolVar) {
+                    if (st->qpDone) end_stage = true;
+                    st->qpDone = 1;
+                    skip_write = true;
+                }
+            }
+            if (!skip_write) st->fmean[(mi - 1) - ((k + 1) % mi)] = fv;
+        }
+        st->iters_stage++;
+        st->total_iters++;
+        if (end_stage) {                 // recompute iteration k from the same x_k, W_k with k = maxIter
+            st->it_apk = 0.5 * ((double)st->maxIter + 1.0);
+            st->it_tau = 2.0 / ((double)st->maxIter + 3.0);
+            st->it_redo = 1;
+            st->it_pending = 1;
+            return;
+        }
+    }
+    // commit iteration k: x_{k+1} and W_{k+1} become current
+    st->cur ^= 1;
+    st->wcur ^= 1;
+    st->stage_start = 0;
+    k = k + 1;
+    if (end_stage || k >= st->maxIter) {
+        if (st->stage - 1 < OCL_MAX_NSTAGES) st->iters[st->stage - 1] = st->iters_stage;
+        st->stage++;
+        st->iters_stage = 0;
+        if (st->stage > st->maxIntIter) {
+            st->active = 0;
+        } else {
+            st->mu = st->mu * st->gamma;
+            st->tolVar = st->tolVar * st->gammat;
+            st->Lmu = st->lambda * (st->normU / st->mu) + st->La;
+            st->Lmu1 = 1.0 / st->Lmu;
+            st->qpDone = 0;
+            st->fmean[mi - 1] = 9223372036854775807.0;
+            st->stage_start = 1;
+            k = 0;
+        }
+    }
+    st->k = k;
+    if (!st->active) {
+        st->it_pending = 0;
+        return;
+    }
+    // coefficients of iteration k for the next pass G, as if its stop test does not end the stage
+    st->it_mu = st->mu;
+    st->it_L1 = st->Lmu1;
+    st->it_apk = 0.5 * ((double)k + 1.0);
+    st->it_tau = 2.0 / ((double)k + 3.0);
+    st->it_cur = st->cur;
+    st->it_wcur = st->wcur;
+    st->it_sstart = st->stage_start;
+    st->it_pending = 1;
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_nesta_S2(const SliceDesc* __restrict__ descs, int nslices) {
+    __shared__ double red[64];
+    if ((int)blockIdx.x >= nslices) return;
+    nesta_gram_step(descs[blockIdx.x], red);
+}
+
+// Pass G of the Gram form over a launch table (PHU > 0: every slice has H = 16 PHU; 0: any four-step H).
+template <int PHU>
+__global__ __launch_bounds__(OCL_NT) void k_nesta_G_t(const SliceDesc* __restrict__ descs,
+                                                      const int* __restrict__ wg_off, int nslices, int mode) {
+    extern __shared__ float2 lds[];
+    __shared__ double red[64];
+    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const SliceDesc& d = descs[si];
+    if (mode == 0 && !d.st->it_pending) return;
+    const int local = blockIdx.x - wg_off[si];
+    switch (PHU > 0 ? PHU : d.fastH) {
+        case 2: nesta_G_fast<2>(d, local, mode, lds, red); break;
+        case 4: nesta_G_fast<4>(d, local, mode, lds, red); break;
+        case 6: nesta_G_fast<6>(d, local, mode, lds, red); break;
+        case 8: nesta_G_fast<8>(d, local, mode, lds, red); break;
+        case 10: nesta_G_fast<10>(d, local, mode, lds, red); break;
+        case 12: nesta_G_fast<12>(d, local, mode, lds, red); break;
+        case 14: nesta_G_fast<14>(d, local, mode, lds, red); break;
+        case 16: nesta_G_fast<16>(d, local, mode, lds, red); break;
+        case 18: nesta_G_fast<18>(d, local, mode, lds, red); break;
+        default: nesta_G_fast<20>(d, local, mode, lds, red); break;
+    }
 }
 
 // One workgroup per slice, between pass B(k) and pass CA(k).  Kernel boundaries order the partial sums written by
@@ -446,6 +586,29 @@ hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode) {
     return hipGetLastError();
 }
 
+static NestaKernel pick_G(int PH) {
+    switch (PH) {
+        case 8: return k_nesta_G_t<8>;
+        case 10: return k_nesta_G_t<10>;
+        case 12: return k_nesta_G_t<12>;
+        case 14: return k_nesta_G_t<14>;
+        case 16: return k_nesta_G_t<16>;
+        case 18: return k_nesta_G_t<18>;
+        case 20: return k_nesta_G_t<20>;
+        default: return k_nesta_G_t<0>;
+    }
+}
+
+hipError_t launch_nesta_G(const NestaLaunch& L, hipStream_t stream, int mode) {
+    hipLaunchKernelGGL(pick_G(L.PH), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_nesta_S2(const NestaLaunch& L, hipStream_t stream) {
+    hipLaunchKernelGGL(k_nesta_S2, dim3(L.ns), dim3(OCL_NT), 0, stream, L.dd, L.ns);
+    return hipGetLastError();
+}
+
 hipError_t launch_nesta_S(const NestaLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_nesta_S, dim3(L.ns), dim3(OCL_NT), 0, stream, L.dd, L.ns);
     return hipGetLastError();
@@ -462,6 +625,7 @@ void nesta_kernel_attributes(int lds_bytes) {
     const int ps[] = {0, 8, 10, 12, 14, 16, 18, 20};
     for (int p : ps) {
         (void)hipFuncSetAttribute((const void*)pick_B(p), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+        (void)hipFuncSetAttribute((const void*)pick_G(p), hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
         for (int e : {4, 8, 16, 32}) {
             for (int g : {0, 1, 2})
                 (void)hipFuncSetAttribute((const void*)pick_CA(p, e, g), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -23,8 +23,9 @@ def run(ctx, sl, prm):
 
 
 # (ca_fast, tile_cols, b_fast, rows_per_wg) on a 160 x 96 slice: H = 16 * 10 and W = 16 * 6 allow every variant
+# (ca_fast 3: the Gram form, with pass B in four-step or Stockham form for its setup)
 VARIANTS = [(1, 16, 1, 16), (2, 16, 1, 16), (0, 4, 0, 4), (0, 8, 1, 16), (1, 16, 0, 8), (2, 16, 0, 16),
-            (0, 16, 0, 16)]
+            (0, 16, 0, 16), (3, 16, 1, 16), (3, 16, 0, 8)]
 
 
 @pytest.mark.parametrize("v", VARIANTS)
@@ -46,8 +47,9 @@ def test_autotune_picks_a_valid_shape_and_is_used(ctx, oracle):
     try:
         t = c2.autotune_shape(192, 128)
         print("tuned 192x128:", t, flush=True)
-        assert t["ms_CA"] > 0 and t["ms_B"] > 0
-        assert (t["ca_fast"], t["tile_cols"]) in {(1, 16), (2, 16), (0, 4), (0, 8), (0, 16)}
+        # the Gram form (ca_fast 3) runs no row pass per iteration: its ms_B is 0
+        assert t["ms_CA"] > 0 and (t["ms_B"] > 0) == (t["ca_fast"] != 3)
+        assert (t["ca_fast"], t["tile_cols"]) in {(1, 16), (2, 16), (0, 4), (0, 8), (0, 16), (3, 16)}
         assert (t["b_fast"], t["rows_per_wg"]) in {(1, 16), (0, 4), (0, 8), (0, 16)}
         assert c2.autotune_shape(192, 128) == t          # cached
         c2.set_autotune(True)

@@ -196,3 +196,34 @@ def test_pixel_mask_split_precision(ctx, oracle):
     err = oracle.rel_l2(x.get(), x64)
     print(f"split form, pixel mask: rel-L2 vs FP64 {err:.2e}", flush=True)
     assert err < 1e-3
+
+
+@pytest.mark.parametrize("H,W,stoptest", [(128, 128, 0), (160, 96, 1), (288, 64, 0), (320, 80, 1)])
+def test_gram_form_matches_row_and_column_passes(oracle, H, W, stoptest):
+    """With a line mask the fused form runs the Gram iteration (one column pass, no row transform per iteration); the
+    same slice with the column and row passes pinned (ca_fast 2) must agree with it and with FP64, and, with the stop
+    test on, take the oracle's iteration counts (a stage ended by the stop test recomputes its last iteration)."""
+    sl = synth.make_cine(H=H, W=W, C=3, F=7, accel=6.0, n_centre=8, slice_id=120 + H)
+    prm = dict(maxIntIter=3, maxIter=40 if stoptest else 12, stoptest=stoptest, miniter=4)
+    x64, r64 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    outs = []
+    for pinned in (False, True):
+        c = capi.Context(0)
+        try:
+            if pinned:                        # W = 80 has no four-step row pass: Stockham rows there
+                c.set_tiles(H, W, 2, 16, *((1, 16) if W % 32 == 0 else (0, 16)))
+            y, S, M, x = upload(c, sl.y, sl.S, sl.M)
+            rep = c.nesta(y, S, M, x, capi.nesta_params(**prm), form=capi.OCL_FORM_FUSED)
+            got = x.get()
+        finally:
+            c.close()
+        err = oracle.rel_l2(got, x64)
+        print(f"{H}x{W} stoptest {stoptest} {'row+column passes' if pinned else 'Gram form'}: iterations "
+              f"{rep['iters']} (oracle {r64['iters']}), rel-L2 vs FP64 {err:.2e}", flush=True)
+        assert rep["iters"] == r64["iters"]
+        assert err < 1e-4
+        np.testing.assert_allclose(rep["trace_f"], r64["trace_f"], rtol=1e-4)
+        outs.append(got)
+    # both are FP32 and within 1e-4 of FP64; they round differently (no row transforms in the Gram form), which
+    # NESTA amplifies over long stages (about 7e-5 apart after a 40-iteration stage at 160 x 96)
+    assert oracle.rel_l2(outs[0], outs[1]) < 2e-4
