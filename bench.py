@@ -2,8 +2,9 @@
 
 Headline workload (one "step"): a batch of B=16 independent cine slices, each F=20 frames x C=8 coils x 256x256,
 variable-density ky-line mask at acceleration 6 with 16 centre lines, reconstructed with NESTA-TV for 5 continuation
-stages x 20 inner iterations (stop test off, fixed work).  The batch is advanced by the fused batched engine (three
-launches per iteration, replayed from a hipGraph); `value` is whole-job recons/s with inputs resident in HBM.
+stages x 20 inner iterations (stop test off, fixed work).  The batch is advanced by the fused batched engine (for
+line masks the Gram form: one column pass and one scalar step per iteration, replayed from a hipGraph); `value` is
+whole-job recons/s with inputs resident in HBM.
 
 Other legs in the same run (each reported under its own key of the JSON line):
   simple_recon  fully-sampled 8-coil 256x256 SimpleMRIRecon (inverse FFT per coil, conj(S) coil combine)
@@ -724,7 +725,8 @@ def main():
         "speedup_vs_plain": g["batched"]["recons_per_s"] / g["plain"]["recons_per_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_nesta_B + k_nesta_S + k_nesta_CA (one fused NESTA iteration of the whole batch)",
+                     "kernel": "k_nesta_G_t<16> + k_nesta_S2 (one fused NESTA iteration of the whole batch in the "
+                               "Gram form: one column pass and its scalar step; HIP events on the library's stream)",
                      "ms_B_plus_S": it["ms_B_plus_S"], "ms_CA": it["ms_CA"],
                      "bytes_alg_per_iter": it["bytes_alg_per_iter"],
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None},
