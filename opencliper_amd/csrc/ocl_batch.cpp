@@ -919,7 +919,8 @@ int tune_shape(ocl_ctx* c, int H, int W) {
     if (rc != OCL_OK) return rc;
     // the direct four-step column pass must win by 5 %: on a tie the prefetching pass, whose latency hiding matters
     // more in the fuller and mixed launches of real batches, is kept.  The Gram form competes with a column pass plus
-    // the fastest row pass, the iteration it replaces.
+    // the fastest row pass, the iteration it replaces, and is kept unless that pair wins by 5 %: it moves fewer bytes,
+    // and a shape that leaves it splits mixed batches into more launches.
     for (size_t i = 0; i < caC.size(); ++i) {
         const double score = ca_ms[i] * (caC[i].first == 1 ? 1.05 : 1.0);
         if (score < best_score) {
@@ -927,7 +928,7 @@ int tune_shape(ocl_ctx* c, int H, int W) {
             best_score = score;
         }
     }
-    if (gram_ms > 0 && gram_ms < best_score + best.ms_B) {
+    if (gram_ms > 0 && gram_ms < 1.05 * (best_score + best.ms_B)) {
         best.ca_fast = 3; best.tile_cols = 16; best.ms_CA = gram_ms; best.ms_B = 0.0;
     }
     c->tiles[{H, W}] = best;

@@ -20,9 +20,12 @@ __host__ __device__ constexpr int fast_twa_len(int P, int TA) { return P == 16 ?
 __host__ __device__ constexpr int fast_ca_lds(int P) {
     return P * FAST_CA_KS + fast_twk_len(P, 16) + fast_twa_len(P, P) + 8 * P;
 }
-// Gram pass G (ocl_nesta_gram.hpp): the column pass's exchange, twiddle and row-slot tables, 16 more for the per-role
-// mask words, and two staging slabs of (nmax + 1) x 16 values of V
-__host__ __device__ constexpr int gram_lds(int P, int nmax) { return fast_ca_lds(P) + 16 + 2 * 16 * (nmax + 1); }
+// Gram pass G (ocl_nesta_gram.hpp): the column pass's exchange, twiddle and row-slot tables, GRAM_SPARE more for the
+// per-row residual factors (H <= 320 floats), and two staging slabs of (nmax + 1) x 16 values of V
+#define GRAM_SPARE 160
+__host__ __device__ constexpr int gram_lds(int P, int nmax) {
+    return fast_ca_lds(P) + GRAM_SPARE + 2 * 16 * (nmax + 1);
+}
 // pass B: row exchange, twiddle tables
 __host__ __device__ constexpr int fast_b_lds(int P) {
     return 16 * FAST_B_RS(P) + fast_twk_len(P, 17) + fast_twa_len(P, P + 1);

@@ -103,12 +103,14 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
     const unsigned lane = (unsigned)(t * W + w), lanec = (unsigned)(t * W + wc);
     const NState* st = d.st;
     // per-role tables (as in the column pass): twk[k1][n2] = W_H^{k1 n2}, twa[a][k1] = W_H^{a k1}, rpk[k1][k2] = slot of
-    // row k1 + P k2 (nmax when the row carries nothing), mbk[k1] bit k2 = row k1 + P k2 is a sampled (mask) row
+    // row k1 + P k2 (nmax when the row carries nothing), mkf[k1][k2] = W / sqrt(N) when that row is a sampled (mask)
+    // row, else 0 (the factor of K in the residual)
     float2* twk = lds + P * FAST_CA_KS;
     float2* twa = P == 16 ? twk : twk + fast_twk_len(P, 16);
     int* rpk = (int*)(twk + fast_twk_len(P, 16) + fast_twa_len(P, P));
-    int* mbk = rpk + H;
-    float2* kst = (float2*)(mbk + 32);
+    float* mkf = (float*)(rpk + H);
+    float2* kst = (float2*)(rpk + H) + GRAM_SPARE;
+    const float sW = (float)W * d.inv_sqrtN;
     const int kss = (nmax + 1) * 16;                   // slab stride; slot nmax of each slab stays zero
     const int nst = d.nrows[f] * 16;                   // staged values per coil
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
@@ -120,14 +122,7 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
         }
         const int slot = d.rowpos[(long)f * H + k1 + P * n2];
         rpk[i] = slot < 0 ? nmax : slot;
-    }
-    if (threadIdx.x < P) {
-        unsigned m = 0;
-        for (int k2 = 0; k2 < 16; ++k2) {
-            const int slot = d.rowpos[(long)f * H + threadIdx.x + P * k2];
-            if (slot >= 0 && d.rowmask[(long)f * nmax + slot]) m |= 1u << k2;
-        }
-        mbk[threadIdx.x] = (int)m;
+        mkf[i] = (slot >= 0 && d.rowmask[(long)f * nmax + slot]) ? sW : 0.f;
     }
     if (threadIdx.x < 32) kst[(threadIdx.x >> 4) * kss + nmax * 16 + (threadIdx.x & 15)] = make_float2(0.f, 0.f);
     // staging of conj(V) of one coil: coalesced 16-column rows, at most FAST_CA_KQ values per thread
@@ -159,7 +154,6 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
     float2 acc[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) acc[i] = make_float2(0.f, 0.f);
-    const float sW = (float)W * d.inv_sqrtN;
     double r2 = 0.0;
     __syncthreads();
 
@@ -192,14 +186,14 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
 #pragma unroll
                     for (int i = 0; i < 16; ++i) u[i] = X[k1 * FAST_CA_KS + i * 16];
                     rf::rdft<16>(u);                              // K at rows k1 + P k2
-                    const unsigned mb = (unsigned)mbk[k1];
+                    float mk[16];
+                    load_rows16((const int*)(mkf + 16 * k1), (int*)mk);
 #pragma unroll
                     for (int k2 = 0; k2 < 16; ++k2) {
                         // conj(rho) = m conj(K) W / sqrt(N) - conj(V); zero at rows that carry nothing
-                        const float m = ((mb >> k2) & 1u) ? sW : 0.f;
                         const float2 cv = ks[jv[k2] * 16];
-                        const float2 rc = make_float2(m * u[k2].x - cv.x, -(m * u[k2].y) - cv.y);
-                        rq += c_abs2(rc);
+                        const float2 rc = make_float2(fmaf(mk[k2], u[k2].x, -cv.x), fmaf(-mk[k2], u[k2].y, -cv.y));
+                        rq = fmaf(rc.x, rc.x, fmaf(rc.y, rc.y, rq));
                         u[k2] = rc;
                     }
                 } else {
@@ -226,8 +220,12 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
 #pragma unroll
         for (int k1 = 0; k1 < P; ++k1) v[k1] = X[k1 * FAST_CA_KS + t * 16];
         rf::rdft<P>(v);
+        // acc += conj(v) conj(S) as four fused multiply-adds
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) acc[n1] = c_add(acc[n1], c_mulconj(c_conj(v[n1]), sv[n1]));
+        for (int n1 = 0; n1 < P; ++n1) {
+            acc[n1].x = fmaf(v[n1].x, sv[n1].x, fmaf(-v[n1].y, sv[n1].y, acc[n1].x));
+            acc[n1].y = fmaf(-v[n1].y, sv[n1].x, fmaf(-v[n1].x, sv[n1].y, acc[n1].y));
+        }
         if (mode != 0) __syncthreads();       // setup: no forward exchange separates the next coil's role phase
     }
     if (mode == 1) {
