@@ -18,13 +18,18 @@
 #pragma once
 #include "ocl_nesta_fast.hpp"
 
+#ifndef GRAM_UPD_ROWS
+#define GRAM_UPD_ROWS 8      // rows of the update whose operands are loaded together
+#endif
+
 // NESTA update of the P pixels h = t + 16 n1 of this thread's column from x_k (in registers), its neighbour frames,
 // x_ref and W_k; accumulates |u_f|^2 and Re(conj(u_f) Ux_f) of x_k for the objective.  SST: first iteration of a
 // stage (x_ref := x_k, W_k := 0).
 template <int P, bool SST>
 __device__ __forceinline__ void gram_update(const SliceDesc& d, const NState* st, int f, unsigned p, bool wv,
                                             const float2* xk, const float2* acc, double* uterm) {
-    constexpr int CH = 4;
+    // loads of CH rows are issued together (the coil loop's registers are free here): each group waits on memory once
+    constexpr int CH = GRAM_UPD_ROWS < P ? GRAM_UPD_ROWS : P;
     const long N = d.N, W = d.W;
     const int fm = f == 0 ? d.F - 1 : f - 1, fp = f == d.F - 1 ? 0 : f + 1;
     const float2* xb = st->it_cur ? d.x1 : d.x0;
