@@ -128,8 +128,15 @@ int ocl_reduce(ocl_ctx* ctx, const void* x, const void* y, long n, int op, doubl
 #define OCL_FORM_FUSED 1   /* fused row pass, scalar step, fused column pass; device-side stop test, no host sync */
 #define OCL_FORM_SPLIT 2   /* as FUSED, with the row and column DFTs as BF16 MFMA GEMMs (3-term hi/lo split product,
                               FP32 accumulation; NESTA scalars and norms in FP64); H, W in {64, 128, 192, 256} */
-/* myNormest (NestaUp.cpp:291-439) for an F-frame image of N pixels, from a fresh POSIX-default drand48 stream. */
+/* myNormest (NestaUp.cpp:291-439) for an F-frame image of N pixels.  Its restarts draw from drand48, which the
+ * reference never seeds (NestaUp.cpp:375): one process-wide stream from the POSIX default state.  By default each
+ * estimate here starts that stream afresh (what the first reconstruction of a process sees; the value is then cached
+ * per (F, N)); ocl_ctx_set_normest_stream(ctx, 1) carries one stream across the context's estimates instead, as the
+ * reference's process carries it across every NestaUp::launch (e.g. NESTA then MC-NESTA, tests/MRIRecon.cpp:232-272),
+ * and resets it to the default state.  Batches and servers on such a context draw in table order. */
 int ocl_normest(ocl_ctx* ctx, int F, long N, double* normU, int* iters);
+int ocl_ctx_set_normest_stream(ocl_ctx* ctx, int carry);
+int ocl_ctx_normest_stream_position(ocl_ctx* ctx, long long* position);   /* draws taken so far */
 int ocl_nesta_run(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_out, int F, int C, int H,
                   int W, const ocl_nesta_params* prm, int form, ocl_nesta_report* report);
 

@@ -121,6 +121,8 @@ def lib():
         "ocl_scal": ([P, P, LG, F32, F32], I), "ocl_axpy": ([P, LG, F32, F32, P, P], I),
         "ocl_reduce": ([P, P, P, LG, I, ctypes.POINTER(D)], I),
         "ocl_normest": ([P, I, LG, ctypes.POINTER(D), ctypes.POINTER(I)], I),
+        "ocl_ctx_set_normest_stream": ([P, I], I),
+        "ocl_ctx_normest_stream_position": ([P, ctypes.POINTER(ctypes.c_longlong)], I),
         "ocl_nesta_run": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(NestaReport)],
                           I),
         "ocl_nesta_run_noalloc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I,
@@ -331,6 +333,16 @@ class Context:
         F, C = y.shape[:2]
         _check(lib().ocl_add_noise_masked(self.handle, y.ptr, M.ptr, F, C, y.size // (F * C), float(sigma),
                                           int(seed) & ((1 << 64) - 1)))
+
+    def set_normest_stream(self, carry=True):
+        """carry: one drand48 stream across this context's normests (the reference's process-wide stream), reset to
+        the default state; False: a fresh stream per estimate."""
+        _check(lib().ocl_ctx_set_normest_stream(self.handle, 1 if carry else 0))
+
+    def normest_stream_position(self):
+        v = ctypes.c_longlong()
+        _check(lib().ocl_ctx_normest_stream_position(self.handle, ctypes.byref(v)))
+        return v.value
 
     def normest(self, F, N):
         v = ctypes.c_double()

@@ -28,6 +28,9 @@ void CLapp::init(const PlatformTraits&, const DeviceTraits& dt) {
     if (ctx_) return;
     if (!dt.useHIP) throw CLError(OCL_E_INVALID, "CLapp::init: this build runs on HIP devices only");
     checkOcl(ocl_ctx_create(dt.deviceNumber, &ctx_), "CLapp::init");
+    // the reference never seeds drand48, so the myNormest restarts of every NestaUp::launch of a process continue one
+    // stream (NestaUp.cpp:375); the context carries that stream the same way
+    checkOcl(ocl_ctx_set_normest_stream(ctx_, 1), "CLapp::init");
 }
 
 CLapp::~CLapp() {

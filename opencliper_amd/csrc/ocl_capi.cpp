@@ -587,7 +587,9 @@ static const unsigned long long DRAND48_X0 = 0x1234ABCD330EULL;
 // recorded norms and takes the value at the iteration where the sequential loop stops (the batch's later iterations
 // only change the discarded vector).  An exact zero inside a batch (where the sequential loop would refill) restarts
 // the whole estimate host-driven, so the result is always that of the sequential loop.
-static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, const SparsityOp* op = nullptr) {
+// start_pos: draws already taken from the drand48 stream; *end_pos: where the stream is after the estimate.
+static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, const SparsityOp* op = nullptr,
+                           long long start_pos = 0, long long* end_pos = nullptr) {
     constexpr int NE_BATCH = 8;
     const long n = (long)F * N;
     float2 *x = nullptr, *Sx = nullptr, *tmp = nullptr;
@@ -614,7 +616,7 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, c
     for (int host_only = 0; host_only < 2; ++host_only) {
         double r[2];
         unsigned cnt = 0;
-        long drand_pos = 0;
+        long long drand_pos = start_pos;
         float e = 0.f, e0 = 0.f;
         bool restart = false;
         hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(OCL_NT), 0, c->stream, x, n, make_float2(1.f, 0.f));
@@ -676,6 +678,7 @@ static int normest_compute(ocl_ctx* c, int F, long N, double* out, int* iters, c
         if (restart) continue;
         *out = e;
         *iters = (int)cnt;
+        if (end_pos) *end_pos = drand_pos;
         break;
     }
     release();
@@ -697,15 +700,42 @@ int normest_mc(ocl_ctx* c, int F, int H, int W, const ocl_mc_args* mc, double* n
     SparsityOp sp{c, F, H, W, mc, {}, aux};
     sp.gatherR = gatherR;
     int rc = mc_geometry(H, W, mc, &sp.g);
-    if (rc == OCL_OK) rc = normest_compute(c, F, (long)H * W, normU, iters, &sp);
+    long long end = 0;
+    const long long start = c->normest_stream ? c->drand_pos : 0;
+    if (rc == OCL_OK) rc = normest_compute(c, F, (long)H * W, normU, iters, &sp, start, &end);
+    if (rc == OCL_OK && c->normest_stream) c->drand_pos = end;
     hipStreamSynchronize(c->stream);
     dev_free(aux);
     return rc;
 }
 
+extern "C" int ocl_ctx_set_normest_stream(ocl_ctx* c, int carry) {
+    CTXCHK(c);
+    c->normest_stream = carry ? 1 : 0;
+    c->drand_pos = 0;
+    return OCL_OK;
+}
+
+extern "C" int ocl_ctx_normest_stream_position(ocl_ctx* c, long long* position) {
+    CTXCHK(c);
+    if (!position) return fail(OCL_E_INVALID, "normest stream position: null output");
+    *position = c->drand_pos;
+    return OCL_OK;
+}
+
 extern "C" int ocl_normest(ocl_ctx* c, int F, long N, double* normU, int* iters) {
     CTXCHK(c);
     if (F < 1 || N < 1 || !normU) return fail(OCL_E_INVALID, "normest: bad arguments");
+    if (c->normest_stream) {             // the carried stream: the value depends on the position, nothing is cached
+        double v = 0;
+        int it = 0;
+        long long end = 0;
+        CHK(normest_compute(c, F, N, &v, &it, nullptr, c->drand_pos, &end));
+        c->drand_pos = end;
+        *normU = v;
+        if (iters) *iters = it;
+        return OCL_OK;
+    }
     {
         std::lock_guard<std::mutex> lk(c->mu);
         auto it = c->normU_cache.find({F, N});
@@ -786,8 +816,13 @@ static int nesta_plain(ocl_ctx* c, const void* y, const void* S, const uint8_t* 
     float mu0 = (float)(0.9 * (double)maxUXref);
     double nU = 0;
     int ne_it = 0;
-    if (mc) CHK(normest_compute(c, F, N, &nU, &ne_it, &sp));
-    else CHK(ocl_normest(c, F, N, &nU, &ne_it));
+    if (mc) {
+        long long end = 0;
+        CHK(normest_compute(c, F, N, &nU, &ne_it, &sp, c->normest_stream ? c->drand_pos : 0, &end));
+        if (c->normest_stream) c->drand_pos = end;
+    } else {
+        CHK(ocl_normest(c, F, N, &nU, &ne_it));
+    }
     const float normU = (float)nU;
     const float muL = lambda / (float)prm.La;
     if (muL > mu0) mu0 = muL;

@@ -35,7 +35,7 @@ int fail(int code, const std::string& msg);
 // Tile shape of the fused FP32 NESTA passes for one (H, W): the register four-step kernels (fast) or the LDS
 // Stockham kernels with tile_cols columns (pass CA) / rows_per_wg rows (pass B) per workgroup.  ca_fast: 0 Stockham,
 // 1 four-step gathering the sampled rows from global memory, 2 four-step prefetching the next coil's sampled rows
-// (through LDS) and sensitivities.
+// (through LDS) and sensitivities, 3 the Gram form (pass G) for line masks, the prefetching pass otherwise.
 struct TileCfg {
     int ca_fast = 2, tile_cols = 16;
     int b_fast = 1, rows_per_wg = 16;
@@ -51,6 +51,11 @@ struct ocl_ctx {
     double* red_out = nullptr;                              // reduction results [8][2]
     double* red_host = nullptr;                             // pinned mirror of red_out
     std::map<std::pair<int, long>, std::pair<double, int>> normU_cache;
+    // drand48 stream of myNormest's restarts: 0 = every normest starts from the POSIX default state (cached per
+    // (F, N)); 1 = one stream carried across the context's normests, as the reference's unseeded process-wide
+    // drand48 (NestaUp.cpp:375) is carried across the myNormest calls of a process (no cache then)
+    int normest_stream = 0;
+    long long drand_pos = 0;                                // draws taken from that stream so far
     std::mutex mu;
     int num_cus = 256;
     int autotune = 0;                                       // 1: tune the tile shape of each new (H, W) on first use

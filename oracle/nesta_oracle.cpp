@@ -430,11 +430,30 @@ template <class T> void op_Ut(const McOp* m, const Cx<T>* u, Cx<T>* out, Cx<T>* 
 // ------------------------------------------------------------------------------------------------------------------
 struct Drand48 {
     uint64_t x = 0x1234ABCD330EULL;
+    long long pos = 0;        // draws taken since the default state
     double next() {
         x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+        ++pos;
         return (double)x / 281474976710656.0;
     }
+    // the state after k draws from the default state: x -> a x + c applied k times, by squaring the affine map
+    void seek(long long k) {
+        const uint64_t m = (1ULL << 48) - 1;
+        uint64_t A = 1, Cc = 0;                 // accumulated map x -> A x + Cc
+        uint64_t a = 0x5DEECE66DULL, c = 0xBULL; // current power of the step map
+        for (long long e = k; e > 0; e >>= 1) {
+            if (e & 1) { A = (a * A) & m; Cc = (a * Cc + c) & m; }
+            c = ((a + 1) * c) & m;
+            a = (a * a) & m;
+        }
+        x = (A * 0x1234ABCD330EULL + Cc) & m;
+        pos = k;
+    }
 };
+// The reference never seeds drand48 (NestaUp.cpp:375), so every myNormest of a process continues one stream from the
+// POSIX default state.  The restatement keeps that stream here; the ctypes binding seeks it before a call (to 0 for a
+// reconstruction that runs first in its process, or to where a previous one left it).
+Drand48 g_drand48;
 
 // myNormest (NestaUp.cpp:291-439): power method for the spectral norm of U = tTV (or tTV o MC with mc, frames of
 // H x W), FP-typed like the reference.
@@ -447,7 +466,7 @@ template <class T> T normest(int F, long N, int* iters_out, const McOp* mc = nul
     T e = nrm2(x.data(), n);
     scal(n, (T)1 / e, x.data());
     T e0 = 0;
-    Drand48 rng;
+    Drand48& rng = g_drand48;
     while ((std::abs(e - e0) > (tol * e)) && (cnt < maxiter)) {
         e0 = e;
         if (mc) op_U(mc, x.data(), Sx.data(), auxMC.data(), F, H, W);
@@ -539,6 +558,12 @@ double orc_normest_f32(int F, long N, int* iters) { return normest<float>(F, N, 
 double orc_normest_f64(int F, long N, int* iters) { return normest<double>(F, N, iters); }
 double orc_drand48_nth(long k) {  // value of the k-th (1-based) drand48() call from the default state
     Drand48 r; double v = 0; for (long i = 0; i < k; ++i) v = r.next(); return v;
+}
+double orc_drand48_nth_seek(long k) {  // the same value through the jump-ahead of the stream (k >= 1)
+    Drand48 r; r.seek(k - 1); return r.next();
+}
+void orc_drand48_seek(long long k) { g_drand48.seek(k); }
+long long orc_drand48_position() { return g_drand48.pos;
 }
 
 }  // extern "C"

@@ -67,7 +67,8 @@ def lib():
             "orc_nrm2_f32": ([P, LG], D), "orc_dotc_re_f32": ([P, P, LG], D), "orc_asum_f32": ([P, LG], D),
             "orc_op_A_f64": ([P, P, P, P, I, I, I, I], None), "orc_op_At_f64": ([P, P, P, I, I, I, I], None),
             "orc_normest_f32": ([I, LG, ctypes.POINTER(I)], D), "orc_normest_f64": ([I, LG, ctypes.POINTER(I)], D),
-            "orc_drand48_nth": ([LG], D),
+            "orc_drand48_nth": ([LG], D), "orc_drand48_nth_seek": ([LG], D),
+            "orc_drand48_seek": ([ctypes.c_longlong], None), "orc_drand48_position": ([], ctypes.c_longlong),
             "orc_nesta_f32": ([P, P, P, P, I, I, I, I, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
             "orc_nesta_f64": ([P, P, P, P, I, I, I, I, ctypes.POINTER(Params), ctypes.POINTER(Report)], I),
             "orc_simple_recon_f32": ([P, P, P, I, I, I, I], I), "orc_simple_recon_f64": ([P, P, P, I, I, I, I], I),
@@ -168,15 +169,27 @@ def op_At(r, S):
     return out
 
 
-def normest(F, N, double=True):
+def drand48_position():
+    """Draws taken from the restatement's drand48 stream (the reference's unseeded, process-wide drand48)."""
+    return int(lib().orc_drand48_position())
+
+
+def normest(F, N, double=True, drand_start=0):
+    """myNormest; drand_start: draws already taken from the process's drand48 stream when it runs (0: the first
+    normest of a process), None: continue where the last call left it."""
+    if drand_start is not None:
+        lib().orc_drand48_seek(int(drand_start))
     it = ctypes.c_int(0)
     f = lib().orc_normest_f64 if double else lib().orc_normest_f32
     v = f(F, N, ctypes.byref(it))
     return v, it.value
 
 
-def nesta(y, S, M, prm: Params | None = None, double=True):
-    """Runs the NESTA-TV reconstruction; returns (x, report dict)."""
+def nesta(y, S, M, prm: Params | None = None, double=True, drand_start=0):
+    """Runs the NESTA-TV reconstruction; returns (x, report dict).  drand_start as for normest; the report's
+    "drand48_position" is where the stream is after the call."""
+    if drand_start is not None:
+        lib().orc_drand48_seek(int(drand_start))
     prm = prm or params()
     dt = np.complex128 if double else np.complex64
     y, S, M = _c(y, dt), _c(S, dt), _c(M, np.uint8)
@@ -187,7 +200,9 @@ def nesta(y, S, M, prm: Params | None = None, double=True):
     rc = f(_p(y), _p(S), _p(M), _p(x), F, C, H, W, ctypes.byref(prm), ctypes.byref(rep))
     if rc != 0:
         raise RuntimeError(f"oracle nesta failed ({rc})")
-    return x, rep.as_dict()
+    out = rep.as_dict()
+    out["drand48_position"] = drand48_position()
+    return x, out
 
 
 def simple_recon(y, S, double=True):
@@ -243,8 +258,10 @@ def mc(x, xn, mesh, bb, adjoint=False, double=True):
     return out
 
 
-def normest_mc(xn, mesh, bb):
-    """myNormest of U = tTV o MC in FP64."""
+def normest_mc(xn, mesh, bb, drand_start=0):
+    """myNormest of U = tTV o MC in FP64 (drand_start as for normest)."""
+    if drand_start is not None:
+        lib().orc_drand48_seek(int(drand_start))
     xn, mesh, bb = _c(xn, np.float32), _c(mesh, np.int32), _c(bb, np.uint32)
     _, F, H, W = xn.shape
     it = ctypes.c_int(0)
@@ -254,8 +271,11 @@ def normest_mc(xn, mesh, bb):
     return v, it.value
 
 
-def nesta_mc(y, S, M, x_ref, xn, mesh, bb, prm: Params | None = None, double=True):
-    """NESTA with motion compensation (argsMC != nullptr), starting from x_ref; returns (x, report dict)."""
+def nesta_mc(y, S, M, x_ref, xn, mesh, bb, prm: Params | None = None, double=True, drand_start=0):
+    """NESTA with motion compensation (argsMC != nullptr), starting from x_ref; returns (x, report dict).  drand_start
+    as for nesta."""
+    if drand_start is not None:
+        lib().orc_drand48_seek(int(drand_start))
     prm = prm or params()
     dt = np.complex128 if double else np.complex64
     y, S, M = _c(y, dt), _c(S, dt), _c(M, np.uint8)
@@ -267,7 +287,9 @@ def nesta_mc(y, S, M, x_ref, xn, mesh, bb, prm: Params | None = None, double=Tru
     rc = f(_p(y), _p(S), _p(M), _p(x), F, C, H, W, _p(xn), _p(mesh), _p(bb), ctypes.byref(prm), ctypes.byref(rep))
     if rc != 0:
         raise RuntimeError(f"oracle nesta_mc failed ({rc})")
-    return x, rep.as_dict()
+    out = rep.as_dict()
+    out["drand48_position"] = drand48_position()
+    return x, out
 
 
 def set_threads(n):

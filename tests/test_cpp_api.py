@@ -90,10 +90,18 @@ def test_driver_motion_compensated_nesta_matches_oracle(oracle, tmp_path):
     assert oracle.rel_l2(oracle.mc(x0, xn, mesh, bb), np.fromfile(outp + ".mc", np.complex64).reshape(shape)) < 1e-6
     assert oracle.rel_l2(oracle.mc(x0, xn, mesh, bb, adjoint=True),
                          np.fromfile(outp + ".mcadj", np.complex64).reshape(shape)) < 1e-6
-    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True)
+    # the C++ CLapp carries the drand48 stream like the reference's process: the MC-NESTA's myNormest continues where
+    # the first NESTA's left it (F N draws), so the restatement runs with the stream carried the same way
+    pos = sl.F * sl.H * sl.W
+    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True, drand_start=pos)
+    _, fresh = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True)
+    print(f"MC normU with the carried stream {rep['normU']:.7f}, from a fresh stream {fresh['normU']:.7f}", flush=True)
+    assert rep["drand48_position"] == 2 * pos
     x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
     assert oracle.rel_l2(x, x64) < 1e-3
     assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout
+    normU = float(r.stdout.split("normU ")[1].split()[0])
+    assert abs(normU - rep["normU"]) < 2e-5 * rep["normU"]
 
 
 @pytest.mark.gpu
@@ -118,7 +126,8 @@ def test_driver_motion_compensated_non_identity_mesh_falls_back_to_plain(oracle,
     shape = (sl.F, sl.H, sl.W)
     x0 = np.fromfile(outp + ".nomc", dtype=np.complex64).reshape(shape)
     prm = oracle.params(maxIntIter=2, maxIter=10, stoptest=1, miniter=7)
-    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True)
+    x64, rep = oracle.nesta_mc(sl.y, sl.S, sl.M, x0, xn, mesh, bb, prm, double=True,
+                               drand_start=sl.F * sl.H * sl.W)
     x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
     assert oracle.rel_l2(x, x64) < 1e-3
     assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout

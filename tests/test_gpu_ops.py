@@ -119,6 +119,55 @@ def test_normest_matches_oracle(ctx, oracle, F, N):
     assert abs(g - e) < 2e-6 * e
 
 
+def test_normest_carried_stream_matches_oracle(oracle):
+    """A context that carries the drand48 stream across estimates, as the reference's process does
+    (NestaUp.cpp:375): two estimates in a row equal the restatement's first and continued ones, and the position
+    counts the draws."""
+    c = capi.Context(0)
+    try:
+        c.set_normest_stream(True)
+        F, N = 20, 65536
+        g1, i1 = c.normest(F, N)
+        assert c.normest_stream_position() == F * N
+        g2, i2 = c.normest(F, N)
+        assert c.normest_stream_position() == 2 * F * N
+    finally:
+        c.close()
+    e1, j1 = oracle.normest(F, N, double=False)
+    e2, j2 = oracle.normest(F, N, double=False, drand_start=None)
+    print(f"carried normest: {g1:.7f} {g2:.7f} (oracle {e1:.7f} {e2:.7f})", flush=True)
+    assert (i1, i2) == (j1, j2)
+    assert abs(g1 - e1) < 2e-6 * e1 and abs(g2 - e2) < 2e-6 * e2
+
+
+@pytest.mark.parametrize("form", [capi.OCL_FORM_PLAIN, capi.OCL_FORM_FUSED])
+def test_repeated_nesta_carries_the_stream(oracle, form):
+    """Two NestaUp::launch calls in one process: the second one's myNormest continues the drand48 stream where the
+    first left it, so its normU (and result) follow the restatement run with the carried stream."""
+    sl = synth.make_cine(H=64, W=64, C=4, F=8, accel=4.0, slice_id=77)
+    prm = dict(maxIntIter=3, maxIter=10, stoptest=0)
+    c = capi.Context(0)
+    try:
+        c.set_normest_stream(True)
+        y, S, M = c.to_device(sl.y), c.to_device(sl.S), c.to_device(sl.M)
+        x = c.empty((sl.F, sl.H, sl.W))
+        r1 = c.nesta(y, S, M, x, capi.nesta_params(**prm), form=form)
+        r2 = c.nesta(y, S, M, x, capi.nesta_params(**prm), form=form)
+        got = x.get()
+        pos = c.normest_stream_position()
+    finally:
+        c.close()
+    x1, o1 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
+    x2, o2 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True, drand_start=None)
+    print(f"form {form}: normU {r1['normU']:.7f} then {r2['normU']:.7f} (oracle {o1['normU']:.7f} "
+          f"{o2['normU']:.7f}), rel-L2 vs FP64 {oracle.rel_l2(got, x2):.2e}", flush=True)
+    assert pos == o2["drand48_position"] == 2 * sl.F * sl.H * sl.W
+    assert abs(r1["normU"] - o1["normU"]) < 2e-6 * o1["normU"]
+    assert abs(r2["normU"] - o2["normU"]) < 2e-6 * o2["normU"]
+    assert r2["normU"] != r1["normU"]
+    assert oracle.rel_l2(got, x2) < 1e-4
+
+
 @pytest.mark.parametrize("H,W,C,F", [(256, 256, 8, 1), (160, 96, 3, 2), (128, 256, 4, 9)])
 def test_simple_mri_recon(ctx, oracle, H, W, C, F):
     sl = synth.make_fully_sampled(H=H, W=W, C=C, F=F)

@@ -73,6 +73,23 @@ def test_drand48_matches_libc(oracle):
     assert got == [ref[0], ref[1], ref[2], ref[499], ref[999]]
 
 
+def test_drand48_jump_ahead_and_carried_stream(oracle):
+    """The restatement keeps the reference's process-wide drand48 stream (NestaUp.cpp:375, never seeded): the jump-ahead
+    to draw k equals k sequential draws (libc's own stream above pins those), a normest of F x N pixels takes F N draws
+    (one restart: U of the constant start vector is zero), and a second normest continuing the stream starts from a
+    different random vector, so its estimate differs in the last digits while staying under sigma_max = 2."""
+    L = oracle.lib()
+    for k in (1, 2, 3, 1000, 123457, 2**22 + 5):
+        assert L.orc_drand48_nth_seek(k) == L.orc_drand48_nth(k)
+    F, N = 20, 4096
+    e1, _ = oracle.normest(F, N)
+    assert oracle.drand48_position() == F * N
+    e2, _ = oracle.normest(F, N, drand_start=None)
+    assert oracle.drand48_position() == 2 * F * N
+    assert e2 != e1 and abs(e2 - e1) < 1e-3 * e1 and e2 <= 2.0
+    assert oracle.normest(F, N, drand_start=F * N)[0] == e2
+
+
 @pytest.mark.parametrize("F", [3, 4, 7, 8, 20, 25])
 def test_normest_power_method(oracle, F):
     N = 256
