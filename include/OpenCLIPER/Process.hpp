@@ -10,6 +10,8 @@ class Process : public ProcessCore {
     virtual ~Process() {}
     const std::shared_ptr<CLapp> getApp() const { return app_; }
     void setApp(const std::shared_ptr<CLapp>& pCLapp) { app_ = pCLapp; }
+    /// The reference names the OpenCL source file a Process compiles at run time.  Here every kernel is compiled ahead
+    /// of time for gfx950 into libocl_amd.so, so no Process has a source file to name: this returns "" for all of them.
     virtual const std::string getKernelFile() const { return ""; }
 
     template <typename T>

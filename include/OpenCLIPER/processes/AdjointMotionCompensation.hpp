@@ -14,7 +14,6 @@ class AdjointMotionCompensation : public Process {
         explicit LaunchParameters(ArgumentsMotionCompensation* args) : argsMC(args) {}
     };
     void launch() override;
-    const std::string getKernelFile() const override { return "optimizer.cl"; }
     OCL_PROCESS_BOILERPLATE(AdjointMotionCompensation)
 };
 

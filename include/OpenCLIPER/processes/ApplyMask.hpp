@@ -12,7 +12,6 @@ class ApplyMask : public Process {
         explicit LaunchParameters(const std::shared_ptr<SamplingMasksData>& m) : samplingMasksData(m) {}
     };
     void launch() override;
-    const std::string getKernelFile() const override { return "applyMask.cl"; }
     OCL_PROCESS_BOILERPLATE(ApplyMask)
 };
 

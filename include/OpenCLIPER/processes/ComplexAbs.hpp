@@ -7,7 +7,6 @@ namespace OpenCLIPER {
 class ComplexAbs : public Process {
   public:
     void launch() override;
-    const std::string getKernelFile() const override { return "complexAbs.cl"; }
     OCL_PROCESS_BOILERPLATE(ComplexAbs)
 };
 

@@ -7,7 +7,6 @@ namespace OpenCLIPER {
 class ComplexAbsPow2 : public Process {
   public:
     void launch() override;
-    const std::string getKernelFile() const override { return "complexAbsPow2.cl"; }
     OCL_PROCESS_BOILERPLATE(ComplexAbsPow2)
 };
 

@@ -15,7 +15,6 @@ class ComplexElementProd : public Process {
             : conjugateSensMap(c), sensitivityMapsData(m) {}
     };
     void launch() override;
-    const std::string getKernelFile() const override { return "complexElementProd.cl"; }
     OCL_PROCESS_BOILERPLATE(ComplexElementProd)
 };
 

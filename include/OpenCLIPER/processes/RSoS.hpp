@@ -8,7 +8,6 @@ namespace OpenCLIPER {
 class RSoS : public Process {
   public:
     void launch() override;
-    const std::string getKernelFile() const override { return "rss.cl"; }
     OCL_PROCESS_BOILERPLATE(RSoS)
 };
 

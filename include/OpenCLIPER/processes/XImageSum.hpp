@@ -7,7 +7,6 @@ namespace OpenCLIPER {
 class XImageSum : public Process {
   public:
     void launch() override;
-    const std::string getKernelFile() const override { return "xImageSum.cl"; }
     OCL_PROCESS_BOILERPLATE(XImageSum)
 };
 

@@ -7,7 +7,6 @@ namespace OpenCLIPER {
 class CopyDataGPU : public Process {
   public:
     void launch() override;
-    const std::string getKernelFile() const override { return "initialize.cl"; }
     OCL_PROCESS_BOILERPLATE(CopyDataGPU)
 };
 

@@ -14,7 +14,6 @@ class TemporalTV : public Process {
     };
     void init() override;
     void launch() override;
-    const std::string getKernelFile() const override { return "temporalTV.cl"; }
     OCL_PROCESS_BOILERPLATE(TemporalTV)
 };
 

@@ -11,7 +11,6 @@ class VectorNormalization : public Process {
         LaunchParameters(float mu) : mu(mu) {}
     };
     void launch() override;
-    const std::string getKernelFile() const override { return "vectorNormalization.cl"; }
     OCL_PROCESS_BOILERPLATE(VectorNormalization)
 };
 

@@ -244,6 +244,10 @@ int ocl_server_wait(ocl_server* s, long request_id, double timeout_s, int* statu
 int ocl_server_request_times(ocl_server* s, long request_id, double* t_submit, double* t_admit, double* t_done);
 int ocl_server_get_stats(ocl_server* s, ocl_server_stats* stats);
 int ocl_server_destroy(ocl_server* s);      /* finishes every submitted request, then stops */
+/* failure handling without a failing device: the worker takes `message` as a device error at its next cycle, as it
+ * takes a failed launch or copy: every running and queued request fails with OCL_E_HIP, the worker stops, and later
+ * submissions are refused with OCL_E_HIP.  For tests of that path. */
+int ocl_server_inject_failure(ocl_server* s, const char* message);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,7 @@ def lib():
         "ocl_server_wait": ([P, LG, D, ctypes.POINTER(I)], I),
         "ocl_server_request_times": ([P, LG, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)], I),
         "ocl_server_get_stats": ([P, ctypes.POINTER(ServerStats)], I),
+        "ocl_server_inject_failure": ([P, ctypes.c_char_p], I),
         "ocl_server_destroy": ([P], I),
     }
     for name, (a, r) in sig.items():
@@ -507,6 +508,9 @@ class Server:
     def report(self, rid):
         rep = self.keep[rid][5]
         return rep.as_dict() if rep is not None else None
+
+    def inject_failure(self, message: str):
+        _check(lib().ocl_server_inject_failure(self.handle, message.encode()))
 
     def stats(self):
         st = ServerStats()

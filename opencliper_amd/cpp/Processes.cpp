@@ -310,11 +310,17 @@ void NestaUp::launch() {
                                   W, &p, OCL_FORM_PLAIN, &m, rep.get());
         checkOcl(rc, "NestaUp::launch");
     } else {
-        checkOcl(ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
+        // sizes the fused or split form does not take (H > 512, W > 1024, a tile beyond LDS, split sizes other than
+        // 64..256) run in the plain form, which takes every size the reference's FFT does
+        const int form = form_ == FUSED ? OCL_FORM_FUSED : form_ == SPLIT ? OCL_FORM_SPLIT : OCL_FORM_PLAIN;
+        int rc = ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
                                (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), F, (int)k->getNCoils(), H, W,
-                               &p, form_ == FUSED ? OCL_FORM_FUSED : form_ == SPLIT ? OCL_FORM_SPLIT : OCL_FORM_PLAIN,
-                               rep.get()),
-                 "NestaUp::launch");
+                               &p, form, rep.get());
+        if (rc == OCL_E_UNSUPPORTED && form != OCL_FORM_PLAIN)
+            rc = ocl_nesta_run(ctxOf(*this), k->getDeviceBuffer(), S->getDeviceBuffer(),
+                               (const uint8_t*)M->getDeviceBuffer(), x->getDeviceBuffer(), F, (int)k->getNCoils(), H, W,
+                               &p, OCL_FORM_PLAIN, rep.get());
+        checkOcl(rc, "NestaUp::launch");
     }
     stopProfiling();
     lp->tolVar = (float)rep->final_tolVar;   // the reference overwrites the caller's tolVar

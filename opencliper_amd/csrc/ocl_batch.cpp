@@ -129,6 +129,8 @@ struct SliceRes {
 #ifndef OCL_GROUP_MIN_WG
 #define OCL_GROUP_MIN_WG 2048
 #endif
+static_assert(OCL_U_CHUNK >= 1, "OCL_U_CHUNK: at least one pixel per temporal-TV workgroup");
+static_assert(OCL_GROUP_MIN_WG >= 1, "OCL_GROUP_MIN_WG: a launch group needs at least one workgroup");
 
 // Tile shape of the generic (non four-step) passes; the four-step passes always use 16 x 16.
 void choose_tiles(long total_cols, const ocl_slice& s, int* tile_cols, int* rows_per_wg) {
@@ -1304,6 +1306,9 @@ struct ocl_server {
     std::condition_variable cv, done_cv;
     std::thread th;
     bool stop = false;
+    bool dead = false;                       // the worker stopped after a device error; no request is accepted
+    std::string dead_msg;
+    std::string injected;                    // a failure the worker takes as a device error (ocl_server_inject_failure)
     std::deque<ServeReq*> queue;
     std::map<long, std::unique_ptr<ServeReq>> reqs;
     long next_id = 1;
@@ -1319,7 +1324,8 @@ struct ocl_server {
 // Marks every request that has not completed as failed (device error in the worker).
 static void server_fail_all(ocl_server* sv, std::vector<ServeReq*>& active) {
     std::lock_guard<std::mutex> lk(sv->mu);
-    const std::string msg = std::string("server: device error: ") + ocl_last_error();
+    const std::string msg = std::string("server: device error: ") + (sv->injected.empty() ? ocl_last_error()
+                                                                                         : sv->injected.c_str());
     for (ServeReq* r : active) {
         r->status = OCL_E_HIP;
         r->err = msg;
@@ -1335,6 +1341,8 @@ static void server_fail_all(ocl_server* sv, std::vector<ServeReq*>& active) {
     sv->queue.clear();
     active.clear();
     sv->used = 0;
+    sv->dead = true;
+    sv->dead_msg = msg;
     sv->done_cv.notify_all();
 }
 
@@ -1347,6 +1355,15 @@ static void server_loop(ocl_server* sv) {
     std::vector<NState> h;
     bool dirty = false;
     for (;;) {
+        bool inject = false;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            inject = !sv->injected.empty();
+        }
+        if (inject) {
+            server_fail_all(sv, active);
+            break;
+        }
         // ---- admission: first come first served while the footprint fits the budget ----
         std::vector<ServeReq*> admitted;
         for (;;) {
@@ -1499,6 +1516,7 @@ extern "C" int ocl_server_submit(ocl_server* sv, const ocl_slice* s, const ocl_n
     r->prm = *prm;
     r->rep = report;
     std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->dead) return fail(OCL_E_HIP, "server_submit: the server stopped after a device error (" + sv->dead_msg + ")");
     if (sv->stop) return fail(OCL_E_INVALID, "server_submit: server is stopping");
     r->id = sv->next_id++;
     r->t_submit = sv->now();
@@ -1558,6 +1576,14 @@ extern "C" int ocl_server_get_stats(ocl_server* sv, ocl_server_stats* st) {
     st->busy_s = sv->busy_s;
     st->elapsed_s = sv->now();
     st->budget_waits = sv->budget_waits;
+    return OCL_OK;
+}
+
+extern "C" int ocl_server_inject_failure(ocl_server* sv, const char* message) {
+    if (!sv || !message || !*message) return fail(OCL_E_INVALID, "server_inject_failure: bad arguments");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->injected = message;
+    sv->cv.notify_all();
     return OCL_OK;
 }
 

@@ -106,6 +106,15 @@ struct SliceDesc {
 #define OCL_MC_TC 64
 #endif
 #define OCL_MC_CHUNK 1024    // pixel-frames per workgroup of the motion-compensation passes
+// The build-time constants above may be overridden with -D; the kernels rely on these properties of them:
+static_assert(OCL_U_FRAMES >= 1, "OCL_U_FRAMES: at least one frame per temporal-TV workgroup");
+static_assert(OCL_MC_TR >= 1 && OCL_MC_TC >= 1 && (OCL_MC_TR * OCL_MC_TC) % OCL_NT == 0,
+              "OCL_MC_TR x OCL_MC_TC: the gather writes TR x TC / OCL_NT pixels per thread, so the tile must be a multiple "
+              "of the workgroup size");
+static_assert(2 * sizeof(float) * 2 * (OCL_MC_TR + 2 * OCL_MC_GATHER_RMAX) * (OCL_MC_TC + 2 * OCL_MC_GATHER_RMAX) <=
+                  64 * 1024,
+              "OCL_MC_TR x OCL_MC_TC: the gather's static LDS staging (positions and values of the tile plus its halo at "
+              "the largest radius) must fit 64 KB");
 
 // kernels (ocl_ops.hip)
 __global__ void k_ttv(const float2* in, float2* out, int F, long N, int dir);

@@ -59,6 +59,24 @@ def test_driver_nesta_matches_oracle(oracle, tmp_path, form):
     assert np.array_equal(v, np.roll(d, -1, axis=0) - d)
 
 
+@pytest.mark.gpu
+def test_driver_fused_falls_back_to_plain_beyond_its_sizes(oracle, tmp_path):
+    """NestaUp asks for the fused form by default; H = 576 is beyond it (H <= 512) but a length the reference's FFT
+    takes (2^6 3^2), so the launch runs in the plain form instead of failing, and matches the FP64 oracle."""
+    sl = synth.make_cine(H=576, W=64, C=2, F=3, accel=4.0, n_centre=8, slice_id=47)
+    inp, outp = str(tmp_path / "in.raw"), str(tmp_path / "out.raw")
+    _write_input(inp, sl)
+    args = [DRIVER, inp, outp, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "2", "5", "0", "7", "1"]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    x = np.fromfile(outp, dtype=np.complex64).reshape(sl.F, sl.H, sl.W)
+    x64, rep = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(maxIntIter=2, maxIter=5, stoptest=0, miniter=7),
+                            double=True)
+    assert oracle.rel_l2(x, x64) < 1e-4
+
+
 DRIVER_MC = os.path.join(PKG, "examples", "mri_recon_mc")
 
 

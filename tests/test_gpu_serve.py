@@ -68,3 +68,24 @@ def test_device_noise_helper_statistics(ctx):
     assert np.all(v[:, :, ::2] == 0)
     assert abs(np.mean(np.abs(on) ** 2) - 0.25) < 0.01       # E|sigma n|^2 = sigma^2
     assert abs(np.mean(on.real)) < 0.01 and abs(np.mean(on.imag)) < 0.01
+
+
+def test_server_refuses_requests_after_a_device_error(ctx):
+    """After a device error the worker fails every running and queued request and stops; a later submission is
+    refused at once (it would otherwise wait forever).  The error is injected from the host, so no kernel fails."""
+    sl = synth.make_cine(H=64, W=64, C=2, F=4, accel=3.0, slice_id=71)
+    dev = (ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M), ctx.empty((sl.F, sl.H, sl.W)))
+    srv = capi.Server(0, budget_bytes=1 << 30, chunk=2)
+    try:
+        rid = srv.submit(*dev, capi.nesta_params(maxIntIter=5, maxIter=200, stoptest=0))
+        srv.inject_failure("injected for the test")
+        with pytest.raises(capi.OclError) as e:
+            srv.wait(rid, 60.0)
+        assert e.value.code == -2 and "injected" in str(e.value)
+        with pytest.raises(capi.OclError) as e:
+            srv.submit(*dev, capi.nesta_params(maxIntIter=1, maxIter=2, stoptest=0))
+        assert e.value.code == -2 and "stopped after a device error" in str(e.value)
+        st = srv.stats()
+        assert st["failed"] == 1 and st["completed"] == 0
+    finally:
+        srv.close()
