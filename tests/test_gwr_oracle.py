@@ -1,0 +1,207 @@
+"""Pins of the groupwise-registration oracle (oracle/gwr_oracle.py), CPU only.
+
+Each test checks the restatement against something the reference's source does not restate: closed forms of the
+B-spline tables, the identity mesh, finite differences of the cost, recovery of known motion, hand-computed step
+logic, and the exact cancellation of the regularisation gradient by the frame-mean projection.
+"""
+import numpy as np
+import pytest
+
+from oracle import gwr_oracle as G
+
+DELTA = 2.0 / 3.0 - 0.6666          # the reference's cubic B-spline uses 0.6666 for 2/3 (bspline.cpp:117)
+
+
+def blobs(N, F, shifts, seed=3, n=14, sig=(3.5, 6.0)):
+    """Frames of one smooth texture (Gaussian blobs around the centre), frame f translated by shifts[f] = (dx, dy)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:N, 0:N].astype(np.float64)
+    c = N / 2
+    spec = [(c + rng.uniform(-N / 5, N / 5), c + rng.uniform(-N / 5, N / 5), rng.uniform(*sig), rng.uniform(0.3, 1))
+            for _ in range(n)]
+    out = []
+    for f in range(F):
+        sx, sy = shifts[f]
+        img = np.zeros((N, N))
+        for cy, cx, s, a in spec:
+            img += a * np.exp(-((yy - cy - sy) ** 2 + (xx - cx - sx) ** 2) / (2 * s * s))
+        out.append(img * np.exp(1j * 0.3))
+    return np.array(out).astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def reg():
+    return G.Registration(64, 4, radius=16, nmax=30)
+
+
+def test_geometry(reg):
+    # circle of radius 16 about (32.5, 32.5): rows/columns 17..48 (1-based), control grid 11 x 11, support 17
+    assert reg.bb == [17, 17, 48, 48]
+    assert (reg.Cp0, reg.dp0) == (11, 17)
+    assert reg.X.sum() == reg.areaX
+    m = reg.X.reshape(64, 64)
+    assert np.array_equal(m, m.T)                 # the ROI is symmetric under transposition
+    assert np.isclose(reg.Wn0, np.float32(1.0) / np.float32(reg.areaX))
+
+
+def test_bspline_known_values():
+    assert G.bspline3(0.0) == np.float32(0.6666)
+    assert np.isclose(G.bspline3(1.0), 1 / 6, rtol=1e-6) and G.bspline3(2.0) == 0
+    assert np.isclose(G.bspline3(0.5), 0.5 * 0.125 - 0.25 + 0.6666, rtol=1e-6)
+    assert G.b2spline3(0.0) == -2 and np.isclose(G.b2spline3(1.5), 0.5)
+
+
+def test_bspline_taps_closed_forms(reg):
+    N, E1 = reg.N, reg.E + 1
+    tab = reg.BBAux.reshape(reg.nt, 3, N, E1)        # [l][value, d1, d2][pixel][tap]
+    for l in range(reg.nt):
+        lo = reg.coef_raw[np.arange(N) + l * N * 2]
+        u = ((np.arange(N) + 1 - reg.c[l]) / reg.Dp[l]).astype(np.float64)
+        args = u[:, None] - (lo[:, None] + np.arange(E1)[None, :])
+        n1 = (np.abs(args) < 1).sum(axis=1)
+        assert np.allclose(tab[l, 0].sum(axis=1), 1 - DELTA * n1, atol=2e-7)
+        assert np.allclose(tab[l, 1].sum(axis=1), 0, atol=2e-6)
+        assert np.allclose(tab[l, 2].sum(axis=1), 0, atol=2e-6)
+        # derivative taps against central differences of the value function
+        h = 1e-3
+        num1 = (G.bspline3(args + h).astype(np.float64) - G.bspline3(args - h)) / (2 * h)
+        assert np.allclose(tab[l, 1], num1, atol=2e-3)
+        num2 = (G.b1spline3(args + h).astype(np.float64) - G.b1spline3(args - h)) / (2 * h)
+        assert np.allclose(tab[l, 2], num2, atol=2e-2)
+
+
+def test_identity_and_constant_displacement(reg):
+    N, F, Cp = reg.N, reg.F, reg.Cp0
+    x = blobs(N, F, np.zeros((F, 2)))
+    I0 = reg.normalize(x)
+    xn = reg.deform(np.zeros(Cp * Cp * F * 2)).reshape(2, F, N, N)
+    r, c = np.mgrid[0:N, 0:N]
+    assert np.array_equal(xn[0], np.broadcast_to(c + 1.0, (F, N, N)))
+    assert np.array_equal(xn[1], np.broadcast_to(r + 1.0, (F, N, N)))
+    assert np.array_equal(reg.interpolator(I0, xn.ravel()), I0)
+    # constant coefficients t_d: box pixels move by t_d * (row tap sum) * (column tap sum)
+    t = (0.37, -0.21)
+    T = np.concatenate([np.full(Cp * Cp * F, t[0]), np.full(Cp * Cp * F, t[1])])
+    xn = reg.deform(T).reshape(2, F, N, N)
+    s = reg.BBAux.reshape(reg.nt, 3, N, reg.E + 1)[0, 0].sum(axis=1).astype(np.float64)
+    box = np.arange(reg.bb[0] - 1, reg.bb[2])
+    rr, cc = np.meshgrid(box, box, indexing="ij")
+    for d in range(2):
+        base = (cc + 1.0) if d == 0 else (rr + 1.0)
+        disp = xn[d][:, rr, cc] - base
+        assert np.allclose(disp, t[d] * s[rr] * s[cc], atol=1e-12)
+    # outside the box the mesh is untouched
+    assert xn[0][0, 0, 0] == 1.0 and xn[1][0, N - 1, 5] == N
+
+
+def test_metric_gradient_matches_finite_differences(reg):
+    N, F, Cp = reg.N, reg.F, reg.Cp0
+    rng = np.random.default_rng(1)
+    x = blobs(N, F, rng.uniform(-1, 1, (F, 2)), seed=5)
+    I0 = reg.normalize(x)
+    T = rng.normal(scale=0.3, size=Cp * Cp * F * 2)
+    g = reg.metric_gradient(I0, T)
+    h = 1e-3
+    H = lambda TT: reg.cost_at(I0, TT)[0]  # noqa: E731
+    big = np.argsort(-np.abs(g))[:200]
+    idx = rng.choice(big, 24, replace=False)
+    num = np.array([(H(T + h * np.eye(1, T.size, e)[0]) - H(T - h * np.eye(1, T.size, e)[0])) / (2 * h)
+                    for e in idx])
+    ana = g[idx]
+    # the reference differentiates with central differences of the image, interpolated: a few percent from the
+    # exact derivative of the bilinear interpolation on smooth images
+    assert np.corrcoef(ana, num)[0, 1] > 0.998
+    assert np.linalg.norm(ana - num) / np.linalg.norm(num) < 0.06
+    # a plausible mistake fails it: the gradient of the other displacement component
+    swapped = np.concatenate([g[Cp * Cp * F:], g[:Cp * Cp * F]])[idx]
+    assert np.linalg.norm(swapped - num) / np.linalg.norm(num) > 0.5
+
+
+def test_known_motion_is_recovered():
+    N, F = 64, 6
+    rng = np.random.default_rng(0)
+    shifts = rng.uniform(-0.8, 0.8, size=(F, 2))
+    x = blobs(N, F, shifts, seed=9, n=24)
+    reg = G.Registration(N, F, radius=16, nmax=80, W=0.05)
+    out = reg.run(x)
+    I0 = out["I0"].reshape(F, N, N)
+    It = out["transformed"].reshape(F, N, N)
+    m = reg.X.reshape(N, N).astype(bool)
+    before = I0.var(axis=0)[m].sum()
+    after = It.var(axis=0)[m].sum()
+    assert after < before / 50
+    xn = out["xn"].reshape(2, F, N, N)
+    r, c = np.mgrid[0:N, 0:N]
+    for d, base in ((0, c + 1.0), (1, r + 1.0)):
+        u = np.median((xn[d] - base)[:, m], axis=1)          # median displacement over the ROI, per frame
+        want = shifts[:, d] - shifts[:, d].mean()            # sample position = pixel + shift, up to the mean
+        # free-form deformation leaves low-texture pixels loosely determined: the per-frame displacement follows the
+        # shift (a swapped axis or a flipped sign would not), within a third of a pixel
+        assert np.corrcoef(u, want)[0, 1] > 0.95, (d, u, want)
+        assert np.abs(u - want).max() < 0.4, (d, u, want)
+    assert out["H"][-1] < out["H"][0]
+
+
+def test_step_logic():
+    Tdif = np.zeros(3, np.float32)
+    Hdif = np.zeros(3, np.float32)
+    H = np.array([10.0, 8.0, 9.0], np.float32)
+    Wn = G.evolution(Tdif, Hdif, np.float32(0.5), np.array([3.0, 4.0]), H, 1, True)
+    assert Tdif[0] == 5 and Hdif[0] == 2 and Wn == np.float32(0.5 * 1.2)
+    Wn = G.evolution(Tdif, Hdif, Wn, np.array([0.0]), H, 2, True)
+    assert Hdif[1] == -1 and Wn == np.float32(np.float32(0.6) / 2)
+    assert G.evolution(Tdif, Hdif, np.float32(0.5), np.array([1.0]), H, 2, False) == np.float32(0.5)
+    # stop at nmax, or when both the step norm and the cost decrease fall below their thresholds
+    assert G.stop_condition(1.0, 1.0, 0.1, 0.1, 5, 5)
+    assert not G.stop_condition(1.0, 0.01, 0.1, 0.1, 5, 4)
+    assert G.stop_condition(0.01, -3.0, 0.1, 0.1, 5, 4)        # a cost increase counts as below the threshold
+    # sumLambda accumulates into an int: the weights of tests/MRIRecon.cpp:125 sum to 0
+    assert G.sum_lambda([0.0, 0.005, 0.0, 0.5]) == 0
+    assert G.sum_lambda([0.6, 0.6, 0.6, 0.6]) == 0
+    assert G.sum_lambda([1.5, 0.7, 0.0, 0.0]) == 1
+
+
+def test_regularisation_terms_of_simple_fields(reg):
+    N, F, Cp = reg.N, reg.F, reg.Cp0
+    box = np.arange(reg.bb[0] - 1, reg.bb[2])
+    rr, cc = np.meshgrid(box, box, indexing="ij")
+
+    def dtau_of(T):
+        TAux = reg.transformationAux(T)
+        P = reg.permuteTAux(TAux)
+        return reg.regularization(P, *reg.shiftTAux(P))
+    dx, dx2, dxy, dt, dt2 = dtau_of(np.zeros(Cp * Cp * F * 2))
+    n4 = N * N * F * 2
+    J = dx.reshape(2, 2, F, N, N)
+    for p in range(2):
+        for d in range(2):
+            assert np.all(J[p, d][:, rr, cc] == (1.0 if p == d else 0.0))
+    assert not dx2.any() and not dxy.any() and not dt.any() and not dt2.any()
+    # coefficients equal over frames: no temporal terms
+    rng = np.random.default_rng(2)
+    T = np.tile(rng.normal(size=(2, 1, Cp * Cp)), (1, F, 1)).ravel()
+    _, _, _, dt, dt2 = dtau_of(T)
+    assert np.abs(dt).max() < 1e-12 and np.abs(dt2).max() < 1e-12
+    assert dx.size == 2 * n4
+
+
+def test_regularisation_gradient_is_removed_by_the_projection():
+    N, F = 64, 4
+    x = blobs(N, F, np.random.default_rng(4).uniform(-1, 1, (F, 2)), seed=4)
+    a = G.Registration(N, F, radius=16, nmax=6, W=0.05, lam=(0.0, 0.005, 0.0, 0.5)).run(x)
+    b = G.Registration(N, F, radius=16, nmax=6, W=0.05, lam=(0.0, 0.0, 0.0, 0.0)).run(x)
+    assert a["iters"] == b["iters"]
+    assert np.allclose(a["H"], b["H"], rtol=1e-6)
+    assert np.linalg.norm(a["T"] - b["T"]) <= 1e-9 * np.linalg.norm(b["T"])
+    # with an int-accumulated weight sum of 1 the regularisation enters the cost (SumLambdaMetric)
+    c = G.Registration(N, F, radius=16, nmax=6, W=0.05, lam=(1.5, 0.0, 0.0, 0.0)).run(x)
+    assert not np.allclose(c["H"][1:], b["H"][1:], rtol=1e-3)
+
+
+def test_unsupported_configurations():
+    with pytest.raises(NotImplementedError):
+        G.Registration(64, 4, radius=0)
+    with pytest.raises(NotImplementedError):
+        G.Registration(64, 4, radius=16, E=2)
+    with pytest.raises(NotImplementedError):
+        G.Registration(64, 4, radius=30)           # supports leave the frame
