@@ -64,7 +64,7 @@ typedef struct {
 /* ---- context: replaces CLapp::create/init (src/CLapp.cpp:139-569): one device, one in-order stream ---- */
 const char* ocl_last_error(void);
 int ocl_ctx_create(int device, ocl_ctx** out);
-int ocl_ctx_destroy(ocl_ctx* ctx);
+int ocl_ctx_destroy(ocl_ctx* ctx);   /* destroy the context's batches and registrations first */
 int ocl_ctx_sync(ocl_ctx* ctx);
 void* ocl_ctx_stream(ocl_ctx* ctx);                       /* CLapp::getCommandQueue -> hipStream_t */
 int ocl_device_info(ocl_ctx* ctx, char* name, size_t name_len, size_t* total_mem, int* num_cus);
@@ -176,6 +176,54 @@ int ocl_adjoint_motion_compensation(ocl_ctx* ctx, const void* in, void* out, int
  * OCL_FORM_FUSED (the original, identity mesh that GroupwiseRegistration produces); OCL_FORM_SPLIT is unsupported. */
 int ocl_nesta_run_mc(ocl_ctx* ctx, const void* y, const void* S, const uint8_t* M, void* x_io, int F, int C, int H,
                      int W, const ocl_nesta_params* prm, int form, const ocl_mc_args* mc, ocl_nesta_report* report);
+
+/* ---- groupwise B-spline registration (GroupwiseRegistration, src/processes/GroupwiseRegistration.cpp:93-615,
+ * groupwiseRegistration/{creabspline,optimizer}.cpp): estimates the motion that ocl_mc_args carries ----
+ * GroupwiseRegistration::InitParameters (include/OpenCLIPER/processes/GroupwiseRegistration.hpp:145-158), same order:
+ * W step weight (Wn = W / ROI area), flagW adaptive Wn, radius of the circular ROI, E spline order, Dp control
+ * spacing, nmax iterations, et step-norm and eh cost-decrease thresholds, lambda regularisation weights. */
+typedef struct {
+    float W;
+    int flagW;
+    unsigned radius, E;
+    int Dp[2];
+    unsigned nmax;
+    float et, eh;
+    float lambda[4];
+} ocl_gwr_params;
+typedef struct ocl_gwr ocl_gwr;
+typedef struct {
+    int Cp;                  /* control points per dimension */
+    int support;             /* pixels per dimension of a control point's support */
+    int box;                 /* side of the ROI's bounding box */
+    int areaX;               /* ROI pixels */
+    int use_lambda;          /* the regularisation enters the cost (int-accumulated weight sum != 0) */
+    unsigned bound_box[4];   /* ArgumentsMotionCompensation::bound_box, 1-based */
+    long T_elems;            /* 2 * F * Cp * Cp floats: T [2][F][Cp][Cp] (the reference's TData {Cp1, Cp0, F, 2}) */
+    long xn_elems;           /* 2 * F * H * W floats: xn [2][F][H][W] (ArgumentsMotionCompensation::xnData) */
+    const int* mesh;         /* device int32 [2][W][H]: ArgumentsMotionCompensation::xData (the identity mesh) */
+} ocl_gwr_info_t;
+#define OCL_GWR_TRACE 1024
+/* What GroupwiseRegistration's optimizer prints per iteration (optimizer.cpp:350-356): the cost H (H[0] before the
+ * loop), the step norm Tdif, the cost decrease Hdif and Wn after adaptation, for the first OCL_GWR_TRACE - 1
+ * iterations. */
+typedef struct {
+    int iters, ntrace;
+    float H0, Wn, oldmax;
+    float H[OCL_GWR_TRACE];
+    float Tdif[OCL_GWR_TRACE], Hdif[OCL_GWR_TRACE], Wn_trace[OCL_GWR_TRACE];
+} ocl_gwr_report;
+/* GroupwiseRegistration::init + creabspline: the ROI, control grid and B-spline tables of an F x H x W series.
+ * Supported: a square image, radius > 0, E = 3, Dp[0] == Dp[1], 2 <= F <= 64, and a geometry whose control points,
+ * supports and interpolation margin stay inside the frame (OCL_E_UNSUPPORTED otherwise, with the reason). */
+int ocl_gwr_create(ocl_ctx* ctx, int F, int H, int W, const ocl_gwr_params* prm, ocl_gwr** out);
+int ocl_gwr_info(const ocl_gwr* g, ocl_gwr_info_t* info);
+/* GroupwiseRegistration::launch: x device complex64 [F][H][W] (the reconstruction); T device float [2][F][Cp][Cp],
+ * read as the starting coefficients when T_in != 0 (LaunchParameters::TData from an earlier motion iteration), else
+ * cleared, and holding the estimate on return; xn device float [2][F][H][W] receives the deformed mesh; registered
+ * (optional) receives the normalised frames resampled at xn.  Synchronous. */
+int ocl_gwr_run(ocl_gwr* g, const void* x, float* T, int T_in, float* xn, float* registered, ocl_gwr_report* report);
+int ocl_gwr_destroy(ocl_gwr* g);
 
 /* ---- batched NESTA-TV: many uneven slices advanced together, one launch per pass per iteration ---- */
 int ocl_batch_create(ocl_ctx* ctx, int nslices, const ocl_slice* slices, const ocl_nesta_params* prm,

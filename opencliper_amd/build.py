@@ -29,7 +29,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-f
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-parameter", "-Wno-unused-variable",
              f"-I{INC}", f"-I{CSRC}"]
-HIP_SOURCES = ["ocl_ops.hip", "ocl_mc.hip", "ocl_nesta.hip", "ocl_nesta_mfma.hip", "ocl_capi.cpp", "ocl_batch.cpp"]
+HIP_SOURCES = ["ocl_ops.hip", "ocl_mc.hip", "ocl_gwr.hip", "ocl_nesta.hip", "ocl_nesta_mfma.hip", "ocl_capi.cpp",
+               "ocl_batch.cpp"]
 CPP_SOURCES = []   # filled below if the C++ API sources exist
 
 

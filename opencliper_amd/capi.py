@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -74,6 +75,46 @@ class McArgs(ctypes.Structure):
     _fields_ = [("xn", ctypes.c_void_p), ("x", ctypes.c_void_p), ("bound_box", ctypes.c_uint * 4)]
 
 
+GWR_TRACE = 1024
+
+
+class GwrParams(ctypes.Structure):
+    """ocl_gwr_params: GroupwiseRegistration::InitParameters, same order."""
+    _fields_ = [("W", ctypes.c_float), ("flagW", ctypes.c_int), ("radius", ctypes.c_uint), ("E", ctypes.c_uint),
+                ("Dp", ctypes.c_int * 2), ("nmax", ctypes.c_uint), ("et", ctypes.c_float), ("eh", ctypes.c_float),
+                ("lambda_", ctypes.c_float * 4)]
+
+
+def gwr_params(W=1.0, flagW=True, radius=40, E=3, Dp=(4, 4), nmax=100, et=0.01, eh=0.005,
+               lam=(0.0, 0.005, 0.0, 0.5)) -> GwrParams:
+    """Defaults are the values of the reference's tests/MRIRecon.cpp:122-130."""
+    p = GwrParams(W, int(bool(flagW)), radius, E)
+    p.Dp[0], p.Dp[1] = Dp
+    p.nmax, p.et, p.eh = nmax, et, eh
+    for i, v in enumerate(lam):
+        p.lambda_[i] = v
+    return p
+
+
+class GwrInfo(ctypes.Structure):
+    _fields_ = [("Cp", ctypes.c_int), ("support", ctypes.c_int), ("box", ctypes.c_int), ("areaX", ctypes.c_int),
+                ("use_lambda", ctypes.c_int), ("bound_box", ctypes.c_uint * 4), ("T_elems", ctypes.c_long),
+                ("xn_elems", ctypes.c_long), ("mesh", ctypes.c_void_p)]
+
+
+class GwrReport(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("ntrace", ctypes.c_int), ("H0", ctypes.c_float), ("Wn", ctypes.c_float),
+                ("oldmax", ctypes.c_float), ("H", ctypes.c_float * GWR_TRACE), ("Tdif", ctypes.c_float * GWR_TRACE),
+                ("Hdif", ctypes.c_float * GWR_TRACE), ("Wn_trace", ctypes.c_float * GWR_TRACE)]
+
+    def as_dict(self):
+        n = self.ntrace
+        return {"iters": self.iters, "H0": self.H0, "Wn": self.Wn, "oldmax": self.oldmax,
+                "H": np.array(self.H[: n + 1], dtype=np.float64), "Tdif": np.array(self.Tdif[:n], dtype=np.float64),
+                "Hdif": np.array(self.Hdif[:n], dtype=np.float64),
+                "Wn_trace": np.array(self.Wn_trace[:n], dtype=np.float64)}
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_long), ("failed", ctypes.c_long), ("queued", ctypes.c_long),
                 ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
@@ -131,6 +172,10 @@ def lib():
         "ocl_adjoint_motion_compensation": ([P, P, P, I, I, I, ctypes.POINTER(McArgs)], I),
         "ocl_nesta_run_mc": ([P, P, P, P, P, I, I, I, I, ctypes.POINTER(NestaParams), I, ctypes.POINTER(McArgs),
                               ctypes.POINTER(NestaReport)], I),
+        "ocl_gwr_create": ([P, I, I, I, ctypes.POINTER(GwrParams), PP], I),
+        "ocl_gwr_info": ([P, ctypes.POINTER(GwrInfo)], I),
+        "ocl_gwr_run": ([P, P, P, I, P, P, ctypes.POINTER(GwrReport)], I),
+        "ocl_gwr_destroy": ([P], I),
         "ocl_batch_create": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), PP], I),
         "ocl_batch_create_mc": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), ctypes.POINTER(McArgs), PP],
                                 I),
@@ -217,9 +262,12 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().ocl_ctx_create(device, ctypes.byref(h)))
         self.handle = h.value
+        self._children = weakref.WeakSet()      # batches and registrations: destroyed before the context
 
     def close(self):
         if self.handle:
+            for ch in list(self._children):
+                ch.close()
             lib().ocl_ctx_destroy(self.handle)
             self.handle = None
 
@@ -394,6 +442,9 @@ class Context:
                                       ctypes.byref(mc), ctypes.byref(rep)))
         return rep.as_dict()
 
+    def registration(self, F, H, W, prm: GwrParams | None = None) -> "Registration":
+        return Registration(self, F, H, W, prm or gwr_params())
+
     def batch(self, slices, prm: NestaParams | None = None, form=OCL_FORM_FUSED, mcs=None) -> "Batch":
         return Batch(self, slices, prm or nesta_params(), form, mcs)
 
@@ -404,6 +455,7 @@ class Batch:
     def __init__(self, ctx: Context, slices, prm: NestaParams, form=OCL_FORM_FUSED, mcs=None):
         """mcs: one McArgs per slice for motion compensation (fused form; each x_out then holds x_ref)."""
         self.ctx = ctx
+        self.handle = None
         self.keep = list(slices)
         arr = (Slice * len(slices))()
         for i, (y, S, M, x) in enumerate(slices):
@@ -418,6 +470,7 @@ class Batch:
             marr = (McArgs * len(mcs))(*mcs)
             _check(lib().ocl_batch_create_mc(ctx.handle, len(slices), arr, ctypes.byref(prm), marr, ctypes.byref(h)))
         self.handle = h.value
+        ctx._children.add(self)
 
     def setup(self):
         _check(lib().ocl_batch_setup(self.handle))
@@ -521,6 +574,54 @@ class Server:
         if self.handle:
             lib().ocl_server_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Registration:
+    """GroupwiseRegistration on the device (ocl_gwr_*): init() at construction, run() = launch()."""
+
+    def __init__(self, ctx: Context, F, H, W, prm: GwrParams):
+        self.ctx = ctx
+        self.handle = None
+        self.F, self.H, self.W = F, H, W
+        h = ctypes.c_void_p()
+        _check(lib().ocl_gwr_create(ctx.handle, F, H, W, ctypes.byref(prm), ctypes.byref(h)))
+        self.handle = h.value
+        ctx._children.add(self)
+        info = GwrInfo()
+        _check(lib().ocl_gwr_info(self.handle, ctypes.byref(info)))
+        self.Cp, self.support, self.box, self.areaX = info.Cp, info.support, info.box, info.areaX
+        self.use_lambda = bool(info.use_lambda)
+        self.bound_box = [int(v) for v in info.bound_box]
+        self.mesh_ptr = info.mesh
+
+    def mesh(self) -> np.ndarray:
+        """The original mesh x, int32 [2][W][H] (1-based)."""
+        out = np.empty((2, self.W, self.H), np.int32)
+        _check(lib().ocl_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.mesh_ptr, out.nbytes))
+        return out
+
+    def mc_args(self, xn: DeviceArray) -> McArgs:
+        a = McArgs(xn.ptr, self.mesh_ptr)
+        for i, v in enumerate(self.bound_box):
+            a.bound_box[i] = v
+        return a
+
+    def run(self, x: DeviceArray, T: DeviceArray, xn: DeviceArray, T_in=False, registered: DeviceArray | None = None):
+        rep = GwrReport()
+        _check(lib().ocl_gwr_run(self.handle, x.ptr, T.ptr, int(bool(T_in)), xn.ptr,
+                                 registered.ptr if registered is not None else None, ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().ocl_gwr_destroy(self.handle)
+        self.handle = None
 
     def __del__(self):
         try:

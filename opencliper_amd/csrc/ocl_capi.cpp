@@ -201,7 +201,7 @@ int make_plan(ocl_ctx* c, int n, FftPlan* P) {
     if (m != 1)
         return fail(OCL_E_UNSUPPORTED,
                     "FFT length " + std::to_string(n) + " is not a product of powers of 2, 3, 5 and 7");
-    if ((int)r.size() > OCL_MAX_STAGES) return fail(OCL_E_UNSUPPORTED, "FFT length has too many factors");
+    if ((int)r.size() > OCL_FFT_MAX_STAGES) return fail(OCL_E_UNSUPPORTED, "FFT length has too many factors");
     std::memset(P, 0, sizeof(*P));
     P->n = n;
     P->nst = (int)r.size();

@@ -6,14 +6,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define OCL_MAX_STAGES 12
+#define OCL_FFT_MAX_STAGES 12   // radix stages of one FFT plan
 #define OCL_NT 256          // threads per workgroup for every kernel of this library (4 wavefronts of 64)
 
 // Factorisation of one DFT length plus its twiddle table tw[m] = exp(-2*pi*i*m/n) (device memory).
 struct FftPlan {
     int n;
     int nst;
-    int radix[OCL_MAX_STAGES];
+    int radix[OCL_FFT_MAX_STAGES];
     const float2* tw;
 };
 

@@ -225,3 +225,21 @@ def motion_fields(H, W, F, amp=1.5, roi=0.6, seed=BASE_SEED, slice_id=0):
     hr, hc = int(roi * H / 2), int(roi * W / 2)
     bb = np.array([W // 2 - hc + 1, H // 2 - hr + 1, W // 2 + hc, H // 2 + hr], dtype=np.uint32)
     return xn, identity_mesh(H, W), bb
+
+
+def shifted_blobs(N, F, shifts, seed=3, n=14, sig=(3.5, 6.0)):
+    """Registration input: F frames of one smooth texture (n Gaussian blobs of width sig about the centre) with
+    frame f translated by shifts[f] = (columns, rows), complex64 [F][N][N] with a constant phase."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:N, 0:N].astype(np.float64)
+    c = N / 2
+    spec = [(c + rng.uniform(-N / 5, N / 5), c + rng.uniform(-N / 5, N / 5), rng.uniform(*sig), rng.uniform(0.3, 1))
+            for _ in range(n)]
+    out = []
+    for f in range(F):
+        sx, sy = shifts[f]
+        img = np.zeros((N, N))
+        for cy, cx, s, a in spec:
+            img += a * np.exp(-((yy - cy - sy) ** 2 + (xx - cx - sx) ** 2) / (2 * s * s))
+        out.append(img * np.exp(1j * 0.3))
+    return np.array(out).astype(np.complex64)

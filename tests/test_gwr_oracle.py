@@ -7,26 +7,14 @@ logic, and the exact cancellation of the regularisation gradient by the frame-me
 import numpy as np
 import pytest
 
+from opencliper_amd import synth
 from oracle import gwr_oracle as G
 
 DELTA = 2.0 / 3.0 - 0.6666          # the reference's cubic B-spline uses 0.6666 for 2/3 (bspline.cpp:117)
 
 
 def blobs(N, F, shifts, seed=3, n=14, sig=(3.5, 6.0)):
-    """Frames of one smooth texture (Gaussian blobs around the centre), frame f translated by shifts[f] = (dx, dy)."""
-    rng = np.random.default_rng(seed)
-    yy, xx = np.mgrid[0:N, 0:N].astype(np.float64)
-    c = N / 2
-    spec = [(c + rng.uniform(-N / 5, N / 5), c + rng.uniform(-N / 5, N / 5), rng.uniform(*sig), rng.uniform(0.3, 1))
-            for _ in range(n)]
-    out = []
-    for f in range(F):
-        sx, sy = shifts[f]
-        img = np.zeros((N, N))
-        for cy, cx, s, a in spec:
-            img += a * np.exp(-((yy - cy - sy) ** 2 + (xx - cx - sx) ** 2) / (2 * s * s))
-        out.append(img * np.exp(1j * 0.3))
-    return np.array(out).astype(np.complex64)
+    return synth.shifted_blobs(N, F, shifts, seed=seed, n=n, sig=sig)
 
 
 @pytest.fixture(scope="module")
