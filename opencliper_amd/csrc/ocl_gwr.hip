@@ -37,8 +37,8 @@
 
 namespace {
 
-constexpr int GWR_PIX = 64;          // box pixels per eval workgroup (one per lane)
-constexpr int GWR_FL = OCL_NT / GWR_PIX;   // frame lanes per eval workgroup
+constexpr int GWR_PIX = 16;          // box pixels per eval workgroup
+constexpr int GWR_FL = OCL_NT / GWR_PIX;   // frame lanes per eval workgroup (frames f = lane, lane + 16, ...)
 constexpr int GWR_FMAX = 64;         // frames (eval keeps It, d1, d2 of its 64 pixels for every frame in LDS)
 constexpr int GWR_SUP_MAX = 32;      // support diameter (the grad pass keeps a lane's support weights in registers)
 
@@ -207,6 +207,7 @@ __device__ float gwr_reg_terms(const GwrDev& g, int R, int C, int f) {
 
 // mode 0: the evaluation before the loop (optimizer.cpp:236-259): the metric of the unwarped frames, dy from them, dx
 // resampled at the starting coefficients.  mode 1: the evaluation that closes iteration it (optimizer.cpp:318-356).
+template <int REG>
 __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
     __shared__ float sI[GWR_FMAX][GWR_PIX], s1[GWR_FMAX][GWR_PIX], s2[GWR_FMAX][GWR_PIX];
     __shared__ float smean[GWR_PIX], sreg[GWR_FL][GWR_PIX];
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
             s2[f][px] = s.z;
             if (g.reg && mode == 1) g.reg[(long)f * N2 + pix] = it;
         }
-        if (g.use_lambda && mode == 1) {
+        if (REG && mode == 1) {
             float a = 0.f;
             for (int f = fl; f < F; f += GWR_FL) a += gwr_reg_terms(g, r, c, f);
             sreg[fl][px] = a;
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
         }
         v = v / (float)F;
         smean[px] = mean;
-        if (g.use_lambda && mode == 1)
+        if (REG && mode == 1)
             for (int q = 0; q < GWR_FL; ++q) v += sreg[q][px];
         hpix = (double)(g.mask[(long)r * N + c] * v);
     }
@@ -276,15 +277,21 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
         last = atomicAdd(&g.st->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last) return;
     __threadfence();
-    // the last workgroup: H, then evolution and stopCondition (GroupwiseRegistration.cpp:456-548)
-    double H = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-        H += __hip_atomic_load(g.partH + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the last workgroup: H and the squared step norm from every block's partial sum (all threads, fixed order), then
+    // evolution and stopCondition (GroupwiseRegistration.cpp:456-548) in one thread
+    double sums[2] = {0.0, 0.0};
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += OCL_NT)
+        sums[0] += __hip_atomic_load(g.partH + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mode == 1)
+        for (int b = threadIdx.x; b < Cp * Cp; b += OCL_NT)
+            sums[1] += __hip_atomic_load(g.partD + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<2>(sums, red);
+    if (threadIdx.x != 0) return;
     GwrState* st = g.st;
     st->ticket = 0;
-    const float Hf = (float)H;
+    const float Hf = (float)sums[0];
     const int nmax = st->nmax;
     if (mode == 0) {
         st->H0 = Hf;
@@ -293,11 +300,8 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
         g.hist[0] = Hf;
         return;
     }
-    double d2 = 0.0;
-    for (int b = 0; b < Cp * Cp; ++b)
-        d2 += __hip_atomic_load(g.partD + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int it = st->it;
-    const float Tdif = sqrtf((float)d2);
+    const float Tdif = sqrtf((float)sums[1]);
     const float Hdif = st->Hprev - Hf;
     float Wn = st->Wn;
     if (st->flagW) Wn = Hdif > 0 ? (float)((double)Wn * 1.2) : Wn / 2;
@@ -309,6 +313,12 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
     st->Hprev = Hf;
     st->stopped = (it == nmax) || (Tdif < st->p2 && Hdif < st->p4);
     st->it = it + 1;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // ---------------------------------------------------------------------------------------------------- the update pass
@@ -341,14 +351,25 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_grad_update(GwrDev g) {
         off[k] = rr * L + cc;
         wt[k] = g.beta[i] * g.beta[j];
     }
-    for (int pr = wave; pr < 2 * F; pr += OCL_NT / 64) {
-        const float* wp = g.w + (long)pr * L2;
-        float a = 0.f;
+    // (d, f) planes handled by a wave four at a time, so their support loads are in flight together
+    constexpr int GB = 4, NW = OCL_NT / 64;
+    for (int pr0 = wave; pr0 < 2 * F; pr0 += NW * GB) {
+        float a[GB] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < K && off[k] >= 0) a = fmaf(wp[off[k]], wt[k], a);
-        const double s = wave_sum((double)a);
-        if (lane == 0) sdH[pr] = (float)s;
+        for (int k = 0; k < KMAX; ++k) {
+            if (k >= K || off[k] < 0) continue;
+#pragma unroll
+            for (int b = 0; b < GB; ++b) {
+                const int pr = pr0 + NW * b;
+                if (pr < 2 * F) a[b] = fmaf(g.w[(long)pr * L2 + off[k]], wt[k], a[b]);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < GB; ++b) {
+            const float sum = wave_sum_f(a[b]);
+            const int pr = pr0 + NW * b;
+            if (lane == 0 && pr < 2 * F) sdH[pr] = sum;
+        }
     }
     __syncthreads();
     double dif2 = 0.0;
@@ -626,15 +647,18 @@ extern "C" int ocl_gwr_run(ocl_gwr* g, const void* x, float* T, int T_in, float*
     hipLaunchKernelGGL(k_gwr_init, dim3(grid), dim3(OCL_NT), 0, c->stream, d, g->Wn0, g->p2, g->eh, g->nmax,
                        g->flagW, T_in ? 0 : 1);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_gwr_absmax, dim3(grid_for(tot)), dim3(OCL_NT), 0, c->stream, (const float2*)x, tot, g->st);
+    // one atomic per workgroup into a single word: few workgroups, each striding over the frames
+    hipLaunchKernelGGL(k_gwr_absmax, dim3(std::min(grid_for(tot), 2 * c->num_cus)), dim3(OCL_NT), 0, c->stream,
+                       (const float2*)x, tot, g->st);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_gwr_prep, dim3(grid_for(tot)), dim3(OCL_NT), 0, c->stream, (const float2*)x, d);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_gwr_eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 0);
+    void (*eval)(GwrDev, int) = g->use_lambda ? k_gwr_eval<1> : k_gwr_eval<0>;
+    hipLaunchKernelGGL(eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 0);
     HIPCHK(hipGetLastError());
     for (int it = 1; it <= g->nmax; ++it) {
         hipLaunchKernelGGL(k_gwr_grad_update, dim3(g->Cp * g->Cp), dim3(OCL_NT), 0, c->stream, d);
-        hipLaunchKernelGGL(k_gwr_eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 1);
+        hipLaunchKernelGGL(eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 1);
     }
     HIPCHK(hipGetLastError());
     GwrState st;
