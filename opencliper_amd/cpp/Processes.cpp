@@ -1,7 +1,7 @@
 // The Processes of the NESTA-TV path in the OpenCLIPER-compatible C++ API, each one call into the C ABI.
 // Reference Processes mirrored: TemporalTV, VectorNormalization, FFT, ComplexElementProd, ApplyMask, XImageSum,
 // ComplexAbs, ComplexAbsPow2, CopyDataGPU, RSoS, SimpleMRIRecon, SimpleMRIReconSOS, MotionCompensation,
-// AdjointMotionCompensation, NestaUp (src/processes/**).
+// AdjointMotionCompensation, GroupwiseRegistration, NestaUp (src/processes/**).
 #include <OpenCLIPER/processes/AdjointMotionCompensation.hpp>
 #include <OpenCLIPER/processes/ApplyMask.hpp>
 #include <OpenCLIPER/processes/MotionCompensation.hpp>
@@ -9,6 +9,7 @@
 #include <OpenCLIPER/processes/ComplexAbsPow2.hpp>
 #include <OpenCLIPER/processes/ComplexElementProd.hpp>
 #include <OpenCLIPER/processes/FFT.hpp>
+#include <OpenCLIPER/processes/GroupwiseRegistration.hpp>
 #include <OpenCLIPER/processes/XImageSum.hpp>
 #include <OpenCLIPER/processes/examples/SimpleMRIRecon.hpp>
 #include <OpenCLIPER/processes/examples/SimpleMRIReconSOS.hpp>
@@ -272,6 +273,91 @@ void AdjointMotionCompensation::launch() {
     startProfiling();
     launchMC(*this, getInput(), getOutput(), lp->argsMC, true, "AdjointMotionCompensation::launch");
     stopProfiling();
+}
+
+// ---- GroupwiseRegistration (src/processes/GroupwiseRegistration.cpp:93-615, groupwiseRegistration/*.cpp) ----
+GroupwiseRegistration::~GroupwiseRegistration() {
+    if (plan_) ocl_gwr_destroy(plan_);
+    delete bound_box_;
+}
+
+void GroupwiseRegistration::init() {
+    auto ip = std::dynamic_pointer_cast<InitParameters>(pInitParameters);
+    if (!ip) throw std::invalid_argument("GroupwiseRegistration::init: InitParameters required");
+    auto in = getInput();
+    if (!in) throw CLError(OCL_E_INVALID, "GroupwiseRegistration::init: init() called before setInput()");
+    if (in->getElementDataType() != TYPEID_COMPLEX)
+        throw std::invalid_argument("GroupwiseRegistration::init: the input must be complex image frames");
+    if (!ip->Dp) throw std::invalid_argument("GroupwiseRegistration::init: null Dp");
+    if (ip->lambda.size() < 4) throw std::invalid_argument("GroupwiseRegistration::init: lambda needs four weights");
+    const NDArray* a = in->getNDArray(0);
+    const int F = (int)in->getNumNDArrays(), H = (int)NDARRAYHEIGHT(a), W = (int)NDARRAYWIDTH(a);
+    ocl_gwr_params prm;
+    prm.W = ip->W;
+    prm.flagW = ip->flagW ? 1 : 0;
+    prm.radius = ip->radius;
+    prm.E = ip->E;
+    prm.Dp[0] = ip->Dp[0];
+    prm.Dp[1] = ip->Dp[1];
+    prm.nmax = ip->nmax;
+    prm.et = ip->et;
+    prm.eh = ip->eh;
+    for (int k = 0; k < 4; ++k) prm.lambda[k] = ip->lambda[k];
+    if (plan_) ocl_gwr_destroy(plan_);
+    plan_ = nullptr;
+    checkOcl(ocl_gwr_create(ctxOf(*this), F, H, W, &prm, &plan_), "GroupwiseRegistration::init");
+    ocl_gwr_info_t info;
+    checkOcl(ocl_gwr_info(plan_, &info), "GroupwiseRegistration::init");
+    F_ = F, N_ = H, Cp_ = info.Cp;
+    // the mesh x (GroupwiseRegistration.cpp:150-164): int32 {H, W, 2}, element [d][c][r]
+    auto* dims = new std::vector<dimIndexType>({(dimIndexType)H, (dimIndexType)W, 2});
+    xData_ = std::make_shared<XData>(getApp(), dims, TYPEID_INT32);
+    checkOcl(ocl_memcpy_d2d(ctxOf(*this), xData_->getDeviceBuffer(), info.mesh, sizeof(int32_t) * 2 * (size_t)H * W),
+             "GroupwiseRegistration::init");
+    checkOcl(ocl_ctx_sync(ctxOf(*this)), "GroupwiseRegistration::init");
+    delete bound_box_;
+    bound_box_ = new std::vector<dimIndexType>(info.bound_box, info.bound_box + 4);
+}
+
+void GroupwiseRegistration::launch() {
+    auto lp = std::dynamic_pointer_cast<LaunchParameters>(pLaunchParameters);
+    if (!lp || !lp->argsMC) throw std::invalid_argument("GroupwiseRegistration::launch: LaunchParameters with argsMC "
+                                                        "required");
+    if (!plan_) throw std::invalid_argument("GroupwiseRegistration::launch: init() not called");
+    auto in = getInput();
+    if (!in || (int)in->getNumNDArrays() != F_ || (int)NDARRAYHEIGHT(in->getNDArray(0)) != N_)
+        throw std::invalid_argument("GroupwiseRegistration::launch: the input changed shape since init()");
+    const size_t nT = 2 * (size_t)F_ * Cp_ * Cp_;
+    // coefficients: the caller's TData (the previous motion iteration's estimate, optimizer.cpp:73-82) or zero
+    std::shared_ptr<Data> T = lp->TData;
+    if (T && (T->getElementDataType() != TYPEID_REAL || T->getDeviceBufferBytes() < nT * sizeof(float)))
+        throw std::invalid_argument("GroupwiseRegistration::launch: TData must hold 2*F*Cp*Cp floats");
+    const bool T_in = (bool)T;
+    if (!T) {
+        auto* dims = new std::vector<dimIndexType>({(dimIndexType)Cp_, (dimIndexType)Cp_, (dimIndexType)F_, 2});
+        T = std::make_shared<XData>(getApp(), dims, TYPEID_REAL);
+    }
+    auto* xdims = new std::vector<dimIndexType>({(dimIndexType)N_, (dimIndexType)N_, (dimIndexType)F_, 2});
+    auto xn = std::make_shared<XData>(getApp(), xdims, TYPEID_REAL);
+    ocl_gwr_report rep;
+    startProfiling();
+    checkOcl(ocl_gwr_run(plan_, in->getDeviceBuffer(), (float*)T->getDeviceBuffer(), T_in ? 1 : 0,
+                         (float*)xn->getDeviceBuffer(), nullptr, &rep),
+             "GroupwiseRegistration::launch");
+    stopProfiling();
+    report_.iters = rep.iters;
+    report_.oldmax = rep.oldmax;
+    report_.H.assign(rep.H, rep.H + rep.ntrace + 1);
+    report_.Tdif.assign(rep.Tdif, rep.Tdif + rep.ntrace);
+    report_.Hdif.assign(rep.Hdif, rep.Hdif + rep.ntrace);
+    report_.Wn.assign(rep.Wn_trace, rep.Wn_trace + rep.ntrace);
+    xnData_ = xn;
+    TData_ = T;
+    // optimizer.cpp:362-365
+    lp->argsMC->xnData = xnData_;
+    lp->argsMC->TData = TData_;
+    lp->argsMC->xData = xData_;
+    lp->argsMC->bound_box = bound_box_;
 }
 
 // ---- NestaUp (src/processes/nesta/NestaUp.cpp:81-106,442-910) ----

@@ -1,16 +1,23 @@
 // Motion-compensated NESTA-TV through the OpenCLIPER-compatible API, chained as the reference's tests/MRIRecon.cpp
-// does it (:232-272): a first NESTA without motion compensation, then NESTA with argsMC starting from that result.
-// The deformation comes from a file instead of GroupwiseRegistration:
+// does it (:209-282): a first NESTA without motion compensation, then NESTA with argsMC starting from that result.
 //   mri_recon_mc <in.raw> <motion.raw> <out.raw> F C H W maxIntIter maxIter stoptest miniter
+//   mri_recon_mc <in.raw> register <out.raw> F C H W maxIntIter maxIter stoptest miniter [motionIters [radius [W]]]
 // in.raw: y [F][C][H][W] complex64, S [C][H][W] complex64, M [F][H][W] uint8 (as mri_recon reads it).
 // motion.raw: xn [2][F][H][W] float32, mesh [2][W][H] int32, bound box 4 x uint32 (1-based).
-// Writes out.raw (the motion-compensated reconstruction), out.raw.nomc (the first reconstruction), and
-// out.raw.mc / out.raw.mcadj (MotionCompensation / AdjointMotionCompensation of the first reconstruction).
+// With "register" the deformation comes from GroupwiseRegistration of the latest reconstruction, motionIters times
+// (default 1), with tests/MRIRecon.cpp:122-130's parameters (radius 40 and W 1 unless given); the coefficients of
+// one motion iteration start the next.  It writes out.raw.xn<i> (xn [2][F][H][W] float32 of motion iteration i) and
+// out.raw.rec<i> (the reconstruction of motion iteration i), and prints one line per registration:
+// "registration <i> iters <n> H <first> <last>".
+// Writes out.raw (the motion-compensated reconstruction), out.raw.nomc (the first reconstruction), and, with a motion
+// file, out.raw.mc / out.raw.mcadj (MotionCompensation / AdjointMotionCompensation of the first reconstruction).
 #include <OpenCLIPER/processes/AdjointMotionCompensation.hpp>
+#include <OpenCLIPER/processes/GroupwiseRegistration.hpp>
 #include <OpenCLIPER/processes/MotionCompensation.hpp>
 #include <OpenCLIPER/processes/nesta/NestaUp.hpp>
 
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 
@@ -68,6 +75,63 @@ int main(int argc, char* argv[]) {
         pKData->setSamplingMasksData(pM);
         auto pOutputXData = std::make_shared<XData>(pCLapp, pKData);
 
+        const float lambda_i = 1e-2f, mu_f = 3e-4f, La = 1.f, tolVar = 3e-4f;
+        const uint32_t maxIntIter = (uint32_t)atoi(argv[8]), stoptest = (uint32_t)atoi(argv[10]),
+                       miniter = (uint32_t)atoi(argv[11]);
+        const int maxIter = atoi(argv[9]);
+        auto nesta = Process::create<NestaUp>(pCLapp);
+        nesta->setForm(NestaUp::FUSED);
+        nesta->setInput(pKData);
+        nesta->setOutput(pOutputXData);
+        nesta->init();
+
+        if (std::strcmp(argv[2], "register") == 0) {
+            const int motionIters = argc > 12 ? atoi(argv[12]) : 1;
+            // tests/MRIRecon.cpp:122-130
+            const unsigned nmax = 100;
+            const float et = 0.01f, eh = 0.005f;
+            const std::vector<float> lambda = {0.f, 0.005f, 0.f, 0.5f};
+            const unsigned radius = argc > 13 ? (unsigned)atoi(argv[13]) : 40;
+            const unsigned E = 3;
+            int Dp[2] = {4, 4};
+            const float Wgt = argc > 14 ? (float)atof(argv[14]) : 1.f;
+            const bool flagW = true;
+            auto gw = Process::create<GroupwiseRegistration>(pCLapp);
+            gw->setInput(pOutputXData);
+            gw->setInitParameters(std::make_shared<GroupwiseRegistration::InitParameters>(Wgt, flagW, radius, E, Dp,
+                                                                                           nmax, et, eh, lambda));
+            gw->init();
+            nesta->setLaunchParameters(std::make_shared<NestaUp::LaunchParameters>(
+                lambda_i, mu_f, La, maxIntIter, tolVar, 1, maxIter, stoptest, miniter, nullptr));
+            nesta->launch();
+            writeX(pOutputXData, std::string(argv[3]) + ".nomc");
+            std::shared_ptr<Data> TData;
+            for (int i = 0; i < motionIters; ++i) {
+                ArgumentsMotionCompensation* argsMC = new ArgumentsMotionCompensation();
+                gw->setLaunchParameters(std::make_shared<GroupwiseRegistration::LaunchParameters>(argsMC, TData, i));
+                gw->launch();
+                TData = argsMC->TData;
+                const auto& g = gw->getReport();
+                std::cout << "registration " << i << " iters " << g.iters << " H " << g.H.front() << " "
+                          << g.H.back() << "\n";
+                argsMC->xnData->device2Host();
+                std::ofstream xo(std::string(argv[3]) + ".xn" + std::to_string(i), std::ios::binary);
+                xo.write((const char*)argsMC->xnData->getHostBuffer(0),
+                         (std::streamsize)(sizeof(float) * 2 * (size_t)F * H * W));
+                nesta->setLaunchParameters(std::make_shared<NestaUp::LaunchParameters>(
+                    lambda_i, mu_f, La, maxIntIter, tolVar, 1, maxIter, stoptest, miniter, argsMC));
+                nesta->launch();
+                writeX(pOutputXData, std::string(argv[3]) + ".rec" + std::to_string(i));
+                delete argsMC;   // bound_box belongs to the registration process, as in the reference
+            }
+            writeX(pOutputXData, argv[3]);
+            const auto& r = nesta->getReport();
+            std::cout << "normU " << r.normU << " iters";
+            for (int n : r.itersPerStage) std::cout << " " << n;
+            std::cout << "\n";
+            return 0;
+        }
+
         // the deformation, in the Data forms GroupwiseRegistration.cpp:150-164,307-314 uses
         std::ifstream mf(argv[2], std::ios::binary);
         if (!mf) throw std::runtime_error("cannot open motion file");
@@ -86,15 +150,6 @@ int main(int argc, char* argv[]) {
         argsMC->xData = std::make_shared<XData>(pCLapp, meshArr, TYPEID_INT32);
         argsMC->bound_box = new std::vector<dimIndexType>(bb);
 
-        const float lambda_i = 1e-2f, mu_f = 3e-4f, La = 1.f, tolVar = 3e-4f;
-        const uint32_t maxIntIter = (uint32_t)atoi(argv[8]), stoptest = (uint32_t)atoi(argv[10]),
-                       miniter = (uint32_t)atoi(argv[11]);
-        const int maxIter = atoi(argv[9]);
-        auto nesta = Process::create<NestaUp>(pCLapp);
-        nesta->setForm(NestaUp::FUSED);
-        nesta->setInput(pKData);
-        nesta->setOutput(pOutputXData);
-        nesta->init();
         // first reconstruction (tests/MRIRecon.cpp:232-235)
         nesta->setLaunchParameters(std::make_shared<NestaUp::LaunchParameters>(lambda_i, mu_f, La, maxIntIter, tolVar, 1,
                                                                                 maxIter, stoptest, miniter, nullptr));

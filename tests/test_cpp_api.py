@@ -149,3 +149,45 @@ def test_driver_motion_compensated_non_identity_mesh_falls_back_to_plain(oracle,
     x = np.fromfile(outp, dtype=np.complex64).reshape(shape)
     assert oracle.rel_l2(x, x64) < 1e-3
     assert "iters " + " ".join(str(i) for i in rep["iters"]) in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_registration_chain(oracle, tmp_path):
+    """tests/MRIRecon.cpp:209-282 through the C++ API: NESTA, GroupwiseRegistration of its result, NESTA with the
+    registration's argsMC (xnData, xData, bound_box), twice, the second registration starting from the first one's
+    TData.  Checked: the initial registration cost against the oracle's metric of the first reconstruction, the cost
+    falling, and each motion-compensated NESTA against the FP64 oracle run with the xn the registration returned."""
+    from oracle import gwr_oracle as G
+    sl = synth.make_cine(H=64, W=64, C=3, F=6, accel=3.0, n_centre=4, slice_id=48)
+    inp, outp = str(tmp_path / "in.raw"), str(tmp_path / "out.raw")
+    _write_input(inp, sl)
+    args = [DRIVER_MC, inp, "register", outp, str(sl.F), str(sl.C), str(sl.H), str(sl.W), "3", "12", "1", "7", "2",
+            "16", "0.05"]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    shape = (sl.F, sl.H, sl.W)
+    x0 = np.fromfile(outp + ".nomc", dtype=np.complex64).reshape(shape)
+    lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("registration")]
+    assert len(lines) == 2
+    R = G.Registration(sl.H, sl.F, radius=16, W=0.05)
+    I0 = R.normalize(x0)
+    H0 = R.cost_at(I0, np.zeros(R.Cp0 * R.Cp0 * sl.F * 2))[0]
+    it0, h_first, h_last = int(lines[0][3]), float(lines[0][5]), float(lines[0][6])
+    assert abs(h_first - H0) <= 1e-5 * H0
+    assert 1 <= it0 <= 100 and h_last < 0.5 * h_first
+    # each motion-compensated NESTA against the oracle run from the same x_ref with the registration's own xn
+    prm = oracle.params(maxIntIter=3, maxIter=12, stoptest=1, miniter=7)
+    mesh = synth.identity_mesh(sl.H, sl.W)
+    bb = np.array(R.bb, np.uint32)
+    pos = sl.F * sl.H * sl.W
+    xref = x0
+    for i in range(2):
+        xn = np.fromfile(outp + f".xn{i}", dtype=np.float32).reshape(2, *shape)
+        want, rep2 = oracle.nesta_mc(sl.y, sl.S, sl.M, xref, xn, mesh, bb, prm, double=True,
+                                     drand_start=(i + 1) * pos)
+        xref = np.fromfile(outp + f".rec{i}", dtype=np.complex64).reshape(shape)
+        assert oracle.rel_l2(xref, want) < 1e-3
+    assert np.array_equal(np.fromfile(outp, dtype=np.complex64).reshape(shape), xref)
+    assert "iters " + " ".join(str(i) for i in rep2["iters"]) in r.stdout
