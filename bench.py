@@ -451,6 +451,50 @@ def leg_mc(ctx, synth, dev0, x_ref, prm, steps, reps=20):
         xs, (xn, mesh, bb)
 
 
+def gwr_bytes(F, L, Cp):
+    """Algorithmic HBM bytes of one registration iteration: the eval pass reads the 16-B {I0, d1, d2} sample of every
+    box pixel-frame and writes its two mesh coordinates and two gradient weights (4 B each), the update pass reads the
+    weights again; the coefficients (2 F Cp^2 floats) are read by both passes and written once."""
+    return F * L * L * (16 + 8 + 8 + 8) + 3 * 2 * F * Cp * Cp * 4
+
+
+def leg_registration(ctx, capi, x, reps=5):
+    """GroupwiseRegistration of the headline slice's plain reconstruction with tests/MRIRecon.cpp's parameters, as
+    the reference's driver runs it before motion-compensated NESTA; then 100 fixed iterations (no stop rule) and one
+    iteration, whose difference gives the time per iteration."""
+    F, H, W = x.shape
+    dx = ctx.to_device(x.astype(np.complex64))
+    out = {}
+    for name, prm in (("mrirecon", capi.gwr_params()), ("fixed100", capi.gwr_params(nmax=100, et=0.0, eh=0.0)),
+                      ("fixed1", capi.gwr_params(nmax=1, et=0.0, eh=0.0))):
+        reg = ctx.registration(F, H, W, prm)
+        T = ctx.empty((2, F, reg.Cp, reg.Cp), np.float32)
+        xn = ctx.empty((2, F, H, W), np.float32)
+        rep = reg.run(dx, T, xn)
+        lat = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            rep = reg.run(dx, T, xn)
+            lat.append(time.perf_counter() - t)
+        out[name] = {"ms": 1e3 * float(np.median(lat)), "iters": rep["iters"], "H0": rep["H0"],
+                     "H_final": float(rep["H"][rep["iters"]])}
+        geo = (reg.Cp, reg.box)
+        reg.close()
+    us_it = 1e3 * (out["fixed100"]["ms"] - out["fixed1"]["ms"]) / 99
+    byts = gwr_bytes(F, geo[1], geo[0])
+    m = out["mrirecon"]
+    return {"workload": f"registration of the plain reconstruction F={F} {H}x{W}: circular ROI radius 40, cubic "
+                        f"B-splines Dp 4 ({geo[0]}x{geo[0]} control points), W 1, flagW, nmax 100, et 0.01, "
+                        f"eh 0.005, lambda (0, 0.005, 0, 0.5) (tests/MRIRecon.cpp:122-130)",
+            "ms": m["ms"], "iters": m["iters"], "H0": m["H0"], "H_final": m["H_final"],
+            "registrations_per_s": 1e3 / m["ms"], "us_per_iteration": us_it,
+            "roofline": {"bound": "hbm", "achieved": byts / (us_it * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": byts / (us_it * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_alg_per_iter": byts,
+                         "kernel": "k_gwr_grad_update + k_gwr_eval (one optimizer iteration, two dependent launches; "
+                                   "wall clock of 100 minus 1 fixed iterations)"}}, x
+
+
 def gpu_legs(args, cine, parts, full, split):
     from opencliper_amd import capi, synth
     ctx = capi.Context(0)
@@ -470,6 +514,9 @@ def gpu_legs(args, cine, parts, full, split):
             log(f"motion-compensated: fused {out['mc']['forms']['fused']['recons_per_s']:.2f} recons/s, plain "
                 f"{out['mc']['forms']['plain']['recons_per_s']:.2f}; MC kernel {k['forward']['achieved_GBs']:.0f} "
                 f"GB/s, adjoint {k['adjoint']['achieved_GBs']:.0f} GB/s")
+        if "reg" in args.legs:
+            log("registration leg")
+            out["registration"], out["x"]["reg_input"] = leg_registration(ctx, capi, out["x"]["plain"])
         out["fused_single"], out["x"]["fused"] = leg_single(ctx, capi, dev[0], prm, capi.OCL_FORM_FUSED, args.steps)
         log(f"fused single: {out['fused_single']['recons_per_s']:.2f} recons/s")
         out["batched"] = leg_batched(ctx, dev, prm, args.steps, args.warmup)
@@ -593,6 +640,22 @@ def cpu_leg(args, slice0, full, g):
                                "cores": threads,
                                "rel_l2_vs_fp64": O.rel_l2(gx["simple"], x64),
                                "rel_l2_vs_truth": O.rel_l2(gx["simple"], full.x_true)}
+    if "reg_input" in gx:
+        from oracle import gwr_oracle as G
+        xr = gx["reg_input"]
+        t = time.perf_counter()
+        R = G.Registration(xr.shape[1], xr.shape[0])
+        t_init = time.perf_counter() - t
+        t = time.perf_counter()
+        R.run(xr, nmax=1)
+        t_it = time.perf_counter() - t
+        iters = g["registration"]["iters"]
+        res["registration"] = {"cpu_s_init": t_init, "cpu_s_first_iteration": t_it,
+                               "cpu_s_per_registration_extrapolated": t_init + iters * t_it, "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle/gwr_oracle.py (numpy, kernel by kernel, FP64) on the same "
+                                         "reconstruction: init + creabspline and one optimizer iteration, "
+                                         f"extrapolated to the GPU run's {iters} iterations"}
     if not args.no_ref:
         prm = O.params(maxIntIter=STAGES, maxIter=ITERS, stoptest=0)
         x64, _ = O.nesta(y, S, M, prm, double=True)
@@ -662,7 +725,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--legs", default="simple,mc,split,uneven,served",
+    ap.add_argument("--legs", default="simple,mc,reg,split,uneven,served",
                     help="comma list of extra legs beside the headline batch: simple, mc, split, uneven, served")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the Poisson trace (300 = 5 min)")
     ap.add_argument("--serve-budget", type=float, default=1.0,
@@ -743,6 +806,13 @@ def main():
                          "achieved": m["kernels"]["forward"]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": m["kernels"]["forward"]["frac_hbm"]}
         line["motion_compensated"] = m
+    if "registration" in g:
+        r = dict(g["registration"])
+        cr = c.get("registration")
+        if cr:
+            r["cpu_baseline"] = cr
+            r["speedup_vs_cpu_oracle"] = cr["cpu_s_per_registration_extrapolated"] / (r["ms"] * 1e-3)
+        line["registration"] = r
     if "uneven_batch" in g:
         u = dict(g["uneven_batch"])
         u.pop("shapes_FCHW", None)
