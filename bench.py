@@ -491,7 +491,7 @@ def leg_registration(ctx, capi, x, reps=5):
             "roofline": {"bound": "hbm", "achieved": byts / (us_it * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": byts / (us_it * 1e-6) / 1e9 / HBM_PEAK_GBS,
                          "bytes_alg_per_iter": byts,
-                         "kernel": "k_gwr_grad_update + k_gwr_eval (one optimizer iteration, two dependent launches; "
+                         "kernel": "k_gwr_grad_update + k_gwr_eval + k_gwr_step (one optimizer iteration, three dependent launches; "
                                    "wall clock of 100 minus 1 fixed iterations)"}}, x
 
 

@@ -21,11 +21,11 @@
 //
 // Layout in HBM: T [2][F][Cp][Cp] (the reference's TData, {Cp1, Cp0, F, nt}); xn [2][F][N][N]; I4 [F][N][N] float4
 // {I0, d1, d2, 0} so one 16-byte load serves the three bilinear samples of a neighbour; w [2][F][L][L] (M dy dx_d over
-// the L x L box).  Per iteration two launches: gwr_grad_update (one workgroup per control point: the support sums,
-// the projection and the update of T, and the block's sum of squared steps) and gwr_eval (64 box pixels x 4 frame
-// lanes per workgroup: xn, It, V, w for the next gradient, the block's share of H); the last eval workgroup to finish
-// takes the step logic (evolution / stopCondition, GroupwiseRegistration.cpp:456-548) on the device, and every later
-// launch of the run returns at once once it has stopped.
+// the L x L box).  Per iteration three launches: gwr_grad_update (one workgroup per control point: the support sums,
+// the projection and the update of T, and the block's sum of squared steps), gwr_eval (16 box pixels x 16 frame lanes
+// per workgroup: xn, It, V, w for the next gradient, the block's share of H) and gwr_step (one workgroup: the step
+// logic, evolution / stopCondition, GroupwiseRegistration.cpp:456-548, on the device).  Every launch after the run
+// has stopped returns at once, so the host enqueues all nmax iterations without synchronising.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -45,7 +45,7 @@ constexpr int GWR_SUP_MAX = 32;      // support diameter (the grad pass keeps a 
 struct GwrState {
     float Wn, p2, p4, eh;
     int it, stopped, nmax, flagW;
-    unsigned ticket, maxbits;
+    unsigned maxbits;
     float H0, Hprev;
 };
 
@@ -212,7 +212,6 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
     __shared__ float sI[GWR_FMAX][GWR_PIX], s1[GWR_FMAX][GWR_PIX], s2[GWR_FMAX][GWR_PIX];
     __shared__ float smean[GWR_PIX], sreg[GWR_FL][GWR_PIX];
     __shared__ double red[16];
-    __shared__ int last;
     if (mode == 1 && g.st->stopped) return;
     const int N = g.N, F = g.F, L = g.L, Cp = g.Cp;
     const long N2 = (long)N * N, L2 = (long)L * L;
@@ -271,26 +270,23 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_eval(GwrDev g, int mode) {
     }
     double v1[1] = {hpix};
     block_sum<1>(v1, red);
-    if (threadIdx.x == 0) {
-        g.partH[blockIdx.x] = v1[0];
-        __threadfence();
-        last = atomicAdd(&g.st->ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // the last workgroup: H and the squared step norm from every block's partial sum (all threads, fixed order), then
-    // evolution and stopCondition (GroupwiseRegistration.cpp:456-548) in one thread
+    if (threadIdx.x == 0) g.partH[blockIdx.x] = v1[0];
+}
+
+// The step logic after an eval pass (evolution / stopCondition, GroupwiseRegistration.cpp:456-548), one workgroup:
+// H and the squared step norm from every block's partial sums in a fixed order.  A launch of its own: taking it in
+// the eval pass's last workgroup would need an agent-scope release fence in every workgroup, which writes back the
+// L2 of its XCD.
+__global__ __launch_bounds__(OCL_NT) void k_gwr_step(GwrDev g, int mode, int nblocks) {
+    __shared__ double red[16];
+    GwrState* st = g.st;
+    if (mode == 1 && st->stopped) return;
     double sums[2] = {0.0, 0.0};
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += OCL_NT)
-        sums[0] += __hip_atomic_load(g.partH + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nblocks; b += OCL_NT) sums[0] += g.partH[b];
     if (mode == 1)
-        for (int b = threadIdx.x; b < Cp * Cp; b += OCL_NT)
-            sums[1] += __hip_atomic_load(g.partD + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int b = threadIdx.x; b < g.Cp * g.Cp; b += OCL_NT) sums[1] += g.partD[b];
     block_sum<2>(sums, red);
     if (threadIdx.x != 0) return;
-    GwrState* st = g.st;
-    st->ticket = 0;
     const float Hf = (float)sums[0];
     const int nmax = st->nmax;
     if (mode == 0) {
@@ -351,10 +347,13 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_grad_update(GwrDev g) {
         off[k] = rr * L + cc;
         wt[k] = g.beta[i] * g.beta[j];
     }
-    // (d, f) planes handled by a wave four at a time, so their support loads are in flight together
-    constexpr int GB = 4, NW = OCL_NT / 64;
+    // (d, f) planes handled by a wave ten at a time (all of them for up to 20 frames), so their support loads are in
+    // flight together
+    constexpr int GB = 10, NW = OCL_NT / 64;
     for (int pr0 = wave; pr0 < 2 * F; pr0 += NW * GB) {
-        float a[GB] = {0.f, 0.f, 0.f, 0.f};
+        float a[GB];
+#pragma unroll
+        for (int b = 0; b < GB; ++b) a[b] = 0.f;
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             if (k >= K || off[k] < 0) continue;
@@ -655,10 +654,12 @@ extern "C" int ocl_gwr_run(ocl_gwr* g, const void* x, float* T, int T_in, float*
     HIPCHK(hipGetLastError());
     void (*eval)(GwrDev, int) = g->use_lambda ? k_gwr_eval<1> : k_gwr_eval<0>;
     hipLaunchKernelGGL(eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 0);
+    hipLaunchKernelGGL(k_gwr_step, dim3(1), dim3(OCL_NT), 0, c->stream, d, 0, g->eval_blocks);
     HIPCHK(hipGetLastError());
     for (int it = 1; it <= g->nmax; ++it) {
         hipLaunchKernelGGL(k_gwr_grad_update, dim3(g->Cp * g->Cp), dim3(OCL_NT), 0, c->stream, d);
         hipLaunchKernelGGL(eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 1);
+        hipLaunchKernelGGL(k_gwr_step, dim3(1), dim3(OCL_NT), 0, c->stream, d, 1, g->eval_blocks);
     }
     HIPCHK(hipGetLastError());
     GwrState st;
