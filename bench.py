@@ -54,15 +54,16 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_nesta_batched_16x20x8x256.json")
 
 
-def split_mfma_flops(F, C, H, W, nmax):
+def split_mfma_flops(F, C, H, W, nmax, gram=True):
     """MFMA flops the split-precision passes execute per inner iteration of one slice: every 16x16x32 complex block
-    is 12 MFMAs of 16*16*32*2 flops.  Pass B: per (frame, coil, 16-row group) a forward and an inverse row DFT of
-    (W/16)(W/32) blocks each; pass CA: per (frame, 16-column tile, coil) an inverse column DFT of (H/16)(kp/32) blocks
-    and a forward one of (H/32)(kp/16) blocks."""
+    is 12 MFMAs of 16*16*32*2 flops.  Per (frame, 16-column tile, coil) an inverse column DFT of (H/16)(kp/32) blocks
+    and a forward one of (H/32)(kp/16) blocks (the Gram pass, and pass CA of the row and column passes); the row and
+    column passes add pass B: per (frame, coil, 16-row group) a forward and an inverse row DFT of (W/16)(W/32) blocks
+    each."""
     blk = 12 * 16 * 16 * 32 * 2
     kp = (nmax + 31) // 32 * 32
     groups = (nmax + 15) // 16
-    b = F * C * groups * 2 * (W // 16) * (W // 32)
+    b = 0 if gram else F * C * groups * 2 * (W // 16) * (W // 32)
     ca = F * (W // 16) * C * ((H // 16) * (kp // 32) + (H // 32) * (kp // 16))
     return blk * (b + ca)
 
@@ -230,7 +231,8 @@ def leg_uneven(ctx, synth, parts, prm, steps, warmup):
 
 
 def leg_split(ctx, synth, parts, prm, steps, warmup):
-    """Split-precision form (BF16-MFMA DFTs, 3-term split) beside the FP32 fused form on the same batch."""
+    """Split-precision form (BF16-MFMA DFTs, 3-term split) beside the FP32 fused form on the same batch: the Gram pass
+    (the default for line masks) and the row and column passes."""
     from opencliper_amd import capi
     dev = []
     for i, (x, S, M) in enumerate(parts):
@@ -239,20 +241,27 @@ def leg_split(ctx, synth, parts, prm, steps, warmup):
     ctx.sync()
     res = {"fp32_fused": leg_batched(ctx, dev, prm, steps, warmup, capi.OCL_FORM_FUSED)}
     x32 = dev[0][3].get()
+    ctx.set_split_gram(False)
+    try:
+        res["split_rowcol"] = leg_batched(ctx, dev, prm, steps, warmup, capi.OCL_FORM_SPLIT)
+    finally:
+        ctx.set_split_gram(True)
+    x_rc = dev[0][3].get()
     res["split"] = leg_batched(ctx, dev, prm, steps, warmup, capi.OCL_FORM_SPLIT)
     xs = dev[0][3].get()
     sh = SPLIT_SHAPE
     nmax = int(round(sh["H"] / sh["accel"]))
-    fl = len(dev) * split_mfma_flops(sh["F"], sh["C"], sh["H"], sh["W"], nmax)
     dense = len(dev) * dense_dft_flops(sh["F"], sh["C"], sh["H"], sh["W"])
-    it = res["split"]["iteration"]
-    it["mfma_flops_per_iter"] = fl
-    it["mfma_TFLOPs"] = fl / (it["ms_iter"] * 1e-3) / 1e12
-    it["frac_mfma"] = it["mfma_TFLOPs"] / MFMA_BF16_PEAK_TFLOPS
-    it["dense_dft_flops_per_iter"] = dense
-    it["dense_dft_TFLOPs_equiv"] = dense / (it["ms_iter"] * 1e-3) / 1e12
+    for key, gram in (("split", True), ("split_rowcol", False)):
+        fl = len(dev) * split_mfma_flops(sh["F"], sh["C"], sh["H"], sh["W"], nmax, gram=gram)
+        it = res[key]["iteration"]
+        it["mfma_flops_per_iter"] = fl
+        it["mfma_TFLOPs"] = fl / (it["ms_iter"] * 1e-3) / 1e12
+        it["frac_mfma"] = it["mfma_TFLOPs"] / MFMA_BF16_PEAK_TFLOPS
+        it["dense_dft_flops_per_iter"] = dense
+        it["dense_dft_TFLOPs_equiv"] = dense / (it["ms_iter"] * 1e-3) / 1e12
     host = (dev[0][0].get(), dev[0][1].get(), dev[0][2].get())
-    return res, {"x_split": xs, "x_fp32": x32, "slice0": host}
+    return res, {"x_split": xs, "x_split_rowcol": x_rc, "x_fp32": x32, "slice0": host}
 
 
 def _serve_trace(ctx, capi, host_pool, arrivals, pick, iters, budget, chunk, check_all=False):
@@ -525,7 +534,8 @@ def gpu_legs(args, cine, parts, full, split):
         if "split" in args.legs:
             out["split"], out["x"]["split"] = leg_split(ctx, synth, split, prm, args.steps, args.warmup)
             sp = out["split"]
-            log(f"split: {sp['split']['recons_per_s']:.1f} recons/s (FP32 fused on the same batch "
+            log(f"split: {sp['split']['recons_per_s']:.1f} recons/s, row and column passes "
+                f"{sp['split_rowcol']['recons_per_s']:.1f} (FP32 fused on the same batch "
                 f"{sp['fp32_fused']['recons_per_s']:.1f}), iteration {sp['split']['iteration']}")
         dev3 = []
         if "uneven" in args.legs:
@@ -672,6 +682,7 @@ def cpu_leg(args, slice0, full, g):
             f32 = np.asarray(g["split"]["fp32_fused"]["reports"][0]["trace_f"])
             rl["split"] = {"shape_FCHW": list(yy.shape), "vs_fp64": O.rel_l2(sx["x_split"], ref),
                            "vs_fp32_form": O.rel_l2(sx["x_split"], sx["x_fp32"]),
+                           "rowcol_vs_fp64": O.rel_l2(sx["x_split_rowcol"], ref),
                            "fp32_form_vs_fp64": O.rel_l2(sx["x_fp32"], ref),
                            "per_iter_f_relerr_vs_fp64_max": float(np.max(np.abs(fs - f64) / np.abs(f64))),
                            "per_iter_f_relerr_vs_fp32_max": float(np.max(np.abs(fs - f32) / np.abs(f32))),
@@ -840,8 +851,11 @@ def main():
                         f"{SPLIT_SHAPE['H']}x{SPLIT_SHAPE['W']} R={SPLIT_SHAPE['accel']:g}, {STAGES} stages x {ITERS} "
                         f"iterations, hipGraph",
             "recons_per_s": sp["split"]["recons_per_s"], "ms_per_step": sp["split"]["ms_per_step"],
+            "form": "Gram pass (column DFT GEMMs only; line masks)",
             "fp32_fused_recons_per_s": sp["fp32_fused"]["recons_per_s"],
             "iteration": sp["split"]["iteration"], "fp32_fused_iteration": sp["fp32_fused"]["iteration"],
+            "rowcol_recons_per_s": sp["split_rowcol"]["recons_per_s"],
+            "rowcol_iteration": sp["split_rowcol"]["iteration"],
             "roofline": {"bound": "mfma", "achieved": sp["split"]["iteration"]["mfma_TFLOPs"],
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": sp["split"]["iteration"]["frac_mfma"]},

@@ -260,6 +260,11 @@ int ocl_batch_destroy(ocl_batch* b);
  *      on a synthetic 4-slice batch and caches the fastest per context.  With autotuning on, batches created on the
  *      context tune every new H x W at setup; otherwise tuned shapes are used when present, else a heuristic. ---- */
 int ocl_ctx_set_autotune(ocl_ctx* ctx, int on);
+/* split-precision form (OCL_FORM_SPLIT, no reference counterpart): 1 (the default) runs slices with line masks (a mask
+ * constant along every k-space row) through the Gram pass, whose only transforms are the column DFTs as BF16-MFMA
+ * GEMMs, where it is instantiated (H in {64, 128, 192} with at most 32 sampled rows per frame, or H = 64 with 64);
+ * 0 keeps the row and column passes for every slice of batches created afterwards */
+int ocl_ctx_set_split_gram(ocl_ctx* ctx, int on);
 /* pins the tile shape of H x W (e.g. a saved tuning): ca_fast 1 / 2 selects the four-step column pass that gathers
  * the sampled rows from global memory / prefetches the next coil's rows and sensitivities, b_fast 1 the four-step
  * row pass; 0 selects the Stockham passes sized by tile_cols / rows_per_wg (powers of two <= 16) */

@@ -181,6 +181,7 @@ def lib():
                                 I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_ctx_set_autotune": ([P, I], I),
+        "ocl_ctx_set_split_gram": ([P, I], I),
         "ocl_rsos": ([P, P, P, I, I, LG], I), "ocl_simple_mri_recon_sos": ([P, P, P, I, I, I, I], I),
         "ocl_ctx_set_tiles": ([P, I, I, I, I, I, I], I),
         "ocl_autotune_shape": ([P, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
@@ -296,6 +297,10 @@ class Context:
     def set_autotune(self, on=True):
         """With autotuning on, batches created on this context tune the tile shape of each new H x W at setup."""
         _check(lib().ocl_ctx_set_autotune(self.handle, 1 if on else 0))
+
+    def set_split_gram(self, on=True):
+        """Split-precision form: Gram pass for line masks where it is instantiated (True) or row and column passes."""
+        _check(lib().ocl_ctx_set_split_gram(self.handle, 1 if on else 0))
 
     def set_tiles(self, H, W, ca_fast, tile_cols, b_fast, rows_per_wg):
         """Pins the tile shape of the FP32 passes for H x W."""

@@ -143,15 +143,19 @@ void choose_tiles(long total_cols, const ocl_slice& s, int* tile_cols, int* rows
 }
 
 size_t lds_B(const SliceDesc& d) {
+    if (d.prec) return mf_b_lds_bytes(d.W);
     return d.fastW ? sizeof(float2) * fast_b_lds(d.fastW)
                    : sizeof(float2) * ((size_t)2 * d.rows_per_wg * (d.W + 1) + d.W);
 }
 size_t lds_CA(const SliceDesc& d) {
+    if (d.prec) return mf_ca_lds_bytes(d.H, d.kp);
     return d.fastH ? sizeof(float2) * (fast_ca_lds(d.fastH) + (d.ca_stage ? 2 * 16 * ((size_t)d.nmax + 1) : 0))
                    : sizeof(float2) * ((size_t)2 * d.H * (d.tile_cols + 1) + d.H);
 }
 
-size_t lds_G(const SliceDesc& d) { return sizeof(float2) * (size_t)gram_lds(d.fastH, d.nmax); }
+size_t lds_G(const SliceDesc& d) {
+    return d.prec ? mf_gs_lds_bytes(d.H, d.kp) : sizeof(float2) * (size_t)gram_lds(d.fastH, d.nmax);
+}
 
 size_t lds_CA_generic(int H, int tc) { return sizeof(float2) * ((size_t)2 * H * (tc + 1) + H); }
 size_t lds_B_generic(int W, int rb) { return sizeof(float2) * ((size_t)2 * rb * (W + 1) + W); }
@@ -225,7 +229,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     if (d.fastH || prec) d.tile_cols = 16;
     if (d.fastW || prec) d.rows_per_wg = 16;
     d.prec = prec;
-    if (lds_B(d) > LDS_MAX_NESTA || lds_CA(d) > LDS_MAX_NESTA)
+    if (!prec && (lds_B(d) > LDS_MAX_NESTA || lds_CA(d) > LDS_MAX_NESTA))
         return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
     // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
     const size_t fb = (size_t)s.F * s.H;
@@ -268,6 +272,14 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     d.kp = (nmax + 31) / 32 * 32;
     if (prec && d.kp > 64)
         return fail(OCL_E_UNSUPPORTED, "NESTA: the split-precision form supports at most 64 sampled rows per frame");
+    // the split-precision Gram pass: line masks, sizes it is instantiated for, unless the context pins the row and
+    // column passes
+    if (prec) {
+        // (its buffer resources address kbuf and S with 32-bit byte offsets)
+        const bool fits32 = 8.0 * s.F * s.C * nmax * s.W < 4294967295.0 && 8.0 * s.C * s.H * s.W < 4294967295.0;
+        d.gram_ok = (c->split_gram && line_mask && fits32 && mfma_gram_supported(s.H, d.kp) &&
+                     mf_gs_lds_bytes(s.H, d.kp) <= LDS_MAX_NESTA) ? 1 : 0;
+    }
     d.ncol_tiles = (s.W + d.tile_cols - 1) / d.tile_cols;
     d.nrow_groups = (nmax + d.rows_per_wg - 1) / d.rows_per_wg;
     d.wg_B = prec ? (s.F * s.C * d.nrow_groups + MF_B_TASKS - 1) / MF_B_TASKS : s.F * s.C * d.nrow_groups;
@@ -370,11 +382,13 @@ int slice_alloc(SliceRes* r) {
         CHK(r->alloc(rowb, &p)); m[1] = (uint16_t*)p;
         CHK(r->alloc(colb, &p)); m[2] = (uint16_t*)p;
         CHK(r->alloc(colb, &p)); m[3] = (uint16_t*)p;
-        for (int kind = 0; kind < 4; ++kind) {
+        for (int slot = 0; slot < 4; ++slot) {
+            // the Gram pass takes its inverse column DFT with the K index permuted (kind 4) in the kind-2 slot
+            const int kind = (slot == 2 && d.gram_ok) ? 4 : slot;
             const int L = kind < 2 ? d.W : d.H;
             const long plane = kind < 2 ? (long)L * L : (long)L * d.kp;
             const dim3 grid((unsigned)std::min<long>((plane + OCL_NT - 1) / OCL_NT, 1024), kind < 2 ? 1 : d.F);
-            hipLaunchKernelGGL(k_dft_split, grid, dim3(OCL_NT), 0, c->stream, m[kind], kind, L, d.kp, d.rows, d.nrows,
+            hipLaunchKernelGGL(k_dft_split, grid, dim3(OCL_NT), 0, c->stream, m[slot], kind, L, d.kp, d.rows, d.nrows,
                                d.nmax);
             HIPCHK(hipGetLastError());
         }
@@ -488,13 +502,13 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     L.maxe = 4;
     int PH = ns ? v[0]->d.fastH : 0, PW = ns ? v[0]->d.fastW : 0;
     L.prec = ns ? v[0]->d.prec : 0;
-    // iteration form of each slice: the Gram form where the slice allows it (split precision and motion compensation
-    // keep passes B, S, CA); a table holding both forms runs each as launch groups of its own
+    // iteration form of each slice: the Gram form where the slice allows it (motion compensation keeps passes B, S,
+    // CA); a table holding both forms runs each as launch groups of its own
     std::vector<char>& isg = t->isg;
     isg.assign(ns, 0);
     int n_g = 0;
     for (int i = 0; i < ns; ++i) {
-        isg[i] = (!L.prec && hd[i].gram_ok && !hd[i].mc_xn) ? 1 : 0;
+        isg[i] = (hd[i].gram_ok && !hd[i].mc_xn) ? 1 : 0;
         n_g += isg[i];
     }
     const bool mixed_forms = n_g > 0 && n_g < ns;
@@ -594,7 +608,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         L.ldsB = 0;
         L.ldsCA = 0;
         for (int i = 0; i < ns; ++i) L.ldsB = std::max(L.ldsB, mf_b_lds_bytes(v[i]->d.W));
-        for (int i = 0; i < ns; ++i) L.ldsCA = std::max(L.ldsCA, mf_ca_lds_bytes(v[i]->d.H, v[i]->d.kp));
+        for (int i = 0; i < ns; ++i) L.ldsCA = std::max(L.ldsCA, lds_col(i));
         if (L.ldsCA > LDS_MAX_NESTA) return fail(OCL_E_UNSUPPORTED, "batch: split-precision column tile exceeds LDS");
     }
     // launch groups of a mixed-size table: descriptors in group order in gdd, per group its B and CA offsets in goff
@@ -612,8 +626,8 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             G.gram = set_gram[si];
             G.maxe = 4;
             G.ldsB = G.ldsCA = 0;
-            G.PH = hd[idx[0]].fastH;
-            G.PW = hd[idx[0]].fastW;
+            G.PH = L.prec ? L.PH : hd[idx[0]].fastH;     // the split-precision kernels are sized by H / 16, W / 16
+            G.PW = L.prec ? L.PW : hd[idx[0]].fastW;
             G.stage = hd[idx[0]].ca_stage;
             go.resize(o0 + 2 * (gn + 1), 0);
             for (int j = 0; j < gn; ++j) {
@@ -624,8 +638,8 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
                 G.ldsB = std::max(G.ldsB, lds_B(d));
                 G.ldsCA = std::max(G.ldsCA, lds_col(idx[j]));
                 if (d.ca_stage != G.stage) G.stage = 2;
-                if (d.fastH != G.PH) G.PH = 0;
-                if (d.fastW != G.PW) G.PW = 0;
+                if (!L.prec && d.fastH != G.PH) G.PH = 0;
+                if (!L.prec && d.fastW != G.PW) G.PW = 0;
                 if (!d.fastH) {
                     const int need = (d.H * d.tile_cols + OCL_NT - 1) / OCL_NT;
                     while (G.maxe < need) G.maxe *= 2;
@@ -1115,6 +1129,12 @@ extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* m
 extern "C" int ocl_ctx_set_autotune(ocl_ctx* c, int on) {
     if (!c) return fail(OCL_E_INVALID, "null context");
     c->autotune = on ? 1 : 0;
+    return OCL_OK;
+}
+
+extern "C" int ocl_ctx_set_split_gram(ocl_ctx* c, int on) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    c->split_gram = on ? 1 : 0;
     return OCL_OK;
 }
 

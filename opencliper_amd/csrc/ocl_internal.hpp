@@ -59,6 +59,9 @@ struct ocl_ctx {
     std::mutex mu;
     int num_cus = 256;
     int autotune = 0;                                       // 1: tune the tile shape of each new (H, W) on first use
+    // split-precision form: 1 runs the Gram pass (column GEMMs only) for line masks where it is instantiated, 0 keeps
+    // the row and column passes for every slice
+    int split_gram = 1;
     std::map<std::pair<int, int>, TileCfg> tiles;           // tuned tile shapes by (H, W)
 };
 

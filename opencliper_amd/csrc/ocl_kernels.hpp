@@ -199,6 +199,9 @@ void nesta_kernel_attributes(int lds_bytes);
 // split-precision passes (ocl_nesta_mfma.hip)
 hipError_t launch_nesta_B_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
 hipError_t launch_nesta_CA_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
+// split-precision Gram pass (line masks): mode 0 iteration, 1 setup x_ref = A^H sqrt(N) y into x0
+hipError_t launch_nesta_G_mfma(const NestaLaunch& L, hipStream_t stream, int mode);
+bool mfma_gram_supported(int H, int kp);   // H and sampled-row capacity the split-precision Gram pass runs
 bool mfma_size_supported(int n);      // H or W the split-precision passes are instantiated for
 void mfma_kernel_attributes(int lds_bytes);
 #define MF_CA_TILES 4                  // column tiles (one wavefront each) per workgroup of the split-precision pass CA
@@ -210,8 +213,23 @@ __host__ __device__ constexpr size_t mf_b_lds_bytes(int W) { return 2 * 4 * (siz
 __host__ __device__ constexpr size_t mf_ca_lds_bytes(int H, int kp) {
     return 2 * 4 * (size_t)((H * mf_lds_ld(kp)) > (kp * mf_lds_ld(H)) ? H * mf_lds_ld(kp) : kp * mf_lds_ld(H));
 }
+// Gram pass of the split-precision form (line masks): row-waves per 16-column tile, each holding 16 H / nr rows of the
+// tile in registers.  Two halve the per-wave registers at H >= 128 at the price of one LDS exchange of the forward
+// partials per coil; H = 64 needs no split.
+#ifndef OCL_GS_NR
+#define OCL_GS_NR 2
+#endif
+static_assert(OCL_GS_NR == 1 || OCL_GS_NR == 2, "OCL_GS_NR: one or two row-waves per column tile");
+__host__ __device__ constexpr int mf_gs_nr(int H) { return H <= 64 ? 1 : OCL_GS_NR; }
+// LDS of the Gram pass: both DFT matrices of a frame (forward [4][kp][H], inverse [4][H][kp], padded rows) and, with
+// several row-waves, one slab of forward partials per wavefront (kp / 16 m-tiles x 8 floats x 64 lanes)
+__host__ __device__ constexpr size_t mf_gs_lds_bytes(int H, int kp) {
+    return 2 * 4 * ((size_t)kp * mf_lds_ld(H) + (size_t)H * mf_lds_ld(kp)) +
+           (mf_gs_nr(H) > 1 ? (size_t)4 * mf_gs_nr(H) * MF_CA_TILES * (kp / 16) * 8 * 64 : 0);
+}
 // fills one DFT matrix of the split-precision passes (kind: 0 row forward, 1 row inverse, 2 column inverse,
-// 3 column forward); rows/nrows/nmax describe the sampled rows of each frame for the column kinds
+// 3 column forward, 4 column inverse with the K index permuted); rows/nrows/nmax describe the sampled rows of each
+// frame for the column kinds
 __global__ void k_dft_split(uint16_t* out, int kind, int L, int kp, const int16_t* rows, const int* nrows, int nmax);
 __global__ void k_setup_max(const SliceDesc* descs, const int* wg_off, int nslices, double* out);
 __global__ void k_finalize(const SliceDesc* descs, int nslices, float2* const* outs);

@@ -600,6 +600,7 @@ static NestaKernel pick_G(int PH) {
 }
 
 hipError_t launch_nesta_G(const NestaLaunch& L, hipStream_t stream, int mode) {
+    if (L.prec) return launch_nesta_G_mfma(L, stream, mode);
     hipLaunchKernelGGL(pick_G(L.PH), dim3(L.totCA), dim3(OCL_NT), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }
