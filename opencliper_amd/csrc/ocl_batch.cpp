@@ -276,7 +276,8 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
     // column passes
     if (prec) {
         // (its buffer resources address kbuf and S with 32-bit byte offsets)
-        const bool fits32 = 8.0 * s.F * s.C * nmax * s.W < 4294967295.0 && 8.0 * s.C * s.H * s.W < 4294967295.0;
+        const bool fits32 = 8.0 * s.F * s.C * ((nmax + 31) / 32 * 32) * s.W < 4294967295.0 &&
+                            8.0 * s.C * s.H * s.W < 4294967295.0;
         d.gram_ok = (c->split_gram && line_mask && fits32 && mfma_gram_supported(s.H, d.kp) &&
                      mf_gs_lds_bytes(s.H, d.kp) <= LDS_MAX_NESTA) ? 1 : 0;
     }

@@ -213,19 +213,14 @@ __host__ __device__ constexpr size_t mf_b_lds_bytes(int W) { return 2 * 4 * (siz
 __host__ __device__ constexpr size_t mf_ca_lds_bytes(int H, int kp) {
     return 2 * 4 * (size_t)((H * mf_lds_ld(kp)) > (kp * mf_lds_ld(H)) ? H * mf_lds_ld(kp) : kp * mf_lds_ld(H));
 }
-// Gram pass of the split-precision form (line masks): row-waves per 16-column tile, each holding 16 H / nr rows of the
-// tile in registers.  Two halve the per-wave registers at H >= 128 at the price of one LDS exchange of the forward
-// partials per coil; H = 64 needs no split.
-#ifndef OCL_GS_NR
-#define OCL_GS_NR 2
-#endif
-static_assert(OCL_GS_NR == 1 || OCL_GS_NR == 2, "OCL_GS_NR: one or two row-waves per column tile");
-__host__ __device__ constexpr int mf_gs_nr(int H) { return H <= 64 ? 1 : OCL_GS_NR; }
+// Gram pass of the split-precision form (line masks): row-waves of mf_gs_rt(MT) m-tiles (16 rows each) per 16-column
+// tile; a tile's two row-waves (H >= 128) exchange their forward partials through LDS
+__host__ __device__ constexpr int mf_gs_rt(int MT) { return MT <= 4 ? MT : MT / 2; }
 // LDS of the Gram pass: both DFT matrices of a frame (forward [4][kp][H], inverse [4][H][kp], padded rows) and, with
-// several row-waves, one slab of forward partials per wavefront (kp / 16 m-tiles x 8 floats x 64 lanes)
+// two row-waves, one slab of forward partials per wavefront (kp / 16 m-tiles x 8 floats x 64 lanes)
 __host__ __device__ constexpr size_t mf_gs_lds_bytes(int H, int kp) {
     return 2 * 4 * ((size_t)kp * mf_lds_ld(H) + (size_t)H * mf_lds_ld(kp)) +
-           (mf_gs_nr(H) > 1 ? (size_t)4 * mf_gs_nr(H) * MF_CA_TILES * (kp / 16) * 8 * 64 : 0);
+           (mf_gs_rt(H / 16) < H / 16 ? (size_t)4 * 2 * MF_CA_TILES * (kp / 16) * 8 * 64 : 0);
 }
 // fills one DFT matrix of the split-precision passes (kind: 0 row forward, 1 row inverse, 2 column inverse,
 // 3 column forward, 4 column inverse with the K index permuted); rows/nrows/nmax describe the sampled rows of each

@@ -20,6 +20,7 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -77,6 +78,47 @@ __device__ __forceinline__ void cmac(f32x4& dr, f32x4& di, const Op8& a, const O
     dr = MF(a.ih, nil, dr);
     di = MF(a.ih, b.rl, di);
     dr = MF(a.il, nih, dr);
+    di = MF(a.il, b.rh, di);
+}
+
+// split8 with one conversion per pair of values: hi = RNE bf16 of both, the exact remainders v - hi (FP32), their RNE
+// bf16 as lo.  The same values as split8, in 3 instructions per value instead of 4.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned cvt_pair(float a, float b) {
+    f32x2_t v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
+}
+__device__ __forceinline__ void split8p(const float2 (&v)[8], Op8& o) {
+    u32x4 rh, rl, ih, il;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const float2 a = v[2 * p], b = v[2 * p + 1];
+        const unsigned hr = cvt_pair(a.x, b.x), hi = cvt_pair(a.y, b.y);
+        rh[p] = hr;
+        ih[p] = hi;
+        rl[p] = cvt_pair(a.x - __builtin_bit_cast(float, hr << 16), b.x - __builtin_bit_cast(float, hr & 0xffff0000u));
+        il[p] = cvt_pair(a.y - __builtin_bit_cast(float, hi << 16), b.y - __builtin_bit_cast(float, hi & 0xffff0000u));
+    }
+    o.rh = __builtin_bit_cast(bf16x8, rh);
+    o.rl = __builtin_bit_cast(bf16x8, rl);
+    o.ih = __builtin_bit_cast(bf16x8, ih);
+    o.il = __builtin_bit_cast(bf16x8, il);
+}
+
+// (dr - dn, di) += A B for one 16 x 16 x 32 complex block, 3-term split product, with the Im A Im B terms summed
+// positive into dn (subtracted once at the end) instead of negating each B operand: three accumulator chains
+__device__ __forceinline__ void cmac3(f32x4& dr, f32x4& dn, f32x4& di, const Op8& a, const Op8& b) {
+    dr = MF(a.rh, b.rh, dr);
+    dn = MF(a.ih, b.ih, dn);
+    di = MF(a.rh, b.ih, di);
+    dr = MF(a.rh, b.rl, dr);
+    dn = MF(a.ih, b.il, dn);
+    di = MF(a.ih, b.rh, di);
+    dr = MF(a.rl, b.rh, dr);
+    dn = MF(a.il, b.ih, dn);
+    di = MF(a.rh, b.il, di);
+    di = MF(a.ih, b.rl, di);
+    di = MF(a.rl, b.ih, di);
     di = MF(a.il, b.rh, di);
 }
 
@@ -428,23 +470,22 @@ NestaKernel pick_CA_mfma(int MT, int KJ) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// Gram pass GS: the Gram form of the fused iteration for line masks (the identities are stated in ocl_nesta_gram.hpp)
-// with both column DFTs as split-precision GEMMs, so no row transform runs in the iteration at all.
-// One workgroup = MF_CA_TILES column tiles of 16 columns of one frame.  Each tile is covered by NR = MT / RT
-// row-waves of RW = 16 RT rows; a wavefront keeps the A^H r accumulator of its RW x 16 pixels in registers, in the
-// MFMA output layout h = h0 + 16 t + 4 g + i (t < RT, i < 4, column n of the tile).  Both DFT matrices of the frame
-// are staged in LDS once per workgroup:
+// Gram pass of the split-precision form (GS): the Gram form of the fused iteration for line masks (the identities are
+// stated in ocl_nesta_gram.hpp) with both column DFTs as split-precision GEMMs, so no row transform runs in the
+// iteration at all.  One workgroup = MF_CA_TILES column tiles of 16 columns of one frame; each tile is covered by
+// NR = MT / RT row-waves of RW = 16 RT rows, and a wavefront keeps the A^H r accumulator of its RW x 16 pixels in
+// registers, in the MFMA output layout h = h0 + 16 t + 4 g + i (t < RT, i < 4, column n of its tile).  Both DFT
+// matrices of the frame are staged in LDS once per workgroup:
 //   DA = dcolF (kind 3) [4][kp][H]: K index = h permuted, so the B operand of k-step s is the lane's own pixels of
 //        m-tiles 2 s and 2 s + 1;
 //   DB = dcolI (kind 4) [4][H][kp]: K index = j permuted, so the B operand is the lane's own forward outputs.
-// Per coil c: K = DA (S_c x_k) over the wave's rows (a partial sum; the two row-waves of a tile exchange their
-// partials through LDS), rho = m K W / sqrt(N) - V_c at the carried rows (V_c = W F_x^{-1} y_c in kbuf; m = 1 on
-// sampled mask rows), ||rho||^2, Z = DB rho, acc += conj(S_c) Z.  x_k and S_c are re-read per coil (L2), one step
-// ahead of their use.  Then the NESTA update of the wave's pixels with the Gram form's ping-ponged x and W, and the
-// partial sums of the scalar step (k_nesta_S2).  mode 1 (setup): x0 = x_ref = A^H sqrt(N) y = sum_c conj(S_c) DB V_c
-// / sqrt(N).
-// Global accesses go through buffer resources: a 32-bit lane offset (4 g W + w, in bytes) and a scalar row offset,
-// so no per-lane 64-bit addresses are formed or kept.
+// Per coil c: K = DA (S_c x_k) over the wave's rows (a partial sum; the two row-waves of a tile exchange their partials
+// through LDS), rho = m K W / sqrt(N) - V_c at the carried rows (V_c = W F_x^{-1} y_c in kbuf; m = 1 on sampled mask
+// rows), ||rho||^2, Z = DB rho, acc += conj(S_c) Z.  Then the NESTA update of the wave's pixels with the Gram form's
+// ping-ponged x and W, and the partial sums of the scalar step (k_nesta_S2).  mode 1 (setup): x0 = x_ref =
+// A^H sqrt(N) y = sum_c conj(S_c) DB V_c / sqrt(N).
+// Global accesses go through buffer resources: a 32-bit lane offset (in bytes) and a scalar row offset, so no per-lane
+// 64-bit addresses are formed or kept.
 // ---------------------------------------------------------------------------------------------------------------------
 typedef __amdgpu_buffer_rsrc_t BufRes;
 __device__ __forceinline__ BufRes buf_res(const void* p, long bytes) {
@@ -530,6 +571,8 @@ __device__ __forceinline__ void gs_update(const SliceDesc& d, const NState* st, 
     uterm[1] += (double)fx;
 }
 
+// The pass itself.  The two row-waves of a tile exchange their forward partials through LDS (own + peer is the same
+// sum in both, as floating-point addition commutes).  S_c and x_k are streamed from L2 one step ahead of each use.
 template <int MT, int KJ, int RT>
 __device__ __forceinline__ void nesta_GS(const SliceDesc& d, int local, int mode, uint16_t* lds, double* red) {
     constexpr int H = 16 * MT, KP = 32 * KJ, MJ = 2 * KJ, RW = 16 * RT, NR = MT / RT, KS1 = RT / 2;
@@ -538,40 +581,40 @@ __device__ __forceinline__ void nesta_GS(const SliceDesc& d, int local, int mode
     static_assert(NR <= 2, "the partial exchange pairs at most two row-waves");
     constexpr int ldA = mf_lds_ld(H), ldB = mf_lds_ld(KP);
     const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
-    // wavefront index as a scalar, so the row offsets derived from it stay wave-uniform
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tcs = wv % TC, q = wv / TC;
+    // consecutive workgroups take consecutive frames of one column-tile group (they share S_c's columns)
     const int ngrp = (d.ncol_tiles + TC - 1) / TC;
-    const int tg = local % ngrp, f = local / ngrp;
+    const int f = local % d.F, tg = local / d.F;
     const int tile = tg * TC + tcs;
-    const bool act = tile < d.ncol_tiles;        // idle wavefronts of a frame's last group still meet every barrier
+    const bool act = tile < d.ncol_tiles;        // idle wavefronts of a last group still meet every barrier
     const int C = d.C, W = d.W, nmax = d.nmax;
     const long N = d.N;
     const int nr = d.nrows[f];
     const int h0 = q * RW;
     const NState* st = d.st;
-    const unsigned W8 = 8u * (unsigned)W;
-    // byte offset of pixel (h0 + 16 t + 4 g + i, w) = so (scalar: frame, row h0 + 16 t + i) + vo (lane)
     const unsigned vo = 8u * (unsigned)(4 * g * W + (act ? tile : 0) * 16 + n);
     const unsigned so0 = (unsigned)(8 * ((long)f * N + (long)h0 * W));
     const BufRes rS = buf_res(d.S, 8 * (long)C * N);
-    const BufRes rX = buf_res(__builtin_amdgcn_readfirstlane(st->it_cur) ? d.x1 : d.x0, 8 * (long)d.F * N);
     const BufRes rV = buf_res(d.kbuf, 8 * (long)d.F * C * nmax * W);
     uint16_t* DA = lds;
     uint16_t* DB = lds + 4 * KP * ldA;
     float* slab = (float*)(DB + 4 * H * ldB);
     if (mode == 0) stage_planes(DA, d.dcolF + (long)f * 4 * KP * H, KP, H);
     stage_planes(DB, d.dcolI + (long)f * 4 * H * KP, H, KP);
+    (void)ngrp;
 
-    // residual factor per carried row j = 16 mj + 4 g + i of this lane: W / sqrt(N) on a sampled mask row, else 0
+    // residual factor W / sqrt(N) at the lane's carried-row slots j = 16 mj + 4 g + i that are sampled mask rows,
+    // as one bit per slot
     const float sW = (float)W * d.inv_sqrtN;
-    float mk[MJ][4];
+    unsigned mkb = 0;
 #pragma unroll
     for (int mj = 0; mj < MJ; ++mj)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = 16 * mj + 4 * g + i;
-            mk[mj][i] = (j < nr && d.rowmask[(long)f * nmax + j]) ? sW : 0.f;
+            if (j < nr && d.rowmask[(long)f * nmax + j]) mkb |= 1u << (4 * mj + i);
         }
+    unsigned W8 = 8u * (unsigned)W;
     auto load_v = [&](int c, float2 (&v)[MJ][4]) {
         const unsigned sb = (unsigned)(8 * ((long)f * C + c) * nmax * W);
 #pragma unroll
@@ -582,148 +625,82 @@ __device__ __forceinline__ void nesta_GS(const SliceDesc& d, int local, int mode
                 v[mj][i] = j < nr ? bld2(rV, vo, sb + (unsigned)(16 * mj + i) * W8) : make_float2(0.f, 0.f);
             }
     };
-    // S_c and x_k rows of m-tile t
-    auto load_s = [&](int c, int t, float2 (&sv)[4]) {
-        const unsigned sb = (unsigned)(8 * ((long)c * N + (long)h0 * W)) + (unsigned)(16 * t) * W8;
+    // S_c at the rows of m-tiles t and t + 1 of this wave
+    auto load_s2 = [&](int c, int t, float2 (&sv)[8]) {
+        const unsigned sb = (unsigned)(8 * ((long)c * N + (long)h0 * W));
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sv[i] = bld2(rS, vo, sb + (unsigned)i * W8);
-    };
-    auto load_x = [&](int t, float2 (&xv)[4]) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) xv[i] = bld2(rX, vo, so0 + (unsigned)(16 * t + i) * W8);
+        for (int e = 0; e < 8; ++e) sv[e] = bld2(rS, vo, sb + (unsigned)(16 * (t + (e >> 2)) + (e & 3)) * W8);
     };
 
-    float2 acc[RT][4], vv[MJ][4], s1[2][4], x1[2][4];
-    if (mode == 0) {
-        load_s(0, 0, s1[0]);
-        load_s(0, 1, s1[1]);
-        load_x(0, x1[0]);
-        load_x(1, x1[1]);
-    }
+    // x_k is re-read per coil (an L2 hit: the workgroups of a column-tile group run on one XCD) instead of being held
+    // in registers across the coil loop
+    const BufRes rX = buf_res(__builtin_amdgcn_readfirstlane(st->it_cur) ? d.x1 : d.x0, 8 * (long)d.F * N);
+    auto load_x2 = [&](int t, float2 (&xv)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[e] = bld2(rX, vo, so0 + (unsigned)(16 * (t + (e >> 2)) + (e & 3)) * W8);
+    };
+    float2 xa[8], xb[8];
+    float2 acc[RT][4], vv[MJ][4], sa[8], sb[8];
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[t][i] = make_float2(0.f, 0.f);
-    load_v(0, vv);
-    double r2 = 0.0;
-    __syncthreads();                              // DA, DB staged
-
-    for (int c = 0; c < C; ++c) {
-        // the DFT fragments are re-read from LDS per coil instead of being hoisted out of the loop into registers
-        asm volatile("" ::: "memory");
-        float2 rho[MJ][4];
-        float2 s2[4];
-        if (mode == 0) {
-            // ---- forward column DFT onto the carried rows (partial over this wave's rows) ----
-            f32x4 kr[MJ], ki[MJ];
+    // inverse column DFT of rho (as the B operands rop) from the carried rows, two m-tiles per step, conj(S_c)
+    // accumulate; sa holds S_c of m-tiles 0, 1 on entry and S of the next coil's m-tiles 0, 1 on exit
+    auto inverse_acc = [&](int c, const Op8 (&rop)[KJ]) {
 #pragma unroll
-            for (int mj = 0; mj < MJ; ++mj) { kr[mj] = f32x4{0.f, 0.f, 0.f, 0.f}; ki[mj] = kr[mj]; }
+        for (int t = 0; t < RT; t += 2) {
+            if (t + 2 < RT) load_s2(c, t + 2, sb);
+            else if (c + 1 < C) load_s2(c + 1, 0, sb);
+            if (mode == 0 && t + 2 >= RT && c + 1 < C) load_x2(0, xa);   // the next coil's first forward operand
+            f32x4 zr[2], zn[2], zi[2];
 #pragma unroll
-            for (int s = 0; s < KS1; ++s) {
-                float2 s1n[2][4], x1n[2][4];
-                if (s + 1 < KS1) {
-                    load_s(c, 2 * s + 2, s1n[0]);
-                    load_s(c, 2 * s + 3, s1n[1]);
-                    load_x(2 * s + 2, x1n[0]);
-                    load_x(2 * s + 3, x1n[1]);
-                } else {
-                    load_s(c, 0, s2);
-                }
-                float2 v[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[e] = c_mul(x1[e >> 2][e & 3], s1[e >> 2][e & 3]);
-                Op8 b;
-                split8(v, b);
-#pragma unroll
-                for (int mj = 0; mj < MJ; ++mj)
-                    cmac(kr[mj], ki[mj], loadA_lds(DA, KP, ldA, 16 * mj + n, h0 + 32 * s + 8 * g), b);
-                if (s + 1 < KS1)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        s1[0][i] = s1n[0][i];
-                        s1[1][i] = s1n[1][i];
-                        x1[0][i] = x1n[0][i];
-                        x1[1][i] = x1n[1][i];
-                    }
-                pipeline_fence();
+            for (int u = 0; u < 2; ++u) {
+                zr[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+                zn[u] = zr[u];
+                zi[u] = zr[u];
             }
-            if constexpr (NR > 1) {
-                // the two row-waves of the tile exchange their partials; own + peer is the same sum in both
-                // (floating-point addition commutes)
-                float* own = slab + (long)wv * (MJ * 8 * 64) + l;
-                const float* peer = slab + (long)((1 - q) * TC + tcs) * (MJ * 8 * 64) + l;
-                __syncthreads();                  // the previous coil's partials have been read
 #pragma unroll
-                for (int mj = 0; mj < MJ; ++mj)
+            for (int s = 0; s < KJ; ++s)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        own[((mj * 4 + i) * 2) * 64] = kr[mj][i];
-                        own[((mj * 4 + i) * 2 + 1) * 64] = ki[mj][i];
-                    }
-                __syncthreads();
+                for (int u = 0; u < 2; ++u)
+                    cmac3(zr[u], zn[u], zi[u], loadA_lds(DB, H, ldB, h0 + 16 * (t + u) + n, 32 * s + 8 * g), rop[s]);
 #pragma unroll
-                for (int mj = 0; mj < MJ; ++mj)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        kr[mj][i] += peer[((mj * 4 + i) * 2) * 64];
-                        ki[mj][i] += peer[((mj * 4 + i) * 2 + 1) * 64];
-                    }
-            }
-            // ---- residual at the carried rows ----
-            float rq = 0.f;
-#pragma unroll
-            for (int mj = 0; mj < MJ; ++mj)
+            for (int u = 0; u < 2; ++u)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float2 rv = make_float2(fmaf(mk[mj][i], kr[mj][i], -vv[mj][i].x),
-                                                  fmaf(mk[mj][i], ki[mj][i], -vv[mj][i].y));
-                    rq = fmaf(rv.x, rv.x, fmaf(rv.y, rv.y, rq));
-                    rho[mj][i] = rv;
+                    const float zx = zr[u][i] - zn[u][i], zy = zi[u][i];
+                    const float2 sv_ = sa[4 * u + i];
+                    acc[t + u][i].x = fmaf(zx, sv_.x, fmaf(zy, sv_.y, acc[t + u][i].x));
+                    acc[t + u][i].y = fmaf(zy, sv_.x, fmaf(-zx, sv_.y, acc[t + u][i].y));
                 }
-            if (q == 0 && act) r2 += (double)rq;
-        } else {
-            load_s(c, 0, s2);
 #pragma unroll
-            for (int mj = 0; mj < MJ; ++mj)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) rho[mj][i] = vv[mj][i];
+            for (int e = 0; e < 8; ++e) sa[e] = sb[e];
+            pipeline_fence();
         }
-        Op8 rop[KJ];
+    };
+    auto split_rho = [&](const float2 (&rho)[MJ][4], Op8 (&rop)[KJ]) {
 #pragma unroll
         for (int s = 0; s < KJ; ++s) {
             float2 v[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) v[e] = rho[2 * s + (e >> 2)][e & 3];
-            split8(v, rop[s]);
+            split8p(v, rop[s]);
         }
-        if (c + 1 < C) load_v(c + 1, vv);
-        // ---- inverse column DFT from the carried rows, conj(S_c) accumulate ----
-#pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            float2 s2n[4];
-            if (t + 1 < RT) {
-                load_s(c, t + 1, s2n);
-            } else if (mode == 0 && c + 1 < C) {          // the next coil's first forward operands
-                load_s(c + 1, 0, s1[0]);
-                load_s(c + 1, 1, s1[1]);
-                load_x(0, x1[0]);
-                load_x(1, x1[1]);
-            }
-            f32x4 zr = {0.f, 0.f, 0.f, 0.f}, zi = zr;
-#pragma unroll
-            for (int s = 0; s < KJ; ++s) cmac(zr, zi, loadA_lds(DB, H, ldB, h0 + 16 * t + n, 32 * s + 8 * g), rop[s]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                acc[t][i].x = fmaf(zr[i], s2[i].x, fmaf(zi[i], s2[i].y, acc[t][i].x));
-                acc[t][i].y = fmaf(zi[i], s2[i].x, fmaf(-zr[i], s2[i].y, acc[t][i].y));
-            }
-            if (t + 1 < RT)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) s2[i] = s2n[i];
-            pipeline_fence();
-        }
-    }
+    };
+
     if (mode == 1) {
+        // ---- setup: x0 = x_ref = sum_c conj(S_c) DB V_c / sqrt(N) ----
+        load_v(0, vv);
+        load_s2(0, 0, sa);
+        __syncthreads();                          // DB staged
+        for (int c = 0; c < C; ++c) {
+            asm volatile("" : "+s"(W8) : : "memory");
+            Op8 rop[KJ];
+            split_rho(vv, rop);
+            if (c + 1 < C) load_v(c + 1, vv);
+            inverse_acc(c, rop);
+        }
         if (act) {
             const BufRes r0 = buf_res(d.x0, 8 * (long)d.F * N);
 #pragma unroll
@@ -733,6 +710,82 @@ __device__ __forceinline__ void nesta_GS(const SliceDesc& d, int local, int mode
                     bst2(r0, vo, so0 + (unsigned)(16 * t + i) * W8, c_scale(acc[t][i], d.inv_sqrtN));
         }
         return;
+    }
+
+    load_s2(0, 0, sa);
+    load_x2(0, xa);
+    double r2 = 0.0;
+    __syncthreads();                              // DA, DB staged
+
+    for (int c = 0; c < C; ++c) {
+        asm volatile("" : "+s"(W8) : : "memory");
+        // ---- forward column DFT onto the carried rows (partial over this wave's rows); sa, xa hold S and x of
+        // k-step s (m-tiles 2 s, 2 s + 1) ----
+        f32x4 kr[MJ], kn[MJ], ki[MJ];
+#pragma unroll
+        for (int mj = 0; mj < MJ; ++mj) {
+            kr[mj] = f32x4{0.f, 0.f, 0.f, 0.f};
+            kn[mj] = kr[mj];
+            ki[mj] = kr[mj];
+        }
+#pragma unroll
+        for (int s = 0; s < KS1; ++s) {
+            // the next operands: k-step s + 1, or S_c of m-tiles 0, 1 for the inverse DFT
+            load_s2(c, s + 1 < KS1 ? 2 * s + 2 : 0, sb);
+            if (s + 1 < KS1) load_x2(2 * s + 2, xb);
+            float2 v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = c_mul(xa[e], sa[e]);
+            Op8 b;
+            split8p(v, b);
+#pragma unroll
+            for (int mj = 0; mj < MJ; ++mj)
+                cmac3(kr[mj], kn[mj], ki[mj], loadA_lds(DA, KP, ldA, 16 * mj + n, h0 + 32 * s + 8 * g), b);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                sa[e] = sb[e];
+                if (s + 1 < KS1) xa[e] = xb[e];
+            }
+            pipeline_fence();
+        }
+#pragma unroll
+        for (int mj = 0; mj < MJ; ++mj) kr[mj] -= kn[mj];
+        load_v(c, vv);                            // V_c, in flight across the partial exchange
+        if constexpr (NR > 1) {
+            float* own = slab + (long)wv * (MJ * 8 * 64) + l;
+            const float* peer = slab + (long)((1 - q) * TC + tcs) * (MJ * 8 * 64) + l;
+            __syncthreads();                      // the previous coil's partials have been read
+#pragma unroll
+            for (int mj = 0; mj < MJ; ++mj)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    own[((mj * 4 + i) * 2) * 64] = kr[mj][i];
+                    own[((mj * 4 + i) * 2 + 1) * 64] = ki[mj][i];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int mj = 0; mj < MJ; ++mj)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    kr[mj][i] += peer[((mj * 4 + i) * 2) * 64];
+                    ki[mj][i] += peer[((mj * 4 + i) * 2 + 1) * 64];
+                }
+        }
+        float2 rho[MJ][4];
+        float rq = 0.f;
+#pragma unroll
+        for (int mj = 0; mj < MJ; ++mj)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float mkv = (mkb >> (4 * mj + i)) & 1u ? sW : 0.f;
+                const float2 rv = make_float2(fmaf(mkv, kr[mj][i], -vv[mj][i].x), fmaf(mkv, ki[mj][i], -vv[mj][i].y));
+                rq = fmaf(rv.x, rv.x, fmaf(rv.y, rv.y, rq));
+                rho[mj][i] = rv;
+            }
+        if (q == 0 && act) r2 += (double)rq;
+        Op8 rop[KJ];
+        split_rho(rho, rop);
+        inverse_acc(c, rop);
     }
     double part[3] = {r2 / (double)W, 0.0, 0.0};
     if (__builtin_amdgcn_readfirstlane(st->it_sstart)) gs_update<RT, true>(d, st, f, so0, vo, act, acc, part + 1);
@@ -745,27 +798,37 @@ __device__ __forceinline__ void nesta_GS(const SliceDesc& d, int local, int mode
     }
 }
 
-// KJM: largest kp / 32 of the table (a slice with kp = 32 in a KJM = 2 launch takes the KJ = 1 path)
+// Logical workgroup of a launch whose physical workgroups are dealt round-robin over the 8 XCDs: XCD x runs the x-th
+// contiguous range of logical workgroups, in order, so workgroups that share data (a column-tile group's frames) share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_logical(int b, int total) {
+    const int q = total >> 3, r = total & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// KJM: largest kp / 32 of the table (a slice with kp = 32 in a KJM = 2 launch takes the KJ = 1 path).  Workgroups
+// are taken in XCD-contiguous order, and consecutive ones are consecutive frames of one column-tile group, so the
+// frames that read the same columns of S_c share an L2.
 template <int MT, int RT, int KJM>
 __global__ __launch_bounds__(64 * MF_CA_TILES * (MT / RT)) void k_nesta_GS(const SliceDesc* __restrict__ descs,
                                                                            const int* __restrict__ wg_off,
                                                                            int nslices, int mode) {
     extern __shared__ uint16_t mlds[];
     __shared__ double red[64];
-    const int si = find_slice(wg_off, nslices, blockIdx.x);
+    const int b = xcd_logical(blockIdx.x, gridDim.x);
+    const int si = find_slice(wg_off, nslices, b);
     const SliceDesc& d = descs[si];
     if (mode == 0 && !d.st->it_pending) return;
-    const int local = blockIdx.x - wg_off[si];
+    const int local = b - wg_off[si];
     if (KJM == 1 || d.kp == 32) nesta_GS<MT, 1, RT>(d, local, mode, mlds, red);
     else nesta_GS<MT, KJM, RT>(d, local, mode, mlds, red);
 }
 
 NestaKernel pick_GS(int MT, int KJ) {
-    constexpr int R128 = 8 / OCL_GS_NR, R192 = 12 / OCL_GS_NR;
     switch (MT) {
-        case 4: return KJ == 1 ? k_nesta_GS<4, 4, 1> : k_nesta_GS<4, 4, 2>;
-        case 8: return KJ == 1 ? k_nesta_GS<8, R128, 1> : nullptr;
-        case 12: return KJ == 1 ? k_nesta_GS<12, R192, 1> : nullptr;
+        case 4: return KJ == 1 ? k_nesta_GS<4, mf_gs_rt(4), 1> : k_nesta_GS<4, mf_gs_rt(4), 2>;
+        case 8: return KJ == 1 ? k_nesta_GS<8, mf_gs_rt(8), 1> : nullptr;
+        case 12: return KJ == 1 ? k_nesta_GS<12, mf_gs_rt(12), 1> : nullptr;
         default: return nullptr;
     }
 }
@@ -777,15 +840,15 @@ bool mfma_size_supported(int n) { return n == 64 || n == 128 || n == 192 || n ==
 // The Gram pass is instantiated for H = 64 (kp 32 or 64), 128 and 192 (kp 32): the sizes whose two DFT matrices and
 // partial slabs fit one workgroup's LDS.
 bool mfma_gram_supported(int H, int kp) {
-    if (!pick_GS(H / 16, kp / 32) || H % 16 != 0 || kp % 32 != 0) return false;
+    if (H % 16 != 0 || kp % 32 != 0 || !pick_GS(H / 16, kp / 32)) return false;
     return mf_gs_lds_bytes(H, kp) + 64 * sizeof(double) <= 160 * 1024;
 }
 
 hipError_t launch_nesta_G_mfma(const NestaLaunch& L, hipStream_t stream, int mode) {
     NestaKernel k = pick_GS(L.PH, L.kj);
     if (!k) return hipErrorInvalidValue;
-    const int threads = 64 * MF_CA_TILES * mf_gs_nr(16 * L.PH);
-    hipLaunchKernelGGL(k, dim3(L.totCA), dim3(threads), L.ldsCA, stream, L.dd, L.offCA, L.ns, mode);
+    hipLaunchKernelGGL(k, dim3(L.totCA), dim3(64 * MF_CA_TILES * (L.PH / mf_gs_rt(L.PH))),
+                       mf_gs_lds_bytes(16 * L.PH, 32 * L.kj), stream, L.dd, L.offCA, L.ns, mode);
     return hipGetLastError();
 }
 
