@@ -104,6 +104,11 @@ int ocl_fft2d(ocl_ctx* ctx, const void* in, void* out, long batch, int H, int W,
 /* SimpleMRIRecon::launch (src/processes/examples/SimpleMRIRecon.cpp:59-101): x = sum_c conj(S_c) IFFT2(y_c).
  * Like the reference it transforms y in place. */
 int ocl_simple_mri_recon(ocl_ctx* ctx, void* y, const void* S, void* x, int F, int C, int H, int W);
+/* n SimpleMRIRecon launches of one shape at once (no reference counterpart: for callers with many reconstructions,
+ * where one launch per reconstruction would be launch-bound).  y[i], S[i], x[i] are the i-th input's device
+ * buffers; each result is what ocl_simple_mri_recon gives, and each y[i] is transformed in place as there. */
+int ocl_simple_mri_recon_batch(ocl_ctx* ctx, int n, void* const* y, const void* const* S, void* const* x, int F, int C,
+                               int H, int W);
 
 /* RSoS::launch (src/processes/RSoS.cpp, kernel src/kernels/rss.cl): out_f = (sqrt(sum_c |in_{f,c}|^2), 0) */
 int ocl_rsos(ocl_ctx* ctx, const void* in, void* out, int F, int C, long N);

@@ -159,6 +159,7 @@ def lib():
         "ocl_ximage_sum": ([P, P, P, I, I, LG], I), "ocl_complex_abs": ([P, P, P, LG, I], I),
         "ocl_copy": ([P, P, P, LG], I), "ocl_fft2d": ([P, P, P, LG, I, I, I], I),
         "ocl_simple_mri_recon": ([P, P, P, P, I, I, I, I], I),
+        "ocl_simple_mri_recon_batch": ([P, I, P, P, P, I, I, I, I], I),
         "ocl_scal": ([P, P, LG, F32, F32], I), "ocl_axpy": ([P, LG, F32, F32, P, P], I),
         "ocl_reduce": ([P, P, P, LG, I, ctypes.POINTER(D)], I),
         "ocl_normest": ([P, I, LG, ctypes.POINTER(D), ctypes.POINTER(I)], I),
@@ -361,6 +362,15 @@ class Context:
     def simple_mri_recon(self, y, S, x):
         F, C, H, W = y.shape
         _check(lib().ocl_simple_mri_recon(self.handle, y.ptr, S.ptr, x.ptr, F, C, H, W))
+
+    def simple_mri_recon_batch(self, items):
+        """SimpleMRIRecon of many inputs of one shape at once: items = [(y, S, x), ...] DeviceArrays."""
+        n = len(items)
+        if n == 0:
+            return
+        F, C, H, W = items[0][0].shape
+        arr = lambda k: (ctypes.c_void_p * n)(*[it[k].ptr for it in items])
+        _check(lib().ocl_simple_mri_recon_batch(self.handle, n, arr(0), arr(1), arr(2), F, C, H, W))
 
     def rsos(self, k, out):
         F, C = k.shape[:2]

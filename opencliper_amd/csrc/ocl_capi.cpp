@@ -482,6 +482,46 @@ extern "C" int ocl_simple_mri_recon(ocl_ctx* c, void* y, const void* S, void* x,
     return OCL_OK;
 }
 
+// Many SimpleMRIRecon launches of one shape at once (no reference counterpart; each entry is what
+// ocl_simple_mri_recon computes): the inverse row DFTs of every input in one launch, then the fused column DFT +
+// conj(S) + coil sum of every frame of every input in another, OCL_RECON_BATCH inputs per launch pair.  Four-step
+// sizes (H, W = 16 P) only; other sizes run ocl_simple_mri_recon per input.
+extern "C" int ocl_simple_mri_recon_batch(ocl_ctx* c, int n, void* const* y, const void* const* S, void* const* x,
+                                          int F, int C, int H, int W) {
+    CTXCHK(c);
+    if (n < 0 || (n > 0 && (!y || !S || !x)) || F < 1 || C < 1 || H < 1 || W < 1)
+        return fail(OCL_E_INVALID, "SimpleMRIRecon batch: bad arguments");
+    for (int i = 0; i < n; ++i)
+        if (!y[i] || !S[i] || !x[i]) return fail(OCL_E_INVALID, "SimpleMRIRecon batch: null input or output");
+    const int PH = fast_p(H), PW = fast_p(W);
+    if (!PH || !PW) {
+        for (int i = 0; i < n; ++i) CHK(ocl_simple_mri_recon(c, y[i], S[i], x[i], F, C, H, W));
+        return OCL_OK;
+    }
+    FftPlan pw, ph;
+    CHK(make_plan(c, W, &pw));
+    CHK(make_plan(c, H, &ph));
+    const long nrows = (long)F * C * H;
+    const size_t ldsr = sizeof(float2) * ((size_t)16 * (PW * 17 + 1) + W);
+    const size_t ldsc = sizeof(float2) * ((size_t)PH * FAST_COLS_KS + H);
+    for (int i0 = 0; i0 < n; i0 += OCL_RECON_BATCH) {
+        const int m = std::min(OCL_RECON_BATCH, n - i0);
+        ReconPtrs p{};
+        for (int j = 0; j < m; ++j) {
+            p.y[j] = (float2*)y[i0 + j];
+            p.S[j] = (const float2*)S[i0 + j];
+            p.x[j] = (float2*)x[i0 + j];
+        }
+        hipLaunchKernelGGL(k_fft_rows_fast_batch, dim3((unsigned)((nrows + 15) / 16), (unsigned)m), dim3(OCL_NT),
+                           ldsr, c->stream, p, nrows, PW, pw.tw, 1, 1.0f);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_simple_cols_fast_batch, dim3((unsigned)((W + 15) / 16), (unsigned)(m * F)), dim3(OCL_NT),
+                           ldsc, c->stream, p, F, C, W, PH, ph.tw, (float)(1.0 / ((double)H * (double)W)));
+        HIPCHK(hipGetLastError());
+    }
+    return OCL_OK;
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // Motion compensation (MotionCompensation.cpp, AdjointMotionCompensation.cpp; kernels in ocl_mc.hip)
 // ---------------------------------------------------------------------------------------------------------------------

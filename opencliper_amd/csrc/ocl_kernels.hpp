@@ -147,6 +147,15 @@ __global__ void k_fft_cols_fast(const float2* in, float2* out, int W, int P, con
                                 float scale);
 __global__ void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P, const float2* tw,
                                    float scale);
+// a batch of SimpleMRIRecon inputs of one shape, passed by value in the kernel argument
+#define OCL_RECON_BATCH 64
+struct ReconPtrs {
+    float2* y[OCL_RECON_BATCH];
+    const float2* S[OCL_RECON_BATCH];
+    float2* x[OCL_RECON_BATCH];
+};
+__global__ void k_simple_cols_fast_batch(ReconPtrs p, int F, int C, int W, int P, const float2* tw, float scale);
+__global__ void k_fft_rows_fast_batch(ReconPtrs p, long nrows, int P, const float2* tw, int conj_io, float scale);
 
 // kernels (ocl_mc.hip): motion compensation over the box rows r0..r0+nr-1, columns c0..c0+nc-1 of every frame
 __global__ void k_mc_interp(const float2* in, float2* out, const float* xn, const int* x, int F, int H, int W, int r0,

@@ -343,13 +343,12 @@ __device__ __forceinline__ void fft_cols_fast(const float2* __restrict__ in, flo
 template <int P>
 __device__ __forceinline__ void simple_cols_fast(float2* __restrict__ y, const float2* __restrict__ S,
                                                  float2* __restrict__ x, int C, int W, const float2* __restrict__ twg,
-                                                 float scale) {
+                                                 float scale, int f) {
     constexpr int H = 16 * P, KS = FAST_COLS_KS;
     constexpr int NQ = P > 16 ? 2 : 1;
     extern __shared__ float2 lds[];
     float2* tw = lds + P * KS;
     const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
-    const int f = blockIdx.y;
     const int w = blockIdx.x * 16 + s;
     const bool wv = w < W;
     const int wc = wv ? w : 0;
@@ -402,19 +401,49 @@ __device__ __forceinline__ void simple_cols_fast(float2* __restrict__ y, const f
     }
 }
 
+__device__ __forceinline__ void simple_cols_fast_p(float2* y, const float2* S, float2* x, int C, int W, int P,
+                                                   const float2* tw, float scale, int f) {
+    switch (P) {
+        case 2: simple_cols_fast<2>(y, S, x, C, W, tw, scale, f); break;
+        case 4: simple_cols_fast<4>(y, S, x, C, W, tw, scale, f); break;
+        case 6: simple_cols_fast<6>(y, S, x, C, W, tw, scale, f); break;
+        case 8: simple_cols_fast<8>(y, S, x, C, W, tw, scale, f); break;
+        case 10: simple_cols_fast<10>(y, S, x, C, W, tw, scale, f); break;
+        case 12: simple_cols_fast<12>(y, S, x, C, W, tw, scale, f); break;
+        case 14: simple_cols_fast<14>(y, S, x, C, W, tw, scale, f); break;
+        case 16: simple_cols_fast<16>(y, S, x, C, W, tw, scale, f); break;
+        case 18: simple_cols_fast<18>(y, S, x, C, W, tw, scale, f); break;
+        default: simple_cols_fast<20>(y, S, x, C, W, tw, scale, f); break;
+    }
+}
+
 __global__ __launch_bounds__(OCL_NT) void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P,
                                                              const float2* tw, float scale) {
+    simple_cols_fast_p(y, S, x, C, W, P, tw, scale, blockIdx.y);
+}
+
+// A batch of reconstructions of one shape: blockIdx.y = recon * F + frame (columns), recon (rows, blockIdx.x the
+// 16-row group within it).  The pointers travel in the kernel argument.
+__global__ __launch_bounds__(OCL_NT) void k_simple_cols_fast_batch(ReconPtrs p, int F, int C, int W, int P,
+                                                                   const float2* tw, float scale) {
+    const int r = blockIdx.y / F, f = blockIdx.y - r * F;
+    simple_cols_fast_p(p.y[r], p.S[r], p.x[r], C, W, P, tw, scale, f);
+}
+
+__global__ __launch_bounds__(OCL_NT) void k_fft_rows_fast_batch(ReconPtrs p, long nrows, int P, const float2* tw,
+                                                                int conj_io, float scale) {
+    float2* y = p.y[blockIdx.y];
     switch (P) {
-        case 2: simple_cols_fast<2>(y, S, x, C, W, tw, scale); break;
-        case 4: simple_cols_fast<4>(y, S, x, C, W, tw, scale); break;
-        case 6: simple_cols_fast<6>(y, S, x, C, W, tw, scale); break;
-        case 8: simple_cols_fast<8>(y, S, x, C, W, tw, scale); break;
-        case 10: simple_cols_fast<10>(y, S, x, C, W, tw, scale); break;
-        case 12: simple_cols_fast<12>(y, S, x, C, W, tw, scale); break;
-        case 14: simple_cols_fast<14>(y, S, x, C, W, tw, scale); break;
-        case 16: simple_cols_fast<16>(y, S, x, C, W, tw, scale); break;
-        case 18: simple_cols_fast<18>(y, S, x, C, W, tw, scale); break;
-        default: simple_cols_fast<20>(y, S, x, C, W, tw, scale); break;
+        case 2: fft_rows_fast<2>(y, y, nrows, tw, conj_io, scale); break;
+        case 4: fft_rows_fast<4>(y, y, nrows, tw, conj_io, scale); break;
+        case 6: fft_rows_fast<6>(y, y, nrows, tw, conj_io, scale); break;
+        case 8: fft_rows_fast<8>(y, y, nrows, tw, conj_io, scale); break;
+        case 10: fft_rows_fast<10>(y, y, nrows, tw, conj_io, scale); break;
+        case 12: fft_rows_fast<12>(y, y, nrows, tw, conj_io, scale); break;
+        case 14: fft_rows_fast<14>(y, y, nrows, tw, conj_io, scale); break;
+        case 16: fft_rows_fast<16>(y, y, nrows, tw, conj_io, scale); break;
+        case 18: fft_rows_fast<18>(y, y, nrows, tw, conj_io, scale); break;
+        default: fft_rows_fast<20>(y, y, nrows, tw, conj_io, scale); break;
     }
 }
 

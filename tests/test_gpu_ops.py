@@ -184,6 +184,25 @@ def test_simple_mri_recon(ctx, oracle, H, W, C, F):
     assert oracle.rel_l2(y.get(), post) < 1e-6
 
 
+@pytest.mark.parametrize("H,W,C,F,n", [(256, 256, 8, 1, 70), (128, 160, 3, 2, 5), (96, 80, 2, 1, 3)])
+def test_simple_mri_recon_batch(ctx, oracle, H, W, C, F, n):
+    """The batched entry point: every input's result and in-place state as ocl_simple_mri_recon gives them; 70
+    inputs span two launch pairs of 64; 96 x 80 (W = 16 P with P odd) takes the per-input fallback."""
+    sls = [synth.make_fully_sampled(H=H, W=W, C=C, F=F, slice_id=300 + i) for i in range(n)]
+    print("simple recon batch", H, W, C, F, n, flush=True)
+    items = [(ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.empty((F, H, W))) for sl in sls]
+    ctx.simple_mri_recon_batch(items)
+    for i in (0, n // 2, n - 1):
+        sl = sls[i]
+        y, S, x = items[i]
+        assert oracle.rel_l2(x.get(), oracle.simple_recon(sl.y, sl.S, double=True)) < 1e-6
+        post = np.fft.ifft2(sl.y.astype(np.complex128), axes=(-2, -1)) * np.conj(sl.S.astype(np.complex128))[None]
+        assert oracle.rel_l2(y.get(), post) < 1e-6
+        y1, S1, x1 = ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.empty((F, H, W))
+        ctx.simple_mri_recon(y1, S1, x1)
+        assert oracle.rel_l2(x.get(), x1.get()) < 1e-6
+
+
 @pytest.mark.parametrize("H,W,C,F", [(256, 256, 8, 1), (96, 160, 5, 3)])
 def test_rsos_and_simple_recon_sos(ctx, oracle, H, W, C, F):
     sl = synth.make_fully_sampled(H=H, W=W, C=C, F=F, slice_id=23)
