@@ -41,6 +41,27 @@ __device__ __forceinline__ uint8_t ldg1(const uint8_t* base, unsigned i) {
     return *(__attribute__((address_space(1))) const uint8_t*)((ocl_gcchar*)base + i);
 }
 
+// Buffer-resource access: a raw buffer (stride 0, range in bytes, gfx9 data-format word) addressed by a 32-bit lane
+// offset in a vector register and a 32-bit offset in a scalar register, both in bytes.  Inside loops this keeps the
+// per-row offsets scalar: pointer-based forms let the compiler hoist one 64-bit per-lane address per row out of the
+// loop.  Ranges stay below 4 GiB.
+typedef __amdgpu_buffer_rsrc_t BufRes;
+__device__ __forceinline__ BufRes buf_res(const void* p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bld2(BufRes r, unsigned vo, unsigned so) {
+    // the 64-bit load's value is taken as a whole (bit cast), never by element
+    const ocl_f2v v = __builtin_bit_cast(ocl_f2v, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void bst2(BufRes r, unsigned vo, unsigned so, float2 x) {
+    ocl_f2v v;
+    v.x = x.x;
+    v.y = x.y;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                          r, vo, so, 0);
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // complex helpers
 // ------------------------------------------------------------------------------------------------------------------

@@ -487,25 +487,6 @@ NestaKernel pick_CA_mfma(int MT, int KJ) {
 // Global accesses go through buffer resources: a 32-bit lane offset (in bytes) and a scalar row offset, so no per-lane
 // 64-bit addresses are formed or kept.
 // ---------------------------------------------------------------------------------------------------------------------
-typedef __amdgpu_buffer_rsrc_t BufRes;
-__device__ __forceinline__ BufRes buf_res(const void* p, long bytes) {
-    // raw buffer: stride 0, range in bytes, gfx9 data-format word
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(unsigned)bytes, 0x00020000);
-}
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 bld2(BufRes r, unsigned vo, unsigned so) {
-    // the 64-bit load's value is taken as a whole (bit cast), never by element
-    const f32x2 v = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
-    return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ void bst2(BufRes r, unsigned vo, unsigned so, float2 x) {
-    f32x2 v;
-    v.x = x.x;
-    v.y = x.y;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
-                                          r, vo, so, 0);
-}
-
 // NESTA update (NestaUp.cpp:696-875) of the wave's pixels from x_k and A^H r (acc, in registers); accumulates |u_f|^2
 // and Re(conj(u_f) Ux_f) of x_k.  SST: first iteration of a stage (x_ref := x_k, W_k := 0).  so0: byte offset of row h0
 // of frame f; vo: the lane's byte offset.
