@@ -524,6 +524,13 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
     // four-step slices of a shared mixed launch all take it, whatever their shape was tuned to alone.  (Few, small
     // groups would serialize launches that each under-fill the GPU, as in a server with a handful of requests.)
     // The Gram form's pass G has no row transform, so its slices are grouped by column size alone.
+    // Within a launch, slices whose column-pass workgroups run longest (coils x column length) come first.  The
+    // dispatcher hands out workgroups in index order, so the long ones then overlap the short ones instead of ending
+    // the launch alone; with coil counts from 4 to 32 in one table that tail was a tenth of some launches.
+    auto wg_cost = [&](int i) { return (long)(hd[i].C + 2) * hd[i].H; };
+    auto longest_first = [&](std::vector<int>& s) {
+        std::stable_sort(s.begin(), s.end(), [&](int a, int b) { return wg_cost(a) > wg_cost(b); });
+    };
     std::vector<std::vector<int>> sets;
     std::vector<char> set_gram;
     for (int form = 0; form < 2; ++form) {
@@ -557,11 +564,25 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             }
         if ((!fsets.empty() || mixed_forms) && !rest.empty()) fsets.push_back(rest);
         for (auto& fs : fsets) {
+            longest_first(fs);
             sets.push_back(fs);
             set_gram.push_back((char)form);
         }
     }
-    const bool grouped = sets.size() > 1;
+    // a table of one set runs as one group of its own when the longest-first order differs from the table's (the
+    // table's descriptor order is the slices' order, which setup and finalize address by slice)
+    if (sets.empty() && ns > 1) {
+        std::vector<int> all(ns);
+        for (int i = 0; i < ns; ++i) all[i] = i;
+        longest_first(all);
+        bool moved = false;
+        for (int i = 0; i < ns; ++i) moved |= all[i] != i;
+        if (moved) {
+            sets.push_back(all);
+            set_gram.push_back((char)(n_g == ns));
+        }
+    }
+    const bool grouped = !sets.empty();
     L.kj = 1;
     L.stage = ns ? hd[0].ca_stage : 0;
     L.mc = ns && v[0]->d.mc_xn ? 1 : 0;

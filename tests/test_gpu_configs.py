@@ -7,7 +7,8 @@ the schedules of tests/MRIRecon.cpp:105-114):
   split precision  25 frames x 16 coils x 192^2, acceleration 8: BF16-MFMA DFT form against FP64
   uneven batch     the 64-slice table of 128..320 matrices, 4..32 coils, 8..30 frames: every slice equal to its solo
                    run, three of them against FP64
-  launch groups    a mixed table whose 256^2 slices fill enough column-pass workgroups for a launch group of their own
+  launch groups    a mixed table whose 256^2 slices fill enough column-pass workgroups for a launch group of their own;
+                   a one-size table with different coil counts, run longest first
   served trace     100 requests over the cine and uneven sizes, Poisson arrivals, an HBM budget that makes requests
                    wait: every served result equal to its solo run
 """
@@ -228,6 +229,29 @@ def test_mixed_table_with_a_launch_group(ctx, oracle):
         assert rel(out[i], x64) < 1e-4
         assert reps[i]["iters"] == r64["iters"]
     b.close()
+
+
+def test_one_size_longest_first(ctx):
+    """One size with 3, 9 and 6 coils: the table runs as one launch group whose workgroups go longest first (9, 6,
+    3 coils), with the stop test on; every slice's iteration counts and result equal its solo run."""
+    sizes = [(128, 128, 3, 8), (128, 128, 9, 6), (128, 128, 6, 10)]
+    print("one-size table: generating 3 slices", flush=True)
+    dev = _device_slices(ctx, sizes, 2500)
+    prm = capi.nesta_params(maxIntIter=3, maxIter=12, stoptest=1, miniter=5)
+    b = ctx.batch(dev, prm)
+    b.setup()
+    info = b.info()
+    print("one-size table", info, flush=True)
+    assert info["launch_groups"] == 1
+    b.run()
+    out = [d[3].get() for d in dev]
+    reps = [b.report(i) for i in range(len(dev))]
+    b.close()
+    for i, d in enumerate(dev):
+        xs, rs = _solo(ctx, d, prm)
+        print(i, sizes[i], reps[i]["iters"], rs["iters"], flush=True)
+        assert reps[i]["iters"] == rs["iters"]
+        assert np.array_equal(out[i], xs)
 
 
 def test_served_trace_equals_solo(ctx):
