@@ -168,14 +168,23 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
         if (c + 1 < C) stage_load(c + 1);
         if (mode == 0) {
             // ---- forward column DFT of S_c x_k (FA phase 1 in registers, twiddle, exchange) ----
+            // all P loads of S_c are issued before the first product: left to itself the scheduler may interleave
+            // load, wait and multiply one value at a time, which serializes P memory latencies
             float2 v[P];
 #pragma unroll
-            for (int n1 = 0; n1 < P; ++n1) v[n1] = c_mul(xk[n1], ldg2(Sc + (long)16 * n1 * W, lanec));
+            for (int n1 = 0; n1 < P; ++n1) v[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int n1 = 0; n1 < P; ++n1) v[n1] = c_mul(xk[n1], v[n1]);
             rf::rdft<P>(v);
             const float2* tr = twa + P * t;                      // tr[0] = 1
+            float2 tw[P];                                        // twiddles read together, as S_c above
+#pragma unroll
+            for (int k1 = 1; k1 < P; ++k1) tw[k1] = tr[k1];
+            __builtin_amdgcn_sched_barrier(0);
             X[t * 16] = v[0];
 #pragma unroll
-            for (int k1 = 1; k1 < P; ++k1) X[k1 * FAST_CA_KS + t * 16] = c_mul(v[k1], tr[k1]);
+            for (int k1 = 1; k1 < P; ++k1) X[k1 * FAST_CA_KS + t * 16] = c_mul(v[k1], tw[k1]);
             __syncthreads();
         }
         // ---- per role k1: (forward FA phase 2,) residual at the carried rows, inverse FB phase 1 ----
