@@ -47,9 +47,10 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
         return fail(OCL_E_HIP, m);
     }
     const void* lds_kernels[] = {(const void*)k_fft_rows, (const void*)k_fft_cols, (const void*)k_fft_rows_fast,
-                                 (const void*)k_fft_cols_fast, (const void*)k_simple_cols_fast};
+                                 (const void*)k_fft_cols_fast};
     if (e == hipSuccess) {
         for (const void* k : lds_kernels) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX);
+        simple_cols_attributes((int)LDS_MAX);
         nesta_kernel_attributes((int)LDS_MAX_NESTA);
         mfma_kernel_attributes((int)LDS_MAX_NESTA);
     }
@@ -468,10 +469,8 @@ extern "C" int ocl_simple_mri_recon(ocl_ctx* c, void* y, const void* S, void* x,
                            (const float2*)y, (float2*)y, nrows, PW, pw.tw, 1, 1.0f);
         HIPCHK(hipGetLastError());
         const size_t ldsc = sizeof(float2) * ((size_t)PH * FAST_COLS_KS + H);
-        hipLaunchKernelGGL(k_simple_cols_fast, dim3((unsigned)((W + 15) / 16), (unsigned)F), dim3(OCL_NT), ldsc,
-                           c->stream, (float2*)y, (const float2*)S, (float2*)x, C, W, PH, ph.tw,
-                           (float)(1.0 / ((double)H * (double)W)));
-        HIPCHK(hipGetLastError());
+        HIPCHK(launch_simple_cols((float2*)y, (const float2*)S, (float2*)x, F, C, W, PH, ph.tw,
+                                  (float)(1.0 / ((double)H * (double)W)), ldsc, c->stream));
         return OCL_OK;
     }
     CHK(fft2d(c, y, y, (long)F * C, H, W, +1));
@@ -515,9 +514,8 @@ extern "C" int ocl_simple_mri_recon_batch(ocl_ctx* c, int n, void* const* y, con
         hipLaunchKernelGGL(k_fft_rows_fast_batch, dim3((unsigned)((nrows + 15) / 16), (unsigned)m), dim3(OCL_NT),
                            ldsr, c->stream, p, nrows, PW, pw.tw, 1, 1.0f);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_simple_cols_fast_batch, dim3((unsigned)((W + 15) / 16), (unsigned)(m * F)), dim3(OCL_NT),
-                           ldsc, c->stream, p, F, C, W, PH, ph.tw, (float)(1.0 / ((double)H * (double)W)));
-        HIPCHK(hipGetLastError());
+        HIPCHK(launch_simple_cols_batch(p, m, F, C, W, PH, ph.tw, (float)(1.0 / ((double)H * (double)W)), ldsc,
+                                        c->stream));
     }
     return OCL_OK;
 }

@@ -145,8 +145,6 @@ __global__ void k_fft_rows_fast(const float2* in, float2* out, long nrows, int P
                                 float scale);
 __global__ void k_fft_cols_fast(const float2* in, float2* out, int W, int P, const float2* tw, int conj_io,
                                 float scale);
-__global__ void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P, const float2* tw,
-                                   float scale);
 // a batch of SimpleMRIRecon inputs of one shape, passed by value in the kernel argument
 #define OCL_RECON_BATCH 64
 struct ReconPtrs {
@@ -154,7 +152,13 @@ struct ReconPtrs {
     const float2* S[OCL_RECON_BATCH];
     float2* x[OCL_RECON_BATCH];
 };
-__global__ void k_simple_cols_fast_batch(ReconPtrs p, int F, int C, int W, int P, const float2* tw, float scale);
+// SimpleMRIRecon's fused column DFT + conj(S) + coil sum, one kernel per column size H = 16 P (ocl_ops.hip): F frames
+// of one input, or m inputs of F frames each
+hipError_t launch_simple_cols(float2* y, const float2* S, float2* x, int F, int C, int W, int P, const float2* tw,
+                              float scale, size_t lds, hipStream_t stream);
+hipError_t launch_simple_cols_batch(const ReconPtrs& p, int m, int F, int C, int W, int P, const float2* tw,
+                                    float scale, size_t lds, hipStream_t stream);
+void simple_cols_attributes(int lds_max);
 __global__ void k_fft_rows_fast_batch(ReconPtrs p, long nrows, int P, const float2* tw, int conj_io, float scale);
 
 // kernels (ocl_mc.hip): motion compensation over the box rows r0..r0+nr-1, columns c0..c0+nc-1 of every frame

@@ -353,6 +353,10 @@ __device__ __forceinline__ void simple_cols_fast(float2* __restrict__ y, const f
     const bool wv = w < W;
     const int wc = wv ? w : 0;
     const long N = (long)H * W;
+    // every access of a coil image goes through a buffer resource over that image (N x 8 bytes < 4 GiB): one 32-bit
+    // lane offset (row t, column wc) and scalar row offsets, instead of 64-bit per-lane addresses kept per row
+    const unsigned vo = (unsigned)(t * W + wc) * 8u;
+    const unsigned rowB = (unsigned)W * 8u;
     for (int i = threadIdx.x; i < H; i += blockDim.x) tw[i] = twg[i];
     float2 acc[NQ][16];
 #pragma unroll
@@ -360,74 +364,107 @@ __device__ __forceinline__ void simple_cols_fast(float2* __restrict__ y, const f
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) acc[q][k2] = make_float2(0.f, 0.f);
     for (int c = 0; c < C; ++c) {
-        float2* I = y + ((long)f * C + c) * N;
-        const float2* Sc = S + (long)c * N;
+        const BufRes rI = buf_res(y + ((long)f * C + c) * N, N * 8);
+        const BufRes rS = buf_res(S + (long)c * N, N * 8);
         float2 v[P];
 #pragma unroll
         for (int b = 0; b < P; ++b) {
-            const float2 a = ldg2(I + (long)16 * b * W, (unsigned)(t * W + wc));
+            const float2 a = bld2(rI, vo, 16u * b * rowB);
             v[b] = wv ? make_float2(a.x, -a.y) : make_float2(0.f, 0.f);    // IFFT = conj(FFT(conj(.)))
         }
         __syncthreads();   // twiddles ready / the previous coil's exchange fully read
         rf::rdft<P>(v);
+        // reads before writes: the twiddles share the LDS array with the exchange, so a read placed after a write
+        // would wait for it, one value at a time
+        float2 twr[P];
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+        for (int k1 = 1; k1 < P; ++k1) twr[k1] = tw[t * k1];
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], twr[k1]);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int k1 = t + 16 * q;
             if (k1 < P) {
-                float2 u[16];
+                // row k1 + P k2 = t + (16 q + P k2): the lane offset vo and a scalar row offset
+                float2 u[16], sv[16];
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) sv[k2] = bld2(rS, vo, (unsigned)(16 * q + P * k2) * rowB);
 #pragma unroll
                 for (int i = 0; i < 16; ++i) u[i] = lds[k1 * KS + i * 16 + s];
                 rf::rdft<16>(u);
 #pragma unroll
                 for (int k2 = 0; k2 < 16; ++k2) {
-                    const unsigned off = (unsigned)((k1 + P * k2) * W + wc);
                     const float2 z = make_float2(u[k2].x * scale, -u[k2].y * scale);
-                    const float2 pz = c_mulconj(z, ldg2(Sc, off));
+                    const float2 pz = c_mulconj(z, sv[k2]);
                     acc[q][k2] = c_add(acc[q][k2], pz);
-                    if (wv) stg2(I, off, pz);
+                    if (wv) bst2(rI, vo, (unsigned)(16 * q + P * k2) * rowB, pz);
                 }
             }
         }
     }
+    const BufRes rX = buf_res(x + (long)f * N, N * 8);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int k1 = t + 16 * q;
         if (k1 < P && wv)
 #pragma unroll
-            for (int k2 = 0; k2 < 16; ++k2) stg2(x + (long)f * N, (unsigned)((k1 + P * k2) * W + w), acc[q][k2]);
+            for (int k2 = 0; k2 < 16; ++k2) bst2(rX, vo, (unsigned)(16 * q + P * k2) * rowB, acc[q][k2]);
     }
 }
 
-__device__ __forceinline__ void simple_cols_fast_p(float2* y, const float2* S, float2* x, int C, int W, int P,
-                                                   const float2* tw, float scale, int f) {
-    switch (P) {
-        case 2: simple_cols_fast<2>(y, S, x, C, W, tw, scale, f); break;
-        case 4: simple_cols_fast<4>(y, S, x, C, W, tw, scale, f); break;
-        case 6: simple_cols_fast<6>(y, S, x, C, W, tw, scale, f); break;
-        case 8: simple_cols_fast<8>(y, S, x, C, W, tw, scale, f); break;
-        case 10: simple_cols_fast<10>(y, S, x, C, W, tw, scale, f); break;
-        case 12: simple_cols_fast<12>(y, S, x, C, W, tw, scale, f); break;
-        case 14: simple_cols_fast<14>(y, S, x, C, W, tw, scale, f); break;
-        case 16: simple_cols_fast<16>(y, S, x, C, W, tw, scale, f); break;
-        case 18: simple_cols_fast<18>(y, S, x, C, W, tw, scale, f); break;
-        default: simple_cols_fast<20>(y, S, x, C, W, tw, scale, f); break;
-    }
-}
-
-__global__ __launch_bounds__(OCL_NT) void k_simple_cols_fast(float2* y, const float2* S, float2* x, int C, int W, int P,
-                                                             const float2* tw, float scale) {
-    simple_cols_fast_p(y, S, x, C, W, P, tw, scale, blockIdx.y);
+// One kernel per column size: a kernel holding every P is allocated the registers of its largest case (P = 20,
+// two 16-value accumulators), which held P = 16 at one wavefront per SIMD.
+template <int P>
+__global__ __launch_bounds__(OCL_NT) void k_simple_cols_t(float2* y, const float2* S, float2* x, int C, int W,
+                                                          const float2* tw, float scale) {
+    simple_cols_fast<P>(y, S, x, C, W, tw, scale, blockIdx.y);
 }
 
 // A batch of reconstructions of one shape: blockIdx.y = recon * F + frame (columns), recon (rows, blockIdx.x the
 // 16-row group within it).  The pointers travel in the kernel argument.
-__global__ __launch_bounds__(OCL_NT) void k_simple_cols_fast_batch(ReconPtrs p, int F, int C, int W, int P,
-                                                                   const float2* tw, float scale) {
+template <int P>
+__global__ __launch_bounds__(OCL_NT) void k_simple_cols_batch_t(ReconPtrs p, int F, int C, int W, const float2* tw,
+                                                                float scale) {
     const int r = blockIdx.y / F, f = blockIdx.y - r * F;
-    simple_cols_fast_p(p.y[r], p.S[r], p.x[r], C, W, P, tw, scale, f);
+    simple_cols_fast<P>(p.y[r], p.S[r], p.x[r], C, W, tw, scale, f);
+}
+
+#define OCL_SIMPLE_COLS_CASES(X) X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20)
+
+hipError_t launch_simple_cols(float2* y, const float2* S, float2* x, int F, int C, int W, int P, const float2* tw,
+                              float scale, size_t lds, hipStream_t stream) {
+    const dim3 grid((unsigned)((W + 15) / 16), (unsigned)F);
+    switch (P) {
+#define OCL_SC_CASE(Q) \
+        case Q: hipLaunchKernelGGL(k_simple_cols_t<Q>, grid, dim3(OCL_NT), lds, stream, y, S, x, C, W, tw, scale); break;
+        OCL_SIMPLE_COLS_CASES(OCL_SC_CASE)
+#undef OCL_SC_CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_simple_cols_batch(const ReconPtrs& p, int m, int F, int C, int W, int P, const float2* tw,
+                                    float scale, size_t lds, hipStream_t stream) {
+    const dim3 grid((unsigned)((W + 15) / 16), (unsigned)(m * F));
+    switch (P) {
+#define OCL_SC_CASE(Q) \
+        case Q: hipLaunchKernelGGL(k_simple_cols_batch_t<Q>, grid, dim3(OCL_NT), lds, stream, p, F, C, W, tw, scale); \
+            break;
+        OCL_SIMPLE_COLS_CASES(OCL_SC_CASE)
+#undef OCL_SC_CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+void simple_cols_attributes(int lds_max) {
+#define OCL_SC_CASE(Q)                                                                                              \
+    (void)hipFuncSetAttribute((const void*)k_simple_cols_t<Q>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max); \
+    (void)hipFuncSetAttribute((const void*)k_simple_cols_batch_t<Q>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    OCL_SIMPLE_COLS_CASES(OCL_SC_CASE)
+#undef OCL_SC_CASE
 }
 
 __global__ __launch_bounds__(OCL_NT) void k_fft_rows_fast_batch(ReconPtrs p, long nrows, int P, const float2* tw,
