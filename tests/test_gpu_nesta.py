@@ -198,11 +198,12 @@ def test_pixel_mask_split_precision(ctx, oracle):
     assert err < 1e-3
 
 
-@pytest.mark.parametrize("H,W,stoptest", [(128, 128, 0), (160, 96, 1), (288, 64, 0), (320, 80, 1)])
+@pytest.mark.parametrize("H,W,stoptest", [(128, 128, 0), (160, 96, 1), (288, 64, 0), (320, 80, 1), (288, 72, 1)])
 def test_gram_form_matches_row_and_column_passes(oracle, H, W, stoptest):
     """With a line mask the fused form runs the Gram iteration (one column pass, no row transform per iteration); the
     same slice with the column and row passes pinned (ca_fast 2) must agree with it and with FP64, and, with the stop
-    test on, take the oracle's iteration counts (a stage ended by the stop test recomputes its last iteration)."""
+    test on, take the oracle's iteration counts (a stage ended by the stop test recomputes its last iteration).
+    H = 288 and 320 give the column pass more DFT jobs than roles (two rounds); 288 x 72 also has a partial column tile."""
     sl = synth.make_cine(H=H, W=W, C=3, F=7, accel=6.0, n_centre=8, slice_id=120 + H)
     prm = dict(maxIntIter=3, maxIter=40 if stoptest else 12, stoptest=stoptest, miniter=4)
     x64, r64 = oracle.nesta(sl.y, sl.S, sl.M, oracle.params(**prm), double=True)
