@@ -321,6 +321,8 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 // One workgroup per control point (P, Q): dH[d][f] = sum over the support of w beta(i) beta(j) (gradientJoint,
 // auxiliarMask, auxiliarCost, cost6DReduction, cost6D: optimizer.cl:442-486, 938-1025, reductions.cl:113-142), then
 // updateTransformation (optimizer.cl:519-535): T -= Wn (dH - mean over frames).
+// KT: support steps per lane, K = ceil(dp^2 / 64) rounded up to 4, 8 or 16 (steps k >= K carry weight 0)
+template <int KT>
 __global__ __launch_bounds__(OCL_NT) void k_gwr_grad_update(GwrDev g) {
     __shared__ float sdH[2 * GWR_FMAX];
     __shared__ double red[16];
@@ -330,12 +332,11 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_grad_update(GwrDev g) {
     const int P = blockIdx.x / Cp, Q = blockIdx.x - P * Cp;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = g.gs[P] - g.box0, c0 = g.gs[Q] - g.box0;
-    constexpr int KMAX = GWR_SUP_MAX * GWR_SUP_MAX / 64;
-    const int K = (dp * dp + 63) / 64;
-    int off[KMAX];
-    float wt[KMAX];
+    const int K = (dp * dp + 63) / 64;            // <= KT (the launch picks KT)
+    int off[KT];
+    float wt[KT];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
+    for (int k = 0; k < KT; ++k) {
         off[k] = -1;
         wt[k] = 0.f;
         if (k >= K) continue;
@@ -354,14 +355,19 @@ __global__ __launch_bounds__(OCL_NT) void k_gwr_grad_update(GwrDev g) {
         float a[GB];
 #pragma unroll
         for (int b = 0; b < GB; ++b) a[b] = 0.f;
+        // branch-free loads (a branch per load puts each in a block of its own, waited on there): a support point
+        // outside the box reads position 0 with weight 0, which adds exactly 0; planes past 2F read the last plane
+        // and their sums are never stored.  Per plane the support terms are summed in k order, as before.
+        int o[KT];
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            if (k >= K || off[k] < 0) continue;
+        for (int k = 0; k < KT; ++k) o[k] = off[k] < 0 ? 0 : off[k];
 #pragma unroll
-            for (int b = 0; b < GB; ++b) {
-                const int pr = pr0 + NW * b;
-                if (pr < 2 * F) a[b] = fmaf(g.w[(long)pr * L2 + off[k]], wt[k], a[b]);
-            }
+        for (int b = 0; b < GB; ++b) {
+            const float* wp = g.w + (long)min(pr0 + NW * b, 2 * F - 1) * L2;
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) s = fmaf(wp[o[k]], wt[k], s);
+            a[b] = s;
         }
 #pragma unroll
         for (int b = 0; b < GB; ++b) {
@@ -657,7 +663,9 @@ extern "C" int ocl_gwr_run(ocl_gwr* g, const void* x, float* T, int T_in, float*
     hipLaunchKernelGGL(k_gwr_step, dim3(1), dim3(OCL_NT), 0, c->stream, d, 0, g->eval_blocks);
     HIPCHK(hipGetLastError());
     for (int it = 1; it <= g->nmax; ++it) {
-        hipLaunchKernelGGL(k_gwr_grad_update, dim3(g->Cp * g->Cp), dim3(OCL_NT), 0, c->stream, d);
+        const int K = (g->dp * g->dp + 63) / 64;     // dp <= GWR_SUP_MAX = 32: K <= 16
+        const auto grad = K <= 4 ? k_gwr_grad_update<4> : K <= 8 ? k_gwr_grad_update<8> : k_gwr_grad_update<16>;
+        hipLaunchKernelGGL(grad, dim3(g->Cp * g->Cp), dim3(OCL_NT), 0, c->stream, d);
         hipLaunchKernelGGL(eval, dim3(g->eval_blocks), dim3(OCL_NT), 0, c->stream, d, 1);
         hipLaunchKernelGGL(k_gwr_step, dim3(1), dim3(OCL_NT), 0, c->stream, d, 1, g->eval_blocks);
     }

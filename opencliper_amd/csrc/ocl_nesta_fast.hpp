@@ -82,11 +82,16 @@ __device__ __forceinline__ double nesta_B_fast(const SliceDesc& d, int local, in
     }
     float2 v[P];
     // ---- FA phase 1 on U (mode 0) or conj(y) (mode 1) ----
+    // (the branches stay outside the loops: a branch per value puts each load in a block of its own, waited on there)
+    if (valid && mode == 0) {
 #pragma unroll
-    for (int b = 0; b < P; ++b) {
-        float2 x = make_float2(0.f, 0.f);
-        if (valid) x = (mode == 0) ? ldg2(kb + 16 * b, kro + a) : c_conj(ldg2(yfc + 16 * b, yro + a));
-        v[b] = x;
+        for (int b = 0; b < P; ++b) v[b] = ldg2(kb + 16 * b, kro + a);
+    } else if (valid) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) v[b] = c_conj(ldg2(yfc + 16 * b, yro + a));
+    } else {
+#pragma unroll
+        for (int b = 0; b < P; ++b) v[b] = make_float2(0.f, 0.f);
     }
     __syncthreads();   // twiddles in LDS
     rf::rdft<P>(v);
