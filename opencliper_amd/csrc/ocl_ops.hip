@@ -260,18 +260,25 @@ __device__ __forceinline__ void fft_rows_fast(const float2* __restrict__ in, flo
     const long row = (long)blockIdx.x * 16 + r;
     const bool valid = row < nrows;
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    // loads from a clamped row, zeroed afterwards (a branch per value would put each load in a block of its own,
+    // waited on there); twiddles read before the exchange writes, which share the LDS array with them
+    const long rowc = valid ? row : 0;
     float2 v[P];
 #pragma unroll
     for (int b = 0; b < P; ++b) {
-        float2 x = valid ? in[row * n + a + 16 * b] : make_float2(0.f, 0.f);
+        float2 x = in[rowc * n + a + 16 * b];
+        if (!valid) x = make_float2(0.f, 0.f);
         if (conj_io) x.y = -x.y;
         v[b] = x;
     }
     __syncthreads();
     rf::rdft<P>(v);
     float2* X = lds + r * RS;
+    float2 twr[P];
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) X[k1 * 17 + a] = (k1 == 0 || a == 0) ? v[k1] : c_mul(v[k1], tw[a * k1]);
+    for (int k1 = 1; k1 < P; ++k1) twr[k1] = tw[a * k1];
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) X[k1 * 17 + a] = (k1 == 0 || a == 0) ? v[k1] : c_mul(v[k1], twr[k1]);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -305,17 +312,23 @@ __device__ __forceinline__ void fft_cols_fast(const float2* __restrict__ in, flo
     const float2* I = in + img * (long)H * W;
     float2* O = out + img * (long)H * W;
     for (int i = threadIdx.x; i < H; i += blockDim.x) tw[i] = twg[i];
+    // as in the row pass: loads from a clamped column, zeroed afterwards; twiddles read before the exchange writes
+    const int wc = wv ? w : 0;
     float2 v[P];
 #pragma unroll
     for (int b = 0; b < P; ++b) {
-        float2 x = wv ? I[(long)(t + 16 * b) * W + w] : make_float2(0.f, 0.f);
+        float2 x = I[(long)(t + 16 * b) * W + wc];
+        if (!wv) x = make_float2(0.f, 0.f);
         if (conj_io) x.y = -x.y;
         v[b] = x;
     }
     __syncthreads();
     rf::rdft<P>(v);
+    float2 twr[P];
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], tw[t * k1]);
+    for (int k1 = 1; k1 < P; ++k1) twr[k1] = tw[t * k1];
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) lds[k1 * KS + t * 16 + s] = (k1 == 0 || t == 0) ? v[k1] : c_mul(v[k1], twr[k1]);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
