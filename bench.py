@@ -14,10 +14,11 @@ Other legs in the same run (each reported under its own key of the JSON line):
                 launch table per iteration, captured in a hipGraph; k-space synthesised on the device; the plain
                 form timed over the same 64 slices beside it
   served        Poisson arrivals (50 req/s, 60 s) mixing the sizes above, 5 stages x 10..30 iterations per request;
-                each request's inputs are uploaded from host memory at its arrival (inside its latency) and it is
-                admitted under a binding HBM budget (1 GiB) into the continuous-batching server; p50/p99 latency and
-                recons/s, a 100-request trace whose every result is compared with its solo run, the capacity (96
-                requests at once) and a second trace at 80% of that capacity; the plain form timed on the same mix
+                each request's inputs are uploaded from page-locked host memory at its arrival (inside its latency)
+                and it is admitted under a binding HBM budget (1 GiB) into the continuous-batching server; p50/p99
+                latency and recons/s, a 100-request trace whose every result is compared with its solo run, the
+                capacity (96 requests at once) and a second trace at 80% of that capacity; the plain form timed on
+                the same mix
   motion_compensated  NESTA-TV with argsMC (U = tTV o MC, fused and plain forms) on the headline slice from the motion-free
                 result, with a smooth synthetic deformation (up to 1.5 px) over a centred bound box, plus the
                 MotionCompensation / AdjointMotionCompensation kernels alone against their HBM roofline
@@ -354,6 +355,7 @@ def _serve_trace(ctx, capi, host_pool, arrivals, pick, iters, budget, chunk, che
            "queue_wait_p99_ms": 1e3 * pct(qwait, 99), "slice_iterations": sum(STAGES * meta[k][1] for k in times),
            "budget_bytes": budget, "peak_used_bytes": stats["peak_used_bytes"],
            "budget_waits": stats["budget_waits"], "launches": stats["launches"], "busy_s": stats["busy_s"],
+           "buffer_cache_hits": stats["cache_hits"], "buffer_cache_misses": stats["cache_misses"],
            "failed": stats["failed"]}
     return res, meta
 
@@ -576,7 +578,10 @@ def gpu_legs(args, cine, parts, full, split):
                 for i, (x, S, M) in enumerate(parts):
                     y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
                     dev3.append((y, dS, dM, None))
-            host_pool = [(y.get(), S.get(), M.get()) for (y, S, M, _) in dev[:4] + dev3[:12]]
+            # request payloads wait in page-locked host memory, as a server's receive buffers would: each upload is
+            # one DMA transfer (from pageable memory the driver stages it, at about a fifth of the rate)
+            host_pool = [(ctx.pinned(y.get()), ctx.pinned(S.get()), ctx.pinned(M.get()))
+                         for (y, S, M, _) in dev[:4] + dev3[:12]]
             out["plain_served_mix"] = leg_plain_mix(ctx, capi, host_pool, 16)
             log(f"plain over the served mix: {out['plain_served_mix']['recons_per_s']:.2f} recons/s")
             budget = int(args.serve_budget * (1 << 30))

@@ -81,6 +81,11 @@ int ocl_memcpy_h2d(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /*
 int ocl_memcpy_d2h(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
 int ocl_memcpy_d2d(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /* stream-ordered */
 int ocl_memset_zero(ocl_ctx* ctx, void* dptr, size_t bytes);                  /* stream-ordered fill kernel */
+/* page-locked host memory for inputs uploaded again and again (a server's request payloads): replaces the reference's
+ * mapped host view of a device buffer (src/DeviceDataProperties.cpp:242, enqueueMapBuffer).  ocl_memcpy_h2d from it
+ * runs as one DMA transfer instead of through the driver's pageable staging.  Freed by ocl_host_free only. */
+int ocl_host_malloc(ocl_ctx* ctx, size_t bytes, void** hptr);
+int ocl_host_free(ocl_ctx* ctx, void* hptr);
 
 /* ---- individual Processes (stream-ordered; outputs may alias inputs where the reference allows it) ---- */
 /* TemporalTV::launch (src/processes/nesta/TemporalTV.cpp:49-86, kernels src/kernels/temporalTV.cl:49-107);
@@ -280,8 +285,10 @@ int ocl_autotune_shape(ocl_ctx* ctx, int H, int W, int* ca_fast, int* tile_cols,
 /* ---- serving: continuous batching of NESTA-TV requests (no counterpart in the reference, which runs one
  *      reconstruction per NestaUp::launch).  A worker thread with its own stream admits queued requests in arrival
  *      order while their device footprint fits the HBM budget, co-batches the iterations of every running request,
- *      and retires each one as its last continuation stage ends.  Inputs and outputs are caller-owned device memory
- *      of the same device and must stay valid until the request is done. ---- */
+ *      and retires each one as its last continuation stage ends.  A finished request's work buffers stay with the
+ *      server and are reused by later requests of the same sizes; buffers in use plus buffers kept never exceed the
+ *      budget.  Inputs and outputs are caller-owned device memory of the same device and must stay valid until the
+ *      request is done. ---- */
 typedef struct ocl_server ocl_server;
 #define OCL_SERVE_QUEUED 0
 #define OCL_SERVE_RUNNING 1
@@ -292,6 +299,7 @@ typedef struct {
     long slice_iterations, launches;
     double busy_s, elapsed_s;
     long budget_waits;   /* requests that reached the head of the queue and waited there for HBM budget */
+    long cache_hits, cache_misses;   /* request buffers reused from the server's cache / newly allocated */
 } ocl_server_stats;
 int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_iters, ocl_server** out);
 /* report may be NULL; when given it is filled when the request completes */

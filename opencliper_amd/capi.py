@@ -119,7 +119,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_long), ("failed", ctypes.c_long), ("queued", ctypes.c_long),
                 ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
                 ("budget_bytes", ctypes.c_size_t), ("slice_iterations", ctypes.c_long), ("launches", ctypes.c_long),
-                ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("budget_waits", ctypes.c_long)]
+                ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("budget_waits", ctypes.c_long),
+                ("cache_hits", ctypes.c_long), ("cache_misses", ctypes.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -152,6 +153,7 @@ def lib():
         "ocl_device_info": ([P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                              ctypes.POINTER(I)], I),
         "ocl_malloc": ([P, ctypes.c_size_t, PP], I), "ocl_free": ([P, P], I),
+        "ocl_host_malloc": ([P, ctypes.c_size_t, PP], I), "ocl_host_free": ([P, P], I),
         "ocl_memcpy_h2d": ([P, P, P, ctypes.c_size_t], I), "ocl_memcpy_d2h": ([P, P, P, ctypes.c_size_t], I),
         "ocl_memcpy_d2d": ([P, P, P, ctypes.c_size_t], I), "ocl_memset_zero": ([P, P, ctypes.c_size_t], I),
         "ocl_ttv": ([P, P, P, I, LG, I], I), "ocl_vecnorm": ([P, P, P, LG, F32], I),
@@ -265,11 +267,15 @@ class Context:
         _check(lib().ocl_ctx_create(device, ctypes.byref(h)))
         self.handle = h.value
         self._children = weakref.WeakSet()      # batches and registrations: destroyed before the context
+        self._pinned = []                       # page-locked host allocations of pinned()
 
     def close(self):
         if self.handle:
             for ch in list(self._children):
                 ch.close()
+            for p in self._pinned:
+                lib().ocl_host_free(self.handle, p)
+            self._pinned = []
             lib().ocl_ctx_destroy(self.handle)
             self.handle = None
 
@@ -320,6 +326,18 @@ class Context:
     # --- memory ---
     def empty(self, shape, dtype=np.complex64) -> DeviceArray:
         return DeviceArray(self, shape, dtype)
+
+    def pinned(self, a: np.ndarray) -> np.ndarray:
+        """A copy of `a` in page-locked host memory (ocl_host_malloc), as a numpy array.  Uploads from it are one
+        DMA transfer.  The memory is released by close(): the array must not be used after that."""
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        _check(lib().ocl_host_malloc(self.handle, max(a.nbytes, 1), ctypes.byref(p)))
+        self._pinned.append(p.value)
+        h = np.frombuffer((ctypes.c_char * max(a.nbytes, 1)).from_address(p.value), dtype=a.dtype, count=a.size)
+        h = h.reshape(a.shape)
+        h[...] = a
+        return h
 
     def to_device(self, a: np.ndarray) -> DeviceArray:
         a = np.ascontiguousarray(a)

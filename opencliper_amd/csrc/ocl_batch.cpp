@@ -86,6 +86,58 @@ struct SliceSetup {
     int ne_iters;
 };
 
+// Device buffers a server keeps between requests, so that admission and completion call neither hipMalloc nor
+// hipFree (hipFree waits for the whole device) while requests of the same shape come and go.  Buffers are reused by
+// exact size.  `held` counts the buffers handed out to live requests; held + bytes stays within `limit` (the server's
+// HBM budget): a miss that would pass it first frees cached buffers.  Used by the server's worker thread only; every
+// user is on the server's one stream, so a buffer given back may be handed out again at once (stream order).
+struct BufCache {
+    std::multimap<size_t, void*> free;
+    size_t bytes = 0, held = 0, limit = 0;
+    long hits = 0, misses = 0;
+    hipStream_t stream = nullptr;
+    hipError_t take(void** p, size_t n) {
+        auto it = free.find(n);
+        if (it != free.end()) {
+            *p = it->second;
+            free.erase(it);
+            bytes -= n;
+            held += n;
+            hits++;
+            return hipSuccess;
+        }
+        if (held + bytes + n > limit && !free.empty()) {
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            while (held + bytes + n > limit && !free.empty()) {
+                auto big = std::prev(free.end());
+                e = dev_free(big->second);
+                bytes -= big->first;
+                free.erase(big);
+                if (e != hipSuccess) return e;
+            }
+        }
+        hipError_t e = dev_malloc(p, n);
+        if (e == hipSuccess) {
+            held += n;
+            misses++;
+        }
+        return e;
+    }
+    void give(void* p, size_t n) {
+        free.emplace(n, p);
+        bytes += n;
+        held -= n;
+    }
+    void clear() {
+        if (free.empty()) return;
+        (void)hipStreamSynchronize(stream);
+        for (auto& kv : free) (void)dev_free(kv.second);
+        free.clear();
+        bytes = 0;
+    }
+};
+
 // Device resources of one slice.
 struct SliceRes {
     ocl_ctx* c = nullptr;
@@ -97,23 +149,30 @@ struct SliceRes {
     std::vector<int> nrows;
     std::vector<uint8_t> rowmask;     // [F][nmax] carried row is a mask row (Gram form)
     std::vector<void*> allocs;
+    std::vector<size_t> sizes;
     size_t bytes = 0;
     bool allocated = false;
     bool has_mc = false;      // motion compensation (argsMC) for this slice
     ocl_mc_args mc{};
+    BufCache* cache = nullptr;   // a server's: buffers come from it and go back to it
 
     ~SliceRes() {
-        if (!allocs.empty()) {
-            hipStreamSynchronize(c->stream);
-            for (void* p : allocs) dev_free(p);
+        if (allocs.empty()) return;
+        if (cache) {
+            for (size_t i = 0; i < allocs.size(); ++i) cache->give(allocs[i], sizes[i]);
+            return;
         }
+        hipStreamSynchronize(c->stream);
+        for (void* p : allocs) dev_free(p);
     }
     int alloc(size_t n, void** p) {
-        hipError_t e = dev_malloc(p, std::max<size_t>(n, 16));
+        n = std::max<size_t>(n, 16);
+        hipError_t e = cache ? cache->take(p, n) : dev_malloc(p, n);
         if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
         HIPCHK(e);
         allocs.push_back(*p);
-        bytes += std::max<size_t>(n, 16);
+        sizes.push_back(n);
+        bytes += n;
         return OCL_OK;
     }
 };
@@ -1356,6 +1415,8 @@ struct ocl_server {
     long next_id = 1;
     size_t used = 0, peak_used = 0;
     long completed = 0, failed = 0, iterations = 0, launches = 0, n_active = 0, budget_waits = 0;
+    long cache_hits = 0, cache_misses = 0;   // copies of cache.hits / misses, taken under mu
+    BufCache cache;                          // request buffers kept between requests (worker thread only)
     double busy_s = 0;
     std::chrono::steady_clock::time_point t0;
     double now() const {
@@ -1417,6 +1478,7 @@ static void server_loop(ocl_server* sv) {
             }
             if (!r->res) {
                 r->res.reset(new SliceRes());
+                r->res->cache = &sv->cache;
                 int rc = slice_plan(c, r->s, r->prm, (long)r->s.F * r->s.W, 0, r->res.get());
                 if (rc == OCL_OK) r->footprint = slice_footprint(*r->res);
                 if (rc == OCL_OK && r->footprint > sv->budget) {
@@ -1455,6 +1517,8 @@ static void server_loop(ocl_server* sv) {
             }
             sv->used += r->footprint;
             sv->peak_used = std::max(sv->peak_used, sv->used);
+            sv->cache_hits = sv->cache.hits;
+            sv->cache_misses = sv->cache.misses;
             r->t_admit = sv->now();
             r->status = OCL_SERVE_RUNNING;
             admitted.push_back(r);
@@ -1542,6 +1606,8 @@ extern "C" int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_
     int rc = ocl_ctx_create(device, &sv->c);
     if (rc != OCL_OK) { delete sv; return rc; }
     sv->budget = hbm_budget_bytes;
+    sv->cache.limit = hbm_budget_bytes;
+    sv->cache.stream = sv->c->stream;
     sv->chunk = chunk_iters > 0 ? chunk_iters : 8;
     sv->t0 = std::chrono::steady_clock::now();
     sv->th = std::thread(server_loop, sv);
@@ -1618,6 +1684,8 @@ extern "C" int ocl_server_get_stats(ocl_server* sv, ocl_server_stats* st) {
     st->busy_s = sv->busy_s;
     st->elapsed_s = sv->now();
     st->budget_waits = sv->budget_waits;
+    st->cache_hits = sv->cache_hits;
+    st->cache_misses = sv->cache_misses;
     return OCL_OK;
 }
 
@@ -1638,6 +1706,7 @@ extern "C" int ocl_server_destroy(ocl_server* sv) {
     }
     if (sv->th.joinable()) sv->th.join();
     sv->reqs.clear();
+    sv->cache.clear();
     ocl_ctx_destroy(sv->c);
     delete sv;
     return OCL_OK;

@@ -151,6 +151,23 @@ extern "C" int ocl_free(ocl_ctx* c, void* p) {
     }
     return OCL_OK;
 }
+extern "C" int ocl_host_malloc(ocl_ctx* c, size_t bytes, void** p) {
+    if (!c || !p) return fail(OCL_E_INVALID, "ocl_host_malloc: null argument");
+    *p = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    hipError_t e = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "ocl_host_malloc: out of page-locked host memory");
+    HIPCHK(e);
+    return OCL_OK;
+}
+extern "C" int ocl_host_free(ocl_ctx* c, void* p) {
+    if (!c) return fail(OCL_E_INVALID, "ocl_host_free: null context");
+    if (p) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipHostFree(p));
+    }
+    return OCL_OK;
+}
 extern "C" int ocl_memcpy_h2d(ocl_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!c) return fail(OCL_E_INVALID, "null context");
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));

@@ -76,6 +76,24 @@ def test_abs_copy(ctx):
     assert np.array_equal(o.get(), x)
 
 
+def test_pinned_host_upload_round_trip():
+    """ocl_host_malloc memory: a page-locked copy holds the same values, and uploads from it land unchanged."""
+    c = capi.Context(0)
+    try:
+        rng = np.random.default_rng(5)
+        a = (rng.standard_normal((3, 4, 64, 96)) + 1j * rng.standard_normal((3, 4, 64, 96))).astype(np.complex64)
+        m = (rng.random((3, 64, 96)) < 0.3).astype(np.uint8)
+        pa, pm = c.pinned(a), c.pinned(m)
+        assert pa.shape == a.shape and pa.dtype == a.dtype and np.array_equal(pa, a)
+        assert np.array_equal(pm, m)
+        da, dm = c.empty(a.shape), c.empty(m.shape, np.uint8)
+        da.set(pa)
+        dm.set(pm)
+        assert np.array_equal(da.get(), a) and np.array_equal(dm.get(), m)
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("H,W", [(8, 8), (12, 10), (128, 128), (160, 192), (224, 256), (256, 256), (288, 320),
                                  (320, 288), (7, 5), (9, 49), (1, 64), (512, 96)])
 def test_fft2d_vs_numpy(ctx, H, W):

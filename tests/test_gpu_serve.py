@@ -46,6 +46,37 @@ def test_served_results_equal_solo_and_budget_is_respected(ctx):
     srv.close()
 
 
+def test_reused_request_buffers_give_solo_results(ctx):
+    """Requests of one shape reuse the work buffers of the ones before them (which still hold their data), and a
+    request of another shape in between makes the server free cached buffers to stay inside its budget.  Every
+    result must equal its solo run bit for bit."""
+    a = synth.make_cine(H=128, W=128, C=4, F=8, accel=3.0, n_centre=4, slice_id=80)
+    b = synth.make_cine(H=96, W=160, C=3, F=6, accel=3.0, n_centre=4, slice_id=81)
+    order = [a, a, b, a, b, b, a]
+    prm = capi.nesta_params(maxIntIter=3, maxIter=8, stoptest=0)
+    ins = {id(sl): (ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M)) for sl in (a, b)}
+    outs = [ctx.empty((sl.F, sl.H, sl.W)) for sl in order]
+    budget = 10 << 20          # holds one request (about 6.6 MiB and 4.4 MiB) at a time, not two
+    srv = capi.Server(0, budget_bytes=budget, chunk=4)
+    try:
+        ids = [srv.submit(*ins[id(sl)], outs[i], prm) for i, sl in enumerate(order)]
+        for rid in ids:
+            assert srv.wait(rid, 300.0) == capi.SERVE_DONE
+        st = srv.stats()
+    finally:
+        srv.close()
+    print(st, flush=True)
+    assert st["completed"] == len(order) and st["failed"] == 0
+    assert st["cache_hits"] > 0 and st["peak_used_bytes"] <= budget
+    solo = {}
+    for sl in (a, b):
+        x = ctx.empty((sl.F, sl.H, sl.W))
+        ctx.nesta(*ins[id(sl)], x, prm, form=capi.OCL_FORM_FUSED)
+        solo[id(sl)] = x.get()
+    for i, sl in enumerate(order):
+        assert np.array_equal(outs[i].get(), solo[id(sl)]), i
+
+
 def test_server_rejects_request_larger_than_budget(ctx):
     sl = synth.make_cine(H=128, W=128, C=4, F=8, accel=3.0, slice_id=70)
     dev = (ctx.to_device(sl.y), ctx.to_device(sl.S), ctx.to_device(sl.M), ctx.empty((sl.F, sl.H, sl.W)))
