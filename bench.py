@@ -356,6 +356,7 @@ def _serve_trace(ctx, capi, host_pool, arrivals, pick, iters, budget, chunk, che
            "budget_bytes": budget, "peak_used_bytes": stats["peak_used_bytes"],
            "budget_waits": stats["budget_waits"], "launches": stats["launches"], "busy_s": stats["busy_s"],
            "buffer_cache_hits": stats["cache_hits"], "buffer_cache_misses": stats["cache_misses"],
+           "admit_s": stats["admit_s"], "retire_s": stats["retire_s"],
            "failed": stats["failed"]}
     return res, meta
 

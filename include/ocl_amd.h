@@ -300,6 +300,8 @@ typedef struct {
     double busy_s, elapsed_s;
     long budget_waits;   /* requests that reached the head of the queue and waited there for HBM budget */
     long cache_hits, cache_misses;   /* request buffers reused from the server's cache / newly allocated */
+    double admit_s, retire_s;        /* worker time admitting (plan, buffers, setup passes, table rebuilds) and
+                                        retiring requests */
 } ocl_server_stats;
 int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_iters, ocl_server** out);
 /* report may be NULL; when given it is filled when the request completes */

@@ -120,7 +120,8 @@ class ServerStats(ctypes.Structure):
                 ("active", ctypes.c_long), ("used_bytes", ctypes.c_size_t), ("peak_used_bytes", ctypes.c_size_t),
                 ("budget_bytes", ctypes.c_size_t), ("slice_iterations", ctypes.c_long), ("launches", ctypes.c_long),
                 ("busy_s", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("budget_waits", ctypes.c_long),
-                ("cache_hits", ctypes.c_long), ("cache_misses", ctypes.c_long)]
+                ("cache_hits", ctypes.c_long), ("cache_misses", ctypes.c_long), ("admit_s", ctypes.c_double),
+                ("retire_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -79,6 +79,64 @@ hipError_t dev_free(void* p) {
     return hipFree(p);
 }
 
+// Small host <-> device transfers through the context's page-locked buffer (ocl_ctx::zc_host), moved by a copy kernel
+// on the context's stream instead of the copy engines, where they would wait behind a caller's large uploads.  Both
+// calls return once the copies are done.  A transfer that does not fit the buffer, or whose size or address is not a
+// multiple of 4 bytes, goes through hipMemcpyAsync instead.
+struct ZcSeg {
+    void* dev;         // device address
+    void* host;        // host address (source of zc_to_device, destination of zc_from_device)
+    size_t bytes;
+};
+
+static bool zc_fits(const std::vector<ZcSeg>& segs) {
+    size_t off = 0;
+    for (const ZcSeg& g : segs) {
+        if (g.bytes % 4 || (uintptr_t)g.dev % 4) return false;
+        off = ((off + 255) & ~(size_t)255) + g.bytes;
+    }
+    return off <= ZC_BYTES;
+}
+
+int zc_to_device(ocl_ctx* c, const std::vector<ZcSeg>& segs) {
+    if (!c->zc_host || !zc_fits(segs)) {
+        for (const ZcSeg& g : segs)
+            if (g.bytes) HIPCHK(hipMemcpyAsync(g.dev, g.host, g.bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return OCL_OK;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));     // the buffer may still feed an earlier copy
+    size_t off = 0;
+    for (const ZcSeg& g : segs) {
+        off = (off + 255) & ~(size_t)255;
+        std::memcpy(c->zc_host + off, g.host, g.bytes);
+        HIPCHK(launch_copy_words(c->zc_dev + off, g.dev, g.bytes, c->stream));
+        off += g.bytes;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCL_OK;
+}
+
+int zc_from_device(ocl_ctx* c, const std::vector<ZcSeg>& segs) {
+    if (!c->zc_host || !zc_fits(segs)) {
+        for (const ZcSeg& g : segs)
+            if (g.bytes) HIPCHK(hipMemcpyAsync(g.host, g.dev, g.bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return OCL_OK;
+    }
+    size_t off = 0;
+    std::vector<size_t> offs;
+    for (const ZcSeg& g : segs) {
+        off = (off + 255) & ~(size_t)255;
+        offs.push_back(off);
+        HIPCHK(launch_copy_words(g.dev, c->zc_dev + off, g.bytes, c->stream));
+        off += g.bytes;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < segs.size(); ++i) std::memcpy(segs[i].host, c->zc_host + offs[i], segs[i].bytes);
+    return OCL_OK;
+}
+
 namespace {
 
 struct SliceSetup {
@@ -86,23 +144,26 @@ struct SliceSetup {
     int ne_iters;
 };
 
-// Device buffers a server keeps between requests, so that admission and completion call neither hipMalloc nor
-// hipFree (hipFree waits for the whole device) while requests of the same shape come and go.  Buffers are reused by
-// exact size.  `held` counts the buffers handed out to live requests; held + bytes stays within `limit` (the server's
-// HBM budget): a miss that would pass it first frees cached buffers.  Used by the server's worker thread only; every
-// user is on the server's one stream, so a buffer given back may be handed out again at once (stream order).
+// Device memory a server keeps between requests, so that admission and completion call neither hipMalloc (about
+// 0.3 ms per buffer) nor hipFree (which waits for the whole device) while requests come and go.  Each request takes
+// one block and carves its work buffers from it; a block given back is reused by a later request that needs at
+// least three quarters of it (the smallest such block).  `held` counts the blocks of live requests, and held + bytes
+// stays within `limit` (the server's HBM budget): a reuse that would pass it is not taken, and a new block first
+// frees cached ones.  Used by the server's worker thread only; every user is on the server's one stream, so a block
+// given back may be handed out again at once (stream order).
 struct BufCache {
     std::multimap<size_t, void*> free;
     size_t bytes = 0, held = 0, limit = 0;
     long hits = 0, misses = 0;
     hipStream_t stream = nullptr;
-    hipError_t take(void** p, size_t n) {
-        auto it = free.find(n);
-        if (it != free.end()) {
+    hipError_t take(void** p, size_t n, size_t* got) {
+        auto it = free.lower_bound(n);
+        if (it != free.end() && it->first <= n + n / 3 && held + it->first <= limit) {
             *p = it->second;
+            *got = it->first;
+            bytes -= it->first;
+            held += it->first;
             free.erase(it);
-            bytes -= n;
-            held += n;
             hits++;
             return hipSuccess;
         }
@@ -119,6 +180,7 @@ struct BufCache {
         }
         hipError_t e = dev_malloc(p, n);
         if (e == hipSuccess) {
+            *got = n;
             held += n;
             misses++;
         }
@@ -149,29 +211,60 @@ struct SliceRes {
     std::vector<int> nrows;
     std::vector<uint8_t> rowmask;     // [F][nmax] carried row is a mask row (Gram form)
     std::vector<void*> allocs;
-    std::vector<size_t> sizes;
     size_t bytes = 0;
     bool allocated = false;
     bool has_mc = false;      // motion compensation (argsMC) for this slice
     ocl_mc_args mc{};
-    BufCache* cache = nullptr;   // a server's: buffers come from it and go back to it
+    // a server's slice: its buffers are carved from one block of the server's cache (blk_need bytes asked for, the
+    // block may be larger); a buffer that does not fit the block takes a block of its own (`extra`)
+    BufCache* cache = nullptr;
+    char* blk = nullptr;
+    size_t blk_need = 0, blk_size = 0, blk_used = 0;
+    std::vector<std::pair<void*, size_t>> extra;
 
     ~SliceRes() {
-        if (allocs.empty()) return;
         if (cache) {
-            for (size_t i = 0; i < allocs.size(); ++i) cache->give(allocs[i], sizes[i]);
+            if (blk) cache->give(blk, blk_size);
+            for (auto& e : extra) cache->give(e.first, e.second);
             return;
         }
+        if (allocs.empty()) return;
         hipStreamSynchronize(c->stream);
         for (void* p : allocs) dev_free(p);
     }
+    size_t device_bytes() const {
+        if (!cache) return bytes;
+        size_t n = blk ? blk_size : 0;
+        for (const auto& e : extra) n += e.second;
+        return n;
+    }
     int alloc(size_t n, void** p) {
         n = std::max<size_t>(n, 16);
-        hipError_t e = cache ? cache->take(p, n) : dev_malloc(p, n);
+        if (cache) {
+            const size_t a = (n + 255) & ~(size_t)255;
+            hipError_t e = hipSuccess;
+            if (!blk && blk_need) {
+                void* b = nullptr;
+                e = cache->take(&b, std::max(blk_need, a), &blk_size);
+                if (e == hipSuccess) blk = (char*)b;
+            }
+            if (e == hipSuccess && blk && blk_used + a <= blk_size) {
+                *p = blk + blk_used;
+                blk_used += a;
+            } else if (e == hipSuccess) {
+                size_t got = 0;
+                e = cache->take(p, a, &got);
+                if (e == hipSuccess) extra.push_back({*p, got});
+            }
+            if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
+            HIPCHK(e);
+            bytes += n;
+            return OCL_OK;
+        }
+        hipError_t e = dev_malloc(p, n);
         if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
         HIPCHK(e);
         allocs.push_back(*p);
-        sizes.push_back(n);
         bytes += n;
         return OCL_OK;
     }
@@ -292,16 +385,25 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
         return fail(OCL_E_UNSUPPORTED, "NESTA: LDS tile too large");
     // rows carrying data: any sampled mask pixel or any non-zero k-space sample in the row
     const size_t fb = (size_t)s.F * s.H;
-    uint8_t* flags_d = nullptr;
-    HIPCHK(dev_malloc((void**)&flags_d, fb));
-    hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F, s.C,
-                       s.H, s.W, flags_d);
     std::vector<uint8_t> flags(fb);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), flags_d, fb, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dev_free(flags_d);
-    HIPCHK(e);
+    if (c->zc_host && fb <= ZC_BYTES) {
+        // straight into the context's page-locked buffer: no device buffer, no copy engine
+        hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F,
+                           s.C, s.H, s.W, (uint8_t*)c->zc_dev);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::memcpy(flags.data(), c->zc_host, fb);
+    } else {
+        uint8_t* flags_d = nullptr;
+        HIPCHK(dev_malloc((void**)&flags_d, fb));
+        hipLaunchKernelGGL(k_rowflags, dim3((unsigned)fb), dim3(OCL_NT), 0, c->stream, (const float2*)s.y, s.M, s.F,
+                           s.C, s.H, s.W, flags_d);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), flags_d, fb, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free(flags_d);
+        HIPCHK(e);
+    }
     r->nrows.assign(s.F, 0);
     int nmax = 1;
     bool line_mask = true;               // every mask row constant along the row
@@ -356,7 +458,7 @@ int slice_plan(ocl_ctx* c, const ocl_slice& s, const ocl_nesta_params& prm, long
 // Device footprint that slice_alloc will take (the server's admission test).
 size_t slice_footprint(const SliceRes& r) {
     const SliceDesc& d = r.d;
-    auto rnd = [](size_t n) { return std::max<size_t>(n, 16); };
+    auto rnd = [](size_t n) { return (std::max<size_t>(n, 16) + 255) & ~(size_t)255; };
     const size_t xb = sizeof(float2) * (size_t)d.F * d.N;
     return 4 * rnd(xb) + rnd(sizeof(float2) * (size_t)d.F * d.C * d.nmax * d.W) + rnd(sizeof(int16_t) * r.rows.size()) +
            rnd(sizeof(int16_t) * r.rowpos.size()) + rnd(sizeof(int) * d.F) + rnd(sizeof(NState)) +
@@ -747,17 +849,18 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
         (L.gram ? t->Lgr : t->Lng) = L;
     }
     if (ns) {
-        HIPCHK(hipMemcpyAsync(t->dd, hd.data(), sizeof(SliceDesc) * ns, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(t->offB, oB.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(t->offCA, oC.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(t->offP, oP.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(t->offX, oX.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(t->offG, oG.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+        // hd, gd and the offset vectors are host temporaries: the copies are complete when this returns
+        std::vector<ZcSeg> segs = {{t->dd, hd.data(), sizeof(SliceDesc) * ns},
+                                   {t->offB, oB.data(), sizeof(int) * (ns + 1)},
+                                   {t->offCA, oC.data(), sizeof(int) * (ns + 1)},
+                                   {t->offP, oP.data(), sizeof(int) * (ns + 1)},
+                                   {t->offX, oX.data(), sizeof(int) * (ns + 1)},
+                                   {t->offG, oG.data(), sizeof(int) * (ns + 1)}};
         if (!gd.empty()) {
-            HIPCHK(hipMemcpyAsync(t->gdd, gd.data(), sizeof(SliceDesc) * gd.size(), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(hipMemcpyAsync(t->goff, go.data(), sizeof(int) * go.size(), hipMemcpyHostToDevice, c->stream));
+            segs.push_back({t->gdd, gd.data(), sizeof(SliceDesc) * gd.size()});
+            segs.push_back({t->goff, go.data(), sizeof(int) * go.size()});
         }
-        HIPCHK(hipStreamSynchronize(c->stream));   // hd, gd and the offset vectors are host temporaries
+        CHK(zc_to_device(c, segs));
     }
     return OCL_OK;
 }
@@ -807,12 +910,20 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
         }
         t.L.mcR = R <= OCL_MC_GATHER_RMAX ? R : 0;
     }
+    std::vector<std::vector<double>> mps(v.size());
+    {
+        std::vector<ZcSeg> segs;
+        for (size_t i = 0; i < v.size(); ++i) {
+            mps[i].resize(2 * v[i]->d.wg_CA);
+            segs.push_back({v[i]->d.maxpart, mps[i].data(), sizeof(double) * mps[i].size()});
+        }
+        CHK(zc_from_device(c, segs));
+    }
+    std::vector<ZcSeg> st_segs;
     for (size_t i = 0; i < v.size(); ++i) {
         SliceRes* r = v[i];
         const SliceDesc& d = r->d;
-        mp.resize(2 * d.wg_CA);
-        HIPCHK(hipMemcpyAsync(mp.data(), d.maxpart, sizeof(double) * 2 * d.wg_CA, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        mp.swap(mps[i]);
         double m0 = (double)LONG_MIN, m1 = (double)LONG_MIN;
         for (int q = 0; q < d.wg_CA; ++q) { m0 = std::max(m0, mp[2 * q]); m1 = std::max(m1, mp[2 * q + 1]); }
         SliceSetup& u = r->setup;
@@ -847,10 +958,9 @@ int slices_setup(ocl_ctx* c, Table& t, std::vector<SliceRes*>& v) {
             S.it_cur = 0; S.it_wcur = 0; S.it_sstart = 1; S.it_redo = 0;
             S.it_pending = S.active;
         }
-        HIPCHK(hipMemcpyAsync(d.st, &S, sizeof(NState), hipMemcpyHostToDevice, c->stream));
+        st_segs.push_back({d.st, &S, sizeof(NState)});
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return OCL_OK;
+    return zc_to_device(c, st_segs);
 }
 
 int table_iterate(ocl_ctx* c, const Table& t, int n) {
@@ -1033,10 +1143,9 @@ int tune_shape(ocl_ctx* c, int H, int W) {
 }
 
 int pull_states(ocl_ctx* c, const std::vector<SliceRes*>& v, NState* h) {
-    for (size_t i = 0; i < v.size(); ++i)
-        HIPCHK(hipMemcpyAsync(&h[i], v[i]->d.st, sizeof(NState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return OCL_OK;
+    std::vector<ZcSeg> segs;
+    for (size_t i = 0; i < v.size(); ++i) segs.push_back({v[i]->d.st, &h[i], sizeof(NState)});
+    return zc_from_device(c, segs);
 }
 
 // Copies the current x of a slice (the buffer its state points at) to the caller's output.
@@ -1416,6 +1525,7 @@ struct ocl_server {
     size_t used = 0, peak_used = 0;
     long completed = 0, failed = 0, iterations = 0, launches = 0, n_active = 0, budget_waits = 0;
     long cache_hits = 0, cache_misses = 0;   // copies of cache.hits / misses, taken under mu
+    double admit_s = 0, retire_s = 0;        // worker time spent admitting (plan, buffers, setup) and retiring
     BufCache cache;                          // request buffers kept between requests (worker thread only)
     double busy_s = 0;
     std::chrono::steady_clock::time_point t0;
@@ -1468,6 +1578,7 @@ static void server_loop(ocl_server* sv) {
             break;
         }
         // ---- admission: first come first served while the footprint fits the budget ----
+        const double ta = sv->now();
         std::vector<ServeReq*> admitted;
         for (;;) {
             ServeReq* r = nullptr;
@@ -1504,6 +1615,7 @@ static void server_loop(ocl_server* sv) {
                 }
                 break;
             }
+            r->res->blk_need = r->footprint;
             int rc = slice_alloc(r->res.get());
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->queue.pop_front();
@@ -1515,6 +1627,7 @@ static void server_loop(ocl_server* sv) {
                 sv->done_cv.notify_all();
                 continue;
             }
+            r->footprint = r->res->device_bytes();   // a reused block may be larger than the request asked for
             sv->used += r->footprint;
             sv->peak_used = std::max(sv->peak_used, sv->used);
             sv->cache_hits = sv->cache.hits;
@@ -1544,6 +1657,10 @@ static void server_loop(ocl_server* sv) {
                 dirty = true;
             }
         }
+        if (!admitted.empty()) {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->admit_s += sv->now() - ta;
+        }
         if (active.empty()) {
             std::unique_lock<std::mutex> lk(sv->mu);
             if (sv->stop && sv->queue.empty()) break;
@@ -1552,11 +1669,14 @@ static void server_loop(ocl_server* sv) {
         }
         // ---- advance every running request by one chunk of iterations ----
         if (dirty) {
+            const double tt = sv->now();
             view.clear();
             for (ServeReq* r : active) view.push_back(r->res.get());
             if (table_build(c, &table, view) != OCL_OK) { server_fail_all(sv, active); break; }
             h.resize(view.size());
             dirty = false;
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->admit_s += sv->now() - tt;     // rebuilding the launch table counts with admission
         }
         const double tb = sv->now();
         if (table_iterate(c, table, sv->chunk) != OCL_OK || pull_states(c, view, h.data()) != OCL_OK) {
@@ -1571,6 +1691,8 @@ static void server_loop(ocl_server* sv) {
             sv->n_active = (long)view.size();
         }
         // ---- completions ----
+        const double tr = sv->now();
+        bool retired = false;
         std::vector<ServeReq*> still;
         for (size_t i = 0; i < active.size(); ++i) {
             ServeReq* r = active[i];
@@ -1592,6 +1714,11 @@ static void server_loop(ocl_server* sv) {
             r->status = OCL_SERVE_DONE;
             sv->completed++;
             dirty = true;
+            retired = true;
+        }
+        if (retired) {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->retire_s += sv->now() - tr;
         }
         if (still.size() != active.size()) {
             active.swap(still);
@@ -1686,6 +1813,8 @@ extern "C" int ocl_server_get_stats(ocl_server* sv, ocl_server_stats* st) {
     st->budget_waits = sv->budget_waits;
     st->cache_hits = sv->cache_hits;
     st->cache_misses = sv->cache_misses;
+    st->admit_s = sv->admit_s;
+    st->retire_s = sv->retire_s;
     return OCL_OK;
 }
 

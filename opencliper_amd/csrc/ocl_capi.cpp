@@ -41,6 +41,9 @@ extern "C" int ocl_ctx_create(int device, ocl_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->red_part, sizeof(double) * 2 * RED_PARTS);
     if (e == hipSuccess) e = hipMalloc(&c->red_out, sizeof(double) * 16);
     if (e == hipSuccess) e = hipHostMalloc(&c->red_host, sizeof(double) * 16, hipHostMallocDefault);
+    // coherent: kernel writes reach host memory without a cache flush, kernel reads see what the host wrote
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->zc_host, ZC_BYTES, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&c->zc_dev, c->zc_host, 0);
     if (e == hipSuccess && scratch_init_device(device) != OCL_OK) {
         const std::string m = ocl_last_error();
         ocl_ctx_destroy(c);
@@ -72,6 +75,7 @@ extern "C" int ocl_ctx_destroy(ocl_ctx* c) {
     if (c->red_part) hipFree(c->red_part);
     if (c->red_out) hipFree(c->red_out);
     if (c->red_host) hipHostFree(c->red_host);
+    if (c->zc_host) hipHostFree(c->zc_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return OCL_OK;

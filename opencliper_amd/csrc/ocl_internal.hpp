@@ -63,7 +63,14 @@ struct ocl_ctx {
     // the row and column passes for every slice
     int split_gram = 1;
     std::map<std::pair<int, int>, TileCfg> tiles;           // tuned tile shapes by (H, W)
+    // page-locked host buffer of ZC_BYTES that kernels read and write directly (zc_dev is its device address): the
+    // small transfers of admission and of each chunk go through it with a copy kernel instead of the copy engines,
+    // where they would queue behind a caller's large uploads
+    char* zc_host = nullptr;
+    char* zc_dev = nullptr;
 };
+
+static const size_t ZC_BYTES = 1 << 20;
 
 static const int RED_PARTS = 1024;
 static const size_t LDS_MAX = 64 * 1024;          // per-workgroup dynamic LDS budget of the standalone FFT launches

@@ -191,6 +191,8 @@ struct NestaLaunch {
     int mcR;            // gather radius of pass G (largest over the table), 0: scatter form; set at setup
     int gram;           // 1: every slice runs the Gram form (pass G over column tiles + scalar step S2)
 };
+// byte copy in 4-byte words by a kernel (addresses and size multiples of 4), for transfers through mapped host memory
+hipError_t launch_copy_words(const void* src, void* dst, size_t bytes, hipStream_t stream);
 hipError_t launch_nesta_B(const NestaLaunch& L, hipStream_t stream, int mode);
 // Gram form: pass G (mode 0 iteration, 1 setup x_ref = A^H sqrt(N) y into x0) and the scalar step after it
 hipError_t launch_nesta_G(const NestaLaunch& L, hipStream_t stream, int mode);

@@ -110,6 +110,21 @@ __global__ void k_abs(const float2* __restrict__ in, float2* __restrict__ out, l
 }
 
 // CopyDataGPU (initialize.cl:76-80) / BLAS copy, scal, axpy with a complex alpha
+// Byte copy in 4-byte words (both addresses and n are multiples of 4): the small host <-> device transfers through
+// page-locked host memory, with either side mapped host memory.
+__global__ void k_copy_words(const unsigned* __restrict__ x, unsigned* __restrict__ y, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+hipError_t launch_copy_words(const void* src, void* dst, size_t bytes, hipStream_t stream) {
+    const long n = (long)(bytes / 4);
+    if (n == 0) return hipSuccess;
+    const long g = (n + OCL_NT - 1) / OCL_NT;
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)(g < 64 ? g : 64)), dim3(OCL_NT), 0, stream,
+                       (const unsigned*)src, (unsigned*)dst, n);
+    return hipGetLastError();
+}
+
 __global__ void k_copy(const float2* __restrict__ x, float2* __restrict__ y, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
 }
