@@ -232,10 +232,12 @@ def leg_batched(ctx, dev, prm, steps, warmup, form=None):
     return res
 
 
-def leg_uneven(ctx, synth, parts, prm, steps, warmup):
+def leg_uneven(ctx, synth, parts, prm, steps, warmup, group_threshold=None):
     """64 uneven slices; the tile shape of each H x W comes from the autotuner (tuned at batch setup, outside the
-    timed steps)."""
+    timed steps).  group_threshold: column-pass workgroups from which a size gets launches of its own."""
     ctx.set_autotune(True)
+    if group_threshold:
+        ctx.set_group_threshold(group_threshold)
     dev = []
     for i, (x, S, M) in enumerate(parts):
         y, dS, dM = synth.device_kspace(ctx, x, S, M, seed=synth.BASE_SEED + 1000 + i)
@@ -565,7 +567,8 @@ def gpu_legs(args, cine, parts, full, split):
                 f"{sp['fp32_fused']['recons_per_s']:.1f}), iteration {sp['split']['iteration']}")
         dev3 = []
         if "uneven" in args.legs:
-            out["uneven_batch"], dev3 = leg_uneven(ctx, synth, parts, prm, args.steps, args.warmup)
+            out["uneven_batch"], dev3 = leg_uneven(ctx, synth, parts, prm, args.steps, args.warmup,
+                                                   args.group_threshold)
             small = min(range(len(dev3)), key=lambda i: b_alg(*dev3[i][0].shape))
             out["x"]["uneven_small"] = (small, dev3[small][0].get(), dev3[small][1].get(), dev3[small][2].get(),
                                         dev3[small][3].get())
@@ -783,6 +786,8 @@ def main():
                     help="server HBM budget in GiB (the default binds: requests wait for memory)")
     ap.add_argument("--serve-chunk", type=int, default=5, help="server iterations between admission points")
     ap.add_argument("--serve-burst", type=int, default=96, help="requests submitted at once for server capacity")
+    ap.add_argument("--group-threshold", type=int, default=None,
+                    help="uneven batch: column-pass workgroups from which a size gets launches of its own")
     ap.add_argument("--serve-burst-budget", type=float, default=0.0,
                     help="HBM budget (GiB) of a second capacity burst (0: skip)")
     ap.add_argument("--serve-load-seconds", type=float, default=30.0,

@@ -275,6 +275,11 @@ int ocl_ctx_set_autotune(ocl_ctx* ctx, int on);
  * GEMMs, where it is instantiated (H in {64, 128, 192} with at most 32 sampled rows per frame, or H = 64 with 64);
  * 0 keeps the row and column passes for every slice of batches created afterwards */
 int ocl_ctx_set_split_gram(ocl_ctx* ctx, int on);
+/* launch grouping of mixed-size tables (no reference counterpart): a column size whose slices have at least
+ * min_workgroups column-pass workgroups runs as a launch of its own with that size's kernels; the other sizes share
+ * one launch of the kernels that hold every size.  Applies to tables built afterwards on this context (its batches;
+ * a server keeps the library default, 2048).  min_workgroups >= 1. */
+int ocl_ctx_set_group_threshold(ocl_ctx* ctx, int min_workgroups);
 /* pins the tile shape of H x W (e.g. a saved tuning): ca_fast 1 / 2 selects the four-step column pass that gathers
  * the sampled rows from global memory / prefetches the next coil's rows and sensitivities, b_fast 1 the four-step
  * row pass; 0 selects the Stockham passes sized by tile_cols / rows_per_wg (powers of two <= 16) */

@@ -185,7 +185,7 @@ def lib():
                                 I),
         "ocl_batch_create_form": ([P, I, ctypes.POINTER(Slice), ctypes.POINTER(NestaParams), I, PP], I),
         "ocl_ctx_set_autotune": ([P, I], I),
-        "ocl_ctx_set_split_gram": ([P, I], I),
+        "ocl_ctx_set_split_gram": ([P, I], I), "ocl_ctx_set_group_threshold": ([P, I], I),
         "ocl_rsos": ([P, P, P, I, I, LG], I), "ocl_simple_mri_recon_sos": ([P, P, P, I, I, I, I], I),
         "ocl_ctx_set_tiles": ([P, I, I, I, I, I, I], I),
         "ocl_autotune_shape": ([P, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
@@ -309,6 +309,10 @@ class Context:
     def set_split_gram(self, on=True):
         """Split-precision form: Gram pass for line masks where it is instantiated (True) or row and column passes."""
         _check(lib().ocl_ctx_set_split_gram(self.handle, 1 if on else 0))
+
+    def set_group_threshold(self, min_workgroups):
+        """Column-pass workgroups from which a size of a mixed table gets launches of its own (tables built later)."""
+        _check(lib().ocl_ctx_set_group_threshold(self.handle, int(min_workgroups)))
 
     def set_tiles(self, H, W, ca_fast, tile_cols, b_fast, rows_per_wg):
         """Pins the tile shape of the FP32 passes for H x W."""

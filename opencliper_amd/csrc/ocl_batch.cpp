@@ -707,7 +707,7 @@ int table_build(ocl_ctx* c, Table* t, const std::vector<SliceRes*>& v) {
             for (const auto& kv : by_size) {
                 long wg = 0;
                 for (int i : kv.second) wg += hd[i].wg_CA;
-                if (wg >= OCL_GROUP_MIN_WG) fsets.push_back(kv.second);
+                if (wg >= (c->group_min_wg > 0 ? c->group_min_wg : OCL_GROUP_MIN_WG)) fsets.push_back(kv.second);
                 else rest.insert(rest.end(), kv.second.begin(), kv.second.end());
             }
         } else {
@@ -1319,6 +1319,13 @@ extern "C" int ocl_batch_time(ocl_batch* b, int n_iters, double* ms_B, double* m
 extern "C" int ocl_ctx_set_autotune(ocl_ctx* c, int on) {
     if (!c) return fail(OCL_E_INVALID, "null context");
     c->autotune = on ? 1 : 0;
+    return OCL_OK;
+}
+
+extern "C" int ocl_ctx_set_group_threshold(ocl_ctx* c, int min_workgroups) {
+    if (!c) return fail(OCL_E_INVALID, "null context");
+    if (min_workgroups < 1) return fail(OCL_E_INVALID, "set_group_threshold: at least one workgroup");
+    c->group_min_wg = min_workgroups;
     return OCL_OK;
 }
 

@@ -62,6 +62,7 @@ struct ocl_ctx {
     // split-precision form: 1 runs the Gram pass (column GEMMs only) for line masks where it is instantiated, 0 keeps
     // the row and column passes for every slice
     int split_gram = 1;
+    int group_min_wg = 0;                                   // launch-group threshold of mixed tables (0: default)
     std::map<std::pair<int, int>, TileCfg> tiles;           // tuned tile shapes by (H, W)
     // page-locked host buffer of ZC_BYTES that kernels read and write directly (zc_dev is its device address): the
     // small transfers of admission and of each chunk go through it with a copy kernel instead of the copy engines,
