@@ -245,6 +245,8 @@ def leg_uneven(ctx, synth, parts, prm, steps, warmup, group_threshold=None):
     ctx.sync()
     t0 = time.perf_counter()
     tiles = {f"{h}x{h}": ctx.autotune_shape(h, h) for h in sorted({x.shape[1] for (x, _, _) in parts})}
+    for t in tiles.values():     # ca_fast 3: the Gram form (one column pass per iteration) beat the row + column passes
+        t["form"] = "gram" if t["ca_fast"] == 3 else "row and column passes"
     tune_s = time.perf_counter() - t0
     res = leg_batched(ctx, dev, prm, steps, warmup)
     res["autotune"] = {"tune_s": tune_s, "tiles": tiles}

@@ -282,7 +282,9 @@ int ocl_ctx_set_split_gram(ocl_ctx* ctx, int on);
 int ocl_ctx_set_group_threshold(ocl_ctx* ctx, int min_workgroups);
 /* pins the tile shape of H x W (e.g. a saved tuning): ca_fast 1 / 2 selects the four-step column pass that gathers
  * the sampled rows from global memory / prefetches the next coil's rows and sensitivities, b_fast 1 the four-step
- * row pass; 0 selects the Stockham passes sized by tile_cols / rows_per_wg (powers of two <= 16) */
+ * row pass; 0 selects the Stockham passes sized by tile_cols / rows_per_wg (powers of two <= 16).  ca_fast 3 selects
+ * the Gram form for slices with line masks (one column pass per iteration, no row pass; other masks fall back to the
+ * prefetching column pass).  ocl_autotune_shape returns the same codes: 3 when the Gram form won. */
 int ocl_ctx_set_tiles(ocl_ctx* ctx, int H, int W, int ca_fast, int tile_cols, int b_fast, int rows_per_wg);
 int ocl_autotune_shape(ocl_ctx* ctx, int H, int W, int* ca_fast, int* tile_cols, int* b_fast, int* rows_per_wg,
                        double* ms_CA_per_slice_iter, double* ms_B_per_slice_iter);

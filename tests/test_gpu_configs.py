@@ -231,6 +231,36 @@ def test_mixed_table_with_a_launch_group(ctx, oracle):
     b.close()
 
 
+def test_every_size_in_a_launch_of_its_own():
+    """With the launch-group threshold at one workgroup (ocl_ctx_set_group_threshold), each of three sizes runs its
+    own specialized kernels in a launch of its own: every slice's iteration counts and result equal its solo run."""
+    sizes = [(160, 160, 6, 9), (192, 192, 4, 12), (128, 128, 5, 10)]
+    c = capi.Context(0)
+    try:
+        with pytest.raises(capi.OclError):
+            c.set_group_threshold(0)
+        c.set_group_threshold(1)
+        print("per-size table: generating 3 slices", flush=True)
+        dev = _device_slices(c, sizes, 2700)
+        prm = capi.nesta_params(maxIntIter=3, maxIter=12, stoptest=1, miniter=5)
+        b = c.batch(dev, prm)
+        b.setup()
+        info = b.info()
+        print("per-size table", info, flush=True)
+        assert info["launch_groups"] == 3
+        b.run()
+        out = [d[3].get() for d in dev]
+        reps = [b.report(i) for i in range(len(dev))]
+        b.close()
+        for i, d in enumerate(dev):
+            xs, rs = _solo(c, d, prm)
+            print(i, sizes[i], reps[i]["iters"], rs["iters"], flush=True)
+            assert reps[i]["iters"] == rs["iters"]
+            assert np.array_equal(out[i], xs)
+    finally:
+        c.close()
+
+
 def test_one_size_longest_first(ctx):
     """One size with 3, 9 and 6 coils: the table runs as one launch group whose workgroups go longest first (9, 6,
     3 coils), with the stop test on; every slice's iteration counts and result equal its solo run."""
