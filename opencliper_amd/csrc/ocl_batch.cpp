@@ -15,6 +15,7 @@
 #include <tuple>
 #include <mutex>
 
+#include "ocl_bufcache.hpp"
 #include "ocl_internal.hpp"
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -144,61 +145,18 @@ struct SliceSetup {
     int ne_iters;
 };
 
-// Device memory a server keeps between requests, so that admission and completion call neither hipMalloc (about
-// 0.3 ms per buffer) nor hipFree (which waits for the whole device) while requests come and go.  Each request takes
-// one block and carves its work buffers from it; a block given back is reused by a later request that needs at
-// least three quarters of it (the smallest such block).  `held` counts the blocks of live requests, and held + bytes
-// stays within `limit` (the server's HBM budget): a reuse that would pass it is not taken, and a new block first
-// frees cached ones.  Used by the server's worker thread only; every user is on the server's one stream, so a block
-// given back may be handed out again at once (stream order).
-struct BufCache {
-    std::multimap<size_t, void*> free;
-    size_t bytes = 0, held = 0, limit = 0;
-    long hits = 0, misses = 0;
+// The server's block cache (ocl_bufcache.hpp) over this library's device allocation and the server's stream.
+struct HipBlockDev {
     hipStream_t stream = nullptr;
-    hipError_t take(void** p, size_t n, size_t* got) {
-        auto it = free.lower_bound(n);
-        if (it != free.end() && it->first <= n + n / 3 && held + it->first <= limit) {
-            *p = it->second;
-            *got = it->first;
-            bytes -= it->first;
-            held += it->first;
-            free.erase(it);
-            hits++;
-            return hipSuccess;
-        }
-        if (held + bytes + n > limit && !free.empty()) {
-            hipError_t e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            while (held + bytes + n > limit && !free.empty()) {
-                auto big = std::prev(free.end());
-                e = dev_free(big->second);
-                bytes -= big->first;
-                free.erase(big);
-                if (e != hipSuccess) return e;
-            }
-        }
-        hipError_t e = dev_malloc(p, n);
-        if (e == hipSuccess) {
-            *got = n;
-            held += n;
-            misses++;
-        }
-        return e;
+    int alloc(void** p, size_t n) {
+        const hipError_t e = dev_malloc(p, n);
+        return e == hipSuccess ? 0 : e == hipErrorOutOfMemory ? kOom : (int)e;
     }
-    void give(void* p, size_t n) {
-        free.emplace(n, p);
-        bytes += n;
-        held -= n;
-    }
-    void clear() {
-        if (free.empty()) return;
-        (void)hipStreamSynchronize(stream);
-        for (auto& kv : free) (void)dev_free(kv.second);
-        free.clear();
-        bytes = 0;
-    }
+    int release(void* p) { return (int)dev_free(p); }
+    int sync() { return (int)hipStreamSynchronize(stream); }
+    static constexpr int kOom = (int)hipErrorOutOfMemory;
 };
+using BufCache = BlockCache<HipBlockDev>;
 
 // Device resources of one slice.
 struct SliceRes {
@@ -245,7 +203,7 @@ struct SliceRes {
             hipError_t e = hipSuccess;
             if (!blk && blk_need) {
                 void* b = nullptr;
-                e = cache->take(&b, std::max(blk_need, a), &blk_size);
+                e = (hipError_t)cache->take(&b, std::max(blk_need, a), &blk_size);
                 if (e == hipSuccess) blk = (char*)b;
             }
             if (e == hipSuccess && blk && blk_used + a <= blk_size) {
@@ -253,7 +211,7 @@ struct SliceRes {
                 blk_used += a;
             } else if (e == hipSuccess) {
                 size_t got = 0;
-                e = cache->take(p, a, &got);
+                e = (hipError_t)cache->take(p, a, &got);
                 if (e == hipSuccess) extra.push_back({*p, got});
             }
             if (e == hipErrorOutOfMemory) return fail(OCL_E_NOMEM, "NESTA: out of device memory");
@@ -1741,7 +1699,7 @@ extern "C" int ocl_server_create(int device, size_t hbm_budget_bytes, int chunk_
     if (rc != OCL_OK) { delete sv; return rc; }
     sv->budget = hbm_budget_bytes;
     sv->cache.limit = hbm_budget_bytes;
-    sv->cache.stream = sv->c->stream;
+    sv->cache.dev.stream = sv->c->stream;
     sv->chunk = chunk_iters > 0 ? chunk_iters : 8;
     sv->t0 = std::chrono::steady_clock::now();
     sv->th = std::thread(server_loop, sv);
