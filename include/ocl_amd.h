@@ -83,7 +83,8 @@ int ocl_memcpy_d2d(ocl_ctx* ctx, void* dst, const void* src, size_t bytes);   /*
 int ocl_memset_zero(ocl_ctx* ctx, void* dptr, size_t bytes);                  /* stream-ordered fill kernel */
 /* page-locked host memory for inputs uploaded again and again (a server's request payloads): replaces the reference's
  * mapped host view of a device buffer (src/DeviceDataProperties.cpp:242, enqueueMapBuffer).  ocl_memcpy_h2d from it
- * runs as one DMA transfer instead of through the driver's pageable staging.  Freed by ocl_host_free only. */
+ * runs as one DMA transfer instead of through the driver's pageable staging.  Freed by ocl_host_free only, which
+ * waits for the context's stream first (ctx may be NULL: the memory does not belong to a context). */
 int ocl_host_malloc(ocl_ctx* ctx, size_t bytes, void** hptr);
 int ocl_host_free(ocl_ctx* ctx, void* hptr);
 

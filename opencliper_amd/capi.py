@@ -268,15 +268,11 @@ class Context:
         _check(lib().ocl_ctx_create(device, ctypes.byref(h)))
         self.handle = h.value
         self._children = weakref.WeakSet()      # batches and registrations: destroyed before the context
-        self._pinned = []                       # page-locked host allocations of pinned()
 
     def close(self):
         if self.handle:
             for ch in list(self._children):
                 ch.close()
-            for p in self._pinned:
-                lib().ocl_host_free(self.handle, p)
-            self._pinned = []
             lib().ocl_ctx_destroy(self.handle)
             self.handle = None
 
@@ -334,13 +330,17 @@ class Context:
 
     def pinned(self, a: np.ndarray) -> np.ndarray:
         """A copy of `a` in page-locked host memory (ocl_host_malloc), as a numpy array.  Uploads from it are one
-        DMA transfer.  The memory is released by close(): the array must not be used after that."""
+        DMA transfer.  The memory is released when the array and every view of it are gone."""
         a = np.ascontiguousarray(a)
+        n = max(a.nbytes, 1)
         p = ctypes.c_void_p()
-        _check(lib().ocl_host_malloc(self.handle, max(a.nbytes, 1), ctypes.byref(p)))
-        self._pinned.append(p.value)
-        h = np.frombuffer((ctypes.c_char * max(a.nbytes, 1)).from_address(p.value), dtype=a.dtype, count=a.size)
-        h = h.reshape(a.shape)
+        _check(lib().ocl_host_malloc(self.handle, n, ctypes.byref(p)))
+        buf = type("PinnedBuffer", (ctypes.c_char * n,), {}).from_address(p.value)
+        # the numpy array keeps `buf` alive; the block is freed (no context needed) once `buf` is collected, and is
+        # left to the process's end rather than freed while the runtime shuts down
+        fin = weakref.finalize(buf, lib().ocl_host_free, None, p.value)
+        fin.atexit = False
+        h = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape)
         h[...] = a
         return h
 

@@ -165,10 +165,9 @@ extern "C" int ocl_host_malloc(ocl_ctx* c, size_t bytes, void** p) {
     return OCL_OK;
 }
 extern "C" int ocl_host_free(ocl_ctx* c, void* p) {
-    if (!c) return fail(OCL_E_INVALID, "ocl_host_free: null context");
     if (p) {
-        HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipHostFree(p));
+        if (c) HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipHostFree(p));        // waits for the device itself when no context is given
     }
     return OCL_OK;
 }

@@ -77,7 +77,8 @@ def test_abs_copy(ctx):
 
 
 def test_pinned_host_upload_round_trip():
-    """ocl_host_malloc memory: a page-locked copy holds the same values, and uploads from it land unchanged."""
+    """ocl_host_malloc memory: a page-locked copy holds the same values, uploads from it land unchanged, and it stays
+    valid after its context is closed (it is freed with its last view, by ocl_host_free without a context)."""
     c = capi.Context(0)
     try:
         rng = np.random.default_rng(5)
@@ -90,8 +91,11 @@ def test_pinned_host_upload_round_trip():
         da.set(pa)
         dm.set(pm)
         assert np.array_equal(da.get(), a) and np.array_equal(dm.get(), m)
+        view = pa[1:]
     finally:
         c.close()
+    assert np.array_equal(view, a[1:])
+    del pa, pm, view
 
 
 @pytest.mark.parametrize("H,W", [(8, 8), (12, 10), (128, 128), (160, 192), (224, 256), (256, 256), (288, 320),
