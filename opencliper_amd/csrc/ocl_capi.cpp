@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "ocl_internal.hpp"
@@ -781,6 +782,11 @@ extern "C" int ocl_ctx_normest_stream_position(ocl_ctx* c, long long* position) 
     return OCL_OK;
 }
 
+// normU from the default drand48 state is a function of (F, N) alone: estimates are shared by every context of a
+// device, so that a new context (a server's, another CLapp) does not repeat a power method another has already run
+static std::mutex g_normU_mu;
+static std::map<std::tuple<int, int, long>, std::pair<double, int>> g_normU;
+
 extern "C" int ocl_normest(ocl_ctx* c, int F, long N, double* normU, int* iters) {
     CTXCHK(c);
     if (F < 1 || N < 1 || !normU) return fail(OCL_E_INVALID, "normest: bad arguments");
@@ -803,13 +809,25 @@ extern "C" int ocl_normest(ocl_ctx* c, int F, long N, double* normU, int* iters)
             return OCL_OK;
         }
     }
-    double v = 0;
-    int it = 0;
-    CHK(normest_compute(c, F, N, &v, &it));
+    std::pair<double, int> e{0.0, 0};
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_normU_mu);
+        auto it = g_normU.find(std::make_tuple(c->device, F, N));
+        if (it != g_normU.end()) {
+            e = it->second;
+            have = true;
+        }
+    }
+    if (!have) {
+        CHK(normest_compute(c, F, N, &e.first, &e.second));
+        std::lock_guard<std::mutex> lk(g_normU_mu);
+        g_normU[std::make_tuple(c->device, F, N)] = e;
+    }
     std::lock_guard<std::mutex> lk(c->mu);
-    c->normU_cache[{F, N}] = {v, it};
-    *normU = v;
-    if (iters) *iters = it;
+    c->normU_cache[{F, N}] = e;
+    *normU = e.first;
+    if (iters) *iters = e.second;
     return OCL_OK;
 }
 
