@@ -32,13 +32,17 @@ __device__ __forceinline__ void gram_update(const SliceDesc& d, const NState* st
     constexpr int CH = GRAM_UPD_ROWS < P ? GRAM_UPD_ROWS : P;
     const long N = d.N, W = d.W;
     const int fm = f == 0 ? d.F - 1 : f - 1, fp = f == d.F - 1 ? 0 : f + 1;
-    const float2* xb = st->it_cur ? d.x1 : d.x0;
-    float2* xo = (st->it_cur ? d.x0 : d.x1) + (long)f * N;
-    const float2* xm = xb + (long)fm * N;
-    const float2* xp = xb + (long)fp * N;
-    const float2* wi = (st->it_wcur ? d.Wk2 : d.Wk) + (long)f * N;
-    float2* wo_ = (st->it_wcur ? d.Wk : d.Wk2) + (long)f * N;
-    float2* xr_ = d.xref + (long)f * N;
+    // buffer resources over the slice's image arrays: one 32-bit lane offset, scalar row offsets (no 64-bit
+    // per-lane address arithmetic on the vector unit)
+    // (the ping-pong flags are read as wavefront-uniform values, so the resources stay in scalar registers)
+    const long xbytes = 8 * (long)d.F * N;
+    const int cur = __builtin_amdgcn_readfirstlane(st->it_cur), wcur = __builtin_amdgcn_readfirstlane(st->it_wcur);
+    const BufRes rxb = buf_res(cur ? d.x1 : d.x0, xbytes), rxo = buf_res(cur ? d.x0 : d.x1, xbytes);
+    const BufRes rwi = buf_res(wcur ? d.Wk2 : d.Wk, xbytes), rwo = buf_res(wcur ? d.Wk : d.Wk2, xbytes);
+    const BufRes rxr = buf_res(d.xref, xbytes);
+    const unsigned vo = p * 8u;
+    const unsigned so_f = (unsigned)(8 * (long)f * N), so_m = (unsigned)(8 * (long)fm * N),
+                   so_p = (unsigned)(8 * (long)fp * N);
     const float mu = (float)st->it_mu, imu = 1.0f / mu, lam = (float)st->lambda, L1 = (float)st->it_L1;
     const float apk = (float)st->it_apk, tau = (float)st->it_tau, sc = d.inv_sqrtN;
     float u2 = 0.f, fx = 0.f;
@@ -48,19 +52,19 @@ __device__ __forceinline__ void gram_update(const SliceDesc& d, const NState* st
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             if (n0 + j >= P) break;
-            const long ub = (long)16 * (n0 + j) * W;
-            a[j] = ldg2(xm + ub, p);
-            b[j] = ldg2(xp + ub, p);
+            const unsigned ub = (unsigned)(8 * 16 * (n0 + j) * W);
+            a[j] = bld2(rxb, vo, so_m + ub);
+            b[j] = bld2(rxb, vo, so_p + ub);
             if constexpr (!SST) {
-                xr[j] = ldg2(xr_ + ub, p);
-                wo[j] = ldg2(wi + ub, p);
+                xr[j] = bld2(rxr, vo, so_f + ub);
+                wo[j] = bld2(rwi, vo, so_f + ub);
             }
         }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             const int n1 = n0 + j;
             if (n1 >= P) break;
-            const long ub = (long)16 * n1 * W;
+            const unsigned ub = (unsigned)(8 * 16 * n1 * W);
             const float2 xf = xk[n1];
             if constexpr (SST) {
                 xr[j] = xf;
@@ -79,9 +83,9 @@ __device__ __forceinline__ void gram_update(const SliceDesc& d, const NState* st
             if (wv) {
                 u2 += c_abs2(uf);
                 fx += uf.x * uxf.x + uf.y * uxf.y;
-                if constexpr (SST) stg2(xr_ + ub, p, xf);
-                stg2(wo_ + ub, p, wn);
-                stg2(xo + ub, p, x1);
+                if constexpr (SST) bst2(rxr, vo, so_f + ub, xf);
+                bst2(rwo, vo, so_f + ub, wn);
+                bst2(rxo, vo, so_f + ub, x1);
             }
         }
     }
@@ -131,13 +135,17 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
     }
     if (threadIdx.x < 32) kst[(threadIdx.x >> 4) * kss + nmax * 16 + (threadIdx.x & 15)] = make_float2(0.f, 0.f);
     // staging of conj(V) of one coil: coalesced 16-column rows, at most FAST_CA_KQ values per thread
+    // buffer resources (one 32-bit lane offset, scalar offsets): the staged k-space, S and x_k
+    const BufRes rV = buf_res(d.kbuf, 8 * (long)d.F * C * nmax * W);
+    const BufRes rS = buf_res(d.S, 8 * (long)C * N);
+    const unsigned vl = lane * 8u, vlc = lanec * 8u, rowB = (unsigned)W * 8u;
     float2 pre[FAST_CA_KQ];
     auto stage_load = [&](int c) {
-        const float2* kb = d.kbuf + ((long)f * C + c) * nmax * W;
+        const unsigned kb = (unsigned)(8 * ((long)f * C + c) * nmax * W);
 #pragma unroll
         for (int q = 0; q < FAST_CA_KQ; ++q) {
             const int e = threadIdx.x + OCL_NT * q;
-            pre[q] = (e < nst && wv) ? ldg2(kb + (long)16 * q * W, lane) : make_float2(0.f, 0.f);
+            pre[q] = (e < nst && wv) ? bld2(rV, vl, kb + 16u * q * rowB) : make_float2(0.f, 0.f);
         }
     };
     auto stage_store = [&](int buf) {
@@ -152,9 +160,10 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
 
     float2 xk[P];
     if (mode == 0) {
-        const float2* xb = (st->it_cur ? d.x1 : d.x0) + (long)f * N;
+        const BufRes rX = buf_res(__builtin_amdgcn_readfirstlane(st->it_cur) ? d.x1 : d.x0, 8 * (long)d.F * N);
+        const unsigned xf = (unsigned)(8 * (long)f * N);
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) xk[n1] = ldg2(xb + (long)16 * n1 * W, lanec);
+        for (int n1 = 0; n1 < P; ++n1) xk[n1] = bld2(rX, vlc, xf + 16u * n1 * rowB);
     }
     float2 acc[P];
 #pragma unroll
@@ -163,7 +172,7 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
     __syncthreads();
 
     for (int c = 0; c < C; ++c) {
-        const float2* Sc = d.S + (long)c * N;
+        const unsigned sc = (unsigned)(8 * (long)c * N);       // byte offset of S_c
         const float2* ks = kst + (c & 1) * kss + s;
         if (c + 1 < C) stage_load(c + 1);
         if (mode == 0) {
@@ -172,7 +181,7 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
             // load, wait and multiply one value at a time, which serializes P memory latencies
             float2 v[P];
 #pragma unroll
-            for (int n1 = 0; n1 < P; ++n1) v[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
+            for (int n1 = 0; n1 < P; ++n1) v[n1] = bld2(rS, vlc, sc + 16u * n1 * rowB);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int n1 = 0; n1 < P; ++n1) v[n1] = c_mul(xk[n1], v[n1]);
@@ -225,7 +234,7 @@ __device__ __forceinline__ void nesta_G_fast(const SliceDesc& d, int local, int 
         // S of this coil for the accumulate, issued before the exchange barrier
         float2 sv[P];
 #pragma unroll
-        for (int n1 = 0; n1 < P; ++n1) sv[n1] = ldg2(Sc + (long)16 * n1 * W, lanec);
+        for (int n1 = 0; n1 < P; ++n1) sv[n1] = bld2(rS, vlc, sc + 16u * n1 * rowB);
         __syncthreads();
         // the other slab was last read in the previous coil's role phase, which the barrier above has closed
         if (c + 1 < C) stage_store((c + 1) & 1);
