@@ -35,6 +35,9 @@ def test_null_arguments_are_rejected_without_gpu():
     assert b"null context" in L.ocl_last_error()
     assert L.ocl_ttv(None, None, None, 1, 1, 1) == -1
     assert L.ocl_batch_setup(None) == -1
+    assert L.ocl_host_malloc(None, 16, None) == -1
+    assert L.ocl_host_free(None, None) == 0          # no context needed; nothing to free
+    assert L.ocl_ctx_set_group_threshold(None, 4) == -1
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="only meaningful without a GPU")
